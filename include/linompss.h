@@ -1,0 +1,152 @@
+/*
+ * linompss.h -- C ABI of liblinompss.so, the MI355X (gfx950) implementation
+ * of Linompss's tiled Cholesky / SPD solve / iterative refinement path.
+ *
+ * Matrices are column-major.  Only the lower triangle of A is read, and only
+ * the lower triangle is overwritten with the factor L (A = L L^T); the strict
+ * upper triangle is left as the caller passed it.
+ *
+ * Return values of the factor / solve entry points:
+ *     0    success
+ *    k>0   LAPACK-style info: the leading minor of order k is not positive
+ *          definite (the reference discards dpotrf's info and returns 0,
+ *          src/blocked_blas/tasks_potrf.c:22-23; no reference caller reads the
+ *          value, so this is compatible)
+ *    -i    argument i is invalid (LAPACK convention, 1-based)
+ *    <= LINOMPSS_ERR_BASE  a device/runtime error (no GPU, HIP failure, the
+ *          device scheduler gave up waiting, device memory exhausted)
+ */
+#ifndef LINOMPSS_H
+#define LINOMPSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LINOMPSS_ERR_BASE   (-1000)
+#define LINOMPSS_ERR_NODEV  (-1001)  /* no usable HIP device */
+#define LINOMPSS_ERR_HIP    (-1002)  /* a HIP call failed */
+#define LINOMPSS_ERR_NOMEM  (-1003)  /* device allocation failed */
+#define LINOMPSS_ERR_SCHED  (-1004)  /* a work unit waited past the scheduler timeout */
+#define LINOMPSS_ERR_PLAN   (-1005)  /* the planner rejected the shape */
+
+/* ---- Drop-in entry points (host memory, synchronous) -------------------- */
+
+/* Replaces ompss_dchol_rl (src/cholesky/dense/ompss_dense_chol.h:38,
+ * ompss_dense_chol.c:38-40 -> chol_rlmain.c:22-53).  Right-looking tiled
+ * Cholesky with tile b; t is accepted and ignored, as in the reference. */
+int ompss_dchol_rl(int n, int b, int t, double *A, int lda);
+/* Replaces ompss_dchol_ll (ompss_dense_chol.h:34 -> chol_llmain.c:23-57). */
+int ompss_dchol_ll(int n, int b, int t, double *A, int lda);
+/* Replaces ompss_schol_rl (ompss_dense_chol.h:36; FP32 build of chol_rlmain.c). */
+int ompss_schol_rl(int n, int b, int t, float *A, int lda);
+/* Replaces ompss_schol_ll (ompss_dense_chol.h:32; FP32 build of chol_llmain.c). */
+int ompss_schol_ll(int n, int b, int t, float *A, int lda);
+
+/* Replaces dspdsolv_main (src/spdsolv/spdsolv_main.h:20, spdsolv_main.c:10-20):
+ * RL factor of the m x m matrix A (lda = m), then the forward and backward
+ * sweeps on the m x n right-hand sides B (ldb = m), in place. */
+int dspdsolv_main(int m, int n, int b, double *A, double *B);
+/* Replaces sspdsolv_main (spdsolv_main.h:19). */
+int sspdsolv_main(int m, int n, int b, float *A, float *B);
+
+/* Replaces the lower branches of bblas_dltrsm (src/blocked_blas/bblas_trsm.h
+ * :116-160), a header-inline routine in the reference: B := alpha op(L)^{-1} B
+ * with op = N ('N') or T ('T'), L the m x m lower factor in A, B m x n.
+ * uplo must be 'L' and diag 'N' (the only branches on the SPD path). */
+int linompss_dltrsm(char uplo, char trans, char diag, int bm, int bn, int m, int n, double alpha,
+                    const double *A, int lda, double *B, int ldb);
+int linompss_sltrsm(char uplo, char trans, char diag, int bm, int bn, int m, int n, float alpha,
+                    const float *A, int lda, float *B, int ldb);
+
+/* Mixed-precision iterative refinement.  The reference's itref has no C API
+ * (only a CLI, src/itref/itref_config.c:130-223); this entry point keeps its
+ * loop contract (itref.c:999-1101: x0 = solve(b); r = b - A x in FP64;
+ * repeat { test; d = solve(r); x += d; r = b - A x }) with the inner solver
+ * an FP32 tiled Cholesky solve.  The normwise backward error
+ * max_c ||r_c||_inf / (||A||_inf ||x_c||_inf) is tested before each
+ * correction; when it stalls or the sweep cap is reached the FP64 factor
+ * solves the system instead. A (lower triangle, symmetric) is not modified. */
+typedef struct {
+    int max_sweeps;       /* corrections before falling back (default 30) */
+    double tol;           /* backward-error target (default 1e-14) */
+    double stall_ratio;   /* ||r_k+1|| > stall_ratio ||r_k|| is a stall (default 0.5) */
+    int update_prec;      /* trailing-update inputs: 0 FP32, 1 BF16, 2 FP16 (default 0) */
+} linompss_ir_opts;
+
+typedef struct {
+    int sweeps;           /* corrections applied */
+    int fallback;         /* 1 when the FP64 factor produced the result */
+    int info;             /* info of the factor that produced the result */
+    double berr;          /* final normwise backward error */
+    double t_factor_s;    /* device time of the low-precision factor */
+    double t_total_s;     /* wall time of the whole solve */
+} linompss_ir_status;
+
+int linompss_dposv_ir(int n, int nrhs, int b, const double *A, int lda, const double *B, int ldb,
+                      double *X, int ldx, const linompss_ir_opts *opts, linompss_ir_status *st);
+
+/* ---- Device-memory entry points ------------------------------------------
+ * Pointers are HIP device pointers; stream is a hipStream_t (NULL = the null
+ * stream).  Work is enqueued on the stream and the call returns after it
+ * completed (the info value needs the result).  dtype: 0 = FP64, 1 = FP32.
+ * variant: 0 = right-looking (CHOL_RL), 1 = left-looking (CHOL_LL).        */
+
+typedef struct {
+    int policy;           /* unit order: 0 reference priorities, 1 critical path (default) */
+    int unit_m, unit_n;   /* GEMM/SYRK work-unit tile (default 128 x 128) */
+    int wg_per_cu;        /* resident workgroups per CU (0 = occupancy limit) */
+    double timeout_s;     /* per-dependency wait limit on the device (default 30 s) */
+} linompss_exec_opts;
+
+typedef struct {
+    double exec_ms;       /* device time of the last executor launch (HIP events) */
+    int units;            /* work units in that launch */
+    int tasks;            /* reference-level tile tasks in that launch */
+    int grid;             /* persistent workgroups launched */
+    double sim_ms;        /* planner's simulated makespan */
+} linompss_exec_stats;
+
+int linompss_chol_dev(int dtype, int variant, int n, int b, void *dA, int64_t lda, void *stream,
+                      const linompss_exec_opts *opts);
+int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, void *dX, int64_t ldx,
+                      void *stream, const linompss_exec_opts *opts);
+int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
+                       int64_t ldx, void *stream, const linompss_exec_opts *opts);
+int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, int64_t lda, const double *dB,
+                          int64_t ldb, double *dX, int64_t ldx, void *stream, const linompss_ir_opts *opts,
+                          linompss_ir_status *st);
+
+/* Statistics of the last executor launch on this thread. */
+int linompss_last_exec_stats(linompss_exec_stats *st);
+
+/* Task graph of a plan (for checking against the reference's recorded DAG,
+ * visualize/TDG/dpotrf_tdg.dot).  what: 0 RL factor, 1 LL factor, 2 RL factor
+ * + both sweeps.  types[t] in {0 POTRF, 1 TRSM, 2 GEMM, 3 SYRK, 4 sweep TRSM,
+ * 5 sweep GEMM}; coords[3t..3t+2] = tile (i, j, k); edges[2e], edges[2e+1] =
+ * (pred, succ).  Returns 0, or -1 when a buffer is too small (counts are still
+ * stored). */
+int linompss_plan_tdg(int what, int n, int b, int nrhs, int *types, int *coords, int max_tasks, int *edges,
+                      int max_edges, int *ntasks, int *nedges);
+
+/* Unit count and simulated makespan of a plan (no device needed). */
+int linompss_plan_info(int what, int n, int b, int nrhs, const linompss_exec_opts *opts, int *units,
+                       int *tasks, double *sim_ms);
+
+const char *linompss_version(void);
+
+/* ---- Device-memory plumbing (so C and ctypes callers need no other HIP
+ * binding).  kind: 1 host->device, 2 device->host, 3 device->device.       */
+int linompss_device_count(void);
+void *linompss_dev_alloc(size_t bytes);
+int linompss_dev_free(void *p);
+int linompss_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int linompss_dev_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LINOMPSS_H */

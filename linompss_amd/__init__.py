@@ -1,0 +1,341 @@
+"""linompss_amd -- MI355X (gfx950) tiled Cholesky / SPD solve / refinement.
+
+Python host mirror of Linompss's C entry points for this path, as ctypes
+calls into ``liblinompss.so`` (built in-tree by :mod:`linompss_amd.build`).
+The names and argument meanings follow the reference's interface:
+
+* ``ompss_dchol_rl(n, b, t, A, lda)`` and its ``ll`` / ``s`` twins
+  (src/cholesky/dense/ompss_dense_chol.h:32-38),
+* ``dspdsolv_main(m, n, b, A, B)`` (src/spdsolv/spdsolv_main.h:20),
+* ``bblas_dltrsm`` lower branches (src/blocked_blas/bblas_trsm.h:116-160),
+* an iterative-refinement solve keeping itref's loop contract
+  (src/itref/itref.c:999-1101).
+
+Host-array functions take column-major (Fortran-ordered) numpy arrays and
+work in place like the reference.  Device functions take :class:`DeviceMatrix`
+objects (column-major device buffers allocated through the library).
+
+There is no CPU fallback: if the library is missing or cannot be loaded,
+importing this package raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "lib", "LinompssError", "ompss_dchol_rl", "ompss_dchol_ll", "ompss_schol_rl", "ompss_schol_ll",
+    "dspdsolv_main", "sspdsolv_main", "dltrsm", "sltrsm", "dposv_ir", "chol_dev", "posv_dev",
+    "potrs_dev", "dposv_ir_dev", "last_exec_stats", "plan_tdg", "plan_info", "ExecOpts", "IrOpts",
+    "IrStatus", "ExecStats", "DeviceMatrix", "device_count",
+]
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("LINOMPSS_LIB", _PKG / "liblinompss.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -m linompss_amd.build` (hipcc, gfx950). "
+        "There is no CPU fallback.")
+# The library binds to the HIP runtime of the ROCm it was built with
+# (/opt/rocm).  PyTorch bundles an older libamdhip64 with the same soname; a
+# process that initialises torch's runtime first would make this library run
+# on that one, a pairing these kernels are not validated with, so the package
+# does not import torch and its device API works on raw device pointers.
+lib = ctypes.CDLL(str(LIB_PATH))
+
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_f = ctypes.c_float
+_p = ctypes.c_void_p
+_c = ctypes.c_char
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class ExecOpts(ctypes.Structure):
+    _fields_ = [("policy", _i), ("unit_m", _i), ("unit_n", _i), ("wg_per_cu", _i), ("timeout_s", _d)]
+
+
+class ExecStats(ctypes.Structure):
+    _fields_ = [("exec_ms", _d), ("units", _i), ("tasks", _i), ("grid", _i), ("sim_ms", _d)]
+
+
+class IrOpts(ctypes.Structure):
+    _fields_ = [("max_sweeps", _i), ("tol", _d), ("stall_ratio", _d), ("update_prec", _i)]
+
+
+class IrStatus(ctypes.Structure):
+    _fields_ = [("sweeps", _i), ("fallback", _i), ("info", _i), ("berr", _d), ("t_factor_s", _d),
+                ("t_total_s", _d)]
+
+
+def _sig(name, restype, *argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+_sig("ompss_dchol_rl", _i, _i, _i, _i, _p, _i)
+_sig("ompss_dchol_ll", _i, _i, _i, _i, _p, _i)
+_sig("ompss_schol_rl", _i, _i, _i, _i, _p, _i)
+_sig("ompss_schol_ll", _i, _i, _i, _i, _p, _i)
+_sig("dspdsolv_main", _i, _i, _i, _i, _p, _p)
+_sig("sspdsolv_main", _i, _i, _i, _i, _p, _p)
+_sig("linompss_dltrsm", _i, _c, _c, _c, _i, _i, _i, _i, _d, _p, _i, _p, _i)
+_sig("linompss_sltrsm", _i, _c, _c, _c, _i, _i, _i, _i, _f, _p, _i, _p, _i)
+_sig("linompss_dposv_ir", _i, _i, _i, _i, _p, _i, _p, _i, _p, _i, ctypes.POINTER(IrOpts),
+     ctypes.POINTER(IrStatus))
+_sig("linompss_chol_dev", _i, _i, _i, _i, _i, _p, _i64, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_posv_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_potrs_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_dposv_ir_dev", _i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p, ctypes.POINTER(IrOpts),
+     ctypes.POINTER(IrStatus))
+_sig("linompss_last_exec_stats", _i, ctypes.POINTER(ExecStats))
+_sig("linompss_plan_tdg", _i, _i, _i, _i, _i, _ip, _ip, _i, _ip, _i, _ip, _ip)
+_sig("linompss_plan_info", _i, _i, _i, _i, _i, ctypes.POINTER(ExecOpts), _ip, _ip, ctypes.POINTER(_d))
+_sig("linompss_version", ctypes.c_char_p)
+
+ERR_BASE = -1000
+ERR_NAMES = {-1001: "no usable HIP device", -1002: "HIP call failed", -1003: "device allocation failed",
+             -1004: "device scheduler timed out", -1005: "planner rejected the shape"}
+
+
+class LinompssError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        what = ERR_NAMES.get(code, "invalid argument %d" % -code if code < 0 else "info %d" % code)
+        super().__init__(f"{fn} returned {code} ({what})")
+        self.code = code
+
+
+def _check(fn: str, rc: int, allow_info: bool = True) -> int:
+    """Raises on argument/runtime errors; returns LAPACK-style info (>= 0)."""
+    if rc < 0 or (rc > 0 and not allow_info):
+        raise LinompssError(fn, rc)
+    return rc
+
+
+def _colmajor(a: np.ndarray, dtype) -> None:
+    if a.dtype != dtype:
+        raise TypeError(f"expected {np.dtype(dtype).name}, got {a.dtype}")
+    if not a.flags.f_contiguous:
+        raise ValueError("expected a column-major (Fortran-ordered) array")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_p)
+
+
+# ---- host-memory drop-in entry points -------------------------------------
+
+def _chol(name, dtype, A: np.ndarray, b: int, t: int = 0, n: int | None = None, lda: int | None = None) -> int:
+    _colmajor(A, dtype)
+    n = A.shape[1] if n is None else n
+    lda = A.shape[0] if lda is None else lda
+    return _check(name, getattr(lib, name)(n, b, t, _ptr(A), lda))
+
+
+def ompss_dchol_rl(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
+    """Right-looking tiled Cholesky of the lower triangle, in place (chol_rlmain.c:22-53)."""
+    return _chol("ompss_dchol_rl", np.float64, A, b, t, n, lda)
+
+
+def ompss_dchol_ll(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
+    """Left-looking tiled Cholesky (chol_llmain.c:23-57)."""
+    return _chol("ompss_dchol_ll", np.float64, A, b, t, n, lda)
+
+
+def ompss_schol_rl(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
+    return _chol("ompss_schol_rl", np.float32, A, b, t, n, lda)
+
+
+def ompss_schol_ll(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
+    return _chol("ompss_schol_ll", np.float32, A, b, t, n, lda)
+
+
+def dspdsolv_main(A: np.ndarray, B: np.ndarray, b: int) -> int:
+    """Factor A (m x m, lda = m) and solve A X = B in place (spdsolv_main.c:10-20)."""
+    _colmajor(A, np.float64)
+    _colmajor(B, np.float64)
+    m = A.shape[0]
+    if A.shape != (m, m) or B.shape[0] != m:
+        raise ValueError("dspdsolv_main expects A m x m and B m x n with lda = ldb = m")
+    nr = B.shape[1] if B.ndim == 2 else 1
+    return _check("dspdsolv_main", lib.dspdsolv_main(m, nr, b, _ptr(A), _ptr(B)))
+
+
+def sspdsolv_main(A: np.ndarray, B: np.ndarray, b: int) -> int:
+    _colmajor(A, np.float32)
+    _colmajor(B, np.float32)
+    m = A.shape[0]
+    nr = B.shape[1] if B.ndim == 2 else 1
+    return _check("sspdsolv_main", lib.sspdsolv_main(m, nr, b, _ptr(A), _ptr(B)))
+
+
+def dltrsm(trans: str, A: np.ndarray, B: np.ndarray, bm: int, alpha: float = 1.0) -> None:
+    """B := alpha op(L)^{-1} B with L the lower triangle of A (bblas_trsm.h:116-160)."""
+    _colmajor(A, np.float64)
+    _colmajor(B, np.float64)
+    m = A.shape[0]
+    bn = B.shape[1] if B.ndim == 2 else 1
+    _check("linompss_dltrsm", lib.linompss_dltrsm(b"L", trans.encode(), b"N", bm, bn, m, bn, alpha, _ptr(A),
+                                                   A.shape[0], _ptr(B), B.shape[0]), allow_info=False)
+
+
+def sltrsm(trans: str, A: np.ndarray, B: np.ndarray, bm: int, alpha: float = 1.0) -> None:
+    _colmajor(A, np.float32)
+    _colmajor(B, np.float32)
+    m = A.shape[0]
+    bn = B.shape[1] if B.ndim == 2 else 1
+    _check("linompss_sltrsm", lib.linompss_sltrsm(b"L", trans.encode(), b"N", bm, bn, m, bn, alpha, _ptr(A),
+                                                   A.shape[0], _ptr(B), B.shape[0]), allow_info=False)
+
+
+def dposv_ir(A: np.ndarray, B: np.ndarray, b: int, opts: IrOpts | None = None):
+    """Mixed-precision refinement solve; returns (X, IrStatus)."""
+    _colmajor(A, np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, nr = A.shape[0], B.shape[1]
+    X = np.zeros((n, nr), dtype=np.float64, order="F")
+    st = IrStatus()
+    rc = lib.linompss_dposv_ir(n, nr, b, _ptr(A), A.shape[0], _ptr(B), n, _ptr(X), n,
+                               ctypes.byref(opts) if opts is not None else None, ctypes.byref(st))
+    _check("linompss_dposv_ir", rc)
+    return X, st
+
+
+# ---- device memory ------------------------------------------------------------
+
+_sig("linompss_device_count", _i)
+_sig("linompss_dev_alloc", _p, ctypes.c_size_t)
+_sig("linompss_dev_free", _i, _p)
+_sig("linompss_memcpy", _i, _p, _p, ctypes.c_size_t, _i)
+_sig("linompss_dev_sync", _i)
+
+
+def device_count() -> int:
+    return int(lib.linompss_device_count())
+
+
+class DeviceMatrix:
+    """Column-major rows x cols matrix in device memory (leading dimension ld)."""
+
+    def __init__(self, rows: int, cols: int, dtype=np.float64, ld: int | None = None):
+        self.rows, self.cols = int(rows), int(cols)
+        self.dtype = np.dtype(dtype)
+        self.ld = max(1, int(ld if ld is not None else rows))
+        self.nbytes = self.ld * max(1, self.cols) * self.dtype.itemsize
+        p = lib.linompss_dev_alloc(self.nbytes)
+        if not p:
+            raise LinompssError("linompss_dev_alloc", -1003)
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceMatrix":
+        a = np.asfortranarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1, order="F")
+        m = cls(a.shape[0], a.shape[1], a.dtype)
+        m.upload(a)
+        return m
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.asfortranarray(a, dtype=self.dtype).reshape(self.rows, -1, order="F")
+        if self.ld != self.rows:
+            raise ValueError("upload needs ld == rows")
+        _check("linompss_memcpy", lib.linompss_memcpy(self.ptr, a.ctypes.data_as(_p), a.nbytes, 1), False)
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty((self.ld, self.cols), dtype=self.dtype, order="F")
+        _check("linompss_memcpy", lib.linompss_memcpy(out.ctypes.data_as(_p), self.ptr, out.nbytes, 2), False)
+        return out[: self.rows]
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib.linompss_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dt(m: DeviceMatrix) -> int:
+    if m.dtype == np.float64:
+        return 0
+    if m.dtype == np.float32:
+        return 1
+    raise TypeError(f"unsupported dtype {m.dtype}")
+
+
+def _s(stream):
+    return ctypes.c_void_p(stream) if stream else None
+
+
+def chol_dev(A: DeviceMatrix, b: int, variant: str = "rl", opts: ExecOpts | None = None, stream=None) -> int:
+    """Factor the lower triangle of the device matrix A in place; returns info."""
+    rc = lib.linompss_chol_dev(_dt(A), 0 if variant == "rl" else 1, A.cols, b, A.ptr, A.ld, _s(stream),
+                               ctypes.byref(opts) if opts is not None else None)
+    return _check("linompss_chol_dev", rc)
+
+
+def posv_dev(A: DeviceMatrix, X: DeviceMatrix, b: int, opts: ExecOpts | None = None, stream=None) -> int:
+    """Factor A and solve A Y = X in place (one persistent launch); returns info."""
+    rc = lib.linompss_posv_dev(_dt(A), A.cols, X.cols, b, A.ptr, A.ld, X.ptr, X.ld, _s(stream),
+                               ctypes.byref(opts) if opts is not None else None)
+    return _check("linompss_posv_dev", rc)
+
+
+def potrs_dev(L: DeviceMatrix, X: DeviceMatrix, b: int, opts: ExecOpts | None = None, stream=None) -> None:
+    rc = lib.linompss_potrs_dev(_dt(L), L.cols, X.cols, b, L.ptr, L.ld, X.ptr, X.ld, _s(stream),
+                                ctypes.byref(opts) if opts is not None else None)
+    _check("linompss_potrs_dev", rc, allow_info=False)
+
+
+def dposv_ir_dev(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, b: int, opts: IrOpts | None = None,
+                 stream=None) -> IrStatus:
+    st = IrStatus()
+    rc = lib.linompss_dposv_ir_dev(A.cols, B.cols, b, A.ptr, A.ld, B.ptr, B.ld, X.ptr, X.ld, _s(stream),
+                                   ctypes.byref(opts) if opts is not None else None, ctypes.byref(st))
+    _check("linompss_dposv_ir_dev", rc)
+    return st
+
+
+def last_exec_stats() -> ExecStats:
+    st = ExecStats()
+    lib.linompss_last_exec_stats(ctypes.byref(st))
+    return st
+
+
+# ---- plan introspection -----------------------------------------------------
+
+def plan_tdg(what: int, n: int, b: int, nrhs: int = 0):
+    """Task graph of a plan: (types[T], coords[T,3], edges[E,2])."""
+    nt, ne = _i(0), _i(0)
+    lib.linompss_plan_tdg(what, n, b, nrhs, None, None, 0, None, 0, ctypes.byref(nt), ctypes.byref(ne))
+    types = np.zeros(nt.value, dtype=np.int32)
+    coords = np.zeros((nt.value, 3), dtype=np.int32)
+    edges = np.zeros((ne.value, 2), dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(_ip)
+    rc = lib.linompss_plan_tdg(what, n, b, nrhs, ip(types), ip(coords), nt.value, ip(edges), ne.value,
+                               ctypes.byref(nt), ctypes.byref(ne))
+    if rc != 0:
+        raise LinompssError("linompss_plan_tdg", rc)
+    return types, coords, edges
+
+
+def plan_info(what: int, n: int, b: int, nrhs: int = 0, opts: ExecOpts | None = None):
+    u, t, s = _i(0), _i(0), _d(0.0)
+    rc = lib.linompss_plan_info(what, n, b, nrhs, ctypes.byref(opts) if opts is not None else None,
+                                ctypes.byref(u), ctypes.byref(t), ctypes.byref(s))
+    if rc != 0:
+        raise LinompssError("linompss_plan_info", rc)
+    return u.value, t.value, s.value

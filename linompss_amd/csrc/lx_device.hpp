@@ -1,0 +1,530 @@
+// lx_device.hpp -- gfx950 tile kernels of the tiled Cholesky path.
+//
+// Every routine here is a *work unit body*: it is executed by one workgroup of
+// LX_WG_THREADS (4 wavefronts of 64 lanes) inside the persistent executor
+// (lx_exec.hip).  Matrices are column-major with 64-bit leading dimensions,
+// exactly the layout the reference hands to its tile tasks
+// (src/blocked_blas/tasks_*.c, one LAPACK/BLAS call per b x b tile).
+//
+// Arithmetic: the trailing updates (reference task_dgemm / task_dsyrk,
+// src/blocked_blas/task_gemm.c:55-58, tasks_syrk.c:19-21) run on the matrix
+// cores with v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32 (FP32);
+// the diagonal factorisation (task_dpotrf, tasks_potrf.c:20-24) and the
+// triangular solves (task_dtrsm, tasks_trsm.c:22-25) use wavefront-level code
+// with their inner updates on the same MFMA core.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LX_WG_THREADS 256
+
+typedef double lx_d4 __attribute__((ext_vector_type(4)));
+typedef float lx_f4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// MFMA 16x16x4 for both precisions.  Operand layout (identical for f64 and
+// f32): lane l supplies A[row = l%16][k = l/16] and B[k = l/16][col = l%16];
+// the 16x16 result is held as 4 values per lane: D[row = 4*(l/16)+i][col = l%16].
+template <typename T> struct lx_mfma;
+template <> struct lx_mfma<double> {
+    typedef lx_d4 acc;
+    static __device__ __forceinline__ acc mac(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct lx_mfma<float> {
+    typedef lx_f4 acc;
+    static __device__ __forceinline__ acc mac(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ int lx_lane() { return threadIdx.x & 63; }
+__device__ __forceinline__ int lx_wave() { return threadIdx.x >> 6; }
+
+// ---------------------------------------------------------------------------
+// Wave-level C_acc += A * B^T over k in [0, K).
+//   A element (r, k) at A[k*lda + r], rows r in [r0, r0 + 16*FM)
+//   B element (c, k) at B[k*ldb + c], cols c in [c0, c0 + 16*FN)
+// Rows >= M and cols >= N are clamped to the last valid one: their products
+// land in accumulator entries the caller never stores.  Columns k >= K read a
+// clamped, in-bounds address and the A operand is zeroed there.
+// Addressing: one uniform base pointer per operand that advances 4*U columns
+// per step, plus a 32-bit per-lane element offset (a unit's operand spans at
+// most 4 columns ahead of the base, so offsets stay far below 2^31).  Loads of
+// the next K-slab are issued before the MFMAs of the current one (register
+// ping-pong), so global-load latency hides behind the matrix core.
+template <typename T, int FM, int FN, int U>
+__device__ __forceinline__ void lx_wave_gemm_nt(typename lx_mfma<T>::acc (&acc)[FM][FN],
+                                                const T *__restrict__ A, long long lda,
+                                                const T *__restrict__ B, long long ldb,
+                                                int r0, int c0, int M, int N, int K)
+{
+    if (K <= 0 || M <= 0 || N <= 0) return;
+    const int lane = lx_lane();
+    const int lr = lane & 15, lk = lane >> 4;
+    uint32_t oa[FM], ob[FN];  // per-lane byte offsets from the slab base
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        int r = r0 + i * 16 + lr;
+        oa[i] = (uint32_t)(((r < M ? r : M - 1) + lk * lda) * (long long)sizeof(T));
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+        int c = c0 + j * 16 + lr;
+        ob[j] = (uint32_t)(((c < N ? c : N - 1) + lk * ldb) * (long long)sizeof(T));
+    }
+    constexpr int KS = 4 * U;
+    const int kfull = K - K % KS;  // columns covered by unmasked steps
+
+    auto load = [&](T(&as)[U][FM], T(&bs)[U][FN], int k0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const char *Ak = (const char *)(A + (long long)(k0 + 4 * u) * lda);
+            const char *Bk = (const char *)(B + (long long)(k0 + 4 * u) * ldb);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) as[u][i] = *(const T *)(Ak + oa[i]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bs[u][j] = *(const T *)(Bk + ob[j]);
+        }
+    };
+    auto compute = [&](const T(&as)[U][FM], const T(&bs)[U][FN]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = lx_mfma<T>::mac(as[u][i], bs[u][j], acc[i][j]);
+    };
+
+    if (kfull > 0) {
+        T ca[U][FM], cb[U][FN];
+        load(ca, cb, 0);
+        for (int k0 = 0; k0 < kfull; k0 += KS) {
+            // prefetch the next slab (the last iteration re-reads its own slab: always in bounds)
+            const int kn = k0 + KS < kfull ? k0 + KS : k0;
+            T na[U][FM], nb[U][FN];
+            load(na, nb, kn);
+            compute(ca, cb);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i) ca[u][i] = na[u][i];
+#pragma unroll
+                for (int j = 0; j < FN; ++j) cb[u][j] = nb[u][j];
+            }
+        }
+    }
+    // ragged tail: steps of 4 columns, out-of-range k masked on the A operand
+    for (int k0 = kfull; k0 < K; k0 += 4) {
+        const bool kv = k0 + lk < K;
+        const int kb = kv ? k0 : K - 1 - lk;  // slab base keeping this lane's column in range
+        const char *Ak = (const char *)(A + (long long)kb * lda);
+        const char *Bk = (const char *)(B + (long long)kb * ldb);
+        T as[FM], bs[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            T v = *(const T *)(Ak + oa[i]);
+            as[i] = kv ? v : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bs[j] = *(const T *)(Bk + ob[j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = lx_mfma<T>::mac(as[i], bs[j], acc[i][j]);
+    }
+}
+
+template <typename T, int FM, int FN>
+__device__ __forceinline__ void lx_acc_zero(typename lx_mfma<T>::acc (&acc)[FM][FN])
+{
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
+}
+
+// C[r][c] -= acc for r < M, c < N (and r >= c + diag when lower).  r0/c0 are
+// the wave block's origin inside C; gdiag is (global row - global col) of
+// C's element (0,0), so the lower mask keeps r + gdiag >= c.
+template <typename T, int FM, int FN>
+__device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)[FM][FN], T *__restrict__ C,
+                                                 long long ldc, int r0, int c0, int M, int N, bool lower,
+                                                 int gdiag)
+{
+    const int lane = lx_lane();
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            int c = c0 + j * 16 + lr;
+            if (c >= N) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int r = r0 + i * 16 + lk * 4 + e;
+                if (r < M && (!lower || r + gdiag >= c)) {
+                    T *p = C + (long long)c * ldc + r;
+                    *p = *p - acc[i][j][e];
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM / SYRK trailing-update unit:  C[M x N] -= A[M x K] * B[N x K]^T  (lower
+// mask on the diagonal of a SYRK tile).  The WG covers up to
+// (2*16*FM) x (2*16*FN) of C with its 2 x 2 wavefronts.  Wave blocks that lie
+// strictly above the diagonal of a SYRK tile are skipped.
+template <typename T, int FM, int FN, int U>
+__device__ __forceinline__ void lx_unit_gemm_nt(T *__restrict__ C, long long ldc, const T *__restrict__ A,
+                                                long long lda, const T *__restrict__ B, long long ldb, int M,
+                                                int N, int K, bool lower, int gdiag)
+{
+    const int w = lx_wave();
+    const int r0 = (w >> 1) * 16 * FM, c0 = (w & 1) * 16 * FN;
+    if (r0 >= M || c0 >= N) return;
+    if (lower && r0 + 16 * FM - 1 + gdiag < c0) return;  // block strictly above the diagonal
+    typename lx_mfma<T>::acc acc[FM][FN];
+    lx_acc_zero<T, FM, FN>(acc);
+    lx_wave_gemm_nt<T, FM, FN, U>(acc, A, lda, B, ldb, r0, c0, M, N, K);
+    lx_acc_sub_store<T, FM, FN>(acc, C, ldc, r0, c0, M, N, lower, gdiag);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-tile Cholesky (reference task_dpotrf -> dpotrf_("L"), tasks_potrf.c
+// :20-24), blocked right-looking inside the tile with panel width PB:
+//   1) factor the PB x PB diagonal block in LDS (one barrier per column);
+//   2) solve the panel below it against that block (one row per thread);
+//   3) update the lower trailing part with the MFMA core.
+// Returns through *info the 1-based column (local) of the first non-positive
+// pivot, as LAPACK's info, and keeps going (results past that column are not
+// meaningful, as in LAPACK).
+template <typename T> struct lx_potrf_cfg { static constexpr int PB = 32; };
+
+template <typename T>
+__device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info_local, char *smem)
+{
+    constexpr int PB = lx_potrf_cfg<T>::PB;
+    T(*sD)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem);
+    T(*sL)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem + sizeof(T) * PB * (PB + 1));
+    __shared__ int s_bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_bad = 0;
+    for (int p = 0; p < M; p += PB) {
+        const int pb = M - p < PB ? M - p : PB;
+        // 1) load diagonal block (lower part used)
+        for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
+            int r = e % PB, c = e / PB;
+            sD[r][c] = (r < pb && c < pb) ? A[(long long)(p + c) * lda + p + r] : T(0);
+            sL[r][c] = T(0);
+        }
+        __syncthreads();
+        for (int c = 0; c < pb; ++c) {
+            T dcc = sD[c][c];
+            bool bad = !(dcc > T(0));
+            T d = sqrt(dcc);  // NaN past a failed pivot, like LAPACK's garbage-after-info
+            if (tid == 0 && bad && s_bad == 0) s_bad = p + c + 1;
+            // column c of L (scaled) -> sL; trailing columns of sD updated
+            for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
+                int r = e % PB, c2 = e / PB;
+                if (r >= pb || c2 >= pb || r < c2) continue;
+                if (c2 == c) {
+                    sL[r][c] = (r == c) ? d : sD[r][c] / d;
+                } else if (c2 > c) {
+                    T lr_ = sD[r][c] / d, lc_ = sD[c2][c] / d;
+                    sD[r][c2] = sD[r][c2] - lr_ * lc_;
+                }
+            }
+            __syncthreads();
+        }
+        // write L_pp back
+        for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
+            int r = e % PB, c = e / PB;
+            if (r < pb && c < pb && r >= c) A[(long long)(p + c) * lda + p + r] = sL[r][c];
+        }
+        // 2) panel rows below: x = a * L_pp^{-T}
+        const int r_beg = p + pb;
+        for (int r = r_beg + tid; r < M; r += LX_WG_THREADS) {
+            T x[PB];
+#pragma unroll
+            for (int c = 0; c < PB; ++c) x[c] = c < pb ? A[(long long)(p + c) * lda + r] : T(0);
+#pragma unroll
+            for (int c = 0; c < PB; ++c) {
+                if (c < pb) {
+                    T s = x[c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) s -= x[c2] * sL[c][c2];
+                    x[c] = s / sL[c][c];
+                }
+                // keep the LDS reads of later columns from being hoisted (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+                if (c < pb) A[(long long)(p + c) * lda + r] = x[c];
+        }
+        __syncthreads();
+        // 3) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel
+        const int mt = M - r_beg;
+        if (mt > 0) {
+            constexpr int FMU = 4, FNU = 4, BS = 2 * 16 * 4;  // 128 x 128 blocks
+            const T *P = A + (long long)p * lda + r_beg;
+            T *Ct = A + (long long)r_beg * lda + r_beg;
+            const int nb = (mt + BS - 1) / BS;
+            for (int bj = 0; bj < nb; ++bj)
+                for (int bi = bj; bi < nb; ++bi) {
+                    int rr = bi * BS, cc = bj * BS;
+                    lx_unit_gemm_nt<T, FMU, FNU, 2>(Ct + (long long)cc * lda + rr, lda, P + rr, lda, P + cc, lda,
+                                                    mt - rr < BS ? mt - rr : BS, mt - cc < BS ? mt - cc : BS, pb,
+                                                    bi == bj, rr - cc);
+                }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *info_local = s_bad;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Panel TRSM unit (reference task_dtrsm(R, L, T, N), chol_rlmain.c:37 ->
+// tasks_trsm.c:22-25):  X[R x N] = B[R x N] * L^{-T}, rows independent.
+// Column-blocked by NB: X_p = (B_p - X_{<p} L_{p,<p}^T) L_pp^{-T}; the update
+// runs on the MFMA core, the NB x NB solve one row per lane.
+template <typename T>
+__device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L, long long ldl,
+                                  int R, int N, char *smem)
+{
+    constexpr int NB = 32, RU = 64;  // R <= RU rows per unit
+    T(*sS)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem);
+    T(*sDg)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem + sizeof(T) * RU * (NB + 1));
+    const int tid = threadIdx.x, w = lx_wave();
+    for (int p = 0; p < N; p += NB) {
+        const int pb = N - p < NB ? N - p : NB;
+        // update: acc = X[:, 0:p] * L[p:p+pb, 0:p]^T  (waves 2 x 2 of 32 x 16)
+        typename lx_mfma<T>::acc acc[2][1];
+        lx_acc_zero<T, 2, 1>(acc);
+        const int r0 = (w >> 1) * 32, c0 = (w & 1) * 16;
+        if (p > 0) lx_wave_gemm_nt<T, 2, 1, 2>(acc, Bm, ldb, L + p, ldl, r0, c0, R, pb, p);
+        // S = B_p - acc -> LDS
+        {
+            const int lane = lx_lane(), lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    int r = r0 + i * 16 + lk * 4 + e, c = c0 + lr;
+                    if (r < R && c < pb) sS[r][c] = Bm[(long long)(p + c) * ldb + r] - acc[i][0][e];
+                }
+        }
+        for (int e = tid; e < NB * NB; e += LX_WG_THREADS) {
+            int r = e % NB, c = e / NB;
+            if (r < pb && c <= r) sDg[r][c] = L[(long long)(p + c) * ldl + p + r];
+        }
+        __syncthreads();
+        if (tid < R) {
+            T x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < pb) {
+                    T s = sS[tid][c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) s -= x[c2] * sDg[c][c2];
+                    x[c] = s / sDg[c][c];
+                } else {
+                    x[c] = T(0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < pb) Bm[(long long)(p + c) * ldb + tid] = x[c];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand-side sweeps (reference bblas_dltrsm lower branches,
+// src/blocked_blas/bblas_trsm.h:119-160): X has nrhs columns (ldx).
+
+// X_i[M x nrhs] := L_ii^{-1} X_i  (trans = 0)  or  L_ii^{-T} X_i  (trans = 1).
+// Blocked by 64 rows; the 64 x 64 diagonal solve runs on one wavefront per
+// right-hand side (lane = row), the off-diagonal update on all 256 threads.
+template <typename T>
+__device__ void lx_unit_trsv_block(T *__restrict__ X, long long ldx, const T *__restrict__ L, long long ldl,
+                                   int M, int nrhs, int trans, char *smem)
+{
+    constexpr int NB = 64;
+    T(*sD)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem);
+    T(*sX)[NB] = reinterpret_cast<T(*)[NB]>(smem + sizeof(T) * NB * (NB + 1));
+    const int tid = threadIdx.x, lane = lx_lane(), w = lx_wave();
+    const int nblk = (M + NB - 1) / NB;
+    for (int cb = 0; cb < nrhs; cb += 16) {
+        const int nc = nrhs - cb < 16 ? nrhs - cb : 16;
+        for (int q = 0; q < nblk; ++q) {
+            const int blk = trans ? nblk - 1 - q : q;
+            const int p = blk * NB;
+            const int pb = M - p < NB ? M - p : NB;
+            // load diagonal block (as L or L^T so the solve is always "lower, forward" in sD)
+            for (int e = tid; e < NB * NB; e += LX_WG_THREADS) {
+                int r = e % NB, c = e / NB;
+                T v = T(0);
+                if (r < pb && c < pb) {
+                    if (!trans) v = (r >= c) ? L[(long long)(p + c) * ldl + p + r] : T(0);
+                    else v = (c >= r) ? L[(long long)(p + r) * ldl + p + c] : T(0);  // (L^T)[r][c]=L[c][r]; upper
+                }
+                sD[r][c] = v;
+            }
+            for (int e = tid; e < 16 * NB; e += LX_WG_THREADS) {
+                int r = e % NB, c = e / NB;
+                sX[c][r] = (r < pb && c < nc) ? X[(long long)(cb + c) * ldx + p + r] : T(0);
+            }
+            __syncthreads();
+            // each wave solves columns c = w, w+4, ...; lane = row
+            for (int c = w; c < nc; c += 4) {
+                T s = sX[c][lane];
+                if (!trans) {
+                    for (int j = 0; j < pb; ++j) {
+                        T xj = __shfl(s, j) / sD[j][j];
+                        if (lane == j) s = xj;
+                        else if (lane > j) s -= sD[lane][j] * xj;
+                    }
+                } else {
+                    // upper-triangular U = L_pp^T: backward substitution
+                    for (int j = pb - 1; j >= 0; --j) {
+                        T xj = __shfl(s, j) / sD[j][j];
+                        if (lane == j) s = xj;
+                        else if (lane < j) s -= sD[lane][j] * xj;
+                    }
+                }
+                sX[c][lane] = s;
+            }
+            __syncthreads();
+            for (int e = tid; e < 16 * NB; e += LX_WG_THREADS) {
+                int r = e % NB, c = e / NB;
+                if (r < pb && c < nc) X[(long long)(cb + c) * ldx + p + r] = sX[c][r];
+            }
+            // update the remaining rows of this block
+            if (!trans) {
+                // rows below: X[r] -= L[r, p:p+pb] x_p,  r in [p+pb, M)
+                for (int r = p + pb + tid; r < M; r += LX_WG_THREADS) {
+                    T s[16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) s[c] = T(0);
+                    for (int j = 0; j < pb; ++j) {
+                        T l = L[(long long)(p + j) * ldl + r];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) s[c] += l * sX[c][j];
+                    }
+                    for (int c = 0; c < nc; ++c) X[(long long)(cb + c) * ldx + r] -= s[c];
+                }
+            } else {
+                // rows above: X[r] -= sum_j L[p+j][r] x_j,  r in [0, p)  (column r of L, rows p..)
+                for (int r = tid; r < p; r += LX_WG_THREADS) {
+                    T s[16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) s[c] = T(0);
+                    for (int j = 0; j < pb; ++j) {
+                        T l = L[(long long)r * ldl + p + j];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) s[c] += l * sX[c][j];
+                    }
+                    for (int c = 0; c < nc; ++c) X[(long long)(cb + c) * ldx + r] -= s[c];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Y[M x nrhs] -= L[M x K] * Z[K x nrhs]   (trans = 0; forward-sweep GEMM)
+// Y[M x nrhs] -= L[K x M]^T * Z[K x nrhs] (trans = 1; backward-sweep GEMM)
+// Memory-bound: L is read once, coalesced along its columns.
+template <typename T>
+__device__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ldy, const T *__restrict__ L, long long ldl,
+                                 const T *__restrict__ Z, long long ldz, int M, int K, int nrhs, int trans,
+                                 char *smem)
+{
+    constexpr int KC = 64;
+    // sRed reuses sT's storage once the K loop of a row block is done
+    T(*sZ)[KC] = reinterpret_cast<T(*)[KC]>(smem);
+    T(*sT)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + sizeof(T) * 16 * KC);
+    T(*sRed)[16][64] = reinterpret_cast<T(*)[16][64]>(smem + sizeof(T) * 16 * KC);
+    const int tid = threadIdx.x;
+    for (int cb = 0; cb < nrhs; cb += 16) {
+        const int nc = nrhs - cb < 16 ? nrhs - cb : 16;
+        if (!trans) {
+            // thread: row r = r0 + (tid & 63) handled by 4 k-groups (tid >> 6)
+            for (int r0 = 0; r0 < M; r0 += 64) {
+                const int r = r0 + (tid & 63), g = tid >> 6;
+                T s[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) s[c] = T(0);
+                for (int k0 = 0; k0 < K; k0 += KC) {
+                    __syncthreads();
+                    for (int e = tid; e < 16 * KC; e += LX_WG_THREADS) {
+                        int k = e % KC, c = e / KC;
+                        sZ[c][k] = (k0 + k < K && c < nc) ? Z[(long long)(cb + c) * ldz + k0 + k] : T(0);
+                    }
+                    __syncthreads();
+                    const int kend = K - k0 < KC ? K - k0 : KC;
+                    if (r < M)
+                        for (int k = g; k < kend; k += 4) {
+                            T l = L[(long long)(k0 + k) * ldl + r];
+#pragma unroll
+                            for (int c = 0; c < 16; ++c) s[c] += l * sZ[c][k];
+                        }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < 16; ++c) sRed[g][c][tid & 63] = s[c];
+                __syncthreads();
+                if (tid < 64 && r0 + tid < M)
+                    for (int c = 0; c < nc; ++c)
+                        Y[(long long)(cb + c) * ldy + r0 + tid] -=
+                            (sRed[0][c][tid] + sRed[1][c][tid]) + (sRed[2][c][tid] + sRed[3][c][tid]);
+                __syncthreads();
+            }
+        } else {
+            // output row r (a column of L); stage L[k0:k0+64, r0:r0+64] transposed in LDS
+            for (int r0 = 0; r0 < M; r0 += 64) {
+                const int rl = tid & 63, g = tid >> 6;
+                T s[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) s[c] = T(0);
+                for (int k0 = 0; k0 < K; k0 += KC) {
+                    __syncthreads();
+                    for (int e = tid; e < 16 * KC; e += LX_WG_THREADS) {
+                        int k = e % KC, c = e / KC;
+                        sZ[c][k] = (k0 + k < K && c < nc) ? Z[(long long)(cb + c) * ldz + k0 + k] : T(0);
+                    }
+                    for (int e = tid; e < 64 * KC; e += LX_WG_THREADS) {
+                        int k = e % KC, rr = e / KC;
+                        sT[rr][k] = (k0 + k < K && r0 + rr < M) ? L[(long long)(r0 + rr) * ldl + k0 + k] : T(0);
+                    }
+                    __syncthreads();
+                    for (int k = g; k < KC; k += 4) {
+                        T l = sT[rl][k];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) s[c] += l * sZ[c][k];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < 16; ++c) sRed[g][c][rl] = s[c];
+                __syncthreads();
+                if (tid < 64 && r0 + tid < M)
+                    for (int c = 0; c < nc; ++c)
+                        Y[(long long)(cb + c) * ldy + r0 + tid] -=
+                            (sRed[0][c][tid] + sRed[1][c][tid]) + (sRed[2][c][tid] + sRed[3][c][tid]);
+                __syncthreads();
+            }
+        }
+    }
+}

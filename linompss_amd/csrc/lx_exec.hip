@@ -1,0 +1,601 @@
+// lx_exec.hip -- persistent tile-DAG executor and the factor / solve entry
+// points of liblinompss.so.
+//
+// The reference runs its tile tasks through the OmpSs runtime: the master
+// thread spawns them (src/cholesky/dense/chol_rlmain.c:25-50), Nanos++ worker
+// threads execute them in priority order once their in/inout tiles are ready,
+// and the caller waits with `#pragma omp taskwait` (chol.c:72).  Here one
+// kernel launch replaces all of that: every workgroup loops
+//   grab the next unit (one atomic) -> wait until each dependency counter
+//   reaches its threshold -> run the tile kernel -> publish (release fence,
+//   counter + 1)
+// until the unit list is exhausted.  Units are listed in a topological order
+// chosen by the host planner, so every wait is on a unit already held by a
+// running workgroup and the launch cannot deadlock, whatever the occupancy.
+// A wait that exceeds the timeout sets an abort flag that every workgroup
+// observes, so each wave reaches the kernel's exit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "linompss.h"
+#include "lx_exec_kernel.hpp"
+#include "lx_internal.hpp"
+#include "lx_planner.hpp"
+
+__global__ void lx_fill_i32_kernel(int32_t *p, int32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+void lx_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(lx_fill_i32_kernel, dim3(blocks), dim3(256), 0, s, p, v, n);
+}
+
+// ---------------------------------------------------------------------------
+// Host side: device context, plan cache, scratch.
+
+namespace {
+
+struct lx_dplan {
+    std::string key;
+    lx_unit *d_units = nullptr;
+    int nunits = 0, ntasks = 0, nctr = 0;
+    double sim_s = 0.0;
+};
+
+struct lx_dev_ctx {
+    int dev = -1;
+    int cus = 0;
+    int wall_khz = 0;
+    int occ[2] = {0, 0};  // resident WGs per CU for the FP64 / FP32 executor
+    std::list<lx_dplan> plans;  // most recent first
+    void *scratch[8] = {};
+    size_t scratch_sz[8] = {};
+};
+
+std::mutex g_mu;
+std::vector<std::unique_ptr<lx_dev_ctx>> g_ctx;
+thread_local linompss_exec_stats g_last_stats = {};
+
+lx_dev_ctx *ctx_get()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    if ((int)g_ctx.size() <= dev) g_ctx.resize(dev + 1);
+    if (!g_ctx[dev]) {
+        auto c = std::make_unique<lx_dev_ctx>();
+        c->dev = dev;
+        if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return nullptr;
+        if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+            c->wall_khz <= 0)
+            c->wall_khz = 100000;
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lx_exec_kernel<double>, LX_WG_THREADS, 0) !=
+            hipSuccess)
+            return nullptr;
+        c->occ[0] = std::max(1, nb);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lx_exec_kernel<float>, LX_WG_THREADS, 0) !=
+            hipSuccess)
+            return nullptr;
+        c->occ[1] = std::max(1, nb);
+        g_ctx[dev] = std::move(c);
+    }
+    return g_ctx[dev].get();
+}
+
+void *scratch_get(lx_dev_ctx *c, int slot, size_t bytes)
+{
+    if (slot < 0 || slot >= 8) return nullptr;
+    if (c->scratch_sz[slot] >= bytes && c->scratch[slot]) return c->scratch[slot];
+    if (c->scratch[slot]) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(c->scratch[slot]);
+        c->scratch[slot] = nullptr;
+        c->scratch_sz[slot] = 0;
+    }
+    size_t sz = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&c->scratch[slot], sz) != hipSuccess) {
+        c->scratch[slot] = nullptr;
+        return nullptr;
+    }
+    c->scratch_sz[slot] = sz;
+    return c->scratch[slot];
+}
+
+void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_opts *o)
+{
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    P.elem_bytes = dtype == 0 ? 8 : 4;
+    P.workers = workers;
+    if (dtype == 1) {
+        P.flop_rate *= 2.0;
+        P.potrf_rate *= 1.5;
+        P.trsm_rate *= 1.5;
+    }
+    if (o) P.policy = o->policy;
+}
+
+bool build_plan(lx_planner &pl, const lx_shape &sh)
+{
+    int s = pl.add_system(sh.n, sh.b, sh.nrhs);
+    switch (sh.what) {
+    case LX_W_FACTOR_RL: pl.factor_rl(s); break;
+    case LX_W_FACTOR_LL: pl.factor_ll(s); break;
+    case LX_W_POSV_RL:
+        pl.factor_rl(s);
+        pl.sweep_forward(s);
+        pl.sweep_backward(s);
+        break;
+    case LX_W_SWEEPS:
+        pl.sweep_forward(s);
+        pl.sweep_backward(s);
+        break;
+    case LX_W_FWD: pl.sweep_forward(s); break;
+    case LX_W_BWD: pl.sweep_backward(s); break;
+    default: return false;
+    }
+    pl.finalize();
+    return true;
+}
+
+// Returns a cached device plan (or builds and uploads one).
+lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linompss_exec_opts *o)
+{
+    lx_plan_params P;
+    fill_params(P, sh.dtype, workers, o);
+    char key[256];
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs,
+             P.policy, P.unit_m, P.unit_n, P.workers);
+    for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
+        if (it->key == key) {
+            c->plans.splice(c->plans.begin(), c->plans, it);
+            return &c->plans.front();
+        }
+    lx_planner pl(P);
+    try {
+        if (!build_plan(pl, sh)) return nullptr;
+    } catch (...) {
+        return nullptr;
+    }
+    lx_dplan d;
+    d.key = key;
+    d.nunits = (int)pl.units().size();
+    d.ntasks = (int)pl.tasks().size();
+    d.nctr = pl.num_counters();
+    d.sim_s = pl.simulated_time();
+    if (d.nunits > 0) {
+        if (hipMalloc(&d.d_units, sizeof(lx_unit) * d.nunits) != hipSuccess) return nullptr;
+        if (hipMemcpy(d.d_units, pl.units().data(), sizeof(lx_unit) * d.nunits, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            (void)hipFree(d.d_units);
+            return nullptr;
+        }
+    }
+    c->plans.push_front(d);
+    while (c->plans.size() > 8) {
+        (void)hipDeviceSynchronize();
+        if (c->plans.back().d_units) (void)hipFree(c->plans.back().d_units);
+        c->plans.pop_back();
+    }
+    return &c->plans.front();
+}
+
+}  // namespace
+
+void *lx_scratch(int slot, size_t bytes)
+{
+    lx_dev_ctx *c = ctx_get();
+    return c ? scratch_get(c, slot, bytes) : nullptr;
+}
+
+// Scratch slots: 0 = executor control (status, counters, info, sys table),
+// 1..3 = host-API staging, 4..7 = refinement workspaces.
+static bool lx_trace_on()
+{
+    static int on = -1;
+    if (on < 0) on = getenv("LINOMPSS_TRACE") ? 1 : 0;
+    return on == 1;
+}
+#define LX_TRACE(...)                                                                                               \
+    do {                                                                                                            \
+        if (lx_trace_on()) {                                                                                        \
+            fprintf(stderr, "[linompss] " __VA_ARGS__);                                                             \
+            fputc('\n', stderr);                                                                                    \
+            fflush(stderr);                                                                                         \
+        }                                                                                                           \
+    } while (0)
+
+int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t ldx, hipStream_t stream,
+                  const linompss_exec_opts *opts, int *info_out)
+{
+    LX_TRACE("run what=%d dtype=%d n=%d b=%d nrhs=%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs);
+    lx_dev_ctx *c = ctx_get();
+    LX_TRACE("ctx %p cus=%d wall_khz=%d occ=%d/%d", (void *)c, c ? c->cus : 0, c ? c->wall_khz : 0,
+             c ? c->occ[0] : 0, c ? c->occ[1] : 0);
+    if (!c) return LINOMPSS_ERR_NODEV;
+    const int occ = c->occ[sh.dtype == 0 ? 0 : 1];
+    int wpc = occ;
+    if (opts && opts->wg_per_cu > 0) wpc = std::min(opts->wg_per_cu, occ);
+    const int workers = c->cus * wpc;
+    lx_dplan *pl = plan_get(c, sh, workers, opts);
+    if (!pl) return LINOMPSS_ERR_PLAN;
+    LX_TRACE("plan units=%d tasks=%d ctr=%d", pl->nunits, pl->ntasks, pl->nctr);
+    if (pl->nunits == 0) {
+        *info_out = 0;
+        return 0;
+    }
+    // control block: status | info | sys | counters
+    const size_t off_info = 64, off_sys = 128, off_ctr = 256;
+    const size_t bytes = off_ctr + sizeof(int32_t) * (size_t)pl->nctr;
+    char *ctl = (char *)scratch_get(c, 0, bytes);
+    if (!ctl) return LINOMPSS_ERR_NOMEM;
+    lx_sys hs = {};
+    hs.A = dA;
+    hs.X = dX;
+    hs.lda = lda;
+    hs.ldx = ldx;
+    hs.n = sh.n;
+    hs.nrhs = sh.nrhs;
+    hs.info_slot = 0;
+    LX_HIP_TRY(hipMemcpyAsync(ctl + off_sys, &hs, sizeof hs, hipMemcpyHostToDevice, stream));
+    lx_fill_i32((int32_t *)ctl, 0, 16, stream);
+    lx_fill_i32((int32_t *)(ctl + off_info), INT_MAX, 1, stream);
+    lx_fill_i32((int32_t *)(ctl + off_ctr), 0, pl->nctr, stream);
+    double tmo = 30.0;
+    if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
+    if (opts && opts->timeout_s > 0) tmo = opts->timeout_s;
+    const long long ticks = (long long)(tmo * c->wall_khz * 1000.0);
+    const int grid = std::max(1, std::min(pl->nunits, workers));
+    hipEvent_t e0, e1;
+    LX_HIP_TRY(hipEventCreate(&e0));
+    LX_HIP_TRY(hipEventCreate(&e1));
+    LX_HIP_TRY(hipEventRecord(e0, stream));
+    if (sh.dtype == 0)
+        hipLaunchKernelGGL(lx_exec_kernel<double>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
+                           pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
+                           (int *)(ctl + off_info), ticks);
+    else
+        hipLaunchKernelGGL(lx_exec_kernel<float>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
+                           pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
+                           (int *)(ctl + off_info), ticks);
+    LX_HIP_TRY(hipGetLastError());
+    LX_TRACE("launched grid=%d", grid);
+    LX_HIP_TRY(hipEventRecord(e1, stream));
+    lx_status hst;
+    int hinfo = 0;
+    LX_HIP_TRY(hipMemcpyAsync(&hst, ctl, sizeof hst, hipMemcpyDeviceToHost, stream));
+    LX_HIP_TRY(hipMemcpyAsync(&hinfo, ctl + off_info, sizeof hinfo, hipMemcpyDeviceToHost, stream));
+    LX_HIP_TRY(hipStreamSynchronize(stream));
+    LX_TRACE("done head=%d abort=%d timeouts=%d done=%d info=%d", hst.head, hst.abort, hst.timeouts, hst.done, hinfo);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_last_stats.exec_ms = ms;
+    g_last_stats.units = pl->nunits;
+    g_last_stats.tasks = pl->ntasks;
+    g_last_stats.grid = grid;
+    g_last_stats.sim_ms = pl->sim_s * 1e3;
+    if (hst.abort || hst.timeouts || hst.done != pl->nunits) return LINOMPSS_ERR_SCHED;
+    *info_out = hinfo == INT_MAX ? 0 : hinfo;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI.
+
+static int check_args(int n, int b, int64_t lda)
+{
+    if (n < 0) return -1;
+    if (b <= 0) return -2;
+    if (lda < std::max(1, n)) return -5;
+    return 0;
+}
+
+extern "C" int linompss_chol_dev(int dtype, int variant, int n, int b, void *dA, int64_t lda, void *stream,
+                                 const linompss_exec_opts *opts)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (variant != 0 && variant != 1) return -2;
+    if (n < 0) return -3;
+    if (b <= 0) return -4;
+    if (lda < std::max(1, n)) return -6;
+    if (n == 0) return 0;
+    std::lock_guard<std::mutex> g(g_mu);
+    lx_shape sh = {variant == 0 ? LX_W_FACTOR_RL : LX_W_FACTOR_LL, dtype, n, b, 0};
+    int info = 0;
+    int rc = lx_run_single(sh, dA, lda, nullptr, 0, (hipStream_t)stream, opts, &info);
+    return rc ? rc : info;
+}
+
+extern "C" int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, void *dX, int64_t ldx,
+                                 void *stream, const linompss_exec_opts *opts)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (b <= 0) return -4;
+    if (lda < std::max(1, n)) return -6;
+    if (ldx < std::max(1, n)) return -8;
+    if (n == 0) return 0;
+    std::lock_guard<std::mutex> g(g_mu);
+    lx_shape sh = {nrhs > 0 ? LX_W_POSV_RL : LX_W_FACTOR_RL, dtype, n, b, nrhs};
+    int info = 0;
+    int rc = lx_run_single(sh, dA, lda, dX, ldx, (hipStream_t)stream, opts, &info);
+    return rc ? rc : info;
+}
+
+extern "C" int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
+                                  int64_t ldx, void *stream, const linompss_exec_opts *opts)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (b <= 0) return -4;
+    if (lda < std::max(1, n)) return -6;
+    if (ldx < std::max(1, n)) return -8;
+    if (n == 0 || nrhs == 0) return 0;
+    std::lock_guard<std::mutex> g(g_mu);
+    lx_shape sh = {LX_W_SWEEPS, dtype, n, b, nrhs};
+    int info = 0;
+    return lx_run_single(sh, const_cast<void *>(dL), lda, dX, ldx, (hipStream_t)stream, opts, &info);
+}
+
+extern "C" int linompss_last_exec_stats(linompss_exec_stats *st)
+{
+    if (!st) return -1;
+    *st = g_last_stats;
+    return 0;
+}
+
+// Host-memory wrappers: stage through device memory with the caller's lda.
+template <typename T>
+static int host_chol(int variant, int n, int b, T *A, int lda)
+{
+    int a = check_args(n, b, lda);
+    if (a) return a;
+    if (n == 0) return 0;
+    const int dtype = sizeof(T) == 8 ? 0 : 1;
+    T *dA = nullptr;
+    if (hipMalloc(&dA, sizeof(T) * (size_t)n * n) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    int rc = 0;
+    if (hipMemcpy2D(dA, sizeof(T) * n, A, sizeof(T) * (size_t)lda, sizeof(T) * n, n, hipMemcpyHostToDevice) !=
+        hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc) rc = linompss_chol_dev(dtype, variant, n, b, dA, n, nullptr, nullptr);
+    if (rc >= 0 &&
+        hipMemcpy2D(A, sizeof(T) * (size_t)lda, dA, sizeof(T) * n, sizeof(T) * n, n, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    (void)hipFree(dA);
+    return rc;
+}
+
+extern "C" int ompss_dchol_rl(int n, int b, int t, double *A, int lda)
+{
+    (void)t;
+    return host_chol<double>(0, n, b, A, lda);
+}
+extern "C" int ompss_dchol_ll(int n, int b, int t, double *A, int lda)
+{
+    (void)t;
+    return host_chol<double>(1, n, b, A, lda);
+}
+extern "C" int ompss_schol_rl(int n, int b, int t, float *A, int lda)
+{
+    (void)t;
+    return host_chol<float>(0, n, b, A, lda);
+}
+extern "C" int ompss_schol_ll(int n, int b, int t, float *A, int lda)
+{
+    (void)t;
+    return host_chol<float>(1, n, b, A, lda);
+}
+
+template <typename T>
+static int host_spdsolv(int m, int n, int b, T *A, T *B)
+{
+    if (m < 0) return -1;
+    if (n < 0) return -2;
+    if (b <= 0) return -3;
+    if (m == 0) return 0;
+    const int dtype = sizeof(T) == 8 ? 0 : 1;
+    T *dA = nullptr, *dB = nullptr;
+    if (hipMalloc(&dA, sizeof(T) * (size_t)m * m) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    if (n > 0 && hipMalloc(&dB, sizeof(T) * (size_t)m * n) != hipSuccess) {
+        (void)hipFree(dA);
+        return LINOMPSS_ERR_NOMEM;
+    }
+    int rc = 0;
+    if (hipMemcpy(dA, A, sizeof(T) * (size_t)m * m, hipMemcpyHostToDevice) != hipSuccess) rc = LINOMPSS_ERR_HIP;
+    if (!rc && n > 0 && hipMemcpy(dB, B, sizeof(T) * (size_t)m * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc) rc = linompss_posv_dev(dtype, m, n, b, dA, m, dB ? (void *)dB : (void *)dA, m, nullptr, nullptr);
+    if (rc >= 0) {
+        if (hipMemcpy(A, dA, sizeof(T) * (size_t)m * m, hipMemcpyDeviceToHost) != hipSuccess) rc = LINOMPSS_ERR_HIP;
+        if (n > 0 && hipMemcpy(B, dB, sizeof(T) * (size_t)m * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = LINOMPSS_ERR_HIP;
+    }
+    (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    return rc;
+}
+
+extern "C" int dspdsolv_main(int m, int n, int b, double *A, double *B) { return host_spdsolv<double>(m, n, b, A, B); }
+extern "C" int sspdsolv_main(int m, int n, int b, float *A, float *B) { return host_spdsolv<float>(m, n, b, A, B); }
+
+template <typename T>
+__global__ void lx_scale_kernel(T *B, int64_t ldb, int m, int n, T alpha)
+{
+    int64_t tot = (int64_t)m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = e / m, r = e % m;
+        B[c * ldb + r] *= alpha;
+    }
+}
+
+template <typename T>
+static int host_ltrsm(char uplo, char trans, char diag, int bm, int bn, int m, int n, T alpha, const T *A, int lda,
+                      T *B, int ldb)
+{
+    (void)n;
+    if (uplo != 'L' && uplo != 'l') return -1;
+    if (trans != 'N' && trans != 'n' && trans != 'T' && trans != 't') return -2;
+    if (diag != 'N' && diag != 'n') return -3;
+    if (bm <= 0) return -4;
+    if (bn < 0) return -5;
+    if (m < 0) return -6;
+    if (lda < std::max(1, m)) return -10;
+    if (ldb < std::max(1, m)) return -12;
+    if (m == 0 || bn == 0) return 0;
+    const int dtype = sizeof(T) == 8 ? 0 : 1;
+    const bool tr = trans == 'T' || trans == 't';
+    T *dA = nullptr, *dB = nullptr;
+    if (hipMalloc(&dA, sizeof(T) * (size_t)m * m) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    if (hipMalloc(&dB, sizeof(T) * (size_t)m * bn) != hipSuccess) {
+        (void)hipFree(dA);
+        return LINOMPSS_ERR_NOMEM;
+    }
+    int rc = 0;
+    if (hipMemcpy2D(dA, sizeof(T) * m, A, sizeof(T) * (size_t)lda, sizeof(T) * m, m, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy2D(dB, sizeof(T) * m, B, sizeof(T) * (size_t)ldb, sizeof(T) * m, bn, hipMemcpyHostToDevice) !=
+            hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc && alpha != T(1)) {
+        hipLaunchKernelGGL(lx_scale_kernel<T>, dim3(256), dim3(256), 0, nullptr, dB, (int64_t)m, m, bn, alpha);
+        if (hipGetLastError() != hipSuccess) rc = LINOMPSS_ERR_HIP;
+    }
+    if (!rc) {
+        std::lock_guard<std::mutex> g(g_mu);
+        lx_shape sh = {tr ? LX_W_BWD : LX_W_FWD, dtype, m, bm, bn};
+        int info = 0;
+        rc = lx_run_single(sh, dA, m, dB, m, nullptr, nullptr, &info);
+    }
+    if (!rc &&
+        hipMemcpy2D(B, sizeof(T) * (size_t)ldb, dB, sizeof(T) * m, sizeof(T) * m, bn, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    return rc;
+}
+
+extern "C" int linompss_dltrsm(char uplo, char trans, char diag, int bm, int bn, int m, int n, double alpha,
+                               const double *A, int lda, double *B, int ldb)
+{
+    return host_ltrsm<double>(uplo, trans, diag, bm, bn, m, n, alpha, A, lda, B, ldb);
+}
+extern "C" int linompss_sltrsm(char uplo, char trans, char diag, int bm, int bn, int m, int n, float alpha,
+                               const float *A, int lda, float *B, int ldb)
+{
+    return host_ltrsm<float>(uplo, trans, diag, bm, bn, m, n, alpha, A, lda, B, ldb);
+}
+
+// ---------------------------------------------------------------------------
+// Plan introspection (host only; no device needed).
+
+static int plan_host(int what, int n, int b, int nrhs, const linompss_exec_opts *o, lx_planner &pl)
+{
+    if (n <= 0 || b <= 0 || nrhs < 0) return -1;
+    lx_shape sh = {what, 0, n, b, nrhs};
+    try {
+        if (!build_plan(pl, sh)) return -1;
+    } catch (...) {
+        return -1;
+    }
+    (void)o;
+    return 0;
+}
+
+extern "C" int linompss_plan_tdg(int what, int n, int b, int nrhs, int *types, int *coords, int max_tasks,
+                                 int *edges, int max_edges, int *ntasks, int *nedges)
+{
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    lx_planner pl(P);
+    int w = what == 2 ? LX_W_POSV_RL : what;
+    if (plan_host(w, n, b, std::max(nrhs, what == 2 ? 1 : 0), nullptr, pl)) return -2;
+    const auto &T = pl.tasks();
+    int nt = (int)T.size(), ne = 0;
+    for (const auto &t : T) ne += (int)t.preds.size();
+    if (ntasks) *ntasks = nt;
+    if (nedges) *nedges = ne;
+    if (nt > max_tasks || ne > max_edges) return -1;
+    int e = 0;
+    for (int q = 0; q < nt; ++q) {
+        if (types) types[q] = T[q].type;
+        if (coords) {
+            coords[3 * q] = T[q].ti;
+            coords[3 * q + 1] = T[q].tj;
+            coords[3 * q + 2] = T[q].tk;
+        }
+        for (int p : T[q].preds) {
+            if (edges) {
+                edges[2 * e] = p;
+                edges[2 * e + 1] = q;
+            }
+            ++e;
+        }
+    }
+    return 0;
+}
+
+extern "C" int linompss_plan_info(int what, int n, int b, int nrhs, const linompss_exec_opts *opts, int *units,
+                                  int *tasks, double *sim_ms)
+{
+    lx_plan_params P;
+    fill_params(P, 0, 512, opts);
+    lx_planner pl(P);
+    if (plan_host(what, n, b, nrhs, opts, pl)) return -2;
+    if (units) *units = (int)pl.units().size();
+    if (tasks) *tasks = (int)pl.tasks().size();
+    if (sim_ms) *sim_ms = pl.simulated_time() * 1e3;
+    return 0;
+}
+
+extern "C" const char *linompss_version(void) { return "linompss-mi355x 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// Device-memory plumbing.
+
+extern "C" int linompss_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" void *linompss_dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+extern "C" int linompss_dev_free(void *p) { return hipFree(p) == hipSuccess ? 0 : LINOMPSS_ERR_HIP; }
+
+extern "C" int linompss_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (kind < 1 || kind > 3) return -4;
+    return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+extern "C" int linompss_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP; }

@@ -1,0 +1,125 @@
+// lx_exec_kernel.hpp -- the persistent tile-DAG executor kernel (device
+// code only; the host side lives in lx_exec.hip).
+//
+// Every workgroup loops: grab the next unit index (one atomic), wait until
+// each of the unit's dependency counters reaches its threshold (relaxed
+// polling, then one acquire fence), run the tile kernel, make its stores
+// visible (release fence in every wave), and bump the unit's tile counter.
+// Units are listed in a topological order chosen by the host planner, so
+// every wait is on a unit that a running workgroup already holds: the launch
+// cannot deadlock, whatever the number of resident workgroups.  A wait that
+// exceeds spin_ticks sets an abort flag that every workgroup observes, so each
+// wave reaches the kernel's exit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lx_device.hpp"
+#include "lx_plan.hpp"
+
+#define LX_SMEM_BYTES (42 * 1024)
+
+template <typename T> struct lx_gemm_cfg;
+template <> struct lx_gemm_cfg<double> { static constexpr int FM = 4, FN = 4, U = 2; };
+template <> struct lx_gemm_cfg<float> { static constexpr int FM = 4, FN = 4, U = 4; };
+static constexpr int LX_UNIT_M = 128, LX_UNIT_N = 128;  // 2 x 2 waves of 16*FM x 16*FN
+
+__device__ __forceinline__ long long lx_off(int64_t rcp, long long ld)
+{
+    return (long long)(rcp >> 32) * ld + (long long)(rcp & 0xffffffffll);
+}
+
+template <typename T>
+__device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__restrict__ sys, int *info, char *smem)
+{
+    const lx_sys s = sys[u.sys];
+    T *A = (T *)s.A, *X = (T *)s.X;
+    T *Cb = u.cbase == LX_BASE_X ? X : A;
+    const T *Ab = u.abase == LX_BASE_X ? X : A;
+    const T *Bb = u.bbase == LX_BASE_X ? X : A;
+    const long long ldc = u.cbase == LX_BASE_X ? s.ldx : s.lda;
+    const long long lda = u.abase == LX_BASE_X ? s.ldx : s.lda;
+    const long long ldb = u.bbase == LX_BASE_X ? s.ldx : s.lda;
+    T *C = Cb + lx_off(u.c, ldc);
+    const T *Ap = Ab + lx_off(u.a, lda);
+    const T *Bp = Bb + lx_off(u.b, ldb);
+    constexpr int FM = lx_gemm_cfg<T>::FM, FN = lx_gemm_cfg<T>::FN, UU = lx_gemm_cfg<T>::U;
+    switch (u.op) {
+    case LX_OP_POTRF: {
+        int loc = 0;
+        lx_unit_potrf<T>(C, ldc, u.M, &loc, smem);
+        if (threadIdx.x == 0 && loc != 0) atomicMin(&info[s.info_slot], u.aux1 + loc);
+        break;
+    }
+    case LX_OP_TRSM_RLTN:
+        lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, u.M, u.N, smem);
+        break;
+    case LX_OP_SYRK:
+    case LX_OP_GEMM_NT:
+        lx_unit_gemm_nt<T, FM, FN, UU>(C, ldc, Ap, lda, Bp, ldb, u.M, u.N, u.K, u.aux0 != 0, u.aux1);
+        break;
+    case LX_OP_TRSV:
+        lx_unit_trsv_block<T>(C, ldc, Ap, lda, u.M, u.N, u.aux0, smem);
+        break;
+    case LX_OP_GEMV_RHS:
+        lx_unit_gemv_rhs<T>(C, ldc, Ap, lda, Bp, ldb, u.M, u.K, u.N, u.aux0, smem);
+        break;
+    default:
+        break;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(LX_WG_THREADS)
+    lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
+                   lx_status *st, int *info, long long spin_ticks)
+{
+    __shared__ __attribute__((aligned(16))) char smem[LX_SMEM_BYTES];
+    __shared__ int s_idx, s_go;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int idx = atomicAdd(&st->head, 1);
+            int go = idx < nunits;
+            if (go && __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) go = 0;
+            if (go) {
+                const int nd = units[idx].nd;
+                for (int d = 0; d < nd && go; ++d) {
+                    const int c = units[idx].dc[d], thr = units[idx].dt[d];
+                    if (__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= thr) continue;
+                    const long long t0 = wall_clock64();
+                    while (__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < thr) {
+                        if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            go = 0;
+                            break;
+                        }
+                        if (wall_clock64() - t0 > spin_ticks) {
+                            atomicAdd(&st->timeouts, 1);
+                            __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            go = 0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            s_idx = idx;
+            s_go = go;
+        }
+        __syncthreads();
+        // s_idx / s_go are the same for every lane; readfirstlane makes that
+        // visible to the compiler, so the loop and the unit's fields are
+        // wave-uniform (scalar loads, scalar branches around every barrier).
+        const int go = __builtin_amdgcn_readfirstlane(s_go);
+        const int idx = __builtin_amdgcn_readfirstlane(s_idx);
+        if (!go) break;
+        const lx_unit u = units[idx];
+        lx_run_unit<T>(u, sys, info, smem);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_add(&ctr[u.ctr], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(&st->done, 1);
+        }
+    }
+}
+

@@ -1,0 +1,40 @@
+// lx_internal.hpp -- host-side helpers shared by the library's translation
+// units (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lx_plan.hpp"
+#include "linompss.h"
+
+// Plan kinds.
+enum lx_what : int {
+    LX_W_FACTOR_RL = 0,
+    LX_W_FACTOR_LL = 1,
+    LX_W_POSV_RL = 2,    // RL factor + forward + backward sweep
+    LX_W_SWEEPS = 3,     // forward + backward sweep on a factored matrix
+    LX_W_FWD = 4,
+    LX_W_BWD = 5
+};
+
+struct lx_shape {
+    int what, dtype, n, b, nrhs;
+};
+
+// Runs one plan over nsys systems sharing the same shape family (the plan was
+// built for exactly these systems).  info_out[s] receives each system's info.
+// Returns 0 or a LINOMPSS_ERR_* code.
+int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t ldx, hipStream_t stream,
+                  const linompss_exec_opts *opts, int *info_out);
+
+// Grow-only device scratch owned by the library (one per slot and device).
+void *lx_scratch(int slot, size_t bytes);
+
+// Elementwise helpers enqueued on a stream.
+void lx_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
+
+#define LX_HIP_TRY(expr)                                                                                    \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return LINOMPSS_ERR_HIP;                                                      \
+    } while (0)

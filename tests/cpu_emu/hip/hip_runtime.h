@@ -1,0 +1,137 @@
+// CPU emulation of the HIP device environment used by the library's kernels
+// (test infrastructure: lets the -m "not gpu" tests execute the exact kernel
+// source on the host, one workgroup of real OS threads at a time).
+//
+// * threadIdx / blockIdx are thread_local; one workgroup runs at a time, so
+//   __shared__ maps to static storage.
+// * __syncthreads is a counting barrier with a timeout: a barrier that not
+//   every thread reaches aborts the run with a diagnostic instead of hanging.
+// * Wavefront collectives (MFMA 16x16x4, __shfl, __shfl_xor) exchange lane
+//   values through a per-wave buffer between two wave barriers; a collective
+//   that only part of a wave reaches is reported the same way.
+// * MFMA operand/result layout: lane l supplies A[l%16][l/16], B[l/16][l%16];
+//   lane l receives D[4*(l/16)+i][l%16] (the layout the kernels assume).
+#pragma once
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <thread>
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct lx_emu_dim3 { unsigned x = 0, y = 0, z = 0; };
+extern thread_local lx_emu_dim3 threadIdx;
+extern thread_local lx_emu_dim3 blockIdx;
+extern lx_emu_dim3 blockDim, gridDim;
+
+struct lx_emu_barrier {
+    std::mutex m;
+    std::condition_variable cv;
+    int n = 0, count = 0;
+    long gen = 0;
+    const char *name = "";
+    void init(int parties, const char *nm) { n = parties; count = 0; gen = 0; name = nm; }
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(m);
+        long g = gen;
+        if (++count == n) {
+            count = 0;
+            ++gen;
+            cv.notify_all();
+            return;
+        }
+        if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return gen != g; })) {
+            fprintf(stderr, "lx_emu: %s barrier timeout (%d of %d arrived) -- mismatched barrier\n", name, count, n);
+            fflush(stderr);
+            std::_Exit(86);
+        }
+    }
+};
+extern lx_emu_barrier lx_emu_wg_barrier;
+extern lx_emu_barrier lx_emu_wave_barrier[4];
+struct lx_emu_wave_buf {
+    double a[64], b[64], c[64][4];
+};
+extern lx_emu_wave_buf lx_emu_wbuf[4];
+
+inline void __syncthreads() { lx_emu_wg_barrier.wait(); }
+
+typedef double lx_emu_d4 __attribute__((ext_vector_type(4)));
+typedef float lx_emu_f4 __attribute__((ext_vector_type(4)));
+
+template <typename V, typename S>
+inline V lx_emu_mfma(S a, S b, V c)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    lx_emu_wave_buf &B = lx_emu_wbuf[w];
+    B.a[lane] = a;
+    B.b[lane] = b;
+    lx_emu_wave_barrier[w].wait();
+    V d = c;
+    const int col = lane & 15;
+    for (int i = 0; i < 4; ++i) {
+        const int row = 4 * (lane >> 4) + i;
+        S acc = 0;
+        for (int k = 0; k < 4; ++k) acc += (S)B.a[k * 16 + row] * (S)B.b[k * 16 + col];
+        d[i] = c[i] + acc;
+    }
+    lx_emu_wave_barrier[w].wait();
+    return d;
+}
+inline lx_emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, lx_emu_d4 c, int, int, int)
+{
+    return lx_emu_mfma<lx_emu_d4, double>(a, b, c);
+}
+inline lx_emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, lx_emu_f4 c, int, int, int)
+{
+    return lx_emu_mfma<lx_emu_f4, float>(a, b, c);
+}
+
+template <typename T>
+inline T lx_emu_xchg(T v, int src_lane)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    lx_emu_wave_buf &B = lx_emu_wbuf[w];
+    B.a[lane] = (double)v;
+    lx_emu_wave_barrier[w].wait();
+    T r = (T)B.a[src_lane & 63];
+    lx_emu_wave_barrier[w].wait();
+    return r;
+}
+template <typename T> inline T __shfl(T v, int src) { return lx_emu_xchg(v, src); }
+template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
+
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
+inline void __builtin_amdgcn_sched_barrier(int) {}
+inline void __builtin_amdgcn_fence_emu() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+#define __builtin_amdgcn_fence(order, scope) __builtin_amdgcn_fence_emu()
+inline long long wall_clock64()
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+               .count() / 10;
+}
+// __hip_atomic_load / _store / _fetch_add are clang builtins on every target;
+// their scope argument takes clang's __MEMORY_SCOPE_* values.
+#define __HIP_MEMORY_SCOPE_AGENT __MEMORY_SCOPE_DEVICE
+#define __HIP_MEMORY_SCOPE_WORKGROUP __MEMORY_SCOPE_WRKGRP
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicMin(int *p, int v)
+{
+    int cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return cur;
+}
+using std::sqrt;

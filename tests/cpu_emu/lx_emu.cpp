@@ -1,0 +1,71 @@
+// CPU execution of the library's persistent executor (test infrastructure).
+// Builds a plan with the product planner and runs the product kernel source
+// (lx_exec_kernel.hpp) as one workgroup of 256 host threads, through the
+// emulation header in this directory.  Exposes a C entry point for ctypes.
+#include "hip/hip_runtime.h"
+
+#include <climits>
+#include <thread>
+#include <vector>
+
+#include "lx_exec_kernel.hpp"
+#include "lx_planner.hpp"
+
+thread_local lx_emu_dim3 threadIdx, blockIdx;
+lx_emu_dim3 blockDim, gridDim;
+lx_emu_barrier lx_emu_wg_barrier;
+lx_emu_barrier lx_emu_wave_barrier[4];
+lx_emu_wave_buf lx_emu_wbuf[4];
+
+extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, long long lda, void *X, long long ldx,
+                          int policy, int *info_out, int *done_out, int *nunits_out)
+{
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    P.policy = policy;
+    lx_planner pl(P);
+    int s = pl.add_system(n, b, nrhs);
+    switch (what) {
+    case 0: pl.factor_rl(s); break;
+    case 1: pl.factor_ll(s); break;
+    case 2: pl.factor_rl(s); pl.sweep_forward(s); pl.sweep_backward(s); break;
+    case 3: pl.sweep_forward(s); pl.sweep_backward(s); break;
+    case 4: pl.sweep_forward(s); break;
+    case 5: pl.sweep_backward(s); break;
+    default: return -2;
+    }
+    pl.finalize();
+    const std::vector<lx_unit> &U = pl.units();
+    std::vector<int> ctr(pl.num_counters(), 0);
+    lx_status st = {};
+    int info = INT_MAX;
+    lx_sys sys = {};
+    sys.A = A;
+    sys.X = X;
+    sys.lda = lda;
+    sys.ldx = ldx;
+    sys.n = n;
+    sys.nrhs = nrhs;
+    blockDim.x = 256;
+    gridDim.x = 1;
+    lx_emu_wg_barrier.init(256, "workgroup");
+    for (int w = 0; w < 4; ++w) lx_emu_wave_barrier[w].init(64, "wavefront");
+    const long long ticks = 100000000LL * 20;  // 20 s at the emulated 100 MHz clock
+    std::vector<std::thread> th;
+    th.reserve(256);
+    for (int t = 0; t < 256; ++t)
+        th.emplace_back([&, t] {
+            threadIdx.x = t;
+            blockIdx.x = 0;
+            if (dtype == 0)
+                lx_exec_kernel<double>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks);
+            else
+                lx_exec_kernel<float>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks);
+        });
+    for (auto &t : th) t.join();
+    *info_out = info == INT_MAX ? 0 : info;
+    *done_out = st.done;
+    *nunits_out = (int)U.size();
+    return (st.abort || st.timeouts) ? -1 : 0;
+}

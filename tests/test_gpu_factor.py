@@ -1,0 +1,105 @@
+"""GPU parity of the tiled factor (ompss_{d,s}chol_{rl,ll}) against the oracle.
+
+Reference behaviour: src/cholesky/dense/chol_rlmain.c:22-53 and
+chol_llmain.c:23-57 (lower factor in place, strict upper triangle untouched),
+checked like tests/dense_tests/potrf_test.c:6-37 (||L L^T - A||/(||A|| N eps)
+< 10) and against the unique Cholesky factor (relative error <= 1e-12 for
+FP64, the plain path's bar).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (n, b): square, ragged last tile, tile larger than n, the reference
+# unit test's 1989/533, and several tile sizes of the sweep in tests/dpotrf.cfg.
+CASES = [(1, 1), (8, 4), (37, 8), (64, 16), (130, 32), (256, 64), (300, 128), (533, 1024),
+         (1024, 128), (1024, 256), (1024, 384), (1989, 533), (2048, 512)]
+
+
+def _spd_gauss(n, seed, alpha):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + alpha * np.eye(n)
+    return np.asfortranarray(A)
+
+
+@pytest.mark.parametrize("variant", ["rl", "ll"])
+@pytest.mark.parametrize("n,b", CASES)
+def test_dchol_matches_lapack(lx, orc, dev, n, b, variant):
+    print(f"[dchol {variant}] n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=0, diag_mode=1)
+    A = A0.copy(order="F")
+    fn = lx.ompss_dchol_rl if variant == "rl" else lx.ompss_dchol_ll
+    assert fn(A, b) == 0
+    Lref = np.linalg.cholesky(A0)
+    rel = np.linalg.norm(np.tril(A) - Lref) / np.linalg.norm(Lref)
+    assert rel <= 1e-12, rel
+    assert np.array_equal(np.triu(A, 1), np.triu(A0, 1)), "strict upper triangle modified"
+    assert orc.potrf_resid(A0, A) < 10.0
+
+
+@pytest.mark.parametrize("n,b", [(96, 32), (500, 128), (1000, 256)])
+def test_dchol_vs_oracle_illconditioned(lx, orc, dev, n, b):
+    print(f"[dchol gauss] n={n} b={b}", flush=True)
+    A0 = _spd_gauss(n, 7, 1e-3)
+    A = A0.copy(order="F")
+    assert lx.ompss_dchol_rl(A, b) == 0
+    Lo = A0.copy(order="F")
+    assert orc.chol(Lo, b, "rl") == 0
+    rel = np.linalg.norm(np.tril(A) - np.tril(Lo)) / np.linalg.norm(np.tril(Lo))
+    # cond(A) ~ 4e3: the factors agree to a few ulps times sqrt(cond)
+    assert rel <= 1e-12, rel
+    assert orc.potrf_resid(A0, A) < 10.0
+
+
+@pytest.mark.parametrize("variant", ["rl", "ll"])
+@pytest.mark.parametrize("n,b", [(8, 4), (130, 32), (533, 128), (1989, 533), (2048, 512)])
+def test_schol(lx, orc, dev, n, b, variant):
+    print(f"[schol {variant}] n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=3, diag_mode=1).astype(np.float32, order="F")
+    A = A0.copy(order="F")
+    fn = lx.ompss_schol_rl if variant == "rl" else lx.ompss_schol_ll
+    assert fn(A, b) == 0
+    Lo = A0.copy(order="F")
+    assert orc.chol(Lo, b, variant) == 0
+    Lref = np.linalg.cholesky(A0.astype(np.float64))
+    rel = np.linalg.norm(np.tril(A).astype(np.float64) - Lref) / np.linalg.norm(Lref)
+    relo = np.linalg.norm(np.tril(Lo).astype(np.float64) - Lref) / np.linalg.norm(Lref)
+    assert rel <= 4 * max(relo, 1e-7), (rel, relo)
+    assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
+
+
+@pytest.mark.parametrize("n,b,k", [(64, 16, 20), (300, 64, 150), (1000, 256, 700)])
+def test_not_spd_reports_info(lx, dev, n, b, k):
+    """A matrix whose leading minor of order k+1 is indefinite: LAPACK info = k+1."""
+    print(f"[info] n={n} b={b} k={k}", flush=True)
+    A = _spd_gauss(n, 11, 1.0)
+    A[k, k] = -abs(A[k, k]) - 10.0 * n
+    A = np.asfortranarray(A)
+    assert lx.ompss_dchol_rl(A.copy(order="F"), b) == k + 1
+    assert lx.ompss_dchol_ll(A.copy(order="F"), b) == k + 1
+
+
+def test_lda_larger_than_n(lx, orc, dev):
+    n, lda, b = 200, 257, 64
+    print("[lda]", flush=True)
+    A0 = orc.genmat_spd(n, seed=5, diag_mode=0, lda=lda)
+    A = A0.copy(order="F")
+    assert lx.ompss_dchol_rl(A, b, n=n, lda=lda) == 0
+    Lref = np.linalg.cholesky(A0[:n, :n])
+    assert np.linalg.norm(np.tril(A[:n]) - Lref) / np.linalg.norm(Lref) <= 1e-12
+    assert np.array_equal(A[n:], A0[n:]), "rows beyond n modified"
+
+
+def test_device_api_factor(lx, orc, dev):
+    n, b = 1536, 256
+    print("[dev api]", flush=True)
+    A0 = orc.genmat_spd(n, seed=9, diag_mode=1)
+    dA = lx.DeviceMatrix.from_host(A0)
+    assert lx.chol_dev(dA, b) == 0
+    L = np.tril(dA.to_host())
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) <= 1e-12
+    st = lx.last_exec_stats()
+    assert st.units > 0 and st.exec_ms > 0

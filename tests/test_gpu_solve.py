@@ -1,0 +1,110 @@
+"""GPU parity of the SPD solve, the blocked sweeps and the refinement loop.
+
+Reference: dspdsolv_main (src/spdsolv/spdsolv_main.c:10-20), bblas_dltrsm
+lower branches (src/blocked_blas/bblas_trsm.h:116-160), itref's loop
+(src/itref/itref.c:999-1101).  Bars: FP64 normwise backward error
+||b - A x||_inf / (||A||_inf ||x||_inf) <= 1e-14; sweeps against the oracle
+within the trsm_test threshold style (relative 1-norm error).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _berr(A, X, B):
+    A = np.asarray(A)
+    R = B - A @ X
+    an = np.abs(A).sum(axis=1).max()
+    return max(np.abs(R[:, c]).max() / (an * np.abs(X[:, c]).max()) for c in range(X.shape[1]))
+
+
+def _spd_gauss(n, seed, alpha):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, n))
+    return np.asfortranarray(G @ G.T / n + alpha * np.eye(n))
+
+
+@pytest.mark.parametrize("m,nr,b", [(8, 1, 4), (130, 1, 32), (512, 3, 128), (1989, 2, 533), (4096, 1, 256)])
+def test_dspdsolv(lx, orc, dev, m, nr, b):
+    print(f"[spdsolv] m={m} nr={nr} b={b}", flush=True)
+    A0 = orc.genmat_spd(m, seed=0, diag_mode=1)
+    X0 = np.ones((m, nr), order="F")
+    B0 = np.asfortranarray(A0 @ X0)
+    A, B = A0.copy(order="F"), B0.copy(order="F")
+    assert lx.dspdsolv_main(A, B, b) == 0
+    assert _berr(A0, B, B0) <= 1e-14
+    # same result as the oracle's restatement to FP64 accuracy
+    Ao, Bo = A0.copy(order="F"), B0.copy(order="F")
+    assert orc.dspdsolv(Ao, Bo, b) == 0
+    assert np.abs(B - Bo).max() / np.abs(Bo).max() <= 1e-13
+    assert np.linalg.norm(np.tril(A) - np.tril(Ao)) / np.linalg.norm(np.tril(Ao)) <= 1e-12
+
+
+@pytest.mark.parametrize("m,b", [(700, 128), (1024, 256)])
+def test_sspdsolv(lx, orc, dev, m, b):
+    print(f"[sspdsolv] m={m} b={b}", flush=True)
+    A0 = orc.genmat_spd(m, seed=1, diag_mode=1)
+    B0 = np.asfortranarray(A0 @ np.ones((m, 2)))
+    A, B = A0.astype(np.float32, order="F"), B0.astype(np.float32, order="F")
+    assert lx.sspdsolv_main(A, B, b) == 0
+    assert _berr(A0, B.astype(np.float64), B0) <= 1e-6
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m,nr,bm", [(600, 5, 121), (500, 1, 128), (64, 3, 64), (1000, 16, 256)])
+def test_dltrsm(lx, orc, dev, trans, m, nr, bm):
+    """trsm_test.c's ragged shapes (600 x 500 with 121 tiles) for the lower branches."""
+    print(f"[ltrsm {trans}] m={m} nr={nr} bm={bm}", flush=True)
+    rng = np.random.default_rng(4)
+    L = np.asfortranarray(np.tril(rng.uniform(0, 1, (m, m))) + m * np.eye(m))
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, nr)))
+    alpha = -2.11
+    B = B0.copy(order="F")
+    lx.dltrsm(trans, L, B, bm, alpha)
+    Bo = B0.copy(order="F")
+    orc.ltrsm_lower(0 if trans == "N" else 1, L, Bo, bm, alpha)
+    rel = np.abs(B - Bo).sum(axis=0).max() / np.abs(Bo).sum(axis=0).max()
+    assert rel <= 1e-14, rel
+
+
+@pytest.mark.parametrize("n,nr,b,alpha", [(512, 1, 128, 1e-2), (1000, 4, 256, 1e-3), (2048, 16, 512, 1e-4)])
+def test_dposv_ir(lx, orc, dev, n, nr, b, alpha):
+    print(f"[ir] n={n} nr={nr} b={b} alpha={alpha}", flush=True)
+    A = _spd_gauss(n, 1, alpha)
+    rng = np.random.default_rng(2)
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, nr)))
+    X, st = lx.dposv_ir(A, B, b)
+    assert st.info == 0
+    assert st.fallback == 0
+    assert st.berr <= 1e-14
+    assert _berr(A, X, B) <= 1e-14
+    Xo, sto, info = orc.dposv_ir(A, B, b)
+    assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 1e-10
+
+
+def test_dposv_ir_fallback(lx, dev):
+    """cond(A) ~ 1e9 is beyond what an FP32 factor can refine: the FP64 factor answers."""
+    n, b = 600, 128
+    print("[ir fallback]", flush=True)
+    rng = np.random.default_rng(5)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.logspace(0, -9, n)
+    A = np.asfortranarray((Q * ev) @ Q.T)
+    A = np.asfortranarray((A + A.T) / 2)
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, 1)))
+    X, st = lx.dposv_ir(A, B, b)
+    assert st.fallback == 1
+    assert _berr(A, X, B) <= 1e-14
+
+
+def test_device_posv_single_launch(lx, orc, dev):
+    n, nr, b = 2048, 2, 256
+    print("[posv dev]", flush=True)
+    A0 = orc.genmat_spd(n, seed=6, diag_mode=1)
+    B0 = np.asfortranarray(A0 @ np.ones((n, nr)))
+    dA = lx.DeviceMatrix.from_host(A0)
+    dX = lx.DeviceMatrix.from_host(B0)
+    assert lx.posv_dev(dA, dX, b) == 0
+    X = dX.to_host()
+    assert _berr(A0, X, B0) <= 1e-14
