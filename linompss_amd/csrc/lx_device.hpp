@@ -202,9 +202,10 @@ __device__ __forceinline__ void lx_unit_gemm_nt(T *__restrict__ C, long long ldc
 //   1) factor the PB x PB diagonal block in LDS (one barrier per column);
 //   2) solve the panel below it against that block (one row per thread);
 //   3) update the lower trailing part with the MFMA core.
-// Returns through *info the 1-based column (local) of the first non-positive
-// pivot, as LAPACK's info, and keeps going (results past that column are not
-// meaningful, as in LAPACK).
+// Returns through *info (in every thread) the 1-based column (local) of the
+// first non-positive pivot, as LAPACK's info, and keeps going (results past
+// that column are not meaningful, as in LAPACK).  Every thread reads the same
+// pivot, so the info is tracked in registers, with no lane-guarded code.
 template <typename T> struct lx_potrf_cfg { static constexpr int PB = 32; };
 
 template <typename T>
@@ -213,9 +214,8 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
     constexpr int PB = lx_potrf_cfg<T>::PB;
     T(*sD)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem);
     T(*sL)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem + sizeof(T) * PB * (PB + 1));
-    __shared__ int s_bad;
     const int tid = threadIdx.x;
-    if (tid == 0) s_bad = 0;
+    int first_bad = 0;
     for (int p = 0; p < M; p += PB) {
         const int pb = M - p < PB ? M - p : PB;
         // 1) load diagonal block (lower part used)
@@ -229,7 +229,7 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
             T dcc = sD[c][c];
             bool bad = !(dcc > T(0));
             T d = sqrt(dcc);  // NaN past a failed pivot, like LAPACK's garbage-after-info
-            if (tid == 0 && bad && s_bad == 0) s_bad = p + c + 1;
+            if (bad && first_bad == 0) first_bad = p + c + 1;
             // column c of L (scaled) -> sL; trailing columns of sD updated
             for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
                 int r = e % PB, c2 = e / PB;
@@ -250,7 +250,12 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
         }
         // 2) panel rows below: x = a * L_pp^{-T}
         const int r_beg = p + pb;
-        for (int r = r_beg + tid; r < M; r += LX_WG_THREADS) {
+        // uniform trip count; rows past M recompute row M-1 and do not store
+        const int nrow = M - r_beg, nq = (nrow + LX_WG_THREADS - 1) / LX_WG_THREADS;
+        for (int q = 0; q < nq; ++q) {
+            const int rr = r_beg + q * LX_WG_THREADS + tid;
+            const bool live = rr < M;
+            const int r = live ? rr : M - 1;
             T x[PB];
 #pragma unroll
             for (int c = 0; c < PB; ++c) x[c] = c < pb ? A[(long long)(p + c) * lda + r] : T(0);
@@ -267,7 +272,7 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
             }
 #pragma unroll
             for (int c = 0; c < PB; ++c)
-                if (c < pb) A[(long long)(p + c) * lda + r] = x[c];
+                if (live && c < pb) A[(long long)(p + c) * lda + r] = x[c];
         }
         __syncthreads();
         // 3) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel
@@ -287,8 +292,7 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
         }
         __syncthreads();
     }
-    if (tid == 0) *info_local = s_bad;
-    __syncthreads();
+    *info_local = first_bad;
 }
 
 // ---------------------------------------------------------------------------
@@ -327,12 +331,15 @@ __device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__
             if (r < pb && c <= r) sDg[r][c] = L[(long long)(p + c) * ldl + p + r];
         }
         __syncthreads();
-        if (tid < R) {
+        {
+            // every thread solves a row (threads past R repeat row R-1 and do not store)
+            const bool live = tid < R;
+            const int rw = live ? tid : R - 1;
             T x[NB];
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
                 if (c < pb) {
-                    T s = sS[tid][c];
+                    T s = sS[rw][c];
 #pragma unroll
                     for (int c2 = 0; c2 < c; ++c2) s -= x[c2] * sDg[c][c2];
                     x[c] = s / sDg[c][c];
@@ -343,7 +350,7 @@ __device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__
             }
 #pragma unroll
             for (int c = 0; c < NB; ++c)
-                if (c < pb) Bm[(long long)(p + c) * ldb + tid] = x[c];
+                if (live && c < pb) Bm[(long long)(p + c) * ldb + rw] = x[c];
         }
         __syncthreads();
     }
@@ -411,31 +418,26 @@ __device__ void lx_unit_trsv_block(T *__restrict__ X, long long ldx, const T *__
                 if (r < pb && c < nc) X[(long long)(cb + c) * ldx + p + r] = sX[c][r];
             }
             // update the remaining rows of this block
-            if (!trans) {
-                // rows below: X[r] -= L[r, p:p+pb] x_p,  r in [p+pb, M)
-                for (int r = p + pb + tid; r < M; r += LX_WG_THREADS) {
+            {
+                // forward: rows below, X[r] -= L[r, p:p+pb] x_p for r in [p+pb, M)
+                // backward: rows above, X[r] -= sum_j L[p+j][r] x_j for r in [0, p)
+                // uniform trip count; out-of-range rows repeat a valid row and do not store
+                const int rlo = trans ? 0 : p + pb, rhi = trans ? p : M;
+                const int nq = (rhi - rlo + LX_WG_THREADS - 1) / LX_WG_THREADS;
+                for (int q = 0; q < nq; ++q) {
+                    const int rr = rlo + q * LX_WG_THREADS + tid;
+                    const bool live = rr < rhi;
+                    const int r = live ? rr : rhi - 1;
                     T s[16];
 #pragma unroll
                     for (int c = 0; c < 16; ++c) s[c] = T(0);
                     for (int j = 0; j < pb; ++j) {
-                        T l = L[(long long)(p + j) * ldl + r];
+                        T l = trans ? L[(long long)r * ldl + p + j] : L[(long long)(p + j) * ldl + r];
 #pragma unroll
                         for (int c = 0; c < 16; ++c) s[c] += l * sX[c][j];
                     }
-                    for (int c = 0; c < nc; ++c) X[(long long)(cb + c) * ldx + r] -= s[c];
-                }
-            } else {
-                // rows above: X[r] -= sum_j L[p+j][r] x_j,  r in [0, p)  (column r of L, rows p..)
-                for (int r = tid; r < p; r += LX_WG_THREADS) {
-                    T s[16];
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) s[c] = T(0);
-                    for (int j = 0; j < pb; ++j) {
-                        T l = L[(long long)r * ldl + p + j];
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) s[c] += l * sX[c][j];
-                    }
-                    for (int c = 0; c < nc; ++c) X[(long long)(cb + c) * ldx + r] -= s[c];
+                    if (live)
+                        for (int c = 0; c < nc; ++c) X[(long long)(cb + c) * ldx + r] -= s[c];
                 }
             }
             __syncthreads();
@@ -474,21 +476,22 @@ __device__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ldy, const T *__re
                     }
                     __syncthreads();
                     const int kend = K - k0 < KC ? K - k0 : KC;
-                    if (r < M)
-                        for (int k = g; k < kend; k += 4) {
-                            T l = L[(long long)(k0 + k) * ldl + r];
+                    const int rc = r < M ? r : M - 1;  // rows past M accumulate row M-1, never stored
+                    for (int k = g; k < kend; k += 4) {
+                        T l = L[(long long)(k0 + k) * ldl + rc];
 #pragma unroll
-                            for (int c = 0; c < 16; ++c) s[c] += l * sZ[c][k];
-                        }
+                        for (int c = 0; c < 16; ++c) s[c] += l * sZ[c][k];
+                    }
                 }
                 __syncthreads();
 #pragma unroll
                 for (int c = 0; c < 16; ++c) sRed[g][c][tid & 63] = s[c];
                 __syncthreads();
-                if (tid < 64 && r0 + tid < M)
-                    for (int c = 0; c < nc; ++c)
-                        Y[(long long)(cb + c) * ldy + r0 + tid] -=
-                            (sRed[0][c][tid] + sRed[1][c][tid]) + (sRed[2][c][tid] + sRed[3][c][tid]);
+                for (int c = 0; c < nc; ++c) {
+                    const int t = tid & 63;
+                    const T v = (sRed[0][c][t] + sRed[1][c][t]) + (sRed[2][c][t] + sRed[3][c][t]);
+                    if (tid < 64 && r0 + t < M) Y[(long long)(cb + c) * ldy + r0 + t] -= v;
+                }
                 __syncthreads();
             }
         } else {
@@ -519,10 +522,11 @@ __device__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ldy, const T *__re
 #pragma unroll
                 for (int c = 0; c < 16; ++c) sRed[g][c][rl] = s[c];
                 __syncthreads();
-                if (tid < 64 && r0 + tid < M)
-                    for (int c = 0; c < nc; ++c)
-                        Y[(long long)(cb + c) * ldy + r0 + tid] -=
-                            (sRed[0][c][tid] + sRed[1][c][tid]) + (sRed[2][c][tid] + sRed[3][c][tid]);
+                for (int c = 0; c < nc; ++c) {
+                    const int t = tid & 63;
+                    const T v = (sRed[0][c][t] + sRed[1][c][t]) + (sRed[2][c][t] + sRed[3][c][t]);
+                    if (tid < 64 && r0 + t < M) Y[(long long)(cb + c) * ldy + r0 + t] -= v;
+                }
                 __syncthreads();
             }
         }

@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "lx_device.hpp"
 #include "lx_plan.hpp"
 
@@ -47,7 +49,8 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     case LX_OP_POTRF: {
         int loc = 0;
         lx_unit_potrf<T>(C, ldc, u.M, &loc, smem);
-        if (threadIdx.x == 0 && loc != 0) atomicMin(&info[s.info_slot], u.aux1 + loc);
+        if (loc != 0)  // loc is the same in every thread
+            atomicMin(&info[s.info_slot], threadIdx.x == 0 ? u.aux1 + loc : INT_MAX);
         break;
     }
     case LX_OP_TRSM_RLTN:
@@ -68,31 +71,48 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     }
 }
 
+// Wave 0 of the workgroup, all 64 lanes active, grabs the next unit and waits
+// for its dependencies; every value that steers its control flow is made
+// wave-uniform with readfirstlane.  No branch in this loop separates the lanes
+// of one wave: code of the form "if (threadIdx.x == 0) { grab; publish to
+// LDS }" lets the compiler send lane 0 and lanes 1..63 of wave 0 around
+// different loop structures, with lanes 1..63 waiting at the next barrier for
+// a value only lane 0 would write -- a deadlock, since the lanes of a wave
+// share one program counter.
+__device__ __forceinline__ int lx_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <typename T>
 __global__ void __launch_bounds__(LX_WG_THREADS)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
                    lx_status *st, int *info, long long spin_ticks)
 {
     __shared__ __attribute__((aligned(16))) char smem[LX_SMEM_BYTES];
-    __shared__ int s_idx, s_go;
+    __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
+    const int lane = lx_lane();
     for (;;) {
-        if (threadIdx.x == 0) {
-            int idx = atomicAdd(&st->head, 1);
-            int go = idx < nunits;
-            if (go && __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) go = 0;
+        if (lx_wave() == 0) {  // wave-uniform branch
+            // one atomic per wave: lane 0 adds 1, the others add 0; lane 0's old value is the index
+            const int idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
+            int go = idx < nunits ? 1 : 0;
+            if (go && lx_uniform(__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                go = 0;
             if (go) {
                 const int nd = units[idx].nd;
                 for (int d = 0; d < nd && go; ++d) {
                     const int c = units[idx].dc[d], thr = units[idx].dt[d];
-                    if (__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= thr) continue;
+                    if (lx_uniform(__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= thr)
+                        continue;
                     const long long t0 = wall_clock64();
-                    while (__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < thr) {
-                        if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    for (;;) {
+                        if (lx_uniform(__hip_atomic_load(&ctr[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= thr)
+                            break;
+                        if (lx_uniform(__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                             go = 0;
                             break;
                         }
                         if (wall_clock64() - t0 > spin_ticks) {
-                            atomicAdd(&st->timeouts, 1);
+                            // every lane stores the same values (no lane branch)
+                            atomicAdd(&st->timeouts, lane == 0 ? 1 : 0);
                             __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             go = 0;
                             break;
@@ -102,24 +122,24 @@ __global__ void __launch_bounds__(LX_WG_THREADS)
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
-            s_idx = idx;
-            s_go = go;
+            // every lane of wave 0 stores the same (uniform) values: no lane branch
+            s_sched[0] = go;
+            s_sched[1] = idx;
         }
         __syncthreads();
-        // s_idx / s_go are the same for every lane; readfirstlane makes that
-        // visible to the compiler, so the loop and the unit's fields are
-        // wave-uniform (scalar loads, scalar branches around every barrier).
-        const int go = __builtin_amdgcn_readfirstlane(s_go);
-        const int idx = __builtin_amdgcn_readfirstlane(s_idx);
+        // the same for every lane; readfirstlane makes the loop and the unit's
+        // fields wave-uniform (scalar loads, scalar branches around barriers)
+        const int go = lx_uniform(s_sched[0]);
+        const int idx = lx_uniform(s_sched[1]);
+        __syncthreads();  // s_sched is rewritten by wave 0 in the next iteration
         if (!go) break;
         const lx_unit u = units[idx];
         lx_run_unit<T>(u, sys, info, smem);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
-        if (threadIdx.x == 0) {
-            __hip_atomic_fetch_add(&ctr[u.ctr], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            atomicAdd(&st->done, 1);
+        if (lx_wave() == 0) {
+            __hip_atomic_fetch_add(&ctr[u.ctr], lane == 0 ? 1 : 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(&st->done, lane == 0 ? 1 : 0);
         }
     }
 }
-

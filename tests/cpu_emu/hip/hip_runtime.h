@@ -6,7 +6,10 @@
 //   __shared__ maps to static storage.
 // * __syncthreads is a counting barrier with a timeout: a barrier that not
 //   every thread reaches aborts the run with a diagnostic instead of hanging.
-// * Wavefront collectives (MFMA 16x16x4, __shfl, __shfl_xor) exchange lane
+// * Lanes run as independent threads, so the emulation checks the kernels'
+//   indexing, barrier matching and dependency waits, not how the compiler
+//   arranges the lanes of a wave around barriers (only the ISA shows that).
+// * Wavefront collectives (MFMA 16x16x4, __shfl, __shfl_xor, readfirstlane) exchange lane
 //   values through a per-wave buffer between two wave barriers; a collective
 //   that only part of a wave reaches is reported the same way.
 // * MFMA operand/result layout: lane l supplies A[l%16][l/16], B[l/16][l%16];
@@ -110,9 +113,11 @@ inline T lx_emu_xchg(T v, int src_lane)
     return r;
 }
 template <typename T> inline T __shfl(T v, int src) { return lx_emu_xchg(v, src); }
+// readfirstlane: every lane of the wave receives lane 0's value (the emulated
+// waves always run with all 64 lanes active)
+inline int __builtin_amdgcn_readfirstlane(int v) { return lx_emu_xchg(v, 0); }
 template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
 
-inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
 inline void __builtin_amdgcn_sched_barrier(int) {}
 inline void __builtin_amdgcn_fence_emu() { std::atomic_thread_fence(std::memory_order_seq_cst); }
