@@ -23,8 +23,11 @@ typedef float lx_f4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // MFMA 16x16x4 for both precisions.  Operand layout (identical for f64 and
-// f32): lane l supplies A[row = l%16][k = l/16] and B[k = l/16][col = l%16];
-// the 16x16 result is held as 4 values per lane: D[row = 4*(l/16)+i][col = l%16].
+// f32): lane l supplies A[row = l%16][k = l/16] and B[k = l/16][col = l%16].
+// The 16x16 result is held as 4 values per lane in column l%16; the rows
+// differ (measured on gfx950 with an in-bounds probe of each builtin):
+//   f32: value i of lane l is row 4*(l/16) + i
+//   f64: value i of lane l is row (l/16) + 4*i
 template <typename T> struct lx_mfma;
 template <> struct lx_mfma<double> {
     typedef lx_d4 acc;
@@ -32,6 +35,7 @@ template <> struct lx_mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int i) { return lk + 4 * i; }
 };
 template <> struct lx_mfma<float> {
     typedef lx_f4 acc;
@@ -39,6 +43,7 @@ template <> struct lx_mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int i) { return 4 * lk + i; }
 };
 
 __device__ __forceinline__ int lx_lane() { return threadIdx.x & 63; }
@@ -167,7 +172,7 @@ __device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)
             if (c >= N) continue;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int r = r0 + i * 16 + lk * 4 + e;
+                int r = r0 + i * 16 + lx_mfma<T>::row(lk, e);
                 if (r < M && (!lower || r + gdiag >= c)) {
                     T *p = C + (long long)c * ldc + r;
                     *p = *p - acc[i][j][e];
@@ -322,7 +327,7 @@ __device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    int r = r0 + i * 16 + lk * 4 + e, c = c0 + lr;
+                    int r = r0 + i * 16 + lx_mfma<T>::row(lk, e), c = c0 + lr;
                     if (r < R && c < pb) sS[r][c] = Bm[(long long)(p + c) * ldb + r] - acc[i][0][e];
                 }
         }

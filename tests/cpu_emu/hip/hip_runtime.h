@@ -13,7 +13,7 @@
 //   values through a per-wave buffer between two wave barriers; a collective
 //   that only part of a wave reaches is reported the same way.
 // * MFMA operand/result layout: lane l supplies A[l%16][l/16], B[l/16][l%16];
-//   lane l receives D[4*(l/16)+i][l%16] (the layout the kernels assume).
+//   lane l receives column l%16, rows 4*(l/16)+i (f32) or (l/16)+4*i (f64).
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -84,7 +84,8 @@ inline V lx_emu_mfma(S a, S b, V c)
     V d = c;
     const int col = lane & 15;
     for (int i = 0; i < 4; ++i) {
-        const int row = 4 * (lane >> 4) + i;
+        // result rows as the gfx950 builtins deliver them (see lx_device.hpp)
+        const int row = sizeof(S) == 8 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
         S acc = 0;
         for (int k = 0; k < 4; ++k) acc += (S)B.a[k * 16 + row] * (S)B.b[k * 16 + col];
         d[i] = c[i] + acc;
