@@ -120,6 +120,11 @@ int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, int64_t lda,
                           int64_t ldb, double *dX, int64_t ldx, void *stream, const linompss_ir_opts *opts,
                           linompss_ir_status *st);
 
+/* Normwise backward error max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf),
+ * computed in FP64 on the device (A symmetric, lower triangle read; nrhs <= 16). */
+int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
+                       const double *dB, int64_t ldb, double *berr);
+
 /* Statistics of the last executor launch on this thread. */
 int linompss_last_exec_stats(linompss_exec_stats *st);
 
@@ -137,6 +142,19 @@ int linompss_plan_info(int what, int n, int b, int nrhs, const linompss_exec_opt
                        int *tasks, double *sim_ms);
 
 const char *linompss_version(void);
+
+/* ---- Workload generators (device memory; used by the benchmark and tests).
+ * linompss_gen_gauss_spd: A = G G^T / n + alpha I (both triangles stored),
+ * G i.i.d. N(0,1) from Philox4x32-10 keyed by seed, alpha = lambda_max /
+ * (kappa - 1) with lambda_max(G G^T / n) from power iteration, so that
+ * cond2(A) ~ kappa.  Needs n*n*8 bytes of temporary device memory.
+ * linompss_gen_uniform: X (m x n, dtype 0 FP64 / 1 FP32) uniform in [lo, hi).
+ * linompss_gen_gauss: G (m x n, FP64) i.i.d. N(0,1), element e = j*m + i
+ * taken from Philox block e/2 (Box-Muller pair).                             */
+int linompss_gen_gauss_spd(int n, uint32_t seed, double kappa, double *dA, int64_t lda, double *alpha,
+                           double *lambda_max);
+int linompss_gen_uniform(int dtype, int m, int n, uint32_t seed, double lo, double hi, void *dX, int64_t ldx);
+int linompss_gen_gauss(int m, int n, uint32_t seed, double *dG, int64_t ldg);
 
 /* ---- Device-memory plumbing (so C and ctypes callers need no other HIP
  * binding).  kind: 1 host->device, 2 device->host, 3 device->device.       */

@@ -309,6 +309,50 @@ def dposv_ir_dev(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, b: int, opts
     return st
 
 
+_sig("linompss_gen_gauss_spd", _i, _i, ctypes.c_uint32, _d, _p, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d))
+_sig("linompss_gen_uniform", _i, _i, _i, _i, ctypes.c_uint32, _d, _d, _p, _i64)
+_sig("linompss_gen_gauss", _i, _i, _i, ctypes.c_uint32, _p, _i64)
+
+
+def gen_gauss_spd(n: int, seed: int, kappa: float) -> tuple[DeviceMatrix, float, float]:
+    """A = G G^T / n + alpha I on the device with cond2(A) ~ kappa; returns (A, alpha, lambda_max)."""
+    A = DeviceMatrix(n, n, np.float64)
+    a, lm = _d(0.0), _d(0.0)
+    _check("linompss_gen_gauss_spd", lib.linompss_gen_gauss_spd(n, seed, kappa, A.ptr, A.ld, ctypes.byref(a),
+                                                                   ctypes.byref(lm)), False)
+    return A, a.value, lm.value
+
+
+def gen_uniform(m: int, n: int, seed: int, lo: float = -1.0, hi: float = 1.0, dtype=np.float64) -> DeviceMatrix:
+    X = DeviceMatrix(m, n, dtype)
+    _check("linompss_gen_uniform", lib.linompss_gen_uniform(0 if np.dtype(dtype) == np.float64 else 1, m, n, seed,
+                                                             lo, hi, X.ptr, X.ld), False)
+    return X
+
+
+def gen_gauss(m: int, n: int, seed: int) -> DeviceMatrix:
+    G = DeviceMatrix(m, n, np.float64)
+    _check("linompss_gen_gauss", lib.linompss_gen_gauss(m, n, seed, G.ptr, G.ld), False)
+    return G
+
+
+_sig("linompss_dberr_dev", _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, ctypes.POINTER(_d))
+
+
+def dberr_dev(A: DeviceMatrix, X: DeviceMatrix, B: DeviceMatrix) -> float:
+    """FP64 normwise backward error of X for A X = B, computed on the device."""
+    e = _d(0.0)
+    _check("linompss_dberr_dev", lib.linompss_dberr_dev(A.cols, X.cols, A.ptr, A.ld, X.ptr, X.ld, B.ptr, B.ld,
+                                                         ctypes.byref(e)), False)
+    return e.value
+
+
+def copy_dev(dst: DeviceMatrix, src: DeviceMatrix) -> None:
+    if dst.nbytes < src.nbytes:
+        raise ValueError("destination too small")
+    _check("linompss_memcpy", lib.linompss_memcpy(dst.ptr, src.ptr, src.nbytes, 3), False)
+
+
 def last_exec_stats() -> ExecStats:
     st = ExecStats()
     lib.linompss_last_exec_stats(ctypes.byref(st))

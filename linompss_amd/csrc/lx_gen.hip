@@ -1,0 +1,227 @@
+// lx_gen.hip -- device-side workload generators.
+//
+// The reference's generators draw from drand48 on the host (GENMAT_SPD,
+// src/matutil/genmat.c:49-70).  The benchmark workloads are SPD matrices
+// A = G G^T / n + alpha I with G of i.i.d. N(0,1) entries, far too large to
+// build on the host, so they are generated here:
+//   * G from a counter-based generator, Philox4x32-10 (Salmon et al., SC'11),
+//     keyed by the seed and counted by element, then Box-Muller;
+//   * A = G G^T / n on the FP64 matrix cores (lower tiles, mirrored);
+//   * lambda_max(G G^T / n) by power iteration, alpha = lambda_max / (kappa - 1)
+//     so that cond2(A) = (lambda_max + alpha) / (lambda_min + alpha) ~ kappa
+//     (lambda_min of a square Wishart matrix is ~1/n^2, negligible);
+//   * right-hand sides uniform in [-1, 1) from the same generator (another key).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "linompss.h"
+#include "lx_device.hpp"
+#include "lx_internal.hpp"
+
+// Philox4x32-10: 10 rounds of (mulhilo by M0/M1, xor with the key), key
+// bumped by the Weyl constants between rounds.
+struct lx_u4 {
+    uint32_t x, y, z, w;
+};
+__host__ __device__ inline lx_u4 lx_philox4x32_10(lx_u4 c, uint32_t k0, uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = lx_u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Two 53-bit uniforms in (0, 1] from one Philox block, then Box-Muller.
+__host__ __device__ inline void lx_gauss2(uint64_t ctr, uint32_t seed, uint32_t stream, double &g0, double &g1)
+{
+    lx_u4 r = lx_philox4x32_10(lx_u4{(uint32_t)ctr, (uint32_t)(ctr >> 32), stream, 0u}, seed, 0x1234567u);
+    const double u0 = ((double)((((uint64_t)r.x) << 21) ^ (r.y >> 11)) + 1.0) * 0x1p-53;
+    const double u1 = ((double)((((uint64_t)r.z) << 21) ^ (r.w >> 11)) + 1.0) * 0x1p-53;
+    const double rad = sqrt(-2.0 * log(u0));
+    const double th = 6.283185307179586 * u1;
+    g0 = rad * cos(th);
+    g1 = rad * sin(th);
+}
+
+// G[i + j*ldg] for element e = j*n + i (column-major order of generation).
+__global__ void lx_gen_gauss_kernel(int m, int n, double *G, int64_t ldg, uint32_t seed)
+{
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e2 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; e2 < tot;
+         e2 += (int64_t)gridDim.x * blockDim.x * 2) {
+        double g0, g1;
+        lx_gauss2((uint64_t)(e2 >> 1), seed, 0u, g0, g1);
+        G[(e2 / m) * ldg + e2 % m] = g0;
+        if (e2 + 1 < tot) G[((e2 + 1) / m) * ldg + (e2 + 1) % m] = g1;
+    }
+}
+
+template <typename T>
+__global__ void lx_gen_uniform_kernel(int m, int n, T *X, int64_t ldx, uint32_t seed, double lo, double hi)
+{
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        lx_u4 r = lx_philox4x32_10(lx_u4{(uint32_t)e, (uint32_t)(e >> 32), 1u, 0u}, seed, 0x89abcdefu);
+        const double u = (double)((((uint64_t)r.x) << 21) ^ (r.y >> 11)) * 0x1p-53;
+        X[(e / m) * ldx + e % m] = (T)(lo + (hi - lo) * u);
+    }
+}
+
+// A = G G^T * scale (+ alpha on the diagonal): one 128 x 128 lower tile per
+// workgroup on the FP64 matrix cores, written to both triangles.
+__global__ void __launch_bounds__(256) lx_gram_kernel(int n, const double *__restrict__ G, int64_t ldg, int k,
+                                                      double *__restrict__ A, int64_t lda, double scale,
+                                                      double alpha)
+{
+    // blockIdx.x enumerates lower tiles (ti >= tj)
+    const int t = blockIdx.x;
+    int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while (ti * (ti + 1) / 2 > t) --ti;
+    const int tj = t - ti * (ti + 1) / 2;
+    const int R0 = ti * 128, C0 = tj * 128;
+    const int M = std::min(128, n - R0), N = std::min(128, n - C0);
+    const int w = threadIdx.x >> 6;
+    const int r0 = (w >> 1) * 64, c0 = (w & 1) * 64;
+    lx_d4 acc[4][4];
+    lx_acc_zero<double, 4, 4>(acc);
+    if (r0 < M && c0 < N) lx_wave_gemm_nt<double, 4, 4, 2>(acc, G + R0, ldg, G + C0, ldg, r0, c0, M, N, k);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + i * 16 + lx_mfma<double>::row(lk, e), c = c0 + j * 16 + lr;
+                if (r < M && c < N) {
+                    const int gr = R0 + r, gc = C0 + c;
+                    double v = acc[i][j][e] * scale + (gr == gc ? alpha : 0.0);
+                    A[(int64_t)gc * lda + gr] = v;
+                    A[(int64_t)gr * lda + gc] = v;
+                }
+            }
+}
+
+__global__ void lx_add_diag_kernel(int n, double *A, int64_t lda, double alpha)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        A[(int64_t)i * lda + i] += alpha;
+}
+
+// y = A x for symmetric A stored full (power iteration on the generated matrix)
+__global__ void __launch_bounds__(256) lx_gemv_full_kernel(int n, const double *__restrict__ A, int64_t lda,
+                                                           const double *__restrict__ x, double *__restrict__ y)
+{
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = blockIdx.x * 64 + lane;
+    const int rr = row < n ? row : n - 1;
+    double s = 0.0;
+    for (int j = w; j < n; j += 4) s += A[(int64_t)j * lda + rr] * x[j];
+    red[w][lane] = s;
+    __syncthreads();
+    const double v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (w == 0 && row < n) y[row] = v;
+}
+
+__global__ void __launch_bounds__(256) lx_norm2_kernel(int n, const double *__restrict__ v, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += v[i] * v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(red[0]);
+}
+
+__global__ void lx_scale_vec_kernel(int n, double *v, const double *nrm)
+{
+    const double s = nrm[0] > 0 ? 1.0 / nrm[0] : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] *= s;
+}
+
+static int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
+
+extern "C" int linompss_gen_gauss_spd(int n, uint32_t seed, double kappa, double *dA, int64_t lda, double *alpha_out,
+                                      double *lmax_out)
+{
+    if (n <= 0) return -1;
+    if (!(kappa > 1.0)) return -3;
+    if (lda < n) return -5;
+    double *G = nullptr;
+    if (hipMalloc(&G, sizeof(double) * (size_t)n * n) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    hipLaunchKernelGGL(lx_gen_gauss_kernel, dim3(grid_for((int64_t)n * n / 2)), dim3(256), 0, 0, n, n, G, (int64_t)n,
+                       seed);
+    const int nt = (n + 127) / 128;
+    const int ntiles = nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(lx_gram_kernel, dim3(ntiles), dim3(256), 0, 0, n, G, (int64_t)n, n, dA, lda, 1.0 / n, 0.0);
+    if (hipGetLastError() != hipSuccess) {
+        (void)hipFree(G);
+        return LINOMPSS_ERR_HIP;
+    }
+    // power iteration for lambda_max (start from a positive vector: the
+    // Perron-like direction of a Wishart matrix is close to it)
+    double *v = G, *y = G + n, *nrm = G + 2 * (size_t)n;  // reuse G's storage
+    (void)hipDeviceSynchronize();
+    std::vector<double> ones(n, 1.0);
+    (void)hipMemcpy(v, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    double lam = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        hipLaunchKernelGGL(lx_norm2_kernel, dim3(1), dim3(256), 0, 0, n, v, nrm);
+        hipLaunchKernelGGL(lx_scale_vec_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, v, nrm);
+        hipLaunchKernelGGL(lx_gemv_full_kernel, dim3((n + 63) / 64), dim3(256), 0, 0, n, dA, lda, v, y);
+        hipLaunchKernelGGL(lx_norm2_kernel, dim3(1), dim3(256), 0, 0, n, y, nrm);
+        double h = 0.0;
+        if (hipMemcpy(&h, nrm, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipFree(G);
+            return LINOMPSS_ERR_HIP;
+        }
+        const double prev = lam;
+        lam = h;
+        std::swap(v, y);
+        if (it > 5 && std::fabs(lam - prev) <= 1e-10 * lam) break;
+    }
+    const double alpha = lam / (kappa - 1.0);
+    hipLaunchKernelGGL(lx_add_diag_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, dA, lda, alpha);
+    const hipError_t e = hipDeviceSynchronize();
+    (void)hipFree(G);
+    if (e != hipSuccess) return LINOMPSS_ERR_HIP;
+    if (alpha_out) *alpha_out = alpha;
+    if (lmax_out) *lmax_out = lam;
+    return 0;
+}
+
+extern "C" int linompss_gen_uniform(int dtype, int m, int n, uint32_t seed, double lo, double hi, void *dX,
+                                    int64_t ldx)
+{
+    if (m < 0 || n < 0) return -2;
+    if (ldx < std::max(1, m)) return -8;
+    if ((int64_t)m * n == 0) return 0;
+    if (dtype == 0)
+        hipLaunchKernelGGL(lx_gen_uniform_kernel<double>, dim3(grid_for((int64_t)m * n)), dim3(256), 0, 0, m, n,
+                           (double *)dX, ldx, seed, lo, hi);
+    else
+        hipLaunchKernelGGL(lx_gen_uniform_kernel<float>, dim3(grid_for((int64_t)m * n)), dim3(256), 0, 0, m, n,
+                           (float *)dX, ldx, seed, lo, hi);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+extern "C" int linompss_gen_gauss(int m, int n, uint32_t seed, double *dG, int64_t ldg)
+{
+    if (m < 0 || n < 0) return -1;
+    if ((int64_t)m * n == 0) return 0;
+    hipLaunchKernelGGL(lx_gen_gauss_kernel, dim3(grid_for((int64_t)m * n / 2 + 1)), dim3(256), 0, 0, m, n, dG, ldg,
+                       seed);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}

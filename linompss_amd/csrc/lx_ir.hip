@@ -380,3 +380,32 @@ extern "C" int linompss_dposv_ir(int n, int nrhs, int b, const double *A, int ld
     (void)hipFree(dX);
     return rc;
 }
+
+// Normwise backward error max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf)
+// in FP64 on the device, A symmetric with its lower triangle stored.
+extern "C" int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
+                                  const double *dB, int64_t ldb, double *berr_out)
+{
+    if (n <= 0 || nrhs <= 0 || nrhs > 16) return -1;
+    std::lock_guard<std::mutex> g(g_ir_mu);
+    const size_t nr = (size_t)n * nrhs;
+    char *ws = (char *)lx_scratch(7, sizeof(double) * (3 * nr + 2 * (size_t)n + 128));
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    double *R = (double *)ws, *Y1 = R + nr, *Y2 = Y1 + nr, *rowabs = Y2 + nr, *colabs = rowabs + n;
+    double *small = colabs + n;
+    double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32;
+    hipStream_t s = nullptr;
+    int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, Y1, Y2, rowabs, colabs, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, dX, ldx, n, xmax);
+    hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
+    double h[16];
+    LX_HIP_TRY(hipMemcpy(h, berr, sizeof(double) * nrhs, hipMemcpyDeviceToHost));
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c)
+        if (!(h[c] <= worst)) worst = h[c];
+    *berr_out = worst;
+    return 0;
+}
