@@ -100,6 +100,8 @@ typedef struct {
     int unit_m, unit_n;   /* GEMM/SYRK work-unit tile (default 128 x 128) */
     int wg_per_cu;        /* resident workgroups per CU (0 = occupancy limit) */
     double timeout_s;     /* per-dependency wait limit on the device (default 30 s) */
+    int fused_solve;      /* posv: 1 = factor and sweeps in one persistent launch (the full
+                             task DAG); 0 (default) = factor launch + sweep kernels */
 } linompss_exec_opts;
 
 typedef struct {

@@ -57,7 +57,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 
 class ExecOpts(ctypes.Structure):
-    _fields_ = [("policy", _i), ("unit_m", _i), ("unit_n", _i), ("wg_per_cu", _i), ("timeout_s", _d)]
+    _fields_ = [("policy", _i), ("unit_m", _i), ("unit_n", _i), ("wg_per_cu", _i), ("timeout_s", _d),
+                ("fused_solve", _i)]
 
 
 class ExecStats(ctypes.Structure):

@@ -265,14 +265,17 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     LX_HIP_TRY(hipEventCreate(&e0));
     LX_HIP_TRY(hipEventCreate(&e1));
     LX_HIP_TRY(hipEventRecord(e0, stream));
+    long long *prof = nullptr;
+    static const bool prof_on = getenv("LINOMPSS_EXEC_PROFILE") != nullptr;
+    if (prof_on) prof = (long long *)scratch_get(c, 2, sizeof(long long) * LX_PROF_WORDS * (size_t)grid);
     if (sh.dtype == 0)
         hipLaunchKernelGGL(lx_exec_kernel<double>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
                            pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
-                           (int *)(ctl + off_info), ticks);
+                           (int *)(ctl + off_info), ticks, prof);
     else
         hipLaunchKernelGGL(lx_exec_kernel<float>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
                            pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
-                           (int *)(ctl + off_info), ticks);
+                           (int *)(ctl + off_info), ticks, prof);
     LX_HIP_TRY(hipGetLastError());
     LX_TRACE("launched grid=%d", grid);
     LX_HIP_TRY(hipEventRecord(e1, stream));
@@ -291,6 +294,23 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     g_last_stats.tasks = pl->ntasks;
     g_last_stats.grid = grid;
     g_last_stats.sim_ms = pl->sim_s * 1e3;
+    if (prof) {
+        std::vector<long long> h((size_t)LX_PROF_WORDS * grid);
+        if (hipMemcpy(h.data(), prof, sizeof(long long) * h.size(), hipMemcpyDeviceToHost) == hipSuccess) {
+            double tot[LX_PROF_WORDS] = {};
+            for (int g2 = 0; g2 < grid; ++g2)
+                for (int q = 0; q < LX_PROF_WORDS; ++q) tot[q] += (double)h[(size_t)g2 * LX_PROF_WORDS + q];
+            const double tick_ms = 1.0 / c->wall_khz;  // per workgroup, summed
+            static const char *nm[LX_OP_NOP] = {"potrf", "trsm", "syrk", "gemm", "trsv", "gemv"};
+            fprintf(stderr, "[linompss profile] what=%d dtype=%d n=%d grid=%d launch=%.2f ms  wait=%.1f%%", sh.what,
+                    sh.dtype, sh.n, grid, ms, 100.0 * tot[0] * tick_ms / (ms * grid));
+            for (int q = 0; q < LX_OP_NOP; ++q)
+                if (tot[1 + LX_OP_NOP + q] > 0)
+                    fprintf(stderr, "  %s: n=%.0f %.1f%% avg=%.1f us", nm[q], tot[1 + LX_OP_NOP + q],
+                            100.0 * tot[1 + q] * tick_ms / (ms * grid), tot[1 + q] * tick_ms * 1e3 / tot[1 + LX_OP_NOP + q]);
+            fprintf(stderr, "\n");
+        }
+    }
     if (hst.abort || hst.timeouts || hst.done != pl->nunits) return LINOMPSS_ERR_SCHED;
     *info_out = hinfo == INT_MAX ? 0 : hinfo;
     return 0;
@@ -334,10 +354,17 @@ extern "C" int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, in
     if (ldx < std::max(1, n)) return -8;
     if (n == 0) return 0;
     std::lock_guard<std::mutex> g(g_mu);
-    lx_shape sh = {nrhs > 0 ? LX_W_POSV_RL : LX_W_FACTOR_RL, dtype, n, b, nrhs};
+    const bool fused = opts && opts->fused_solve;
+    lx_shape sh = {(nrhs > 0 && fused) ? LX_W_POSV_RL : LX_W_FACTOR_RL, dtype, n, b, nrhs};
     int info = 0;
     int rc = lx_run_single(sh, dA, lda, dX, ldx, (hipStream_t)stream, opts, &info);
-    return rc ? rc : info;
+    if (rc) return rc;
+    if (nrhs > 0 && !fused && info == 0) {
+        rc = lx_sweeps_dev(dtype, n, nrhs, dA, lda, dX, ldx, 3, (hipStream_t)stream);
+        if (rc) return rc;
+        LX_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return info;
 }
 
 extern "C" int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
@@ -351,9 +378,15 @@ extern "C" int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void 
     if (ldx < std::max(1, n)) return -8;
     if (n == 0 || nrhs == 0) return 0;
     std::lock_guard<std::mutex> g(g_mu);
-    lx_shape sh = {LX_W_SWEEPS, dtype, n, b, nrhs};
-    int info = 0;
-    return lx_run_single(sh, const_cast<void *>(dL), lda, dX, ldx, (hipStream_t)stream, opts, &info);
+    if (opts && opts->fused_solve) {  // the sweeps as a task DAG on the persistent executor
+        lx_shape sh = {LX_W_SWEEPS, dtype, n, b, nrhs};
+        int info = 0;
+        return lx_run_single(sh, const_cast<void *>(dL), lda, dX, ldx, (hipStream_t)stream, opts, &info);
+    }
+    int rc = lx_sweeps_dev(dtype, n, nrhs, dL, lda, dX, ldx, 3, (hipStream_t)stream);
+    if (rc) return rc;
+    LX_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
 }
 
 extern "C" int linompss_last_exec_stats(linompss_exec_stats *st)
@@ -483,9 +516,8 @@ static int host_ltrsm(char uplo, char trans, char diag, int bm, int bn, int m, i
     }
     if (!rc) {
         std::lock_guard<std::mutex> g(g_mu);
-        lx_shape sh = {tr ? LX_W_BWD : LX_W_FWD, dtype, m, bm, bn};
-        int info = 0;
-        rc = lx_run_single(sh, dA, m, dB, m, nullptr, nullptr, &info);
+        rc = lx_sweeps_dev(dtype, m, bn, dA, m, dB, m, tr ? 2 : 1, nullptr);
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = LINOMPSS_ERR_HIP;
     }
     if (!rc &&
         hipMemcpy2D(B, sizeof(T) * (size_t)ldb, dB, sizeof(T) * m, sizeof(T) * m, bn, hipMemcpyDeviceToHost) !=

@@ -81,15 +81,26 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
 // share one program counter.
 __device__ __forceinline__ int lx_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Optional profile (prof != null): per workgroup, wave 0 accumulates
+// wall-clock ticks spent waiting for dependencies and running each op type,
+// written at exit as LX_PROF_WORDS int64 words per workgroup:
+//   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
+#define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
+
 template <typename T>
 __global__ void __launch_bounds__(LX_WG_THREADS)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
-                   lx_status *st, int *info, long long spin_ticks)
+                   lx_status *st, int *info, long long spin_ticks, long long *prof)
 {
     __shared__ __attribute__((aligned(16))) char smem[LX_SMEM_BYTES];
     __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
     const int lane = lx_lane();
+    long long p_acc[LX_PROF_WORDS];
+#pragma unroll
+    for (int q = 0; q < LX_PROF_WORDS; ++q) p_acc[q] = 0;
+    long long t_grab = 0, t_ready = 0;
     for (;;) {
+        if (prof) t_grab = wall_clock64();
         if (lx_wave() == 0) {  // wave-uniform branch
             // one atomic per wave: lane 0 adds 1, the others add 0; lane 0's old value is the index
             const int idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
@@ -133,6 +144,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS)
         const int idx = lx_uniform(s_sched[1]);
         __syncthreads();  // s_sched is rewritten by wave 0 in the next iteration
         if (!go) break;
+        if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
         lx_run_unit<T>(u, sys, info, smem);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -141,5 +153,17 @@ __global__ void __launch_bounds__(LX_WG_THREADS)
             __hip_atomic_fetch_add(&ctr[u.ctr], lane == 0 ? 1 : 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             atomicAdd(&st->done, lane == 0 ? 1 : 0);
         }
+        if (prof) {
+            const long long t_done = wall_clock64();
+            p_acc[0] += t_ready - t_grab;
+#pragma unroll
+            for (int q = 0; q < LX_OP_NOP; ++q)
+                if (u.op == q) {
+                    p_acc[1 + q] += t_done - t_ready;
+                    p_acc[1 + LX_OP_NOP + q] += 1;
+                }
+        }
     }
+    if (prof && threadIdx.x == 0)
+        for (int q = 0; q < LX_PROF_WORDS; ++q) prof[(long long)blockIdx.x * LX_PROF_WORDS + q] = p_acc[q];
 }

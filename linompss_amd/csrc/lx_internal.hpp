@@ -27,6 +27,11 @@ struct lx_shape {
 int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t ldx, hipStream_t stream,
                   const linompss_exec_opts *opts, int *info_out);
 
+// Triangular sweeps with dedicated kernels (lx_sweep.hip): which = 1 forward
+// (L Y = X), 2 backward (L^T Z = Y), 3 both.  Enqueued on the stream.
+int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
+                  hipStream_t s);
+
 // Grow-only device scratch owned by the library (one per slot and device).
 void *lx_scratch(int slot, size_t bytes);
 

@@ -264,7 +264,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         hipLaunchKernelGGL(lx_d2s_lower_kernel, grid, dim3(256), 0, s, dA, lda, n, Af);
         LX_HIP_TRY(hipGetLastError());
     }
-    linompss_exec_opts eo = {1, 128, 128, 0, 30.0};
+    linompss_exec_opts eo = {1, 128, 128, 0, 30.0, 0};
     int info = 0;
     int rc = linompss_chol_dev(1, 0, n, b, Af, n, s, &eo);
     linompss_exec_stats es;

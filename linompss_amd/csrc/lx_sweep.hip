@@ -1,0 +1,182 @@
+// lx_sweep.hip -- blocked triangular sweeps on the device: L Y = B and
+// L^T X = Y for a lower factor L, nrhs <= 16 right-hand sides.
+//
+// Same arithmetic as the reference's bblas_dltrsm lower branches
+// (src/blocked_blas/bblas_trsm.h:119-160): for each row block, subtract the
+// products with the blocks already solved (task_dgemm), then solve with the
+// diagonal block (task_dtrsm).  The reference runs these as a chain of OmpSs
+// tasks over b-row blocks; on the GPU the sweep is bandwidth- and
+// latency-bound (L is read once, n^2/2 elements, and each block depends on
+// the previous one), so it runs as one launch with 64-row blocks:
+//   * a workgroup takes the next block by ticket (blocks are handed out in
+//     dependency order, so every wait is on a block already held by a running
+//     workgroup);
+//   * its 4 wavefronts stream the L blocks of its row (forward) or column
+//     (backward), each waiting on the ready flag of the block it consumes;
+//     the backward sweep transposes each 64 x 16 slab through LDS so that the
+//     global reads stay coalesced;
+//   * wavefront 0 solves the 64 x 64 diagonal block (lane = row, shuffles
+//     broadcast the solved entries), stores the block and publishes its flag.
+// All branches that steer a wavefront are wave-uniform.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "linompss.h"
+#include "lx_device.hpp"
+#include "lx_internal.hpp"
+
+#define LX_SW_NR 16
+
+__device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename T>
+__device__ __forceinline__ void lx_sw_wait(const int *flag)
+{
+    while (lx_sw_uniform(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0)
+        __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X is
+// overwritten in place.  flags[nblk] and *ticket must be zero on entry.
+template <typename T, int TRANS>
+__global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T *__restrict__ L, int64_t ldl,
+                                                       T *__restrict__ X, int64_t ldx, int *flags, int *ticket)
+{
+    constexpr int KC = 16;  // backward: rows of an L block staged per LDS slab
+    // per wave: the solved block it consumes, then (after its last block) its partial sums
+    __shared__ T sx[4][LX_SW_NR][64];
+    __shared__ T stile[TRANS ? 4 : 1][64][KC + 1];
+    __shared__ int s_blk;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nblk = (n + 63) / 64;
+    if (w == 0) {
+        const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
+        s_blk = t;  // every lane of wave 0 stores the same value
+    }
+    __syncthreads();
+    const int t = lx_sw_uniform(s_blk);
+    const int ib = TRANS ? nblk - 1 - t : t;
+    const int r0 = ib * 64;
+    const int hb = min(64, n - r0);
+    const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb recompute the last row, never store
+    T acc[LX_SW_NR];
+#pragma unroll
+    for (int c = 0; c < LX_SW_NR; ++c) acc[c] = T(0);
+
+    // contributions of the blocks solved before this one
+    const int jfirst = TRANS ? ib + 1 : 0, jlast = TRANS ? nblk : ib;
+    for (int j = jfirst + w; j < jlast; j += 4) {
+        lx_sw_wait<T>(&flags[j]);
+        const int c0 = j * 64, hj = min(64, n - c0);
+        // stage x_j (solved rows c0 .. c0+hj) in LDS
+        for (int c = 0; c < nrhs; ++c) sx[w][c][lane] = lane < hj ? X[(int64_t)c * ldx + c0 + lane] : T(0);
+        __builtin_amdgcn_wave_barrier();
+        if (!TRANS) {
+            // acc[r] += sum_k L[row, c0+k] x_j[k]  (column k of the block: 64 contiguous rows)
+            for (int k = 0; k < hj; ++k) {
+                const T l = L[(int64_t)(c0 + k) * ldl + row];
+#pragma unroll
+                for (int c = 0; c < LX_SW_NR; ++c)
+                    if (c < nrhs) acc[c] += l * sx[w][c][k];
+            }
+        } else {
+            // acc[r] += sum_k L[c0+k, r0+r] x_j[k]: block (j, ib) read coalesced along its
+            // columns (lanes = k), transposed through LDS, then lane r walks its row
+            for (int k0 = 0; k0 < hj; k0 += KC) {
+                const int kk = lane & (KC - 1), cq = lane / KC;  // 4 columns per pass, 16 rows each
+                for (int cc = cq; cc < 64; cc += 64 / KC) {
+                    const int colg = r0 + min(cc, hb - 1), rowg = c0 + min(k0 + kk, hj - 1);
+                    stile[w][cc][kk] = L[(int64_t)colg * ldl + rowg];
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int kend = min(KC, hj - k0);
+                for (int k = 0; k < kend; ++k) {
+                    const T l = stile[w][lane][k];
+#pragma unroll
+                    for (int c = 0; c < LX_SW_NR; ++c)
+                        if (c < nrhs) acc[c] += l * sx[w][c][k0 + k];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // sx[w] is this wave's own buffer: safe to reuse for its partial sums
+#pragma unroll
+    for (int c = 0; c < LX_SW_NR; ++c) sx[w][c][lane] = acc[c];
+    __syncthreads();
+    if (w == 0) {
+        // right-hand side of this block minus the contributions
+        T s[LX_SW_NR];
+#pragma unroll
+        for (int c = 0; c < LX_SW_NR; ++c)
+            s[c] = c < nrhs ? X[(int64_t)c * ldx + row] -
+                                  ((sx[0][c][lane] + sx[1][c][lane]) + (sx[2][c][lane] + sx[3][c][lane]))
+                            : T(0);
+        // diagonal block solve, lane = row
+        if (!TRANS) {
+            for (int jj = 0; jj < hb; ++jj) {
+                const T piv = L[(int64_t)(r0 + jj) * ldl + r0 + jj];
+                const T ljj = L[(int64_t)(r0 + jj) * ldl + row];  // L[row, r0+jj]
+#pragma unroll
+                for (int c = 0; c < LX_SW_NR; ++c) {
+                    if (c < nrhs) {
+                        const T xj = __shfl(s[c], jj) / piv;
+                        s[c] = lane == jj ? xj : (lane > jj ? s[c] - ljj * xj : s[c]);
+                    }
+                }
+            }
+        } else {
+            for (int jj = hb - 1; jj >= 0; --jj) {
+                const T piv = L[(int64_t)(r0 + jj) * ldl + r0 + jj];
+                const T ljj = L[(int64_t)row * ldl + r0 + jj];  // L[r0+jj, row] = (L^T)[row, r0+jj]
+#pragma unroll
+                for (int c = 0; c < LX_SW_NR; ++c) {
+                    if (c < nrhs) {
+                        const T xj = __shfl(s[c], jj) / piv;
+                        s[c] = lane == jj ? xj : (lane < jj ? s[c] - ljj * xj : s[c]);
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < nrhs; ++c)
+            if (lane < hb) X[(int64_t)c * ldx + r0 + lane] = s[c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[ib], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+#ifndef LX_SWEEP_KERNELS_ONLY
+template <typename T>
+static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, int which, hipStream_t s)
+{
+    const int nblk = (n + 63) / 64;
+    int *ctl = (int *)lx_scratch(3, sizeof(int) * (size_t)(2 * nblk + 64));
+    if (!ctl) return LINOMPSS_ERR_NOMEM;
+    for (int c0 = 0; c0 < nrhs; c0 += LX_SW_NR) {
+        const int nc = std::min(LX_SW_NR, nrhs - c0);
+        if (which & 1) {
+            lx_fill_i32(ctl, 0, nblk + 64, s);
+            hipLaunchKernelGGL((lx_sweep_kernel<T, 0>), dim3(nblk), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,
+                               ctl + 64, ctl);
+        }
+        if (which & 2) {
+            lx_fill_i32(ctl, 0, nblk + 64, s);
+            hipLaunchKernelGGL((lx_sweep_kernel<T, 1>), dim3(nblk), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,
+                               ctl + 64, ctl);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+// which: 1 forward, 2 backward, 3 both.  Enqueued on the stream (no sync).
+int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
+                  hipStream_t s)
+{
+    if (n <= 0 || nrhs <= 0) return 0;
+    if (dtype == 0) return sweeps<double>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, which, s);
+    return sweeps<float>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, s);
+}
+#endif  // LX_SWEEP_KERNELS_ONLY

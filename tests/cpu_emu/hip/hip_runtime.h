@@ -33,6 +33,7 @@
 #define __shared__ static
 
 struct lx_emu_dim3 { unsigned x = 0, y = 0, z = 0; };
+typedef struct lx_emu_stream *hipStream_t;  // host-side declarations only
 extern thread_local lx_emu_dim3 threadIdx;
 extern thread_local lx_emu_dim3 blockIdx;
 extern lx_emu_dim3 blockDim, gridDim;
@@ -120,6 +121,8 @@ inline int __builtin_amdgcn_readfirstlane(int v) { return lx_emu_xchg(v, 0); }
 template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
 
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
+// lanes are separate threads here, so a wave barrier must really synchronise them
+inline void __builtin_amdgcn_wave_barrier() { lx_emu_wave_barrier[threadIdx.x >> 6].wait(); }
 inline void __builtin_amdgcn_sched_barrier(int) {}
 inline void __builtin_amdgcn_fence_emu() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 #define __builtin_amdgcn_fence(order, scope) __builtin_amdgcn_fence_emu()
@@ -141,3 +144,5 @@ inline int atomicMin(int *p, int v)
     return cur;
 }
 using std::sqrt;
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }

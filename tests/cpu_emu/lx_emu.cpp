@@ -11,6 +11,11 @@
 #include "lx_exec_kernel.hpp"
 #include "lx_planner.hpp"
 
+// the sweep kernels' device code (the .hip file also holds host code that
+// needs the HIP runtime; compile only its kernels here)
+#define LX_SWEEP_KERNELS_ONLY
+#include "lx_sweep.hip"
+
 thread_local lx_emu_dim3 threadIdx, blockIdx;
 lx_emu_dim3 blockDim, gridDim;
 lx_emu_barrier lx_emu_wg_barrier;
@@ -59,13 +64,57 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
             threadIdx.x = t;
             blockIdx.x = 0;
             if (dtype == 0)
-                lx_exec_kernel<double>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks);
+                lx_exec_kernel<double>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
             else
-                lx_exec_kernel<float>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks);
+                lx_exec_kernel<float>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
     *done_out = st.done;
     *nunits_out = (int)U.size();
     return (st.abort || st.timeouts) ? -1 : 0;
+}
+
+template <typename F>
+static void run_wg(F &&body)
+{
+    lx_emu_wg_barrier.init(256, "workgroup");
+    for (int w = 0; w < 4; ++w) lx_emu_wave_barrier[w].init(64, "wavefront");
+    std::vector<std::thread> th;
+    th.reserve(256);
+    for (int t = 0; t < 256; ++t)
+        th.emplace_back([&, t] {
+            threadIdx.x = t;
+            blockIdx.x = 0;
+            body();
+        });
+    for (auto &t : th) t.join();
+}
+
+// One sweep (trans 0: L Y = X, 1: L^T Z = X) through lx_sweep_kernel, one
+// workgroup at a time in ticket order.
+extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,
+                            long long ldx)
+{
+    const int nblk = (n + 63) / 64;
+    std::vector<int> flags(nblk, 0);
+    int ticket = 0;
+    for (int b = 0; b < nblk; ++b) {
+        run_wg([&] {
+            if (dtype == 0) {
+                if (trans)
+                    lx_sweep_kernel<double, 1>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, flags.data(), &ticket);
+                else
+                    lx_sweep_kernel<double, 0>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, flags.data(), &ticket);
+            } else {
+                if (trans)
+                    lx_sweep_kernel<float, 1>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, flags.data(), &ticket);
+                else
+                    lx_sweep_kernel<float, 0>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, flags.data(), &ticket);
+            }
+        });
+    }
+    for (int b = 0; b < nblk; ++b)
+        if (flags[b] != 1) return -1;
+    return ticket == nblk ? 0 : -2;
 }

@@ -84,3 +84,34 @@ def test_info_on_indefinite(emu, orc):
     A[30, 30] = -5.0
     assert emu(0, 0, n, b, 0, A.copy(order="F")) == 31
     assert emu(0, 1, n, b, 0, A.copy(order="F")) == 31
+
+
+@pytest.fixture(scope="module")
+def emu_sweep(emu):
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E.lx_emu_sweep.restype = ctypes.c_int
+    E.lx_emu_sweep.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+                                                   ctypes.c_longlong]
+
+    def run(trans, L, X):
+        dtype = 0 if L.dtype == np.float64 else 1
+        rc = E.lx_emu_sweep(dtype, trans, L.shape[0], X.shape[1], L.ctypes.data_as(ctypes.c_void_p), L.shape[0],
+                            X.ctypes.data_as(ctypes.c_void_p), X.shape[0])
+        assert rc == 0
+    return run
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,nr", [(1, 1), (64, 2), (100, 3), (200, 16)])
+def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
+    """lx_sweep.hip's forward and backward kernels against the oracle's blocked sweeps."""
+    rng = np.random.default_rng(n + nr)
+    L = np.asfortranarray(np.tril(rng.uniform(0, 1, (n, n))) + n * np.eye(n)).astype(dtype, order="F")
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (n, nr))).astype(dtype, order="F")
+    for trans in (0, 1):
+        X = B0.copy(order="F")
+        emu_sweep(trans, L, X)
+        Xo = B0.astype(np.float64, order="F")
+        orc.ltrsm_lower(trans, np.asfortranarray(L.astype(np.float64)), Xo, 32)
+        tol = 1e-13 if dtype == np.float64 else 1e-5
+        assert np.abs(X - Xo).max() <= tol * np.abs(Xo).max()
