@@ -214,7 +214,7 @@ __device__ __forceinline__ void lx_unit_gemm_nt(T *__restrict__ C, long long ldc
 template <typename T> struct lx_potrf_cfg { static constexpr int PB = 32; };
 
 template <typename T>
-__device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info_local, char *smem)
+__device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info_local, char *smem)
 {
     constexpr int PB = lx_potrf_cfg<T>::PB;
     T(*sD)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem);
@@ -306,7 +306,7 @@ __device__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info
 // Column-blocked by NB: X_p = (B_p - X_{<p} L_{p,<p}^T) L_pp^{-T}; the update
 // runs on the MFMA core, the NB x NB solve one row per lane.
 template <typename T>
-__device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L, long long ldl,
+__device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L, long long ldl,
                                   int R, int N, char *smem)
 {
     constexpr int NB = 32, RU = 64;  // R <= RU rows per unit
@@ -369,7 +369,7 @@ __device__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__
 // Blocked by 64 rows; the 64 x 64 diagonal solve runs on one wavefront per
 // right-hand side (lane = row), the off-diagonal update on all 256 threads.
 template <typename T>
-__device__ void lx_unit_trsv_block(T *__restrict__ X, long long ldx, const T *__restrict__ L, long long ldl,
+__device__ __forceinline__ void lx_unit_trsv_block(T *__restrict__ X, long long ldx, const T *__restrict__ L, long long ldl,
                                    int M, int nrhs, int trans, char *smem)
 {
     constexpr int NB = 64;
@@ -454,7 +454,7 @@ __device__ void lx_unit_trsv_block(T *__restrict__ X, long long ldx, const T *__
 // Y[M x nrhs] -= L[K x M]^T * Z[K x nrhs] (trans = 1; backward-sweep GEMM)
 // Memory-bound: L is read once, coalesced along its columns.
 template <typename T>
-__device__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ldy, const T *__restrict__ L, long long ldl,
+__device__ __forceinline__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ldy, const T *__restrict__ L, long long ldl,
                                  const T *__restrict__ Z, long long ldz, int M, int K, int nrhs, int trans,
                                  char *smem)
 {

@@ -61,7 +61,7 @@ struct lx_dev_ctx {
     int dev = -1;
     int cus = 0;
     int wall_khz = 0;
-    int occ[2] = {0, 0};  // resident WGs per CU for the FP64 / FP32 executor
+    int occ[2][3] = {};  // resident WGs per CU: [FP64 / FP32][op family]
     std::list<lx_dplan> plans;  // most recent first
     void *scratch[8] = {};
     size_t scratch_sz[8] = {};
@@ -83,15 +83,16 @@ lx_dev_ctx *ctx_get()
         if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
             c->wall_khz <= 0)
             c->wall_khz = 100000;
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lx_exec_kernel<double>, LX_WG_THREADS, 0) !=
-            hipSuccess)
-            return nullptr;
-        c->occ[0] = std::max(1, nb);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lx_exec_kernel<float>, LX_WG_THREADS, 0) !=
-            hipSuccess)
-            return nullptr;
-        c->occ[1] = std::max(1, nb);
+        int nb[6] = {};
+        hipError_t e = hipSuccess;
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[0], lx_exec_kernel<double, LX_FAM_FACTOR>, LX_WG_THREADS, 0);
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[1], lx_exec_kernel<double, LX_FAM_SOLVE>, LX_WG_THREADS, 0);
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[2], lx_exec_kernel<double, LX_FAM_ALL>, LX_WG_THREADS, 0);
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[3], lx_exec_kernel<float, LX_FAM_FACTOR>, LX_WG_THREADS, 0);
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[4], lx_exec_kernel<float, LX_FAM_SOLVE>, LX_WG_THREADS, 0);
+        e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[5], lx_exec_kernel<float, LX_FAM_ALL>, LX_WG_THREADS, 0);
+        if (e != hipSuccess) return nullptr;
+        for (int q = 0; q < 6; ++q) c->occ[q / 3][q % 3] = std::max(1, nb[q]);
         g_ctx[dev] = std::move(c);
     }
     return g_ctx[dev].get();
@@ -225,10 +226,12 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
 {
     LX_TRACE("run what=%d dtype=%d n=%d b=%d nrhs=%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs);
     lx_dev_ctx *c = ctx_get();
-    LX_TRACE("ctx %p cus=%d wall_khz=%d occ=%d/%d", (void *)c, c ? c->cus : 0, c ? c->wall_khz : 0,
-             c ? c->occ[0] : 0, c ? c->occ[1] : 0);
     if (!c) return LINOMPSS_ERR_NODEV;
-    const int occ = c->occ[sh.dtype == 0 ? 0 : 1];
+    const int fam = (sh.what == LX_W_FACTOR_RL || sh.what == LX_W_FACTOR_LL) ? LX_FAM_FACTOR
+                    : sh.what == LX_W_POSV_RL                                 ? LX_FAM_ALL
+                                                                              : LX_FAM_SOLVE;
+    const int occ = c->occ[sh.dtype == 0 ? 0 : 1][fam];
+    LX_TRACE("ctx %p cus=%d wall_khz=%d family=%d occ=%d", (void *)c, c->cus, c->wall_khz, fam, occ);
     int wpc = occ;
     if (opts && opts->wg_per_cu > 0) wpc = std::min(opts->wg_per_cu, occ);
     const int workers = c->cus * wpc;
@@ -268,14 +271,20 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     long long *prof = nullptr;
     static const bool prof_on = getenv("LINOMPSS_EXEC_PROFILE") != nullptr;
     if (prof_on) prof = (long long *)scratch_get(c, 2, sizeof(long long) * LX_PROF_WORDS * (size_t)grid);
-    if (sh.dtype == 0)
-        hipLaunchKernelGGL(lx_exec_kernel<double>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
-                           pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
-                           (int *)(ctl + off_info), ticks, prof);
-    else
-        hipLaunchKernelGGL(lx_exec_kernel<float>, dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units,
-                           pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,
-                           (int *)(ctl + off_info), ticks, prof);
+#define LX_LAUNCH(TT, FF)                                                                                          \
+    hipLaunchKernelGGL((lx_exec_kernel<TT, FF>), dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units, pl->nunits, \
+                       (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,                   \
+                       (int *)(ctl + off_info), ticks, prof)
+    if (sh.dtype == 0) {
+        if (fam == LX_FAM_FACTOR) LX_LAUNCH(double, LX_FAM_FACTOR);
+        else if (fam == LX_FAM_SOLVE) LX_LAUNCH(double, LX_FAM_SOLVE);
+        else LX_LAUNCH(double, LX_FAM_ALL);
+    } else {
+        if (fam == LX_FAM_FACTOR) LX_LAUNCH(float, LX_FAM_FACTOR);
+        else if (fam == LX_FAM_SOLVE) LX_LAUNCH(float, LX_FAM_SOLVE);
+        else LX_LAUNCH(float, LX_FAM_ALL);
+    }
+#undef LX_LAUNCH
     LX_HIP_TRY(hipGetLastError());
     LX_TRACE("launched grid=%d", grid);
     LX_HIP_TRY(hipEventRecord(e1, stream));

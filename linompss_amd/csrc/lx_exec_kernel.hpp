@@ -18,7 +18,11 @@
 #include "lx_device.hpp"
 #include "lx_plan.hpp"
 
-#define LX_SMEM_BYTES (42 * 1024)
+// Op families compiled into one executor kernel.  Each family is its own
+// kernel symbol, so a launch carries only the registers and LDS its units use.
+enum lx_family : int { LX_FAM_FACTOR = 0, LX_FAM_SOLVE = 1, LX_FAM_ALL = 2 };
+template <int FAM> struct lx_fam_smem { static constexpr int bytes = FAM == LX_FAM_FACTOR ? 25 * 1024 + 512 : 42 * 1024; };
+template <int FAM> struct lx_fam_occ { static constexpr int waves_per_simd = FAM == LX_FAM_FACTOR ? 2 : 1; };
 
 template <typename T> struct lx_gemm_cfg;
 template <> struct lx_gemm_cfg<double> { static constexpr int FM = 4, FN = 4, U = 2; };
@@ -30,7 +34,7 @@ __device__ __forceinline__ long long lx_off(int64_t rcp, long long ld)
     return (long long)(rcp >> 32) * ld + (long long)(rcp & 0xffffffffll);
 }
 
-template <typename T>
+template <typename T, int FAM>
 __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__restrict__ sys, int *info, char *smem)
 {
     const lx_sys s = sys[u.sys];
@@ -45,8 +49,10 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     const T *Ap = Ab + lx_off(u.a, lda);
     const T *Bp = Bb + lx_off(u.b, ldb);
     constexpr int FM = lx_gemm_cfg<T>::FM, FN = lx_gemm_cfg<T>::FN, UU = lx_gemm_cfg<T>::U;
+    constexpr bool F = FAM != LX_FAM_SOLVE, S = FAM != LX_FAM_FACTOR;
     switch (u.op) {
     case LX_OP_POTRF: {
+        if constexpr (!F) break;
         int loc = 0;
         lx_unit_potrf<T>(C, ldc, u.M, &loc, smem);
         if (loc != 0)  // loc is the same in every thread
@@ -54,17 +60,17 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
         break;
     }
     case LX_OP_TRSM_RLTN:
-        lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, u.M, u.N, smem);
+        if constexpr (F) lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, u.M, u.N, smem);
         break;
     case LX_OP_SYRK:
     case LX_OP_GEMM_NT:
-        lx_unit_gemm_nt<T, FM, FN, UU>(C, ldc, Ap, lda, Bp, ldb, u.M, u.N, u.K, u.aux0 != 0, u.aux1);
+        if constexpr (F) lx_unit_gemm_nt<T, FM, FN, UU>(C, ldc, Ap, lda, Bp, ldb, u.M, u.N, u.K, u.aux0 != 0, u.aux1);
         break;
     case LX_OP_TRSV:
-        lx_unit_trsv_block<T>(C, ldc, Ap, lda, u.M, u.N, u.aux0, smem);
+        if constexpr (S) lx_unit_trsv_block<T>(C, ldc, Ap, lda, u.M, u.N, u.aux0, smem);
         break;
     case LX_OP_GEMV_RHS:
-        lx_unit_gemv_rhs<T>(C, ldc, Ap, lda, Bp, ldb, u.M, u.K, u.N, u.aux0, smem);
+        if constexpr (S) lx_unit_gemv_rhs<T>(C, ldc, Ap, lda, Bp, ldb, u.M, u.K, u.N, u.aux0, smem);
         break;
     default:
         break;
@@ -87,12 +93,12 @@ __device__ __forceinline__ int lx_uniform(int v) { return __builtin_amdgcn_readf
 //   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
 #define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
 
-template <typename T>
-__global__ void __launch_bounds__(LX_WG_THREADS)
+template <typename T, int FAM>
+__global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
                    lx_status *st, int *info, long long spin_ticks, long long *prof)
 {
-    __shared__ __attribute__((aligned(16))) char smem[LX_SMEM_BYTES];
+    __shared__ __attribute__((aligned(16))) char smem[lx_fam_smem<FAM>::bytes];
     __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
     const int lane = lx_lane();
     long long p_acc[LX_PROF_WORDS];
@@ -146,7 +152,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS)
         if (!go) break;
         if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
-        lx_run_unit<T>(u, sys, info, smem);
+        lx_run_unit<T, FAM>(u, sys, info, smem);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (lx_wave() == 0) {

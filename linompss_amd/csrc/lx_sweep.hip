@@ -40,16 +40,22 @@ __device__ __forceinline__ void lx_sw_wait(const int *flag)
 
 // trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X is
 // overwritten in place.  flags[nblk] and *ticket must be zero on entry.
+//
+// The 64 x 64 diagonal block is read into registers (16 values per thread,
+// coalesced) before the workgroup waits on anything, and moved to LDS for the
+// solve, so the serial part of the chain touches no global memory.
 template <typename T, int TRANS>
 __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T *__restrict__ L, int64_t ldl,
                                                        T *__restrict__ X, int64_t ldx, int *flags, int *ticket)
 {
-    constexpr int KC = 16;  // backward: rows of an L block staged per LDS slab
+    constexpr int KC = 128 / sizeof(T);  // backward: L rows per LDS slab (one 128-byte line per column)
+    constexpr int TILE = 4 * 64 * (KC + 1), DIAG = 64 * 65;
     // per wave: the solved block it consumes, then (after its last block) its partial sums
     __shared__ T sx[4][LX_SW_NR][64];
-    __shared__ T stile[TRANS ? 4 : 1][64][KC + 1];
+    // backward transpose slabs during the accumulation, the diagonal block for the solve
+    __shared__ T sbuf[(TRANS ? TILE : 0) > DIAG ? (TRANS ? TILE : 0) : DIAG];
     __shared__ int s_blk;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nblk = (n + 63) / 64;
     if (w == 0) {
         const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
@@ -61,6 +67,15 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
     const int r0 = ib * 64;
     const int hb = min(64, n - r0);
     const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb recompute the last row, never store
+
+    // diagonal block -> registers: thread (lane, w) holds column 4*q + w, row lane
+    T dreg[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = min(4 * q + w, hb - 1);
+        dreg[q] = L[(int64_t)(r0 + col) * ldl + row];
+    }
+
     T acc[LX_SW_NR];
 #pragma unroll
     for (int c = 0; c < LX_SW_NR; ++c) acc[c] = T(0);
@@ -75,25 +90,46 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
         __builtin_amdgcn_wave_barrier();
         if (!TRANS) {
             // acc[r] += sum_k L[row, c0+k] x_j[k]  (column k of the block: 64 contiguous rows)
-            for (int k = 0; k < hj; ++k) {
-                const T l = L[(int64_t)(c0 + k) * ldl + row];
+            const T *Lb = L + (int64_t)c0 * ldl + row;
+            if (hj == 64) {
+                for (int k0 = 0; k0 < 64; k0 += 8) {
+                    T l[8];
 #pragma unroll
-                for (int c = 0; c < LX_SW_NR; ++c)
-                    if (c < nrhs) acc[c] += l * sx[w][c][k];
+                    for (int u = 0; u < 8; ++u) l[u] = Lb[(int64_t)(k0 + u) * ldl];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+#pragma unroll
+                        for (int c = 0; c < LX_SW_NR; ++c)
+                            if (c < nrhs) acc[c] += l[u] * sx[w][c][k0 + u];
+                }
+            } else {
+                for (int k = 0; k < hj; ++k) {
+                    const T l = Lb[(int64_t)k * ldl];
+#pragma unroll
+                    for (int c = 0; c < LX_SW_NR; ++c)
+                        if (c < nrhs) acc[c] += l * sx[w][c][k];
+                }
             }
         } else {
-            // acc[r] += sum_k L[c0+k, r0+r] x_j[k]: block (j, ib) read coalesced along its
-            // columns (lanes = k), transposed through LDS, then lane r walks its row
+            // acc[r] += sum_k L[c0+k, r0+r] x_j[k]: block (j, ib) read along its columns
+            // (one 128-byte line per column and instruction), transposed through LDS
+            T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(sbuf + w * 64 * (KC + 1));
+            constexpr int CPI = 64 / KC;  // columns per load instruction
             for (int k0 = 0; k0 < hj; k0 += KC) {
-                const int kk = lane & (KC - 1), cq = lane / KC;  // 4 columns per pass, 16 rows each
-                for (int cc = cq; cc < 64; cc += 64 / KC) {
+                const int kk = lane % KC, cq = lane / KC;
+                T v[KC];  // KC >= 16: 64 / CPI loads in flight
+#pragma unroll
+                for (int q = 0; q < 64 / CPI; ++q) {
+                    const int cc = q * CPI + cq;
                     const int colg = r0 + min(cc, hb - 1), rowg = c0 + min(k0 + kk, hj - 1);
-                    stile[w][cc][kk] = L[(int64_t)colg * ldl + rowg];
+                    v[q] = L[(int64_t)colg * ldl + rowg];
                 }
+#pragma unroll
+                for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = v[q];
                 __builtin_amdgcn_wave_barrier();
                 const int kend = min(KC, hj - k0);
                 for (int k = 0; k < kend; ++k) {
-                    const T l = stile[w][lane][k];
+                    const T l = stile[lane][k];
 #pragma unroll
                     for (int c = 0; c < LX_SW_NR; ++c)
                         if (c < nrhs) acc[c] += l * sx[w][c][k0 + k];
@@ -106,6 +142,10 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
     // sx[w] is this wave's own buffer: safe to reuse for its partial sums
 #pragma unroll
     for (int c = 0; c < LX_SW_NR; ++c) sx[w][c][lane] = acc[c];
+    __syncthreads();  // all transpose slabs consumed: sbuf now takes the diagonal block
+    T(*sdiag)[65] = reinterpret_cast<T(*)[65]>(sbuf);  // sdiag[r][c] = L[r0+r, r0+c]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sdiag[lane][4 * q + w] = dreg[q];
     __syncthreads();
     if (w == 0) {
         // right-hand side of this block minus the contributions
@@ -115,11 +155,12 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
             s[c] = c < nrhs ? X[(int64_t)c * ldx + row] -
                                   ((sx[0][c][lane] + sx[1][c][lane]) + (sx[2][c][lane] + sx[3][c][lane]))
                             : T(0);
+        const int lr = min(lane, hb - 1);
         // diagonal block solve, lane = row
         if (!TRANS) {
             for (int jj = 0; jj < hb; ++jj) {
-                const T piv = L[(int64_t)(r0 + jj) * ldl + r0 + jj];
-                const T ljj = L[(int64_t)(r0 + jj) * ldl + row];  // L[row, r0+jj]
+                const T piv = sdiag[jj][jj];
+                const T ljj = sdiag[lr][jj];  // L[row, r0+jj]
 #pragma unroll
                 for (int c = 0; c < LX_SW_NR; ++c) {
                     if (c < nrhs) {
@@ -130,8 +171,8 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
             }
         } else {
             for (int jj = hb - 1; jj >= 0; --jj) {
-                const T piv = L[(int64_t)(r0 + jj) * ldl + r0 + jj];
-                const T ljj = L[(int64_t)row * ldl + r0 + jj];  // L[r0+jj, row] = (L^T)[row, r0+jj]
+                const T piv = sdiag[jj][jj];
+                const T ljj = sdiag[jj][lr];  // L[r0+jj, row] = (L^T)[row, r0+jj]
 #pragma unroll
                 for (int c = 0; c < LX_SW_NR; ++c) {
                     if (c < nrhs) {

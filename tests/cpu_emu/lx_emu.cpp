@@ -64,9 +64,9 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
             threadIdx.x = t;
             blockIdx.x = 0;
             if (dtype == 0)
-                lx_exec_kernel<double>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
+                lx_exec_kernel<double, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
             else
-                lx_exec_kernel<float>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
+                lx_exec_kernel<float, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
