@@ -46,7 +46,20 @@ template <> struct lx_mfma<float> {
     static __device__ __forceinline__ int row(int lk, int i) { return 4 * lk + i; }
 };
 
-__device__ __forceinline__ int lx_lane() { return threadIdx.x & 63; }
+// Thread index passed through an empty asm statement.  The executor runs
+// many unit bodies inside one persistent loop; with the plain index the
+// compiler hoists every lane-derived address of every body out of that loop
+// and keeps them all live at once (hundreds of registers, then spills).  The
+// opaque copy is recomputed where it is used.
+__device__ __forceinline__ int lx_tid()
+{
+    int t = threadIdx.x;
+#if defined(__AMDGCN__)
+    asm volatile("" : "+v"(t));
+#endif
+    return t;
+}
+__device__ __forceinline__ int lx_lane() { return lx_tid() & 63; }
 __device__ __forceinline__ int lx_wave() { return threadIdx.x >> 6; }
 
 // ---------------------------------------------------------------------------
@@ -202,97 +215,187 @@ __device__ __forceinline__ void lx_unit_gemm_nt(T *__restrict__ C, long long ldc
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-tile Cholesky (reference task_dpotrf -> dpotrf_("L"), tasks_potrf.c
-// :20-24), blocked right-looking inside the tile with panel width PB:
-//   1) factor the PB x PB diagonal block in LDS (one barrier per column);
-//   2) solve the panel below it against that block (one row per thread);
-//   3) update the lower trailing part with the MFMA core.
-// Returns through *info (in every thread) the 1-based column (local) of the
-// first non-positive pivot, as LAPACK's info, and keeps going (results past
-// that column are not meaningful, as in LAPACK).  Every thread reads the same
-// pivot, so the info is tracked in registers, with no lane-guarded code.
-template <typename T> struct lx_potrf_cfg { static constexpr int PB = 32; };
-
-template <typename T>
-__device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, int *info_local, char *smem)
+// Wave-level C_acc += A * B^T with A from global memory (rows r0.. of a
+// column-major panel, clamped past M) and B^T from LDS: element (k, c) of B^T
+// is sB[c * ldsb + k] (row c of a row-major 64 x K block).  16 rows x 16*FN
+// columns, K a multiple of 4 (columns >= K must hold zeros in sB).
+template <typename T, int FN>
+__device__ __forceinline__ void lx_wave_mm_ldsb(typename lx_mfma<T>::acc (&acc)[1][FN], const T *__restrict__ A,
+                                                long long lda, int r0, int M, const T *sB, int ldsb, int K)
 {
-    constexpr int PB = lx_potrf_cfg<T>::PB;
-    T(*sD)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem);
-    T(*sL)[PB + 1] = reinterpret_cast<T(*)[PB + 1]>(smem + sizeof(T) * PB * (PB + 1));
-    const int tid = threadIdx.x;
+    const int lane = lx_lane(), lr = lane & 15, lk = lane >> 4;
+    const int r = r0 + lr < M ? r0 + lr : M - 1;
+    const T *pa = A + r;
+#pragma unroll 2
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const T a = pa[(long long)(k0 + lk) * lda];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[0][j] = lx_mfma<T>::mac(a, sB[(j * 16 + lr) * ldsb + k0 + lk], acc[0][j]);
+    }
+}
+
+// Value of v in lane l (l wave-uniform), for every lane: v_readlane, no LDS.
+__device__ __forceinline__ float lx_readlane(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lx_readlane(double v, int l)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-tile Cholesky (reference task_dpotrf -> dpotrf_("L"), tasks_potrf.c
+// :20-24), blocked right-looking inside the tile with 64-wide panels.  For
+// each panel the 64 x 64 diagonal block D sits in LDS (rows past the tile are
+// an identity pad) and:
+//   1) D is factored left-looking in four 16-column blocks: the matrix cores
+//      subtract the earlier columns (wave w owns rows 16w..16w+15), then
+//      wavefront 0 factors the 16 columns with one row per lane, held in
+//      registers (v_readlane broadcasts the pivots and the scaled column);
+//   2) Z = D^-1 by blocks: each wave inverts one 16 x 16 diagonal block, then
+//      Z_IJ = -Z_II sum_{K=J}^{I-1} L_IK Z_KJ on the matrix cores, level by
+//      level; Z goes to LDS and to the workspace W (the panel TRSM units of
+//      this step read it there);
+//   3) the rows below D become A_panel Z^T on the matrix cores (each wave owns
+//      16-row chunks, in place);
+//   4) the lower trailing part is updated with K = 64 on the matrix cores.
+// Returns through *info (same value in every thread) the 1-based local column
+// of the first non-positive pivot, as LAPACK's info; the factorisation keeps
+// going past it (as LAPACK, results there are not meaningful).
+// LDS: 2 * 64 * 65 * sizeof(T) + 16 bytes.
+template <typename T>
+__device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, T *__restrict__ W,
+                                              long long ldw, int *info_local, char *smem)
+{
+    constexpr int PB = 64, LD = 65;
+    T(*sD)[LD] = reinterpret_cast<T(*)[LD]>(smem);
+    T(*sZ)[LD] = reinterpret_cast<T(*)[LD]>(smem + sizeof(T) * PB * LD);
+    int *s_bad = reinterpret_cast<int *>(smem + 2 * sizeof(T) * PB * LD);
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
     int first_bad = 0;
     for (int p = 0; p < M; p += PB) {
         const int pb = M - p < PB ? M - p : PB;
-        // 1) load diagonal block (lower part used)
+        // load D (full square, identity pad past pb); clear Z
         for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
-            int r = e % PB, c = e / PB;
-            sD[r][c] = (r < pb && c < pb) ? A[(long long)(p + c) * lda + p + r] : T(0);
-            sL[r][c] = T(0);
+            const int r = e % PB, c = e / PB;
+            sD[r][c] = (r < pb && c < pb) ? A[(long long)(p + c) * lda + p + r] : (r == c ? T(1) : T(0));
+            sZ[r][c] = T(0);
         }
         __syncthreads();
-        for (int c = 0; c < pb; ++c) {
-            T dcc = sD[c][c];
-            bool bad = !(dcc > T(0));
-            T d = sqrt(dcc);  // NaN past a failed pivot, like LAPACK's garbage-after-info
-            if (bad && first_bad == 0) first_bad = p + c + 1;
-            // column c of L (scaled) -> sL; trailing columns of sD updated
-            for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
-                int r = e % PB, c2 = e / PB;
-                if (r >= pb || c2 >= pb || r < c2) continue;
-                if (c2 == c) {
-                    sL[r][c] = (r == c) ? d : sD[r][c] / d;
-                } else if (c2 > c) {
-                    T lr_ = sD[r][c] / d, lc_ = sD[c2][c] / d;
-                    sD[r][c2] = sD[r][c2] - lr_ * lc_;
+        // 1) factor D, 16 columns at a time
+        for (int J = 0; J < 4; ++J) {
+            const int j0 = 16 * J;
+            if (J > 0 && w >= J) {  // wave-uniform: D[rows of wave w, j0:j0+16] -= L[., 0:j0] L[j0:j0+16, 0:j0]^T
+                typename lx_mfma<T>::acc acc[1][1];
+                lx_acc_zero<T, 1, 1>(acc);
+                for (int k0 = 0; k0 < j0; k0 += 4)
+                    acc[0][0] = lx_mfma<T>::mac(sD[16 * w + lr][k0 + lk], sD[j0 + lr][k0 + lk], acc[0][0]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sD[16 * w + lx_mfma<T>::row(lk, i)][j0 + lr] -= acc[0][0][i];
+            }
+            __syncthreads();
+            if (w == 0) {
+                T a[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) a[j] = sD[lane][j0 + j];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int cc = j0 + c;
+                    const T piv = lx_readlane(a[c], cc);
+                    if (cc < pb && !(piv > T(0)) && first_bad == 0) first_bad = p + cc + 1;  // uniform
+                    const T d = sqrt(piv), rd = T(1) / d;
+                    a[c] = lane == cc ? d : (lane > cc ? a[c] * rd : a[c]);
+#pragma unroll
+                    for (int c2 = c + 1; c2 < 16; ++c2) {
+                        const T l2 = lx_readlane(a[c], j0 + c2);
+                        a[c2] = lane >= j0 + c2 ? a[c2] - a[c] * l2 : a[c2];
+                    }
                 }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) sD[lane][j0 + j] = a[j];
+                s_bad[0] = first_bad;  // every lane of wave 0 stores the same value
             }
             __syncthreads();
         }
-        // write L_pp back
-        for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
-            int r = e % PB, c = e / PB;
-            if (r < pb && c < pb && r >= c) A[(long long)(p + c) * lda + p + r] = sL[r][c];
-        }
-        // 2) panel rows below: x = a * L_pp^{-T}
-        const int r_beg = p + pb;
-        // uniform trip count; rows past M recompute row M-1 and do not store
-        const int nrow = M - r_beg, nq = (nrow + LX_WG_THREADS - 1) / LX_WG_THREADS;
-        for (int q = 0; q < nq; ++q) {
-            const int rr = r_beg + q * LX_WG_THREADS + tid;
-            const bool live = rr < M;
-            const int r = live ? rr : M - 1;
-            T x[PB];
+        first_bad = s_bad[0];
+        // 2a) wave J inverts the 16 x 16 diagonal block J, lane = column (lanes 0..15 store)
+        {
+            const int b0 = 16 * w, c = lr;
+            T z[16];
 #pragma unroll
-            for (int c = 0; c < PB; ++c) x[c] = c < pb ? A[(long long)(p + c) * lda + r] : T(0);
+            for (int r = 0; r < 16; ++r) {
+                T s = r == c ? T(1) : T(0);
 #pragma unroll
-            for (int c = 0; c < PB; ++c) {
-                if (c < pb) {
-                    T s = x[c];
-#pragma unroll
-                    for (int c2 = 0; c2 < c; ++c2) s -= x[c2] * sL[c][c2];
-                    x[c] = s / sL[c][c];
-                }
-                // keep the LDS reads of later columns from being hoisted (register pressure)
-                __builtin_amdgcn_sched_barrier(0);
+                for (int k = 0; k < r; ++k) s -= sD[b0 + r][b0 + k] * z[k];
+                z[r] = r >= c ? s / sD[b0 + r][b0 + r] : T(0);
             }
+            if (lane < 16) {
 #pragma unroll
-            for (int c = 0; c < PB; ++c)
-                if (live && c < pb) A[(long long)(p + c) * lda + r] = x[c];
+                for (int r = 0; r < 16; ++r) sZ[b0 + r][b0 + c] = z[r];
+            }
         }
         __syncthreads();
-        // 3) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel
-        const int mt = M - r_beg;
-        if (mt > 0) {
+        // 2b) off-diagonal blocks, level I = 1..3; wave J < I forms Z_IJ.  The
+        // partial sum S goes through D's unused upper block (J, I).
+        for (int I = 1; I < 4; ++I) {
+            if (w < I) {  // wave-uniform
+                const int J = w;
+                typename lx_mfma<T>::acc S[1][1], R[1][1];
+                lx_acc_zero<T, 1, 1>(S);
+                lx_acc_zero<T, 1, 1>(R);
+                for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
+                    S[0][0] = lx_mfma<T>::mac(sD[16 * I + lr][16 * J + k0 + lk], sZ[16 * J + k0 + lk][16 * J + lr], S[0][0]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sD[16 * J + lx_mfma<T>::row(lk, i)][16 * I + lr] = S[0][0][i];
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int k0 = 0; k0 < 16; k0 += 4)
+                    R[0][0] = lx_mfma<T>::mac(sZ[16 * I + lr][16 * I + k0 + lk], sD[16 * J + k0 + lk][16 * I + lr], R[0][0]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sZ[16 * I + lx_mfma<T>::row(lk, i)][16 * J + lr] = -R[0][0][i];
+            }
+            __syncthreads();
+        }
+        // Z -> W (the full 64 x 64 block; W holds n + 64 rows, so p + 63 stays
+        // inside), L_pp (lower) -> A
+        for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
+            const int r = e % PB, c = e / PB;
+            W[(long long)c * ldw + p + r] = sZ[r][c];
+            if (r < pb && c < pb && r >= c) A[(long long)(p + c) * lda + p + r] = sD[r][c];
+        }
+        // 3) panel rows below: P := P * Z^T, 16-row chunks per wave, in place
+        const int r_beg = p + pb, nrow = M - r_beg;
+        const int nchunk = (nrow + 15) / 16;
+        T *P = A + (long long)p * lda;
+        for (int ch = w; ch < nchunk; ch += 4) {
+            typename lx_mfma<T>::acc acc[1][4];
+            lx_acc_zero<T, 1, 4>(acc);
+            lx_wave_mm_ldsb<T, 4>(acc, P, lda, r_beg + ch * 16, M, &sZ[0][0], LD, PB);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = r_beg + ch * 16 + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                    if (r < M && c < pb) P[(long long)c * lda + r] = acc[0][j][e];
+                }
+        }
+        __syncthreads();
+        // 4) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel, K = pb
+        if (nrow > 0) {
             constexpr int FMU = 4, FNU = 4, BS = 2 * 16 * 4;  // 128 x 128 blocks
-            const T *P = A + (long long)p * lda + r_beg;
+            const T *Pp = A + (long long)p * lda + r_beg;
             T *Ct = A + (long long)r_beg * lda + r_beg;
-            const int nb = (mt + BS - 1) / BS;
+            const int nb = (nrow + BS - 1) / BS;
             for (int bj = 0; bj < nb; ++bj)
                 for (int bi = bj; bi < nb; ++bi) {
-                    int rr = bi * BS, cc = bj * BS;
-                    lx_unit_gemm_nt<T, FMU, FNU, 2>(Ct + (long long)cc * lda + rr, lda, P + rr, lda, P + cc, lda,
-                                                    mt - rr < BS ? mt - rr : BS, mt - cc < BS ? mt - cc : BS, pb,
-                                                    bi == bj, rr - cc);
+                    const int rr = bi * BS, cc = bj * BS;
+                    lx_unit_gemm_nt<T, FMU, FNU, sizeof(T) == 8 ? 1 : 2>(
+                        Ct + (long long)cc * lda + rr, lda, Pp + rr, lda, Pp + cc, lda, nrow - rr < BS ? nrow - rr : BS,
+                        nrow - cc < BS ? nrow - cc : BS, pb, bi == bj, rr - cc);
                 }
         }
         __syncthreads();
@@ -302,62 +405,77 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
 
 // ---------------------------------------------------------------------------
 // Panel TRSM unit (reference task_dtrsm(R, L, T, N), chol_rlmain.c:37 ->
-// tasks_trsm.c:22-25):  X[R x N] = B[R x N] * L^{-T}, rows independent.
-// Column-blocked by NB: X_p = (B_p - X_{<p} L_{p,<p}^T) L_pp^{-T}; the update
-// runs on the MFMA core, the NB x NB solve one row per lane.
+// tasks_trsm.c:22-25):  X[R x N] = B[R x N] * L^{-T}, R <= 64, rows
+// independent.  Wavefront w owns rows 16w..16w+15 and the workgroup walks the
+// 64-column blocks q left to right:
+//   Y = B_q - X_{<q} L[q, <q]^T   (matrix cores, K = 64 q, operands from L2)
+//   X_q = Y Z_q^T                  (Z_q = inverse of the diagonal 64 x 64
+//                                   block, written to W by the POTRF unit and
+//                                   staged in LDS once per block)
+// Every wave runs every block (waves past R compute clamped rows and store
+// nothing), so the two barriers per block match in all four.
+// LDS: (64 * 65 + 4 * 16 * 65) * sizeof(T).
 template <typename T>
-__device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L, long long ldl,
-                                  int R, int N, char *smem)
+__device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L,
+                                                  long long ldl, const T *__restrict__ W, long long ldw, int R, int N,
+                                                  char *smem)
 {
-    constexpr int NB = 32, RU = 64;  // R <= RU rows per unit
-    T(*sS)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem);
-    T(*sDg)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem + sizeof(T) * RU * (NB + 1));
-    const int tid = threadIdx.x, w = lx_wave();
-    for (int p = 0; p < N; p += NB) {
-        const int pb = N - p < NB ? N - p : NB;
-        // update: acc = X[:, 0:p] * L[p:p+pb, 0:p]^T  (waves 2 x 2 of 32 x 16)
-        typename lx_mfma<T>::acc acc[2][1];
-        lx_acc_zero<T, 2, 1>(acc);
-        const int r0 = (w >> 1) * 32, c0 = (w & 1) * 16;
-        if (p > 0) lx_wave_gemm_nt<T, 2, 1, 2>(acc, Bm, ldb, L + p, ldl, r0, c0, R, pb, p);
-        // S = B_p - acc -> LDS
-        {
-            const int lane = lx_lane(), lr = lane & 15, lk = lane >> 4;
+    constexpr int QB = 64, LD = 65;
+    constexpr int U = sizeof(T) == 8 ? 2 : 4;
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    T(*sZq)[LD] = reinterpret_cast<T(*)[LD]>(smem);  // sZq[c][k] = Z_q[c][k]
+    T(*sY)[LD] = reinterpret_cast<T(*)[LD]>(smem + sizeof(T) * (QB * LD + 16 * LD * w));
+    const int r0 = w * 16;
+    const bool active = r0 < R;  // wave-uniform
+    for (int q = 0; q < N; q += QB) {
+        const int qb = N - q < QB ? N - q : QB;
+        // stage Z_q (column-major in W: Z[r][c] at W[c * ldw + q + r])
+        for (int e = tid; e < QB * QB; e += LX_WG_THREADS) {
+            const int r = e % QB, c = e / QB;
+            sZq[r][c] = W[(long long)c * ldw + q + r];
+        }
+        // this block's right-hand sides, loaded before the long update
+        T bq[4][4];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rl = lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                const int rg = r0 + rl < R ? r0 + rl : R - 1;
+                bq[j][e] = Bm[(long long)(q + (c < qb ? c : qb - 1)) * ldb + rg];
+            }
+        typename lx_mfma<T>::acc acc[1][4];
+        lx_acc_zero<T, 1, 4>(acc);
+        // X_{<q} L[q, <q]^T
+        if (active && q > 0) lx_wave_gemm_nt<T, 1, 4, U>(acc, Bm, ldb, L + q, ldl, r0, 0, R, qb, q);
+        // Y = B_q - acc -> LDS (row-major, zero past the block)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rl = lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                sY[rl][c] = c < qb ? bq[j][e] - acc[0][j][e] : T(0);
+            }
+        __syncthreads();  // sZq staged, sY written
+        // X_q = Y Z_q^T
+        typename lx_mfma<T>::acc x[1][4];
+        lx_acc_zero<T, 1, 4>(x);
+#pragma unroll 2
+        for (int k0 = 0; k0 < QB; k0 += 4) {
+            const T a = sY[lr][k0 + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[0][j] = lx_mfma<T>::mac(a, sZq[j * 16 + lr][k0 + lk], x[0][j]);
+        }
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    int r = r0 + i * 16 + lx_mfma<T>::row(lk, e), c = c0 + lr;
-                    if (r < R && c < pb) sS[r][c] = Bm[(long long)(p + c) * ldb + r] - acc[i][0][e];
+                    const int r = r0 + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                    if (r < R && c < qb) Bm[(long long)(q + c) * ldb + r] = x[0][j][e];
                 }
         }
-        for (int e = tid; e < NB * NB; e += LX_WG_THREADS) {
-            int r = e % NB, c = e / NB;
-            if (r < pb && c <= r) sDg[r][c] = L[(long long)(p + c) * ldl + p + r];
-        }
-        __syncthreads();
-        {
-            // every thread solves a row (threads past R repeat row R-1 and do not store)
-            const bool live = tid < R;
-            const int rw = live ? tid : R - 1;
-            T x[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                if (c < pb) {
-                    T s = sS[rw][c];
-#pragma unroll
-                    for (int c2 = 0; c2 < c; ++c2) s -= x[c2] * sDg[c][c2];
-                    x[c] = s / sDg[c][c];
-                } else {
-                    x[c] = T(0);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-#pragma unroll
-            for (int c = 0; c < NB; ++c)
-                if (live && c < pb) Bm[(long long)(p + c) * ldb + rw] = x[c];
-        }
-        __syncthreads();
+        __syncthreads();  // sZq and sY are rewritten by the next block
     }
 }
 
@@ -375,7 +493,7 @@ __device__ __forceinline__ void lx_unit_trsv_block(T *__restrict__ X, long long 
     constexpr int NB = 64;
     T(*sD)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(smem);
     T(*sX)[NB] = reinterpret_cast<T(*)[NB]>(smem + sizeof(T) * NB * (NB + 1));
-    const int tid = threadIdx.x, lane = lx_lane(), w = lx_wave();
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
     const int nblk = (M + NB - 1) / NB;
     for (int cb = 0; cb < nrhs; cb += 16) {
         const int nc = nrhs - cb < 16 ? nrhs - cb : 16;
@@ -463,7 +581,7 @@ __device__ __forceinline__ void lx_unit_gemv_rhs(T *__restrict__ Y, long long ld
     T(*sZ)[KC] = reinterpret_cast<T(*)[KC]>(smem);
     T(*sT)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + sizeof(T) * 16 * KC);
     T(*sRed)[16][64] = reinterpret_cast<T(*)[16][64]>(smem + sizeof(T) * 16 * KC);
-    const int tid = threadIdx.x;
+    const int tid = lx_tid();
     for (int cb = 0; cb < nrhs; cb += 16) {
         const int nc = nrhs - cb < 16 ? nrhs - cb : 16;
         if (!trans) {

@@ -205,7 +205,8 @@ void *lx_scratch(int slot, size_t bytes)
 }
 
 // Scratch slots: 0 = executor control (status, counters, info, sys table),
-// 1..3 = host-API staging, 4..7 = refinement workspaces.
+// 1 = factor workspace (inverse diagonal blocks), 2 = executor profile,
+// 3 = sweep flags, 4..7 = refinement workspaces.
 static bool lx_trace_on()
 {
     static int on = -1;
@@ -255,6 +256,11 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     hs.n = sh.n;
     hs.nrhs = sh.nrhs;
     hs.info_slot = 0;
+    if (fam != LX_FAM_SOLVE) {
+        hs.ldw = (int64_t)sh.n + 64;
+        hs.W = scratch_get(c, 1, (size_t)hs.ldw * 64 * (sh.dtype == 0 ? sizeof(double) : sizeof(float)));
+        if (!hs.W) return LINOMPSS_ERR_NOMEM;
+    }
     LX_HIP_TRY(hipMemcpyAsync(ctl + off_sys, &hs, sizeof hs, hipMemcpyHostToDevice, stream));
     lx_fill_i32((int32_t *)ctl, 0, 16, stream);
     lx_fill_i32((int32_t *)(ctl + off_info), INT_MAX, 1, stream);

@@ -21,7 +21,11 @@
 // Op families compiled into one executor kernel.  Each family is its own
 // kernel symbol, so a launch carries only the registers and LDS its units use.
 enum lx_family : int { LX_FAM_FACTOR = 0, LX_FAM_SOLVE = 1, LX_FAM_ALL = 2 };
-template <int FAM> struct lx_fam_smem { static constexpr int bytes = FAM == LX_FAM_FACTOR ? 25 * 1024 + 512 : 42 * 1024; };
+template <typename T, int FAM> struct lx_fam_smem {
+    static constexpr int factor = 2 * 64 * 65 * (int)sizeof(T) + 16;  // POTRF; TRSM takes 2 * 64 * 65 * sizeof(T)
+    static constexpr int solve = 42 * 1024;
+    static constexpr int bytes = FAM == LX_FAM_FACTOR ? factor : FAM == LX_FAM_SOLVE ? solve : (factor > solve ? factor : solve);
+};
 template <int FAM> struct lx_fam_occ { static constexpr int waves_per_simd = FAM == LX_FAM_FACTOR ? 2 : 1; };
 
 template <typename T> struct lx_gemm_cfg;
@@ -54,13 +58,15 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     case LX_OP_POTRF: {
         if constexpr (!F) break;
         int loc = 0;
-        lx_unit_potrf<T>(C, ldc, u.M, &loc, smem);
+        lx_unit_potrf<T>(C, ldc, u.M, (T *)s.W + u.aux1, s.ldw, &loc, smem);
         if (loc != 0)  // loc is the same in every thread
             atomicMin(&info[s.info_slot], threadIdx.x == 0 ? u.aux1 + loc : INT_MAX);
         break;
     }
     case LX_OP_TRSM_RLTN:
-        if constexpr (F) lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, u.M, u.N, smem);
+        // the diagonal tile's inverse blocks start at W row = its row in A
+        if constexpr (F)
+            lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, (const T *)s.W + (u.a & 0xffffffffll), s.ldw, u.M, u.N, smem);
         break;
     case LX_OP_SYRK:
     case LX_OP_GEMM_NT:
@@ -98,12 +104,11 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
                    lx_status *st, int *info, long long spin_ticks, long long *prof)
 {
-    __shared__ __attribute__((aligned(16))) char smem[lx_fam_smem<FAM>::bytes];
+    __shared__ __attribute__((aligned(16))) char smem[lx_fam_smem<T, FAM>::bytes];
     __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
     const int lane = lx_lane();
-    long long p_acc[LX_PROF_WORDS];
-#pragma unroll
-    for (int q = 0; q < LX_PROF_WORDS; ++q) p_acc[q] = 0;
+    __shared__ long long s_prof[LX_PROF_WORDS];  // in LDS: keeps the profile out of the registers
+    if (threadIdx.x < LX_PROF_WORDS) s_prof[threadIdx.x] = 0;  // ordered by the loop's first barrier
     long long t_grab = 0, t_ready = 0;
     for (;;) {
         if (prof) t_grab = wall_clock64();
@@ -159,17 +164,13 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
             __hip_atomic_fetch_add(&ctr[u.ctr], lane == 0 ? 1 : 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             atomicAdd(&st->done, lane == 0 ? 1 : 0);
         }
-        if (prof) {
+        if (prof && threadIdx.x == 0 && u.op >= 0 && u.op < LX_OP_NOP) {  // no barrier inside
             const long long t_done = wall_clock64();
-            p_acc[0] += t_ready - t_grab;
-#pragma unroll
-            for (int q = 0; q < LX_OP_NOP; ++q)
-                if (u.op == q) {
-                    p_acc[1 + q] += t_done - t_ready;
-                    p_acc[1 + LX_OP_NOP + q] += 1;
-                }
+            s_prof[0] += t_ready - t_grab;
+            s_prof[1 + u.op] += t_done - t_ready;
+            s_prof[1 + LX_OP_NOP + u.op] += 1;
         }
     }
     if (prof && threadIdx.x == 0)
-        for (int q = 0; q < LX_PROF_WORDS; ++q) prof[(long long)blockIdx.x * LX_PROF_WORDS + q] = p_acc[q];
+        for (int q = 0; q < LX_PROF_WORDS; ++q) prof[(long long)blockIdx.x * LX_PROF_WORDS + q] = s_prof[q];
 }

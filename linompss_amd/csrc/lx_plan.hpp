@@ -53,6 +53,9 @@ struct lx_sys {
     int32_t n, nrhs;
     int32_t info_slot;     // index in the info array
     int32_t pad_;
+    void *W;               // factor workspace (T*), column-major (n + 64) x 64:
+                           // inverses of the 64 x 64 diagonal blocks, rows as in A
+    int64_t ldw;
 };
 
 // Device-side status words (one per launch).

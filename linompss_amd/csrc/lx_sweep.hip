@@ -80,9 +80,12 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
 #pragma unroll
     for (int c = 0; c < LX_SW_NR; ++c) acc[c] = T(0);
 
-    // contributions of the blocks solved before this one
-    const int jfirst = TRANS ? ib + 1 : 0, jlast = TRANS ? nblk : ib;
-    for (int j = jfirst + w; j < jlast; j += 4) {
+    // contributions of the blocks solved before this one, oldest first (forward:
+    // j = 0, 1, ...; backward: j = nblk-1, nblk-2, ...) so that each wave only
+    // ever waits for the newest block at the end of its list
+    const int nj = TRANS ? nblk - 1 - ib : ib;
+    for (int q = w; q < nj; q += 4) {
+        const int j = TRANS ? nblk - 1 - q : q;
         lx_sw_wait<T>(&flags[j]);
         const int c0 = j * 64, hj = min(64, n - c0);
         // stage x_j (solved rows c0 .. c0+hj) in LDS

@@ -118,6 +118,7 @@ template <typename T> inline T __shfl(T v, int src) { return lx_emu_xchg(v, src)
 // readfirstlane: every lane of the wave receives lane 0's value (the emulated
 // waves always run with all 64 lanes active)
 inline int __builtin_amdgcn_readfirstlane(int v) { return lx_emu_xchg(v, 0); }
+inline int __builtin_amdgcn_readlane(int v, int l) { return lx_emu_xchg(v, l); }
 template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
 
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }

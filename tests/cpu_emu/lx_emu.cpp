@@ -52,6 +52,9 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     sys.ldx = ldx;
     sys.n = n;
     sys.nrhs = nrhs;
+    sys.ldw = (int64_t)n + 64;
+    std::vector<double> wbuf((size_t)sys.ldw * 64);  // large enough for either precision
+    sys.W = wbuf.data();
     blockDim.x = 256;
     gridDim.x = 1;
     lx_emu_wg_barrier.init(256, "workgroup");
