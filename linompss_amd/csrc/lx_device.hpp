@@ -248,6 +248,56 @@ __device__ __forceinline__ double lx_readlane(double v, int l)
 }
 
 // ---------------------------------------------------------------------------
+// Z = L^-1 for a 64 x 64 lower-triangular L held in LDS (sL[r][c], row
+// stride 65; the upper blocks of sL are used as scratch and lost), by blocks:
+// each wave inverts one 16 x 16 diagonal block (lane = column), then
+// Z_IJ = -Z_II sum_{K=J}^{I-1} L_IK Z_KJ on the matrix cores for I = 1..3.
+// sZ must be zero on entry and sL/sZ visible to the whole workgroup; all four
+// waves must call it (it contains workgroup barriers); sZ is complete and
+// visible on return.
+template <typename T>
+__device__ __forceinline__ void lx_tri_inv64(T (*sL)[65], T (*sZ)[65])
+{
+    const int lane = lx_lane(), w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    {
+        const int b0 = 16 * w, c = lr;
+        T z[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            T s = r == c ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < r; ++k) s -= sL[b0 + r][b0 + k] * z[k];
+            z[r] = r >= c ? s / sL[b0 + r][b0 + r] : T(0);
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) sZ[b0 + r][b0 + c] = z[r];
+        }
+    }
+    __syncthreads();
+    // the partial sum S of block (I, J) goes through sL's unused upper block (J, I)
+    for (int I = 1; I < 4; ++I) {
+        if (w < I) {  // wave-uniform
+            const int J = w;
+            typename lx_mfma<T>::acc S[1][1], R[1][1];
+            lx_acc_zero<T, 1, 1>(S);
+            lx_acc_zero<T, 1, 1>(R);
+            for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
+                S[0][0] = lx_mfma<T>::mac(sL[16 * I + lr][16 * J + k0 + lk], sZ[16 * J + k0 + lk][16 * J + lr], S[0][0]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sL[16 * J + lx_mfma<T>::row(lk, i)][16 * I + lr] = S[0][0][i];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k0 = 0; k0 < 16; k0 += 4)
+                R[0][0] = lx_mfma<T>::mac(sZ[16 * I + lr][16 * I + k0 + lk], sL[16 * J + k0 + lk][16 * I + lr], R[0][0]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sZ[16 * I + lx_mfma<T>::row(lk, i)][16 * J + lr] = -R[0][0][i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Diagonal-tile Cholesky (reference task_dpotrf -> dpotrf_("L"), tasks_potrf.c
 // :20-24), blocked right-looking inside the tile with 64-wide panels.  For
 // each panel the 64 x 64 diagonal block D sits in LDS (rows past the tile are
@@ -256,10 +306,8 @@ __device__ __forceinline__ double lx_readlane(double v, int l)
 //      subtract the earlier columns (wave w owns rows 16w..16w+15), then
 //      wavefront 0 factors the 16 columns with one row per lane, held in
 //      registers (v_readlane broadcasts the pivots and the scaled column);
-//   2) Z = D^-1 by blocks: each wave inverts one 16 x 16 diagonal block, then
-//      Z_IJ = -Z_II sum_{K=J}^{I-1} L_IK Z_KJ on the matrix cores, level by
-//      level; Z goes to LDS and to the workspace W (the panel TRSM units of
-//      this step read it there);
+//   2) Z = D^-1 by blocks (lx_tri_inv64); Z goes to LDS and to the workspace
+//      W (the panel TRSM units of this step read it there);
 //   3) the rows below D become A_panel Z^T on the matrix cores (each wave owns
 //      16-row chunks, in place);
 //   4) the lower trailing part is updated with K = 64 on the matrix cores.
@@ -322,44 +370,8 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
             __syncthreads();
         }
         first_bad = s_bad[0];
-        // 2a) wave J inverts the 16 x 16 diagonal block J, lane = column (lanes 0..15 store)
-        {
-            const int b0 = 16 * w, c = lr;
-            T z[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                T s = r == c ? T(1) : T(0);
-#pragma unroll
-                for (int k = 0; k < r; ++k) s -= sD[b0 + r][b0 + k] * z[k];
-                z[r] = r >= c ? s / sD[b0 + r][b0 + r] : T(0);
-            }
-            if (lane < 16) {
-#pragma unroll
-                for (int r = 0; r < 16; ++r) sZ[b0 + r][b0 + c] = z[r];
-            }
-        }
-        __syncthreads();
-        // 2b) off-diagonal blocks, level I = 1..3; wave J < I forms Z_IJ.  The
-        // partial sum S goes through D's unused upper block (J, I).
-        for (int I = 1; I < 4; ++I) {
-            if (w < I) {  // wave-uniform
-                const int J = w;
-                typename lx_mfma<T>::acc S[1][1], R[1][1];
-                lx_acc_zero<T, 1, 1>(S);
-                lx_acc_zero<T, 1, 1>(R);
-                for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
-                    S[0][0] = lx_mfma<T>::mac(sD[16 * I + lr][16 * J + k0 + lk], sZ[16 * J + k0 + lk][16 * J + lr], S[0][0]);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) sD[16 * J + lx_mfma<T>::row(lk, i)][16 * I + lr] = S[0][0][i];
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int k0 = 0; k0 < 16; k0 += 4)
-                    R[0][0] = lx_mfma<T>::mac(sZ[16 * I + lr][16 * I + k0 + lk], sD[16 * J + k0 + lk][16 * I + lr], R[0][0]);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) sZ[16 * I + lx_mfma<T>::row(lk, i)][16 * J + lr] = -R[0][0][i];
-            }
-            __syncthreads();
-        }
+        // 2) Z = D^-1 (uses D's upper blocks as scratch)
+        lx_tri_inv64<T>(sD, sZ);
         // Z -> W (the full 64 x 64 block; W holds n + 64 rows, so p + 63 stays
         // inside), L_pp (lower) -> A
         for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {

@@ -28,7 +28,8 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
                   const linompss_exec_opts *opts, int *info_out);
 
 // Triangular sweeps with dedicated kernels (lx_sweep.hip): which = 1 forward
-// (L Y = X), 2 backward (L^T Z = Y), 3 both.  Enqueued on the stream.
+// (L Y = X), 2 backward (L^T Z = Y), 3 both.  Synchronises the stream; returns
+// LINOMPSS_ERR_SCHED if a wait inside a sweep timed out.
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
                   hipStream_t s);
 

@@ -1,26 +1,26 @@
 // lx_sweep.hip -- blocked triangular sweeps on the device: L Y = B and
-// L^T X = Y for a lower factor L, nrhs <= 16 right-hand sides.
+// L^T X = Y for a lower factor L, nrhs <= 16 right-hand sides per launch.
 //
 // Same arithmetic as the reference's bblas_dltrsm lower branches
 // (src/blocked_blas/bblas_trsm.h:119-160): for each row block, subtract the
 // products with the blocks already solved (task_dgemm), then solve with the
 // diagonal block (task_dtrsm).  The reference runs these as a chain of OmpSs
-// tasks over b-row blocks; on the GPU the sweep is bandwidth- and
-// latency-bound (L is read once, n^2/2 elements, and each block depends on
-// the previous one), so it runs as one launch with 64-row blocks:
-//   * a workgroup takes the next block by ticket (blocks are handed out in
-//     dependency order, so every wait is on a block already held by a running
-//     workgroup);
-//   * its 4 wavefronts stream the L blocks of its row (forward) or column
-//     (backward), each waiting on the ready flag of the block it consumes;
-//     the backward sweep transposes each 64 x 16 slab through LDS so that the
-//     global reads stay coalesced;
-//   * wavefront 0 solves the 64 x 64 diagonal block (lane = row, shuffles
-//     broadcast the solved entries), stores the block and publishes its flag.
-// All branches that steer a wavefront are wave-uniform.
+// tasks over b-row blocks.  On the GPU a sweep reads L once (n^2/2 elements,
+// HBM-bound) along a chain of dependent diagonal solves (latency-bound), so
+// one persistent launch splits it into 64-row steps:
+//   * the products with blocks solved two or more steps earlier are partial
+//     sums computed by many workgroups in parallel (C blocks per unit), each
+//     written to its own slot and added in a fixed order (deterministic);
+//   * the solve unit of a step computes the inverse of its 64 x 64 diagonal
+//     block and loads the block coupling it to the previous step before it
+//     waits, so the serial part of the chain is one 64 x 64 product with the
+//     previous step's x, the sum of the partials and one 64 x 64 product with
+//     the inverse.
+// All branches that steer a wavefront are wave-uniform; every wait is bounded.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "linompss.h"
 #include "lx_device.hpp"
@@ -30,192 +30,349 @@
 
 __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <typename T>
-__device__ __forceinline__ void lx_sw_wait(const int *flag)
+// Work units of one sweep over nb 64-row blocks, in "step" order s = 0..nb-1
+// (block s forward, block nb-1-s backward):
+//   S(s)     the solve unit of step s: b_s minus the partial sums of step s,
+//            minus the product with the block solved at step s-1, times the
+//            inverse of the diagonal block; publishes flag[s];
+//   P(s, c)  c >= 1: the partial sum of step s over the steps
+//            [max(0, s-1-cC), s-1-(c-1)C); publishes by counting pcnt[s].
+// Units are listed by the newest step they read (S(s): s-1, P(s, c):
+// s-2-(c-1)C), S first within a key.  Every wait is on a unit listed earlier,
+// so a persistent grid that takes units in list order cannot deadlock.
+// Unit code: (s << 16) | c.
+__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s >= 2 ? (s - 1 + C - 1) / C : 0; }
+static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase)
 {
-    while (lx_sw_uniform(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0)
-        __builtin_amdgcn_s_sleep(1);
+    units.clear();
+    rowbase.assign(nb + 1, 0);
+    for (int s = 0; s < nb; ++s) rowbase[s + 1] = rowbase[s] + lx_sw_nparts(s, C);
+    for (int key = -1; key <= nb - 2; ++key) {
+        if (key + 1 < nb) units.push_back((key + 1) << 16);
+        if (key < 0) continue;
+        for (int c = 1;; ++c) {
+            const long long st = (long long)key + 2 + (long long)(c - 1) * C;
+            if (st >= nb) break;
+            units.push_back((int)(st << 16) | c);
+        }
+    }
+}
+
+// Poll until *p >= need (wave-uniform; every lane of the calling wave must be
+// active).  Gives up after spin_ticks of the constant wall clock, then sets
+// *abort so that every other wait ends too.  Followed by an acquire fence.
+__device__ __forceinline__ void lx_sw_wait_ge(const int *p, int need, int *abort, long long spin_ticks)
+{
+    if (lx_sw_uniform(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+        const long long t0 = wall_clock64();
+        while (lx_sw_uniform(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+            if (lx_sw_uniform(__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+            if (wall_clock64() - t0 > spin_ticks) {
+                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X is
-// overwritten in place.  flags[nblk] and *ticket must be zero on entry.
-//
-// The 64 x 64 diagonal block is read into registers (16 values per thread,
-// coalesced) before the workgroup waits on anything, and moved to LDS for the
-// solve, so the serial part of the chain touches no global memory.
+// trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X (n x nrhs,
+// nrhs <= 16) is overwritten in place.  ctl = [ticket, abort, pad..., flags[nb],
+// pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64 values.
 template <typename T, int TRANS>
 __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T *__restrict__ L, int64_t ldl,
-                                                       T *__restrict__ X, int64_t ldx, int *flags, int *ticket)
+                                                       T *__restrict__ X, int64_t ldx, const int *__restrict__ units,
+                                                       int nunits, const int *__restrict__ rowbase, int C,
+                                                       T *__restrict__ part, int *ctl, long long spin_ticks)
 {
-    constexpr int KC = 128 / sizeof(T);  // backward: L rows per LDS slab (one 128-byte line per column)
-    constexpr int TILE = 4 * 64 * (KC + 1), DIAG = 64 * 65;
-    // per wave: the solved block it consumes, then (after its last block) its partial sums
-    __shared__ T sx[4][LX_SW_NR][64];
-    // backward transpose slabs during the accumulation, the diagonal block for the solve
-    __shared__ T sbuf[(TRANS ? TILE : 0) > DIAG ? (TRANS ? TILE : 0) : DIAG];
-    __shared__ int s_blk;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nblk = (n + 63) / 64;
-    if (w == 0) {
-        const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
-        s_blk = t;  // every lane of wave 0 stores the same value
-    }
-    __syncthreads();
-    const int t = lx_sw_uniform(s_blk);
-    const int ib = TRANS ? nblk - 1 - t : t;
-    const int r0 = ib * 64;
-    const int hb = min(64, n - r0);
-    const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb recompute the last row, never store
-
-    // diagonal block -> registers: thread (lane, w) holds column 4*q + w, row lane
-    T dreg[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int col = min(4 * q + w, hb - 1);
-        dreg[q] = L[(int64_t)(r0 + col) * ldl + row];
-    }
-
-    T acc[LX_SW_NR];
-#pragma unroll
-    for (int c = 0; c < LX_SW_NR; ++c) acc[c] = T(0);
-
-    // contributions of the blocks solved before this one, oldest first (forward:
-    // j = 0, 1, ...; backward: j = nblk-1, nblk-2, ...) so that each wave only
-    // ever waits for the newest block at the end of its list
-    const int nj = TRANS ? nblk - 1 - ib : ib;
-    for (int q = w; q < nj; q += 4) {
-        const int j = TRANS ? nblk - 1 - q : q;
-        lx_sw_wait<T>(&flags[j]);
-        const int c0 = j * 64, hj = min(64, n - c0);
-        // stage x_j (solved rows c0 .. c0+hj) in LDS
-        for (int c = 0; c < nrhs; ++c) sx[w][c][lane] = lane < hj ? X[(int64_t)c * ldx + c0 + lane] : T(0);
-        __builtin_amdgcn_wave_barrier();
-        if (!TRANS) {
-            // acc[r] += sum_k L[row, c0+k] x_j[k]  (column k of the block: 64 contiguous rows)
-            const T *Lb = L + (int64_t)c0 * ldl + row;
-            if (hj == 64) {
-                for (int k0 = 0; k0 < 64; k0 += 8) {
-                    T l[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) l[u] = Lb[(int64_t)(k0 + u) * ldl];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-#pragma unroll
-                        for (int c = 0; c < LX_SW_NR; ++c)
-                            if (c < nrhs) acc[c] += l[u] * sx[w][c][k0 + u];
-                }
-            } else {
-                for (int k = 0; k < hj; ++k) {
-                    const T l = Lb[(int64_t)k * ldl];
-#pragma unroll
-                    for (int c = 0; c < LX_SW_NR; ++c)
-                        if (c < nrhs) acc[c] += l * sx[w][c][k];
-                }
-            }
-        } else {
-            // acc[r] += sum_k L[c0+k, r0+r] x_j[k]: block (j, ib) read along its columns
-            // (one 128-byte line per column and instruction), transposed through LDS
-            T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(sbuf + w * 64 * (KC + 1));
-            constexpr int CPI = 64 / KC;  // columns per load instruction
-            for (int k0 = 0; k0 < hj; k0 += KC) {
-                const int kk = lane % KC, cq = lane / KC;
-                T v[KC];  // KC >= 16: 64 / CPI loads in flight
-#pragma unroll
-                for (int q = 0; q < 64 / CPI; ++q) {
-                    const int cc = q * CPI + cq;
-                    const int colg = r0 + min(cc, hb - 1), rowg = c0 + min(k0 + kk, hj - 1);
-                    v[q] = L[(int64_t)colg * ldl + rowg];
-                }
-#pragma unroll
-                for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = v[q];
-                __builtin_amdgcn_wave_barrier();
-                const int kend = min(KC, hj - k0);
-                for (int k = 0; k < kend; ++k) {
-                    const T l = stile[lane][k];
-#pragma unroll
-                    for (int c = 0; c < LX_SW_NR; ++c)
-                        if (c < nrhs) acc[c] += l * sx[w][c][k0 + k];
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
+    constexpr int KC = 128 / sizeof(T);  // backward partial sums: L rows per LDS slab
+    constexpr int S_BYTES = 2 * 64 * 65 * (int)sizeof(T);
+    constexpr int P_BYTES = (4 * 64 * (KC + 1) + 4 * LX_SW_NR * 64) * (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) char smem[S_BYTES > P_BYTES ? S_BYTES : P_BYTES];
+    __shared__ T svec[2][64][LX_SW_NR + 1];  // S unit: [0] = b, then y; [1] = previous block's x
+    __shared__ int s_unit;
+    const int nb = (n + 63) / 64;
+    int *ticket = ctl, *abort = ctl + 1, *flags = ctl + 64, *pcnt = ctl + 64 + nb;
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    for (;;) {
+        if (w == 0) {
+            const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
+            s_unit = t < nunits ? units[t] : -1;  // every lane stores the same value
         }
-        __builtin_amdgcn_wave_barrier();
-    }
-    // sx[w] is this wave's own buffer: safe to reuse for its partial sums
+        __syncthreads();
+        const int code = lx_sw_uniform(s_unit);
+        __syncthreads();
+        if (code < 0) break;
+        const int s = code >> 16, c = code & 0xffff;
+        const int ib = TRANS ? nb - 1 - s : s;
+        const int r0 = ib * 64, hb = min(64, n - r0);
+        if (c == 0) {
+            // ---------------- S(s) ----------------
+            T(*sA)[65] = reinterpret_cast<T(*)[65]>(smem);
+            T(*sZ)[65] = reinterpret_cast<T(*)[65]>(smem + 64 * 65 * sizeof(T));
+            // diagonal block (identity pad) and b: nothing here waits
+            for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+                const int r = e % 64, k = e / 64;
+                sA[r][k] = (r < hb && k < hb) ? L[(int64_t)(r0 + k) * ldl + r0 + r] : (r == k ? T(1) : T(0));
+                sZ[r][k] = T(0);
+            }
+            for (int e = tid; e < 64 * LX_SW_NR; e += LX_WG_THREADS) {
+                const int r = e % 64, cc = e / 64;
+                svec[0][r][cc] = (r < hb && cc < nrhs) ? X[(int64_t)cc * ldx + r0 + r] : T(0);
+            }
+            __syncthreads();
+            lx_tri_inv64<T>(sA, sZ);  // ends with a barrier
+            // the block that couples this step to the previous one
+            const int ip = TRANS ? ib + 1 : ib - 1;
+            const int hp = s > 0 ? min(64, n - ip * 64) : 0;
+            if (s > 0) {
+                for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+                    const int i0 = e % 64, i1 = e / 64;  // i0 runs along a column of L: coalesced
+                    if (!TRANS)  // sA[r][k] = L[r0 + r][ip*64 + k]
+                        sA[i0][i1] = i0 < hb ? L[(int64_t)(ip * 64 + i1) * ldl + r0 + i0] : T(0);
+                    else  // sA[r][k] = L[ip*64 + k][r0 + r]
+                        sA[i1][i0] = i0 < hp ? L[(int64_t)(r0 + i1) * ldl + ip * 64 + i0] : T(0);
+                }
+            }
+            // b minus the partial sums (in part order): they only need steps <= s-2,
+            // so they are subtracted before the wait on step s-1.  The host picks
+            // C so that a step has at most 8 parts; lane = row, wave w owns rhs
+            // w, w+4, ...; all loads of a column are issued together.
+            const int need = lx_sw_nparts(s, C);
+            if (need > 0) {
+                if (w == 0) lx_sw_wait_ge(&pcnt[s], need, abort, spin_ticks);  // wave-uniform
+                __syncthreads();
+                const T *pp = part + (size_t)rowbase[s] * LX_SW_NR * 64;
+                for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
+                    T y = svec[0][lane][cc];
+                    for (int q0 = 0; q0 < need; q0 += 8) {
+                        T v[8];
 #pragma unroll
-    for (int c = 0; c < LX_SW_NR; ++c) sx[w][c][lane] = acc[c];
-    __syncthreads();  // all transpose slabs consumed: sbuf now takes the diagonal block
-    T(*sdiag)[65] = reinterpret_cast<T(*)[65]>(sbuf);  // sdiag[r][c] = L[r0+r, r0+c]
+                        for (int q = 0; q < 8; ++q)
+                            v[q] = q0 + q < need ? pp[((size_t)(q0 + q) * LX_SW_NR + cc) * 64 + lane] : T(0);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) sdiag[lane][4 * q + w] = dreg[q];
-    __syncthreads();
-    if (w == 0) {
-        // right-hand side of this block minus the contributions
-        T s[LX_SW_NR];
+                        for (int q = 0; q < 8; ++q) y -= v[q];
+                    }
+                    svec[0][lane][cc] = y;
+                }
+            }
+            if (s > 0) {
+                if (w == 0) lx_sw_wait_ge(&flags[s - 1], 1, abort, spin_ticks);  // wave-uniform
+                __syncthreads();
+                for (int e = tid; e < 64 * LX_SW_NR; e += LX_WG_THREADS) {
+                    const int k = e % 64, cc = e / 64;
+                    svec[1][k][cc] = (k < hp && cc < nrhs) ? X[(int64_t)cc * ldx + ip * 64 + k] : T(0);
+                }
+            }
+            __syncthreads();
+            // y = (b - parts) - A_crit x_prev; x = Z y (forward) or Z^T y (backward);
+            // lane = row, wave w owns rhs w, w+4, ...
+            for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
+                T y = svec[0][lane][cc];
+                if (s > 0) {
+                    T a0 = T(0), a1 = T(0);
+#pragma unroll 8
+                    for (int k = 0; k < 64; k += 2) {
+                        a0 += sA[lane][k] * svec[1][k][cc];
+                        a1 += sA[lane][k + 1] * svec[1][k + 1][cc];
+                    }
+                    y -= a0 + a1;
+                }
+                svec[0][lane][cc] = y;  // column cc is this wave's own
+                __builtin_amdgcn_wave_barrier();
+                T x0 = T(0), x1 = T(0);
+#pragma unroll 8
+                for (int k = 0; k < 64; k += 2) {
+                    x0 += (TRANS ? sZ[k][lane] : sZ[lane][k]) * svec[0][k][cc];
+                    x1 += (TRANS ? sZ[k + 1][lane] : sZ[lane][k + 1]) * svec[0][k + 1][cc];
+                }
+                if (lane < hb) X[(int64_t)cc * ldx + r0 + lane] = x0 + x1;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (w == 0) __hip_atomic_store(&flags[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            // ---------------- P(s, c) ----------------
+            T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + w * 64 * (KC + 1) * sizeof(T));
+            T(*sx)[64] = reinterpret_cast<T(*)[64]>(smem + (4 * 64 * (KC + 1) + w * LX_SW_NR * 64) * sizeof(T));
+            const int t_hi = s - 1 - (c - 1) * C, t_lo = max(0, t_hi - C);
+            const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb never store
+            T acc[LX_SW_NR];
 #pragma unroll
-        for (int c = 0; c < LX_SW_NR; ++c)
-            s[c] = c < nrhs ? X[(int64_t)c * ldx + row] -
-                                  ((sx[0][c][lane] + sx[1][c][lane]) + (sx[2][c][lane] + sx[3][c][lane]))
-                            : T(0);
-        const int lr = min(lane, hb - 1);
-        // diagonal block solve, lane = row
-        if (!TRANS) {
-            for (int jj = 0; jj < hb; ++jj) {
-                const T piv = sdiag[jj][jj];
-                const T ljj = sdiag[lr][jj];  // L[row, r0+jj]
+            for (int cc = 0; cc < LX_SW_NR; ++cc) acc[cc] = T(0);
+            for (int t = t_lo + w; t < t_hi; t += 4) {  // wave-uniform
+                const int jb = TRANS ? nb - 1 - t : t;
+                const int c0 = jb * 64, hj = min(64, n - c0);
+                lx_sw_wait_ge(&flags[t], 1, abort, spin_ticks);
+                for (int cc = 0; cc < nrhs; ++cc) sx[cc][lane] = lane < hj ? X[(int64_t)cc * ldx + c0 + lane] : T(0);
+                __builtin_amdgcn_wave_barrier();
+                if (!TRANS) {
+                    // acc += L[row, c0 + k] x_j[k]: column k of the block is 64 contiguous rows
+                    const T *Lb = L + (int64_t)c0 * ldl + row;
+                    for (int k0 = 0; k0 < 64; k0 += 8) {  // forward: block jb < ib is full
+                        T l[8];
 #pragma unroll
-                for (int c = 0; c < LX_SW_NR; ++c) {
-                    if (c < nrhs) {
-                        const T xj = __shfl(s[c], jj) / piv;
-                        s[c] = lane == jj ? xj : (lane > jj ? s[c] - ljj * xj : s[c]);
+                        for (int u = 0; u < 8; ++u) l[u] = Lb[(int64_t)(k0 + u) * ldl];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+#pragma unroll
+                            for (int cc = 0; cc < LX_SW_NR; ++cc)
+                                if (cc < nrhs) acc[cc] += l[u] * sx[cc][k0 + u];
+                    }
+                } else {
+                    // acc += L[c0 + k, row] x_j[k]: block (jb, ib) read along its columns
+                    // (one 128-byte line per column and instruction), transposed through LDS
+                    constexpr int CPI = 64 / KC;  // columns per load instruction
+                    for (int k0 = 0; k0 < hj; k0 += KC) {
+                        const int kk = lane % KC, cq = lane / KC;
+                        T v[64 / CPI];
+#pragma unroll
+                        for (int q = 0; q < 64 / CPI; ++q) {
+                            const int col = q * CPI + cq;
+                            const int colg = r0 + min(col, hb - 1), rowg = c0 + min(k0 + kk, hj - 1);
+                            v[q] = L[(int64_t)colg * ldl + rowg];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = v[q];
+                        __builtin_amdgcn_wave_barrier();
+                        const int kend = min(KC, hj - k0);
+                        for (int k = 0; k < kend; ++k) {
+                            const T l = stile[lane][k];
+#pragma unroll
+                            for (int cc = 0; cc < LX_SW_NR; ++cc)
+                                if (cc < nrhs) acc[cc] += l * sx[cc][k0 + k];
+                        }
+                        __builtin_amdgcn_wave_barrier();
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
             }
-        } else {
-            for (int jj = hb - 1; jj >= 0; --jj) {
-                const T piv = sdiag[jj][jj];
-                const T ljj = sdiag[jj][lr];  // L[r0+jj, row] = (L^T)[row, r0+jj]
+            // reduce the four waves in a fixed order through sx (each wave's own buffer)
 #pragma unroll
-                for (int c = 0; c < LX_SW_NR; ++c) {
-                    if (c < nrhs) {
-                        const T xj = __shfl(s[c], jj) / piv;
-                        s[c] = lane == jj ? xj : (lane < jj ? s[c] - ljj * xj : s[c]);
-                    }
-                }
+            for (int cc = 0; cc < LX_SW_NR; ++cc) sx[cc][lane] = acc[cc];
+            __syncthreads();
+            T(*sx0)[LX_SW_NR][64] = reinterpret_cast<T(*)[LX_SW_NR][64]>(smem + 4 * 64 * (KC + 1) * sizeof(T));
+            T *pp = part + ((size_t)rowbase[s] + (c - 1)) * LX_SW_NR * 64;
+            for (int cc = w; cc < nrhs; cc += 4) {
+                const T v = (sx0[0][cc][lane] + sx0[1][cc][lane]) + (sx0[2][cc][lane] + sx0[3][cc][lane]);
+                pp[(size_t)cc * 64 + lane] = lane < hb ? v : T(0);
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (w == 0) atomicAdd(&pcnt[s], lane == 0 ? 1 : 0);
         }
-        for (int c = 0; c < nrhs; ++c)
-            if (lane < hb) X[(int64_t)c * ldx + r0 + lane] = s[c];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&flags[ib], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 #ifndef LX_SWEEP_KERNELS_ONLY
+#include <map>
+#include <mutex>
+
+namespace {
+
+// 64-row blocks per partial-sum unit: at least 8, and at most 8 parts per step
+int lx_sw_chunk(int nb) { return std::max(8, (nb - 1 + 7) / 8); }
+
+struct lx_sw_plan {
+    int *d_units = nullptr, *d_rowbase = nullptr;
+    int nunits = 0, nparts = 0;
+};
+struct lx_sw_dev {
+    int cus = 0, wall_khz = 0, occ[2][2] = {};
+    std::map<int, lx_sw_plan> plans;  // by number of blocks
+};
+std::mutex lx_sw_mu;
+std::map<int, lx_sw_dev> lx_sw_devs;
+
+template <typename T, int TR>
+int lx_sw_occ()
+{
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lx_sweep_kernel<T, TR>, 256, 0) != hipSuccess || o < 1) o = 1;
+    return o;
+}
+
+// Device properties, occupancy and the unit list for nb blocks (built once).
+lx_sw_dev *lx_sw_get(int nb, const lx_sw_plan **plan)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(lx_sw_mu);
+    lx_sw_dev &d = lx_sw_devs[dev];
+    if (d.cus == 0) {
+        if (hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return nullptr;
+        if (hipDeviceGetAttribute(&d.wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || d.wall_khz <= 0)
+            d.wall_khz = 100000;
+        d.occ[0][0] = lx_sw_occ<double, 0>();
+        d.occ[0][1] = lx_sw_occ<double, 1>();
+        d.occ[1][0] = lx_sw_occ<float, 0>();
+        d.occ[1][1] = lx_sw_occ<float, 1>();
+    }
+    auto it = d.plans.find(nb);
+    if (it == d.plans.end()) {
+        std::vector<int> units, rowbase;
+        lx_sw_build_units(nb, lx_sw_chunk(nb), units, rowbase);
+        lx_sw_plan pl;
+        pl.nunits = (int)units.size();
+        pl.nparts = rowbase[nb];
+        if (hipMalloc(&pl.d_units, sizeof(int) * units.size()) != hipSuccess) return nullptr;
+        if (hipMalloc(&pl.d_rowbase, sizeof(int) * rowbase.size()) != hipSuccess) {
+            (void)hipFree(pl.d_units);
+            return nullptr;
+        }
+        if (hipMemcpy(pl.d_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(pl.d_rowbase, rowbase.data(), sizeof(int) * rowbase.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return nullptr;
+        it = d.plans.emplace(nb, pl).first;
+    }
+    *plan = &it->second;
+    return &d;
+}
+}  // namespace
+
 template <typename T>
 static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, int which, hipStream_t s)
 {
-    const int nblk = (n + 63) / 64;
-    int *ctl = (int *)lx_scratch(3, sizeof(int) * (size_t)(2 * nblk + 64));
-    if (!ctl) return LINOMPSS_ERR_NOMEM;
+    const int nb = (n + 63) / 64;
+    const lx_sw_plan *pl = nullptr;
+    lx_sw_dev *d = lx_sw_get(nb, &pl);
+    if (!d) return LINOMPSS_ERR_HIP;
+    const size_t ctl_ints = 64 + 2 * (size_t)nb;
+    const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
+    char *ws = (char *)lx_scratch(3, part_off + sizeof(T) * (size_t)std::max(1, pl->nparts) * LX_SW_NR * 64);
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    int *ctl = (int *)ws;
+    T *part = (T *)(ws + part_off);
+    double tmo = 30.0;
+    if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
+    const long long ticks = (long long)(tmo * d->wall_khz * 1000.0);
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    int aborted = 0;
     for (int c0 = 0; c0 < nrhs; c0 += LX_SW_NR) {
         const int nc = std::min(LX_SW_NR, nrhs - c0);
-        if (which & 1) {
-            lx_fill_i32(ctl, 0, nblk + 64, s);
-            hipLaunchKernelGGL((lx_sweep_kernel<T, 0>), dim3(nblk), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,
-                               ctl + 64, ctl);
-        }
-        if (which & 2) {
-            lx_fill_i32(ctl, 0, nblk + 64, s);
-            hipLaunchKernelGGL((lx_sweep_kernel<T, 1>), dim3(nblk), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,
-                               ctl + 64, ctl);
+        for (int tr = 0; tr < 2; ++tr) {
+            if (!(which & (1 << tr))) continue;
+            const int grid = std::max(1, std::min(pl->nunits, d->cus * d->occ[ti][tr]));
+            lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
+            if (tr == 0)
+                hipLaunchKernelGGL((lx_sweep_kernel<T, 0>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx,
+                                   ldx, pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks);
+            else
+                hipLaunchKernelGGL((lx_sweep_kernel<T, 1>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx,
+                                   ldx, pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks);
+            if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
+            int ab = 0;
+            LX_HIP_TRY(hipMemcpyAsync(&ab, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            LX_HIP_TRY(hipStreamSynchronize(s));
+            aborted |= ab;
         }
     }
-    return hipGetLastError() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+    return aborted ? LINOMPSS_ERR_SCHED : 0;
 }
 
-// which: 1 forward, 2 backward, 3 both.  Enqueued on the stream (no sync).
+// which: 1 forward, 2 backward, 3 both.  Returns after the sweeps finished
+// (each launch is checked for a timed-out wait).
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
                   hipStream_t s)
 {

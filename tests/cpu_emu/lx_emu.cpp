@@ -99,25 +99,32 @@ static void run_wg(F &&body)
 extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,
                             long long ldx)
 {
-    const int nblk = (n + 63) / 64;
-    std::vector<int> flags(nblk, 0);
-    int ticket = 0;
-    for (int b = 0; b < nblk; ++b) {
-        run_wg([&] {
-            if (dtype == 0) {
-                if (trans)
-                    lx_sweep_kernel<double, 1>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, flags.data(), &ticket);
-                else
-                    lx_sweep_kernel<double, 0>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, flags.data(), &ticket);
-            } else {
-                if (trans)
-                    lx_sweep_kernel<float, 1>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, flags.data(), &ticket);
-                else
-                    lx_sweep_kernel<float, 0>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, flags.data(), &ticket);
-            }
-        });
-    }
-    for (int b = 0; b < nblk; ++b)
-        if (flags[b] != 1) return -1;
-    return ticket == nblk ? 0 : -2;
+    // one emulated workgroup takes every unit in list order
+    const int nb = (n + 63) / 64, C = 3;  // small chunks: several partial-sum units per step
+    std::vector<int> units, rowbase;
+    lx_sw_build_units(nb, C, units, rowbase);
+    std::vector<int> ctl(64 + 2 * nb, 0);
+    std::vector<double> part((size_t)std::max(1, rowbase[nb]) * LX_SW_NR * 64);  // large enough for either type
+    const long long ticks = 100000000LL * 20;
+    run_wg([&] {
+        if (dtype == 0) {
+            if (trans)
+                lx_sweep_kernel<double, 1>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, units.data(),
+                                           (int)units.size(), rowbase.data(), C, part.data(), ctl.data(), ticks);
+            else
+                lx_sweep_kernel<double, 0>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, units.data(),
+                                           (int)units.size(), rowbase.data(), C, part.data(), ctl.data(), ticks);
+        } else {
+            if (trans)
+                lx_sweep_kernel<float, 1>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, units.data(),
+                                          (int)units.size(), rowbase.data(), C, (float *)part.data(), ctl.data(), ticks);
+            else
+                lx_sweep_kernel<float, 0>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, units.data(),
+                                          (int)units.size(), rowbase.data(), C, (float *)part.data(), ctl.data(), ticks);
+        }
+    });
+    if (ctl[1] != 0) return -3;  // a wait timed out
+    for (int b = 0; b < nb; ++b)
+        if (ctl[64 + b] != 1 || ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;
+    return ctl[0] == (int)units.size() + 1 ? 0 : -2;
 }
