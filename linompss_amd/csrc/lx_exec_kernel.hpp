@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
                         if (wall_clock64() - t0 > spin_ticks) {
                             // every lane stores the same values (no lane branch)
                             atomicAdd(&st->timeouts, lane == 0 ? 1 : 0);
-                            __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            atomicOr(&st->abort, lane == 0 ? 1 : 0);  // read-modify-write: seen at once
                             go = 0;
                             break;
                         }

@@ -33,25 +33,27 @@ __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_re
 // Work units of one sweep over nb 64-row blocks, in "step" order s = 0..nb-1
 // (block s forward, block nb-1-s backward):
 //   S(s)     the solve unit of step s: b_s minus the partial sums of step s,
-//            minus the product with the block solved at step s-1, times the
+//            minus the products with the blocks solved at steps s-2 and s-1
+//            (each subtracted as soon as that step is published), times the
 //            inverse of the diagonal block; publishes flag[s];
 //   P(s, c)  c >= 1: the partial sum of step s over the steps
-//            [max(0, s-1-cC), s-1-(c-1)C); publishes by counting pcnt[s].
-// Units are listed by the newest step they read (S(s): s-1, P(s, c):
-// s-2-(c-1)C), S first within a key.  Every wait is on a unit listed earlier,
-// so a persistent grid that takes units in list order cannot deadlock.
-// Unit code: (s << 16) | c.
-__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s >= 2 ? (s - 1 + C - 1) / C : 0; }
+//            [max(0, s-2-cC), s-2-(c-1)C); publishes by counting pcnt[s].
+// A partial sum never reads the two newest steps, so its units run a step
+// ahead of the chain of solve units.  Units are listed by the newest step they
+// read (S(s): s-1, P(s, c): s-3-(c-1)C), S first within a key.  Every wait is
+// on a unit listed earlier, so a persistent grid that takes units in list
+// order cannot deadlock.  Unit code: (s << 16) | c.
+__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s >= 3 ? (s - 2 + C - 1) / C : 0; }
 static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase)
 {
     units.clear();
     rowbase.assign(nb + 1, 0);
     for (int s = 0; s < nb; ++s) rowbase[s + 1] = rowbase[s] + lx_sw_nparts(s, C);
     for (int key = -1; key <= nb - 2; ++key) {
-        if (key + 1 < nb) units.push_back((key + 1) << 16);
+        units.push_back((key + 1) << 16);
         if (key < 0) continue;
         for (int c = 1;; ++c) {
-            const long long st = (long long)key + 2 + (long long)(c - 1) * C;
+            const long long st = (long long)key + 3 + (long long)(c - 1) * C;
             if (st >= nb) break;
             units.push_back((int)(st << 16) | c);
         }
@@ -68,7 +70,7 @@ __device__ __forceinline__ void lx_sw_wait_ge(const int *p, int need, int *abort
         while (lx_sw_uniform(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
             if (lx_sw_uniform(__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
             if (wall_clock64() - t0 > spin_ticks) {
-                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicOr(abort, (threadIdx.x & 63) == 0 ? 1 : 0);  // read-modify-write: seen at once
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -78,19 +80,20 @@ __device__ __forceinline__ void lx_sw_wait_ge(const int *p, int need, int *abort
 }
 
 // trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X (n x nrhs,
-// nrhs <= 16) is overwritten in place.  ctl = [ticket, abort, pad..., flags[nb],
-// pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64 values.
-template <typename T, int TRANS>
+// nrhs <= NR <= 16) is overwritten in place.  ctl = [ticket, abort, pad...,
+// flags[nb], pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64
+// values.
+template <typename T, int TRANS, int NR>
 __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T *__restrict__ L, int64_t ldl,
                                                        T *__restrict__ X, int64_t ldx, const int *__restrict__ units,
                                                        int nunits, const int *__restrict__ rowbase, int C,
                                                        T *__restrict__ part, int *ctl, long long spin_ticks)
 {
     constexpr int KC = 128 / sizeof(T);  // backward partial sums: L rows per LDS slab
-    constexpr int S_BYTES = 2 * 64 * 65 * (int)sizeof(T);
-    constexpr int P_BYTES = (4 * 64 * (KC + 1) + 4 * LX_SW_NR * 64) * (int)sizeof(T);
+    constexpr int S_BYTES = 3 * 64 * 65 * (int)sizeof(T);
+    constexpr int P_BYTES = (4 * 64 * (KC + 1) + 4 * NR * 64) * (int)sizeof(T);
     __shared__ __attribute__((aligned(16))) char smem[S_BYTES > P_BYTES ? S_BYTES : P_BYTES];
-    __shared__ T svec[2][64][LX_SW_NR + 1];  // S unit: [0] = b, then y; [1] = previous block's x
+    __shared__ T svec[2][64][NR + 1];  // S unit: [0] = b, then y; [1] = a solved block's x
     __shared__ int s_unit;
     const int nb = (n + 63) / 64;
     int *ticket = ctl, *abort = ctl + 1, *flags = ctl + 64, *pcnt = ctl + 64 + nb;
@@ -109,36 +112,41 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
         const int r0 = ib * 64, hb = min(64, n - r0);
         if (c == 0) {
             // ---------------- S(s) ----------------
+            // sA: the diagonal block, then the block coupling step s-2;
+            // sZ: its inverse; sC: the block coupling step s-1
             T(*sA)[65] = reinterpret_cast<T(*)[65]>(smem);
             T(*sZ)[65] = reinterpret_cast<T(*)[65]>(smem + 64 * 65 * sizeof(T));
-            // diagonal block (identity pad) and b: nothing here waits
+            T(*sC)[65] = reinterpret_cast<T(*)[65]>(smem + 2 * 64 * 65 * sizeof(T));
+            // coupling block with step s-d into sB: sB[r][k] = L[r0 + r][ip*64 + k]
+            // (forward) or L[ip*64 + k][r0 + r] (backward); reads run down columns
+            auto load_coupling = [&](T(*sB)[65], int d) {
+                const int ip = TRANS ? ib + d : ib - d;
+                const int hp = min(64, n - ip * 64);
+                for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+                    const int i0 = e % 64, i1 = e / 64;
+                    if (!TRANS)
+                        sB[i0][i1] = i0 < hb ? L[(int64_t)(ip * 64 + i1) * ldl + r0 + i0] : T(0);
+                    else
+                        sB[i1][i0] = i0 < hp ? L[(int64_t)(r0 + i1) * ldl + ip * 64 + i0] : T(0);
+                }
+            };
+            // nothing before the waits depends on another unit
             for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
                 const int r = e % 64, k = e / 64;
                 sA[r][k] = (r < hb && k < hb) ? L[(int64_t)(r0 + k) * ldl + r0 + r] : (r == k ? T(1) : T(0));
                 sZ[r][k] = T(0);
             }
-            for (int e = tid; e < 64 * LX_SW_NR; e += LX_WG_THREADS) {
+            for (int e = tid; e < 64 * NR; e += LX_WG_THREADS) {
                 const int r = e % 64, cc = e / 64;
                 svec[0][r][cc] = (r < hb && cc < nrhs) ? X[(int64_t)cc * ldx + r0 + r] : T(0);
             }
+            if (s >= 1) load_coupling(sC, 1);
             __syncthreads();
             lx_tri_inv64<T>(sA, sZ);  // ends with a barrier
-            // the block that couples this step to the previous one
-            const int ip = TRANS ? ib + 1 : ib - 1;
-            const int hp = s > 0 ? min(64, n - ip * 64) : 0;
-            if (s > 0) {
-                for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
-                    const int i0 = e % 64, i1 = e / 64;  // i0 runs along a column of L: coalesced
-                    if (!TRANS)  // sA[r][k] = L[r0 + r][ip*64 + k]
-                        sA[i0][i1] = i0 < hb ? L[(int64_t)(ip * 64 + i1) * ldl + r0 + i0] : T(0);
-                    else  // sA[r][k] = L[ip*64 + k][r0 + r]
-                        sA[i1][i0] = i0 < hp ? L[(int64_t)(r0 + i1) * ldl + ip * 64 + i0] : T(0);
-                }
-            }
-            // b minus the partial sums (in part order): they only need steps <= s-2,
-            // so they are subtracted before the wait on step s-1.  The host picks
-            // C so that a step has at most 8 parts; lane = row, wave w owns rhs
-            // w, w+4, ...; all loads of a column are issued together.
+            if (s >= 2) load_coupling(sA, 2);
+            // b minus the partial sums (in part order): they only read steps <= s-3.
+            // The host picks C so that a step has at most 8 parts; lane = row,
+            // wave w owns rhs w, w+4, ...; all loads of a column are issued together.
             const int need = lx_sw_nparts(s, C);
             if (need > 0) {
                 if (w == 0) lx_sw_wait_ge(&pcnt[s], need, abort, spin_ticks);  // wave-uniform
@@ -157,30 +165,33 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                     svec[0][lane][cc] = y;
                 }
             }
-            if (s > 0) {
-                if (w == 0) lx_sw_wait_ge(&flags[s - 1], 1, abort, spin_ticks);  // wave-uniform
-                __syncthreads();
-                for (int e = tid; e < 64 * LX_SW_NR; e += LX_WG_THREADS) {
+            // the products with the blocks solved at steps s-2, then s-1, each
+            // subtracted as soon as its step is published
+            for (int d = 2; d >= 1; --d) {
+                if (s < d) continue;  // uniform
+                const int ip = TRANS ? ib + d : ib - d;
+                const int hp = min(64, n - ip * 64);
+                T(*sB)[65] = d == 2 ? sA : sC;
+                if (w == 0) lx_sw_wait_ge(&flags[s - d], 1, abort, spin_ticks);  // wave-uniform
+                __syncthreads();  // also: svec[1] is free, sB is loaded
+                for (int e = tid; e < 64 * NR; e += LX_WG_THREADS) {
                     const int k = e % 64, cc = e / 64;
                     svec[1][k][cc] = (k < hp && cc < nrhs) ? X[(int64_t)cc * ldx + ip * 64 + k] : T(0);
                 }
-            }
-            __syncthreads();
-            // y = (b - parts) - A_crit x_prev; x = Z y (forward) or Z^T y (backward);
-            // lane = row, wave w owns rhs w, w+4, ...
-            for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
-                T y = svec[0][lane][cc];
-                if (s > 0) {
+                __syncthreads();
+                for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
                     T a0 = T(0), a1 = T(0);
 #pragma unroll 8
                     for (int k = 0; k < 64; k += 2) {
-                        a0 += sA[lane][k] * svec[1][k][cc];
-                        a1 += sA[lane][k + 1] * svec[1][k + 1][cc];
+                        a0 += sB[lane][k] * svec[1][k][cc];
+                        a1 += sB[lane][k + 1] * svec[1][k + 1][cc];
                     }
-                    y -= a0 + a1;
+                    svec[0][lane][cc] -= a0 + a1;
                 }
-                svec[0][lane][cc] = y;  // column cc is this wave's own
-                __builtin_amdgcn_wave_barrier();
+            }
+            // x = Z y (forward) or Z^T y (backward); lane = row, wave w owns rhs w, w+4, ...
+            __builtin_amdgcn_wave_barrier();
+            for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform; column cc is this wave's own
                 T x0 = T(0), x1 = T(0);
 #pragma unroll 8
                 for (int k = 0; k < 64; k += 2) {
@@ -191,57 +202,68 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
-            if (w == 0) __hip_atomic_store(&flags[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // published with a read-modify-write: it is performed at the shared level
+            // at once, where a plain atomic store can sit in this XCD's L2 for
+            // microseconds before other XCDs see it
+            if (w == 0) atomicAdd(&flags[s], lane == 0 ? 1 : 0);
         } else {
             // ---------------- P(s, c) ----------------
             T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + w * 64 * (KC + 1) * sizeof(T));
-            T(*sx)[64] = reinterpret_cast<T(*)[64]>(smem + (4 * 64 * (KC + 1) + w * LX_SW_NR * 64) * sizeof(T));
-            const int t_hi = s - 1 - (c - 1) * C, t_lo = max(0, t_hi - C);
+            T(*sx)[64] = reinterpret_cast<T(*)[64]>(smem + (4 * 64 * (KC + 1) + w * NR * 64) * sizeof(T));
+            const int t_hi = s - 2 - (c - 1) * C, t_lo = max(0, t_hi - C);
             const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb never store
-            T acc[LX_SW_NR];
+            T acc[NR];
 #pragma unroll
-            for (int cc = 0; cc < LX_SW_NR; ++cc) acc[cc] = T(0);
+            for (int cc = 0; cc < NR; ++cc) acc[cc] = T(0);
             for (int t = t_lo + w; t < t_hi; t += 4) {  // wave-uniform
                 const int jb = TRANS ? nb - 1 - t : t;
                 const int c0 = jb * 64, hj = min(64, n - c0);
+                // the block of L is read before the wait: only x_j depends on step t
+                T lv[64];
+                if (!TRANS) {
+                    // lv[k] = L[row, c0 + k]: column k of the block is 64 contiguous rows
+                    // (forward: block jb < ib is full)
+                    const T *Lb = L + (int64_t)c0 * ldl + row;
+#pragma unroll
+                    for (int k = 0; k < 64; ++k) lv[k] = Lb[(int64_t)k * ldl];
+                } else {
+                    // slab k0 = KC*m: lv[m*(64/CPI) + q] = L[c0 + k0 + kk, r0 + q*CPI + cq], one
+                    // 128-byte line per column and instruction
+                    constexpr int CPI = 64 / KC;
+                    const int kk = lane % KC, cq = lane / KC;
+#pragma unroll
+                    for (int m = 0; m < 64 / KC; ++m)
+#pragma unroll
+                        for (int q = 0; q < 64 / CPI; ++q) {
+                            const int colg = r0 + min(q * CPI + cq, hb - 1), rowg = c0 + min(m * KC + kk, hj - 1);
+                            lv[m * (64 / CPI) + q] = L[(int64_t)colg * ldl + rowg];
+                        }
+                }
                 lx_sw_wait_ge(&flags[t], 1, abort, spin_ticks);
                 for (int cc = 0; cc < nrhs; ++cc) sx[cc][lane] = lane < hj ? X[(int64_t)cc * ldx + c0 + lane] : T(0);
                 __builtin_amdgcn_wave_barrier();
                 if (!TRANS) {
-                    // acc += L[row, c0 + k] x_j[k]: column k of the block is 64 contiguous rows
-                    const T *Lb = L + (int64_t)c0 * ldl + row;
-                    for (int k0 = 0; k0 < 64; k0 += 8) {  // forward: block jb < ib is full
-                        T l[8];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) l[u] = Lb[(int64_t)(k0 + u) * ldl];
+                    for (int k = 0; k < 64; ++k)
 #pragma unroll
-                        for (int u = 0; u < 8; ++u)
-#pragma unroll
-                            for (int cc = 0; cc < LX_SW_NR; ++cc)
-                                if (cc < nrhs) acc[cc] += l[u] * sx[cc][k0 + u];
-                    }
+                        for (int cc = 0; cc < NR; ++cc)
+                            if (cc < nrhs) acc[cc] += lv[k] * sx[cc][k];
                 } else {
-                    // acc += L[c0 + k, row] x_j[k]: block (jb, ib) read along its columns
-                    // (one 128-byte line per column and instruction), transposed through LDS
-                    constexpr int CPI = 64 / KC;  // columns per load instruction
-                    for (int k0 = 0; k0 < hj; k0 += KC) {
-                        const int kk = lane % KC, cq = lane / KC;
-                        T v[64 / CPI];
+                    // transpose each slab through LDS: stile[col][kk]
+                    constexpr int CPI = 64 / KC;
+                    const int kk = lane % KC, cq = lane / KC;
 #pragma unroll
-                        for (int q = 0; q < 64 / CPI; ++q) {
-                            const int col = q * CPI + cq;
-                            const int colg = r0 + min(col, hb - 1), rowg = c0 + min(k0 + kk, hj - 1);
-                            v[q] = L[(int64_t)colg * ldl + rowg];
-                        }
+                    for (int m = 0; m < 64 / KC; ++m) {
 #pragma unroll
-                        for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = v[q];
+                        for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = lv[m * (64 / CPI) + q];
                         __builtin_amdgcn_wave_barrier();
-                        const int kend = min(KC, hj - k0);
-                        for (int k = 0; k < kend; ++k) {
+                        // rows past hj: sx is zero there
+#pragma unroll 8
+                        for (int k = 0; k < KC; ++k) {
                             const T l = stile[lane][k];
 #pragma unroll
-                            for (int cc = 0; cc < LX_SW_NR; ++cc)
-                                if (cc < nrhs) acc[cc] += l * sx[cc][k0 + k];
+                            for (int cc = 0; cc < NR; ++cc)
+                                if (cc < nrhs) acc[cc] += l * sx[cc][m * KC + k];
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -250,9 +272,9 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
             }
             // reduce the four waves in a fixed order through sx (each wave's own buffer)
 #pragma unroll
-            for (int cc = 0; cc < LX_SW_NR; ++cc) sx[cc][lane] = acc[cc];
+            for (int cc = 0; cc < NR; ++cc) sx[cc][lane] = acc[cc];
             __syncthreads();
-            T(*sx0)[LX_SW_NR][64] = reinterpret_cast<T(*)[LX_SW_NR][64]>(smem + 4 * 64 * (KC + 1) * sizeof(T));
+            T(*sx0)[NR][64] = reinterpret_cast<T(*)[NR][64]>(smem + 4 * 64 * (KC + 1) * sizeof(T));
             T *pp = part + ((size_t)rowbase[s] + (c - 1)) * LX_SW_NR * 64;
             for (int cc = w; cc < nrhs; cc += 4) {
                 const T v = (sx0[0][cc][lane] + sx0[1][cc][lane]) + (sx0[2][cc][lane] + sx0[3][cc][lane]);
@@ -279,18 +301,29 @@ struct lx_sw_plan {
     int nunits = 0, nparts = 0;
 };
 struct lx_sw_dev {
-    int cus = 0, wall_khz = 0, occ[2][2] = {};
+    int cus = 0, wall_khz = 0, occ[2][2][3] = {};  // [type][trans][NR 1, 4, 16]
     std::map<int, lx_sw_plan> plans;  // by number of blocks
 };
 std::mutex lx_sw_mu;
 std::map<int, lx_sw_dev> lx_sw_devs;
 
-template <typename T, int TR>
+template <typename T, int TR, int NR>
 int lx_sw_occ()
 {
     int o = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lx_sweep_kernel<T, TR>, 256, 0) != hipSuccess || o < 1) o = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lx_sweep_kernel<T, TR, NR>, 256, 0) != hipSuccess || o < 1)
+        o = 1;
     return o;
+}
+template <typename T>
+void lx_sw_occ_all(int (&o)[2][3])
+{
+    o[0][0] = lx_sw_occ<T, 0, 1>();
+    o[0][1] = lx_sw_occ<T, 0, 4>();
+    o[0][2] = lx_sw_occ<T, 0, 16>();
+    o[1][0] = lx_sw_occ<T, 1, 1>();
+    o[1][1] = lx_sw_occ<T, 1, 4>();
+    o[1][2] = lx_sw_occ<T, 1, 16>();
 }
 
 // Device properties, occupancy and the unit list for nb blocks (built once).
@@ -304,10 +337,8 @@ lx_sw_dev *lx_sw_get(int nb, const lx_sw_plan **plan)
         if (hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return nullptr;
         if (hipDeviceGetAttribute(&d.wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || d.wall_khz <= 0)
             d.wall_khz = 100000;
-        d.occ[0][0] = lx_sw_occ<double, 0>();
-        d.occ[0][1] = lx_sw_occ<double, 1>();
-        d.occ[1][0] = lx_sw_occ<float, 0>();
-        d.occ[1][1] = lx_sw_occ<float, 1>();
+        lx_sw_occ_all<double>(d.occ[0]);
+        lx_sw_occ_all<float>(d.occ[1]);
     }
     auto it = d.plans.find(nb);
     if (it == d.plans.end()) {
@@ -351,16 +382,24 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
     int aborted = 0;
     for (int c0 = 0; c0 < nrhs; c0 += LX_SW_NR) {
         const int nc = std::min(LX_SW_NR, nrhs - c0);
+        const int ri = nc <= 1 ? 0 : nc <= 4 ? 1 : 2;
         for (int tr = 0; tr < 2; ++tr) {
             if (!(which & (1 << tr))) continue;
-            const int grid = std::max(1, std::min(pl->nunits, d->cus * d->occ[ti][tr]));
+            const int grid = std::max(1, std::min(pl->nunits, d->cus * d->occ[ti][tr][ri]));
             lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
-            if (tr == 0)
-                hipLaunchKernelGGL((lx_sweep_kernel<T, 0>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx,
-                                   ldx, pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks);
-            else
-                hipLaunchKernelGGL((lx_sweep_kernel<T, 1>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx,
-                                   ldx, pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks);
+#define LX_SW_LAUNCH(TR, NR)                                                                                       \
+    hipLaunchKernelGGL((lx_sweep_kernel<T, TR, NR>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,  \
+                       pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks)
+            if (tr == 0) {
+                if (ri == 0) LX_SW_LAUNCH(0, 1);
+                else if (ri == 1) LX_SW_LAUNCH(0, 4);
+                else LX_SW_LAUNCH(0, 16);
+            } else {
+                if (ri == 0) LX_SW_LAUNCH(1, 1);
+                else if (ri == 1) LX_SW_LAUNCH(1, 4);
+                else LX_SW_LAUNCH(1, 16);
+            }
+#undef LX_SW_LAUNCH
             if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
             int ab = 0;
             LX_HIP_TRY(hipMemcpyAsync(&ab, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, s));

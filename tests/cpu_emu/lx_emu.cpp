@@ -1,3 +1,4 @@
+#include <type_traits>
 // CPU execution of the library's persistent executor (test infrastructure).
 // Builds a plan with the product planner and runs the product kernel source
 // (lx_exec_kernel.hpp) as one workgroup of 256 host threads, through the
@@ -106,23 +107,23 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     std::vector<int> ctl(64 + 2 * nb, 0);
     std::vector<double> part((size_t)std::max(1, rowbase[nb]) * LX_SW_NR * 64);  // large enough for either type
     const long long ticks = 100000000LL * 20;
-    run_wg([&] {
-        if (dtype == 0) {
-            if (trans)
-                lx_sweep_kernel<double, 1>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, units.data(),
-                                           (int)units.size(), rowbase.data(), C, part.data(), ctl.data(), ticks);
-            else
-                lx_sweep_kernel<double, 0>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, units.data(),
-                                           (int)units.size(), rowbase.data(), C, part.data(), ctl.data(), ticks);
-        } else {
-            if (trans)
-                lx_sweep_kernel<float, 1>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, units.data(),
-                                          (int)units.size(), rowbase.data(), C, (float *)part.data(), ctl.data(), ticks);
-            else
-                lx_sweep_kernel<float, 0>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, units.data(),
-                                          (int)units.size(), rowbase.data(), C, (float *)part.data(), ctl.data(), ticks);
-        }
-    });
+    auto go = [&](auto tp, auto tr, auto nr) {
+        using T = decltype(tp);
+        run_wg([&] {
+            lx_sweep_kernel<T, decltype(tr)::value, decltype(nr)::value>(
+                n, nrhs, (const T *)L, ldl, (T *)X, ldx, units.data(), (int)units.size(), rowbase.data(), C,
+                (T *)part.data(), ctl.data(), ticks);
+        });
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    auto by_nr = [&](auto tp, auto tr) {
+        if (nrhs <= 1) go(tp, tr, std::integral_constant<int, 1>());
+        else if (nrhs <= 4) go(tp, tr, std::integral_constant<int, 4>());
+        else go(tp, tr, std::integral_constant<int, 16>());
+    };
+    if (dtype == 0) { if (trans) by_nr(double(), I1()); else by_nr(double(), I0()); }
+    else { if (trans) by_nr(float(), I1()); else by_nr(float(), I0()); }
     if (ctl[1] != 0) return -3;  // a wait timed out
     for (int b = 0; b < nb; ++b)
         if (ctl[64 + b] != 1 || ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;

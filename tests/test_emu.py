@@ -102,7 +102,7 @@ def emu_sweep(emu):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("n,nr", [(1, 1), (64, 2), (100, 3), (200, 16), (700, 2), (1000, 1)])
+@pytest.mark.parametrize("n,nr", [(1, 1), (64, 2), (100, 3), (200, 16), (700, 2), (1000, 1), (300, 7)])
 def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
     """lx_sweep.hip's forward and backward kernels against the oracle's blocked sweeps."""
     rng = np.random.default_rng(n + nr)
