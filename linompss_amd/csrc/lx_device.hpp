@@ -178,7 +178,7 @@ __device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)
     const int lane = lx_lane();
     const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             int c = c0 + j * 16 + lr;
@@ -192,6 +192,10 @@ __device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)
                 }
             }
         }
+        // one row of tiles at a time: keeps the loaded C values from piling up
+        // on top of the accumulators
+        __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -212,6 +216,124 @@ __device__ __forceinline__ void lx_unit_gemm_nt(T *__restrict__ C, long long ldc
     lx_acc_zero<T, FM, FN>(acc);
     lx_wave_gemm_nt<T, FM, FN, U>(acc, A, lda, B, ldb, r0, c0, M, N, K);
     lx_acc_sub_store<T, FM, FN>(acc, C, ldc, r0, c0, M, N, lower, gdiag);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged GEMM / SYRK unit for the trailing updates (same contract as
+// lx_unit_gemm_nt, M, N <= 128):  C[M x N] -= A[M x K] * B[N x K]^T.
+// The workgroup stages K-slabs of A and B (128 x KS each) in LDS, double
+// buffered: the global loads of slab s+1 (16-byte vectors when the unit is
+// full) are in flight while the matrix cores consume slab s, and each operand
+// element crosses the memory hierarchy once per workgroup instead of once per
+// wave.  LDS layout [k][row] with a row stride of 144 elements, so the four
+// 16-lane groups of an MFMA operand read hit disjoint banks.
+template <typename T> struct lx_lgemm_cfg;
+template <> struct lx_lgemm_cfg<float> { static constexpr int KS = 32; };
+template <> struct lx_lgemm_cfg<double> { static constexpr int KS = 16; };
+#define LX_LG_STR 144
+template <typename T>
+struct lx_lgemm_smem {
+    static constexpr int bytes = 2 * 2 * lx_lgemm_cfg<T>::KS * LX_LG_STR * (int)sizeof(T);
+};
+
+typedef float lx_f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double lx_d2u __attribute__((ext_vector_type(2), aligned(8)));
+template <typename T> struct lx_vec16;
+template <> struct lx_vec16<float> { typedef lx_f4u type; static constexpr int n = 4; };
+template <> struct lx_vec16<double> { typedef lx_d2u type; static constexpr int n = 2; };
+
+template <typename T>
+__device__ __forceinline__ void lx_unit_gemm_lds(T *__restrict__ C, long long ldc, const T *__restrict__ A,
+                                                 long long lda, const T *__restrict__ B, long long ldb, int M, int N,
+                                                 int K, bool lower, int gdiag, char *smem)
+{
+    constexpr int KS = lx_lgemm_cfg<T>::KS, VN = lx_vec16<T>::n;
+    typedef typename lx_vec16<T>::type V;
+    constexpr int NV = 128 * KS / VN / LX_WG_THREADS;  // 16-byte vectors per thread per operand and slab
+    constexpr int VPC = 128 / VN;                      // vectors per slab column
+    T(*sA)[KS][LX_LG_STR] = reinterpret_cast<T(*)[KS][LX_LG_STR]>(smem);
+    T(*sB)[KS][LX_LG_STR] = reinterpret_cast<T(*)[KS][LX_LG_STR]>(smem + 2 * KS * LX_LG_STR * sizeof(T));
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+    // a wave block strictly above the diagonal of a SYRK tile does no arithmetic
+    const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
+    const bool full = M == 128 && N == 128 && K % KS == 0;  // uniform
+    const int nslab = (K + KS - 1) / KS;
+    typename lx_mfma<T>::acc acc[4][4];
+    lx_acc_zero<T, 4, 4>(acc);
+    auto compute = [&](int buf) {
+        if (!active) return;
+        // two k-steps of operands at a time (the barrier stops the compiler from
+        // reading the whole slab's operands into registers up front)
+#pragma unroll
+        for (int k2 = 0; k2 < KS; k2 += 8) {
+#pragma unroll
+            for (int kk = k2; kk < k2 + 8; kk += 4) {
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = sA[buf][kk + lk][wr + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = sB[buf][kk + lk][wc + j * 16 + lr];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = lx_mfma<T>::mac(a[i], b[j], acc[i][j]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    if (full) {
+        // 16-byte loads of slab s+1 in flight while slab s is multiplied
+        V ra[NV], rb[NV];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+                ra[i] = *(const V *)(A + (long long)(k0 + kk) * lda + r);
+                rb[i] = *(const V *)(B + (long long)(k0 + kk) * ldb + r);
+            }
+        };
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    sA[buf][kk][r + e] = ra[i][e];
+                    sB[buf][kk][r + e] = rb[i][e];
+                }
+            }
+        };
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int buf = sl & 1;
+            if (sl + 1 < nslab) gload((sl + 1) * KS);
+            compute(buf);
+            if (sl + 1 < nslab) lstore(buf ^ 1);
+            __syncthreads();
+        }
+    } else {
+        // ragged unit (edge tiles): element loads straight into LDS, rows past
+        // M / N clamped (never stored), columns past K zero
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int k0 = sl * KS;
+            for (int e = tid; e < 128 * KS; e += LX_WG_THREADS) {
+                const int r = e % 128, kk = e / 128;
+                const bool kv = k0 + kk < K;
+                const int kc = kv ? k0 + kk : K - 1;
+                const T va = A[(long long)kc * lda + (r < M ? r : M - 1)];
+                const T vb = B[(long long)kc * ldb + (r < N ? r : N - 1)];
+                sA[0][kk][r] = kv ? va : T(0);
+                sB[0][kk][r] = kv ? vb : T(0);
+            }
+            __syncthreads();
+            compute(0);
+            __syncthreads();
+        }
+    }
+    if (active) lx_acc_sub_store<T, 4, 4>(acc, C, ldc, wr, wc, M, N, lower, gdiag);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,7 +436,7 @@ __device__ __forceinline__ void lx_tri_inv64(T (*sL)[65], T (*sZ)[65])
 // Returns through *info (same value in every thread) the 1-based local column
 // of the first non-positive pivot, as LAPACK's info; the factorisation keeps
 // going past it (as LAPACK, results there are not meaningful).
-// LDS: 2 * 64 * 65 * sizeof(T) + 16 bytes.
+// LDS: max(2 * 64 * 65 * sizeof(T) + 16, lx_lgemm_smem<T>::bytes) bytes.
 template <typename T>
 __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, T *__restrict__ W,
                                               long long ldw, int *info_local, char *smem)
@@ -396,18 +518,20 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
                 }
         }
         __syncthreads();
-        // 4) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel, K = pb
+        // 4) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel, K = pb,
+        // in staged 128 x 128 blocks (D and Z are no longer needed: the blocks
+        // take the whole LDS arena)
         if (nrow > 0) {
-            constexpr int FMU = 4, FNU = 4, BS = 2 * 16 * 4;  // 128 x 128 blocks
+            constexpr int BS = 128;
             const T *Pp = A + (long long)p * lda + r_beg;
             T *Ct = A + (long long)r_beg * lda + r_beg;
             const int nb = (nrow + BS - 1) / BS;
             for (int bj = 0; bj < nb; ++bj)
                 for (int bi = bj; bi < nb; ++bi) {
                     const int rr = bi * BS, cc = bj * BS;
-                    lx_unit_gemm_nt<T, FMU, FNU, sizeof(T) == 8 ? 1 : 2>(
-                        Ct + (long long)cc * lda + rr, lda, Pp + rr, lda, Pp + cc, lda, nrow - rr < BS ? nrow - rr : BS,
-                        nrow - cc < BS ? nrow - cc : BS, pb, bi == bj, rr - cc);
+                    lx_unit_gemm_lds<T>(Ct + (long long)cc * lda + rr, lda, Pp + rr, lda, Pp + cc, lda,
+                                        nrow - rr < BS ? nrow - rr : BS, nrow - cc < BS ? nrow - cc : BS, pb, bi == bj,
+                                        rr - cc, smem);
                 }
         }
         __syncthreads();

@@ -22,7 +22,10 @@
 // kernel symbol, so a launch carries only the registers and LDS its units use.
 enum lx_family : int { LX_FAM_FACTOR = 0, LX_FAM_SOLVE = 1, LX_FAM_ALL = 2 };
 template <typename T, int FAM> struct lx_fam_smem {
-    static constexpr int factor = 2 * 64 * 65 * (int)sizeof(T) + 16;  // POTRF; TRSM takes 2 * 64 * 65 * sizeof(T)
+    // POTRF (2 * 64 * 65 * sizeof(T) + 16; its trailing update reuses the space
+    // for the staged GEMM), TRSM (2 * 64 * 65 * sizeof(T)), staged GEMM
+    static constexpr int potrf = 2 * 64 * 65 * (int)sizeof(T) + 16;
+    static constexpr int factor = potrf > lx_lgemm_smem<T>::bytes ? potrf : lx_lgemm_smem<T>::bytes;
     static constexpr int solve = 42 * 1024;
     static constexpr int bytes = FAM == LX_FAM_FACTOR ? factor : FAM == LX_FAM_SOLVE ? solve : (factor > solve ? factor : solve);
 };
@@ -52,7 +55,6 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     T *C = Cb + lx_off(u.c, ldc);
     const T *Ap = Ab + lx_off(u.a, lda);
     const T *Bp = Bb + lx_off(u.b, ldb);
-    constexpr int FM = lx_gemm_cfg<T>::FM, FN = lx_gemm_cfg<T>::FN, UU = lx_gemm_cfg<T>::U;
     constexpr bool F = FAM != LX_FAM_SOLVE, S = FAM != LX_FAM_FACTOR;
     switch (u.op) {
     case LX_OP_POTRF: {
@@ -70,7 +72,12 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
         break;
     case LX_OP_SYRK:
     case LX_OP_GEMM_NT:
-        if constexpr (F) lx_unit_gemm_nt<T, FM, FN, UU>(C, ldc, Ap, lda, Bp, ldb, u.M, u.N, u.K, u.aux0 != 0, u.aux1);
+        // a unit covers up to M = 128 rows and N = 128 * m columns, done as m
+        // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0))
+        if constexpr (F)
+            for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N)  // uniform
+                lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M,
+                                    u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N, u.K, u.aux0 != 0, u.aux1 - c0, smem);
         break;
     case LX_OP_TRSV:
         if constexpr (S) lx_unit_trsv_block<T>(C, ldc, Ap, lda, u.M, u.N, u.aux0, smem);

@@ -129,11 +129,14 @@ void lx_planner::factor_rl(int s)
                 const int jbs = std::min(b, n - j * b);
                 int outc = tile_ctr(s, i, j), in[2] = {tile_ctr(s, i, k), tile_ctr(s, j, k)};
                 int t = new_task(LX_T_GEMM, s, i, j, k, PRIO_LO, outc, in, 2);
-                for (int c0 = 0; c0 < jbs; c0 += P.unit_n)
+                // the next panel's column (j = k+1) feeds the critical path: fine units;
+                // elsewhere wide units, so that one release fence covers more work
+                const int un = j == k + 1 ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
+                for (int c0 = 0; c0 < jbs; c0 += un)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         lx_unit u = proto_unit(LX_OP_GEMM_NT, s);
                         u.M = std::min(P.unit_m, ibs - r0);
-                        u.N = std::min(P.unit_n, jbs - c0);
+                        u.N = std::min(un, jbs - c0);
                         u.K = bs;
                         u.ctr = outc;
                         u.c = rc(i * b + r0, j * b + c0);
@@ -141,7 +144,7 @@ void lx_planner::factor_rl(int s)
                         u.b = rc(j * b + c0, k * b);
                         add_unit(t, u);
                     }
-                Tk[t].unit_cost = 2.0 * P.unit_m * P.unit_n * bs / P.flop_rate + P.unit_overhead;
+                Tk[t].unit_cost = 2.0 * P.unit_m * un * bs / P.flop_rate + P.unit_overhead;
                 issued[outc] += Tk[t].nunits;
                 last_writer[outc] = t;
             }

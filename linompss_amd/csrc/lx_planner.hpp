@@ -18,6 +18,7 @@
 struct lx_plan_params {
     int unit_m = 128;       // GEMM / SYRK unit rows (one workgroup tile)
     int unit_n = 128;       // GEMM / SYRK unit cols
+    int gemm_wide_n = 512;  // GEMM unit cols off the lookahead column (one fence per 128 x 512)
     int trsm_rows = 64;     // panel TRSM rows per unit
     int gemv_rows = 128;    // RHS GEMV output rows per unit
     int policy = 1;         // 0: reference priorities + FIFO, 1: critical-path (bottom level)
