@@ -73,6 +73,29 @@ void lx_planner::add_unit(int task, const lx_unit &proto)
     Tk[task].nunits++;
 }
 
+void lx_planner::add_unit_deps(int task, const lx_unit &proto, const std::vector<std::pair<int32_t, int32_t>> &deps)
+{
+    if ((int)deps.size() > LX_MAX_DEPS) throw std::logic_error("too many dependencies");
+    lx_unit u = proto;
+    u.nd = (int)deps.size();
+    for (int d = 0; d < LX_MAX_DEPS; ++d) {
+        u.dc[d] = d < u.nd ? deps[d].first : 0;
+        u.dt[d] = d < u.nd ? deps[d].second : 0;
+    }
+    U.push_back(u);
+    unit_task.push_back(task);
+    Tk[task].nunits++;
+}
+
+int lx_planner::alloc_counters(int count)
+{
+    const int base = nctr;
+    nctr += count;
+    issued.resize(nctr, 0);
+    last_writer.resize(nctr, -1);
+    return base;
+}
+
 static lx_unit proto_unit(int op, int sys)
 {
     lx_unit u = {};
@@ -87,6 +110,99 @@ static lx_unit proto_unit(int op, int sys)
 // packed as row + col * 2^32 so that one plan serves any lda.
 static inline int64_t rc(int64_t r, int64_t c) { return (c << 32) | r; }
 
+// POTRF(k): one unit when the tile is at most potrf_sub wide; otherwise the
+// tile is factored as a right-looking sub-DAG of potrf_sub-wide sub-tiles --
+// POTRF, TRSM and SYRK/GEMM units with counters of their own -- so that the
+// diagonal factorisation, the step every other task of the next panel waits
+// for, is spread over several workgroups.  Every sub-unit waits (directly or
+// through the first sub-POTRF) for the tile's earlier updates; only the last
+// sub-POTRF bumps the tile's counter, and it runs after every other sub-unit
+// (each one precedes it in the sub-DAG), so that bump publishes the whole tile.
+void lx_planner::potrf_units(int t, int s, int kb, int bs, int outc)
+{
+    const std::vector<std::pair<int32_t, int32_t>> tile_deps = task_deps[t];
+    const int sb = P.potrf_sub;
+    if (bs <= sb || sb <= 0) {
+        lx_unit u = proto_unit(LX_OP_POTRF, s);
+        u.M = u.N = u.K = bs;
+        u.ctr = outc;
+        u.c = rc(kb, kb);
+        u.aux1 = kb;
+        Tk[t].unit_cost = (double)bs * bs * bs / 3.0 / P.potrf_rate + P.unit_overhead;
+        add_unit_deps(t, u, tile_deps);
+        return;
+    }
+    const int ns = (bs + sb - 1) / sb;
+    const int base = alloc_counters(ns * (ns + 1) / 2);
+    auto sub = [&](int i, int j) { return base + i * (i + 1) / 2 + j; };
+    std::vector<int32_t> cnt(ns * (ns + 1) / 2, 0);  // units issued per sub-counter
+    auto cnt_of = [&](int i, int j) -> int32_t & { return cnt[i * (i + 1) / 2 + j]; };
+    for (int kk = 0; kk < ns; ++kk) {
+        const int k0 = kb + kk * sb, kbs = std::min(sb, bs - kk * sb);
+        {   // sub-POTRF(kk)
+            std::vector<std::pair<int32_t, int32_t>> d = kk == 0 ? tile_deps : std::vector<std::pair<int32_t, int32_t>>();
+            if (cnt_of(kk, kk) > 0) d.push_back({sub(kk, kk), cnt_of(kk, kk)});
+            lx_unit u = proto_unit(LX_OP_POTRF, s);
+            u.M = u.N = u.K = kbs;
+            const bool last = kk == ns - 1;
+            u.ctr = last ? outc : sub(kk, kk);
+            u.c = rc(k0, k0);
+            u.aux1 = k0;
+            add_unit_deps(t, u, d);
+            if (!last) cnt_of(kk, kk) += 1;
+        }
+        for (int ii = kk + 1; ii < ns; ++ii) {  // sub-TRSM(ii, kk)
+            const int i0 = kb + ii * sb, ibs = std::min(sb, bs - ii * sb);
+            std::vector<std::pair<int32_t, int32_t>> d = {{sub(kk, kk), cnt_of(kk, kk)}};
+            if (cnt_of(ii, kk) > 0) d.push_back({sub(ii, kk), cnt_of(ii, kk)});
+            int nu = 0;
+            for (int r0 = 0; r0 < ibs; r0 += P.trsm_rows, ++nu) {
+                lx_unit u = proto_unit(LX_OP_TRSM_RLTN, s);
+                u.M = std::min(P.trsm_rows, ibs - r0);
+                u.N = kbs;
+                u.ctr = sub(ii, kk);
+                u.c = rc(i0 + r0, k0);
+                u.a = rc(k0, k0);
+                add_unit_deps(t, u, d);
+            }
+            cnt_of(ii, kk) += nu;
+        }
+        for (int ii = kk + 1; ii < ns; ++ii) {  // sub-SYRK(ii, kk) and sub-GEMM(ii, jj, kk)
+            const int i0 = kb + ii * sb, ibs = std::min(sb, bs - ii * sb);
+            for (int jj = kk + 1; jj <= ii; ++jj) {
+                const int j0 = kb + jj * sb, jbs = std::min(sb, bs - jj * sb);
+                std::vector<std::pair<int32_t, int32_t>> d = {{sub(ii, kk), cnt_of(ii, kk)}};
+                if (jj != ii) d.push_back({sub(jj, kk), cnt_of(jj, kk)});
+                if (cnt_of(ii, jj) > 0) d.push_back({sub(ii, jj), cnt_of(ii, jj)});
+                int nu = 0;
+                for (int c0 = 0; c0 < jbs; c0 += P.unit_n)
+                    for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
+                        if (ii == jj && r0 + P.unit_m - 1 < c0) continue;  // strictly upper
+                        lx_unit u = proto_unit(ii == jj ? LX_OP_SYRK : LX_OP_GEMM_NT, s);
+                        u.M = std::min(P.unit_m, ibs - r0);
+                        u.N = std::min(P.unit_n, jbs - c0);
+                        u.K = kbs;
+                        u.ctr = sub(ii, jj);
+                        u.c = rc(i0 + r0, j0 + c0);
+                        u.a = rc(i0 + r0, k0);
+                        u.b = rc(j0 + c0, k0);
+                        if (ii == jj) {
+                            u.aux0 = 1;
+                            u.aux1 = r0 - c0;
+                        }
+                        add_unit_deps(t, u, d);
+                        ++nu;
+                    }
+                cnt_of(ii, jj) += nu;
+            }
+        }
+    }
+    // simulated cost per unit: the sub-DAG's chain spread over its units
+    const double chain = ns * ((double)sb * sb * sb / 3.0 / P.potrf_rate + (double)sb * sb * sb / P.trsm_rate +
+                               2.0 * sb * sb * sb / P.flop_rate + 3 * P.unit_overhead);
+    Tk[t].unit_cost = chain / std::max(1, Tk[t].nunits);
+}
+
 void lx_planner::factor_rl(int s)
 {
     const sysinfo &sy = S[s];
@@ -96,14 +212,8 @@ void lx_planner::factor_rl(int s)
         {   // POTRF(k)  chol_rlmain.c:30
             int outc = tile_ctr(s, k, k);
             int t = new_task(LX_T_POTRF, s, k, k, k, PRIO_HI, outc, nullptr, 0);
-            lx_unit u = proto_unit(LX_OP_POTRF, s);
-            u.M = u.N = u.K = bs;
-            u.ctr = outc;
-            u.c = rc(k * b, k * b);
-            u.aux1 = k * b;
-            Tk[t].unit_cost = (double)bs * bs * bs / 3.0 / P.potrf_rate + P.unit_overhead;
-            add_unit(t, u);
-            issued[outc] += Tk[t].nunits;
+            potrf_units(t, s, k * b, bs, outc);
+            issued[outc] += 1;  // one unit (the last) bumps the tile's counter
             last_writer[outc] = t;
         }
         for (int i = k + 1; i < mt; ++i) {  // TRSM(i,k)  chol_rlmain.c:35-38
@@ -205,14 +315,8 @@ void lx_planner::factor_ll(int s)
         {   // POTRF(k)  chol_llmain.c:38
             int outc = tile_ctr(s, k, k);
             int t = new_task(LX_T_POTRF, s, k, k, k, PRIO_HI, outc, nullptr, 0);
-            lx_unit u = proto_unit(LX_OP_POTRF, s);
-            u.M = u.N = u.K = bs;
-            u.ctr = outc;
-            u.c = rc(k * b, k * b);
-            u.aux1 = k * b;
-            Tk[t].unit_cost = (double)bs * bs * bs / 3.0 / P.potrf_rate + P.unit_overhead;
-            add_unit(t, u);
-            issued[outc] += Tk[t].nunits;
+            potrf_units(t, s, k * b, bs, outc);
+            issued[outc] += 1;  // one unit (the last) bumps the tile's counter
             last_writer[outc] = t;
         }
         for (int i = k + 1; i < mt; ++i) {

@@ -19,6 +19,7 @@ struct lx_plan_params {
     int unit_m = 128;       // GEMM / SYRK unit rows (one workgroup tile)
     int unit_n = 128;       // GEMM / SYRK unit cols
     int gemm_wide_n = 512;  // GEMM unit cols off the lookahead column (one fence per 128 x 512)
+    int potrf_sub = 128;    // diagonal tiles wider than this factor as a sub-DAG of sub-tiles of this size
     int trsm_rows = 64;     // panel TRSM rows per unit
     int gemv_rows = 128;    // RHS GEMV output rows per unit
     int policy = 1;         // 0: reference priorities + FIFO, 1: critical-path (bottom level)
@@ -82,6 +83,13 @@ class lx_planner {
     int rhs_ctr(int s, int i) const { return S[s].ctr_base + S[s].mt * (S[s].mt + 1) / 2 + i; }
     int new_task(int type, int sys, int ti, int tj, int tk, int prio, int out_ctr, const int *in_ctrs, int nin);
     void add_unit(int task, const lx_unit &proto);
+    // A unit with its own dependency list (units inside one task that wait on
+    // each other, created in a topological order).
+    void add_unit_deps(int task, const lx_unit &proto, const std::vector<std::pair<int32_t, int32_t>> &deps);
+    int alloc_counters(int count);
+    // Units of the diagonal-tile factorisation POTRF(k) of task t (its output
+    // counter outc, global column offset k*b, width bs).
+    void potrf_units(int t, int s, int kb, int bs, int outc);
 
     lx_plan_params P;
     std::vector<sysinfo> S;

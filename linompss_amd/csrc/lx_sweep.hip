@@ -79,6 +79,25 @@ __device__ __forceinline__ void lx_sw_wait_ge(const int *p, int need, int *abort
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// *p += 1 once the workgroup's stores are visible device-wide: each wave waits
+// for its own stores, then wave 0 alone issues the agent-scope release
+// read-modify-write (one L2 write-back for the workgroup, not one per wave).
+__device__ __forceinline__ void lx_sw_publish(int *p)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (lx_wave() == 0) {  // wave-uniform; lane 0 adds 1, the others 0
+        // the release as a fence plus an explicit wait, then a relaxed add: with a
+        // release read-modify-write the compiler dropped the wait between the L2
+        // write-back and the atomic as redundant (nothing else was outstanding),
+        // letting the counter overtake the data
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_fetch_add(p, (threadIdx.x & 63) == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X (n x nrhs,
 // nrhs <= NR <= 16) is overwritten in place.  ctl = [ticket, abort, pad...,
 // flags[nb], pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64
@@ -200,12 +219,10 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                 }
                 if (lane < hb) X[(int64_t)cc * ldx + r0 + lane] = x0 + x1;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __syncthreads();
             // published with a read-modify-write: it is performed at the shared level
             // at once, where a plain atomic store can sit in this XCD's L2 for
             // microseconds before other XCDs see it
-            if (w == 0) atomicAdd(&flags[s], lane == 0 ? 1 : 0);
+            lx_sw_publish(&flags[s]);
         } else {
             // ---------------- P(s, c) ----------------
             T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + w * 64 * (KC + 1) * sizeof(T));
@@ -280,9 +297,7 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                 const T v = (sx0[0][cc][lane] + sx0[1][cc][lane]) + (sx0[2][cc][lane] + sx0[3][cc][lane]);
                 pp[(size_t)cc * 64 + lane] = lane < hb ? v : T(0);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __syncthreads();
-            if (w == 0) atomicAdd(&pcnt[s], lane == 0 ? 1 : 0);
+            lx_sw_publish(&pcnt[s]);
         }
     }
 }

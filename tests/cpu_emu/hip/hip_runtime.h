@@ -127,6 +127,7 @@ inline void __builtin_amdgcn_wave_barrier() { lx_emu_wave_barrier[threadIdx.x >>
 inline void __builtin_amdgcn_sched_barrier(int) {}
 inline void __builtin_amdgcn_fence_emu() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 #define __builtin_amdgcn_fence(order, scope) __builtin_amdgcn_fence_emu()
+inline void __builtin_amdgcn_s_waitcnt(int) {}
 inline long long wall_clock64()
 {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
