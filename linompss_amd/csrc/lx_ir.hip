@@ -7,8 +7,8 @@
 // Here solve() is the FP32 tiled Cholesky factor + sweeps of this library,
 // and the FP64 pieces (residual, update, backward-error test) are the kernels
 // below.  A is symmetric and only its lower triangle is read: the residual is
-// r = b - (tril(A) x + stril(A)^T x), two coalesced passes over the lower
-// triangle that also produce ||A||_inf on the first sweep.
+// r = b - (tril(A) x + stril(A)^T x), one pass over the lower triangle that
+// also produces ||A||_inf on the first refinement step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,91 +20,136 @@
 #include "linompss.h"
 #include "lx_internal.hpp"
 
-#define LX_IR_NR 16  // right-hand sides per kernel pass
 
-// y1 = tril(A) X, rowabs = sum_{j<=i} |a_ij|  (rows: lanes; columns split by wave)
-__global__ void __launch_bounds__(256)
-    lx_symv_rows_kernel(const double *__restrict__ A, int64_t lda, int n, const double *__restrict__ X, int64_t ldx,
-                        int nrhs, double *__restrict__ Y1, double *__restrict__ rowabs)
+// Residual pass over the lower triangle, read once.  The triangle is cut into
+// 256 x 256 super-blocks (4 x 4 blocks of 64); the workgroup of super-block
+// (SI, SJ), SJ <= SI, forms for each of its blocks A_IJ both
+//   the row part    A_IJ x_J      (contributes to y_I)
+//   the column part A_IJ^T x_I    (contributes to y_J; on a diagonal block the
+//                                  strictly lower part only)
+// and, when ABS, the matching sums of |a_ij| (the rows of |A| for ||A||_inf).
+// Its sums over the super-block go to P[SI][SJ] (row parts) and P[SJ][SI]
+// (column parts; both parts of a diagonal super-block go to P[SI][SI]), and
+// y_S = sum over SJ of P[S][SJ] in a fixed order (lx_symv_reduce_kernel):
+// deterministic, no atomics.  Each 64 x 64 block is staged in LDS (coalesced
+// column reads); thread (l, q) owns row l / column l of the block and the
+// quarter q of the other index.  P holds ns*ns slots of 256 x NC values
+// (NC = NR, plus 1 for the |A| sums).
+#define LX_SY_BS 64
+#define LX_SY_SB 4  // blocks per super-block side
+template <int NR, bool ABS>
+__global__ void __launch_bounds__(256) lx_symv_blk_kernel(const double *__restrict__ A, int64_t lda, int n,
+                                                          const double *__restrict__ X, int64_t ldx, int nrhs,
+                                                          double *__restrict__ P)
 {
-    __shared__ double red[4][LX_IR_NR + 1][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = blockIdx.x * 64, row = r0 + lane;
-    const int jend = std::min(n, r0 + 64);  // columns 0 .. jend-1 (masked j <= row)
-    const int chunk = (jend + 3) / 4;
-    const int jb = w * chunk, je = std::min(jend, jb + chunk);
-    double acc[LX_IR_NR];
-#pragma unroll
-    for (int c = 0; c < LX_IR_NR; ++c) acc[c] = 0.0;
-    double aabs = 0.0;
-    if (row < n) {
-        const double *Ar = A + row;
-        int j = jb;
-        for (; j < je; ++j) {
-            double a = (j <= row) ? Ar[(int64_t)j * lda] : 0.0;
-            aabs += fabs(a);
-#pragma unroll
-            for (int c = 0; c < LX_IR_NR; ++c)
-                if (c < nrhs) acc[c] += a * X[(int64_t)c * ldx + j];
+    constexpr int NC = NR + (ABS ? 1 : 0), SB = LX_SY_SB, SN = SB * LX_SY_BS;
+    __shared__ double sA[64][65];           // sA[c][r] = A[I*64 + r, J*64 + c]
+    __shared__ double sxI[SB][NR][64];      // x of the super-block's row blocks
+    __shared__ double sxJ[SB][NR][64];      // x of its column blocks
+    __shared__ double sred[4][NC][65];
+    const int nb = (n + 63) / 64, ns = (nb + SB - 1) / SB;
+    const int64_t nsb = (int64_t)ns * (ns + 1) / 2;
+    const int tid = threadIdx.x, l = tid & 63, q = tid >> 6;
+    for (int64_t sbk = blockIdx.x; sbk < nsb; sbk += gridDim.x) {
+        int SI = (int)((sqrt(8.0 * (double)sbk + 1.0) - 1.0) * 0.5);
+        while ((int64_t)SI * (SI + 1) / 2 > sbk) --SI;
+        while ((int64_t)(SI + 1) * (SI + 2) / 2 <= sbk) ++SI;
+        const int SJ = (int)(sbk - (int64_t)SI * (SI + 1) / 2);
+        __syncthreads();  // the previous super-block is done with the LDS
+        for (int e = tid; e < SB * NR * 64; e += 256) {
+            const int k = e % 64, c = (e / 64) % NR, b = e / (64 * NR);
+            const int ri = SI * SN + b * 64 + k, rj = SJ * SN + b * 64 + k;
+            sxI[b][c][k] = (c < nrhs && ri < n) ? X[(int64_t)c * ldx + ri] : 0.0;
+            sxJ[b][c][k] = (c < nrhs && rj < n) ? X[(int64_t)c * ldx + rj] : 0.0;
         }
-    }
+        double accR[SB][NC], accC[SB][NC];
 #pragma unroll
-    for (int c = 0; c < LX_IR_NR; ++c) red[w][c][lane] = acc[c];
-    red[w][LX_IR_NR][lane] = aabs;
-    __syncthreads();
-    if (w == 0 && row < n) {
-        for (int c = 0; c < nrhs; ++c)
-            Y1[(int64_t)c * n + row] = (red[0][c][lane] + red[1][c][lane]) + (red[2][c][lane] + red[3][c][lane]);
-        if (rowabs)
-            rowabs[row] = (red[0][LX_IR_NR][lane] + red[1][LX_IR_NR][lane]) +
-                          (red[2][LX_IR_NR][lane] + red[3][LX_IR_NR][lane]);
+        for (int b = 0; b < SB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) accR[b][c] = accC[b][c] = 0.0;
+#pragma unroll
+        for (int bi = 0; bi < SB; ++bi) {
+#pragma unroll
+            for (int bj = 0; bj < SB; ++bj) {
+                const int I = SI * SB + bi, J = SJ * SB + bj;
+                if (J > I || I >= nb) continue;  // uniform
+                const int r0 = I * 64, c0 = J * 64, hr = min(64, n - r0), hc = min(64, n - c0);
+                __syncthreads();  // sA is free; also orders the sx writes above
+                for (int e = tid; e < 64 * 64; e += 256) {
+                    const int r = e & 63, c = e >> 6;
+                    sA[c][r] = (r < hr && c < hc && (I != J || r >= c)) ? A[(int64_t)(c0 + c) * lda + r0 + r] : 0.0;
+                }
+                __syncthreads();
+                // row part: row l, columns 16q .. 16q+15
+#pragma unroll 4
+                for (int k = 16 * q; k < 16 * q + 16; ++k) {
+                    const double a = sA[k][l];
+#pragma unroll
+                    for (int c = 0; c < NR; ++c) accR[bi][c] += a * sxJ[bj][c][k];
+                    if (ABS) accR[bi][NR] += fabs(a);
+                }
+                // column part: column l, rows 16q .. 16q+15 (diagonal block: rows > l only)
+#pragma unroll 4
+                for (int r = 16 * q; r < 16 * q + 16; ++r) {
+                    const double a = (I != J || r > l) ? sA[l][r] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < NR; ++c) accC[bj][c] += a * sxI[bi][c][r];
+                    if (ABS) accC[bj][NR] += fabs(a);
+                }
+            }
+        }
+        // quarter sums in a fixed order, then out to P
+        double *Prow = P + ((size_t)SI * ns + SJ) * SN * NC;
+        double *Pcol = P + ((size_t)SJ * ns + SI) * SN * NC;
+        for (int b = 0; b < SB; ++b) {
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; ++c) sred[q][c][l] = accR[b][c];
+            __syncthreads();
+            if (SI == SJ) {  // uniform: both parts land in P[SI][SI]
+#pragma unroll
+                for (int c = 0; c < NC; ++c) sred[q][c][l] += accC[b][c];
+                __syncthreads();
+            }
+            for (int e = tid; e < NC * 64; e += 256) {
+                const int r = e % 64, c = e / 64;
+                Prow[(size_t)c * SN + b * 64 + r] = (sred[0][c][r] + sred[1][c][r]) + (sred[2][c][r] + sred[3][c][r]);
+            }
+            if (SI != SJ) {
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < NC; ++c) sred[q][c][l] = accC[b][c];
+                __syncthreads();
+                for (int e = tid; e < NC * 64; e += 256) {
+                    const int r = e % 64, c = e / 64;
+                    Pcol[(size_t)c * SN + b * 64 + r] =
+                        (sred[0][c][r] + sred[1][c][r]) + (sred[2][c][r] + sred[3][c][r]);
+                }
+            }
+        }
     }
 }
 
-// y2_j = sum_{i>j} a_ij x_i, colabs_j = sum_{i>j} |a_ij|  (one wave per column)
-__global__ void __launch_bounds__(256)
-    lx_symv_cols_kernel(const double *__restrict__ A, int64_t lda, int n, const double *__restrict__ X, int64_t ldx,
-                        int nrhs, double *__restrict__ Y2, double *__restrict__ colabs)
+// R = B - y with y_S = sum over SJ = 0.. of P[S][SJ]; rowabs (when non-null)
+// = the |A| row sums (column NR of P).  Thread = one row of one right-hand side.
+template <int NR, bool ABS>
+__global__ void __launch_bounds__(256) lx_symv_reduce_kernel(int n, int nrhs, const double *__restrict__ P,
+                                                             const double *__restrict__ B, int64_t ldb,
+                                                             double *__restrict__ R, double *__restrict__ rowabs)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int j = blockIdx.x * 4 + w; j < n; j += gridDim.x * 4) {
-        double acc[LX_IR_NR];
-#pragma unroll
-        for (int c = 0; c < LX_IR_NR; ++c) acc[c] = 0.0;
-        double aabs = 0.0;
-        const double *Aj = A + (int64_t)j * lda;
-        for (int i = j + 1 + lane; i < n; i += 64) {
-            double a = Aj[i];
-            aabs += fabs(a);
-#pragma unroll
-            for (int c = 0; c < LX_IR_NR; ++c)
-                if (c < nrhs) acc[c] += a * X[(int64_t)c * ldx + i];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-            for (int c = 0; c < LX_IR_NR; ++c)
-                if (c < nrhs) acc[c] += __shfl_xor(acc[c], off);
-            aabs += __shfl_xor(aabs, off);
-        }
-        if (lane == 0) {
-            for (int c = 0; c < nrhs; ++c) Y2[(int64_t)c * n + j] = acc[c];
-            if (colabs) colabs[j] = aabs;
-        }
-    }
-}
-
-// R = B - (Y1 + Y2); optional rowsum = rowabs + colabs
-__global__ void lx_resid_combine_kernel(int n, int nrhs, const double *__restrict__ B, int64_t ldb,
-                                        const double *__restrict__ Y1, const double *__restrict__ Y2,
-                                        double *__restrict__ R, double *__restrict__ rowabs,
-                                        const double *__restrict__ colabs)
-{
-    int64_t tot = (int64_t)n * nrhs;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c = e / n, i = e % n;
-        R[c * n + i] = B[c * ldb + i] - (Y1[c * n + i] + Y2[c * n + i]);
-        if (rowabs && c == 0) rowabs[i] += colabs[i];
+    constexpr int NC = NR + (ABS ? 1 : 0), SN = LX_SY_SB * LX_SY_BS;
+    const int nb = (n + 63) / 64, ns = (nb + LX_SY_SB - 1) / LX_SY_SB;
+    const int64_t tot = (int64_t)ns * SN * NC;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+        const int r = (int)(e % SN), c = (int)((e / SN) % NC), S = (int)(e / ((int64_t)SN * NC));
+        const int row = S * SN + r;
+        if (row >= n) continue;
+        if (c < NR && c >= nrhs) continue;
+        const double *p = P + (size_t)S * ns * SN * NC + (size_t)c * SN + r;
+        double y = 0.0;
+        for (int sj = 0; sj < ns; ++sj) y += p[(size_t)sj * SN * NC];
+        if (c < NR) R[(int64_t)c * n + row] = B[(int64_t)c * ldb + row] - y;
+        else if (rowabs) rowabs[row] = y;
     }
 }
 
@@ -203,20 +248,42 @@ __global__ void lx_max_kernel(int n, const double *__restrict__ v, double *__res
 
 static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
-// r = b - A x for nrhs <= LX_IR_NR columns at a time; rowsum (when non-null)
-// receives ||A||'s row sums.
-static int residual(int n, int nrhs, const double *A, int64_t lda, const double *X, int64_t ldx, const double *B,
-                    int64_t ldb, double *R, double *Y1, double *Y2, double *rowabs, double *colabs, hipStream_t s)
+// r = b - A x, four right-hand sides per pass (one for a single column);
+// rowabs (when non-null) receives the row sums of |A|.
+template <int NR>
+static int residual_nr(int n, int nc, const double *A, int64_t lda, const double *X, int64_t ldx, const double *B,
+                       int64_t ldb, double *R, double *rowabs, double *P, hipStream_t s)
 {
-    for (int c0 = 0; c0 < nrhs; c0 += LX_IR_NR) {
-        int nc = std::min(LX_IR_NR, nrhs - c0);
-        bool nrm = rowabs && c0 == 0;
-        hipLaunchKernelGGL(lx_symv_rows_kernel, dim3((n + 63) / 64), dim3(256), 0, s, A, lda, n, X + c0 * ldx, ldx,
-                           nc, Y1, nrm ? rowabs : nullptr);
-        hipLaunchKernelGGL(lx_symv_cols_kernel, dim3(std::min(8192, (n + 3) / 4)), dim3(256), 0, s, A, lda, n,
-                           X + c0 * ldx, ldx, nc, Y2, nrm ? colabs : nullptr);
-        hipLaunchKernelGGL(lx_resid_combine_kernel, dim3(grid_for((int64_t)n * nc)), dim3(256), 0, s, n, nc,
-                           B + c0 * ldb, ldb, Y1, Y2, R + (int64_t)c0 * n, nrm ? rowabs : nullptr, colabs);
+    const int nb = (n + 63) / 64, ns = (nb + LX_SY_SB - 1) / LX_SY_SB;
+    const int64_t nsb = (int64_t)ns * (ns + 1) / 2;
+    const int grid = (int)std::min<int64_t>(nsb, 2048);
+    const int64_t red = (int64_t)ns * LX_SY_SB * LX_SY_BS * (NR + 1);
+    if (rowabs) {
+        hipLaunchKernelGGL((lx_symv_blk_kernel<NR, true>), dim3(grid), dim3(256), 0, s, A, lda, n, X, ldx, nc, P);
+        hipLaunchKernelGGL((lx_symv_reduce_kernel<NR, true>), dim3(grid_for(red)), dim3(256), 0, s, n, nc, P, B, ldb, R,
+                           rowabs);
+    } else {
+        hipLaunchKernelGGL((lx_symv_blk_kernel<NR, false>), dim3(grid), dim3(256), 0, s, A, lda, n, X, ldx, nc, P);
+        hipLaunchKernelGGL((lx_symv_reduce_kernel<NR, false>), dim3(grid_for(red)), dim3(256), 0, s, n, nc, P, B, ldb,
+                           R, nullptr);
+    }
+    return 0;
+}
+
+static int residual(int n, int nrhs, const double *A, int64_t lda, const double *X, int64_t ldx, const double *B,
+                    int64_t ldb, double *R, double *rowabs, hipStream_t s)
+{
+    const int nb = (n + 63) / 64, ns = (nb + LX_SY_SB - 1) / LX_SY_SB;
+    const int nrp = nrhs == 1 ? 1 : 4;
+    double *P = (double *)lx_scratch(8, sizeof(double) * (size_t)ns * ns * LX_SY_SB * LX_SY_BS * (nrp + 1));
+    if (!P) return LINOMPSS_ERR_NOMEM;
+    for (int c0 = 0; c0 < nrhs; c0 += nrp) {
+        const int nc = std::min(nrp, nrhs - c0);
+        double *ra = c0 == 0 ? rowabs : nullptr;
+        if (nrp == 1)
+            residual_nr<1>(n, nc, A, lda, X + c0 * ldx, ldx, B + c0 * ldb, ldb, R + (int64_t)c0 * n, ra, P, s);
+        else
+            residual_nr<4>(n, nc, A, lda, X + c0 * ldx, ldx, B + c0 * ldb, ldb, R + (int64_t)c0 * n, ra, P, s);
     }
     LX_HIP_TRY(hipGetLastError());
     return 0;
@@ -249,9 +316,8 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     char *ws = (char *)lx_scratch(5, sizeof(double) * (3 * nr + 2 * (size_t)n + 64) + sizeof(float) * nr + 1024);
     if (!Af || !ws) return LINOMPSS_ERR_NOMEM;
     double *R = (double *)ws;
-    double *Y1 = R + nr, *Y2 = Y1 + nr;
-    double *rowabs = Y2 + nr, *colabs = rowabs + n;
-    double *small = colabs + n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
+    double *rowabs = R + nr;
+    double *small = rowabs + n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
     float *Sf = (float *)(small + 128);
     double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32, *bmax = small + 1 + 48;
     if (nrhs > 16) {
@@ -285,7 +351,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, bmax, dX, ldx,
                            0);
         for (int it = 0;; ++it) {
-            rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, Y1, Y2, it == 0 ? rowabs : nullptr, colabs, s);
+            rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, it == 0 ? rowabs : nullptr, s);
             if (rc) return rc;
             if (it == 0) hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
             hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
@@ -328,7 +394,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         rc = linompss_posv_dev(0, n, nrhs, b, Ad, n, dX, ldx, s, &eo);
         if (rc < 0) return rc;
         info = rc;
-        rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, Y1, Y2, rowabs, colabs, s);
+        rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
         if (rc) return rc;
         hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
         hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
@@ -391,11 +457,11 @@ extern "C" int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda
     const size_t nr = (size_t)n * nrhs;
     char *ws = (char *)lx_scratch(7, sizeof(double) * (3 * nr + 2 * (size_t)n + 128));
     if (!ws) return LINOMPSS_ERR_NOMEM;
-    double *R = (double *)ws, *Y1 = R + nr, *Y2 = Y1 + nr, *rowabs = Y2 + nr, *colabs = rowabs + n;
-    double *small = colabs + n;
+    double *R = (double *)ws, *rowabs = R + nr;
+    double *small = rowabs + n;
     double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32;
     hipStream_t s = nullptr;
-    int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, Y1, Y2, rowabs, colabs, s);
+    int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
     if (rc) return rc;
     hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
     hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
