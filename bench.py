@@ -52,6 +52,45 @@ def cpu_baseline(n, b, threads):
                       f"1 RHS, {t:.2f} s, correct={ok}"}
 
 
+def traffic_from(fetch_csv, write_csv, out, n):
+    """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
+    same counter reads for the device copy of A (lx_copy_kernel: n*n*8 bytes
+    read and written) in the same run."""
+    import csv
+    known_kib = n * n * 8 / 1024.0
+
+    def read(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copy = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("lx_copy_kernel")]
+        fac = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("void lx_exec_kernel<float, 0>")]
+        if not copy or not fac:
+            raise SystemExit(f"{path}: no lx_copy_kernel / FP32 factor dispatch with {counter}")
+        return max(copy), sum(fac) / len(fac), len(fac)
+
+    fc, ff, nf = read(fetch_csv, "FETCH_SIZE")
+    wc, wf, nw = read(write_csv, "WRITE_SIZE")
+    fs, ws = known_kib / fc, known_kib / wc
+    res = {
+        "kernel": "lx_exec_kernel<float, 0> (FP32 tiled factor, one persistent launch)",
+        "workload": f"n={n}, b=512, FP32 factor inside the mixed-precision solve",
+        "copy_calibration": {"bytes": n * n * 8, "fetch_kib": fc, "write_kib": wc,
+                             "fetch_scale": fs, "write_scale": ws},
+        "factor_fetch_kib": ff, "factor_write_kib": wf, "dispatches": [nf, nw],
+        "hbm_read_bytes": ff * fs * 1024, "hbm_write_bytes": wf * ws * 1024,
+        "hbm_bytes_per_launch": round((ff * fs + wf * ws) * 1024),
+        "commands": [
+            "rocprofv3 --pmc FETCH_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
+            "--warmup 0 --fp64-steps 1 --no-cpu",
+            "rocprofv3 --pmc WRITE_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
+            "--warmup 0 --fp64-steps 1 --no-cpu",
+            "python3 bench.py --traffic-from <fetch>/run_counter_collection.csv "
+            "<write>/run_counter_collection.csv --traffic-out profiles/traffic_fp32_factor_n32768.json"],
+    }
+    Path(out).write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +105,13 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
+    ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the tiled Cholesky path here runs on a single MI355X "
@@ -97,11 +142,14 @@ def main():
         log(f"warmup {w}: sweeps={st.sweeps} fallback={st.fallback} berr={st.berr:.2e} "
             f"factor={st.t_factor_s * 1e3:.1f} ms total={st.t_total_s * 1e3:.1f} ms")
     lx.lib.linompss_dev_sync()
-    t_fac, sweeps, berrs, fallbacks = [], [], [], []
+    t_fac, t_launch, sweeps, berrs, fallbacks = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = lx.dposv_ir_dev(A, B, X, b)
         t_fac.append(st.t_factor_s)
+        # HIP events around the FP32 factor launch, on its stream (the last
+        # executor launch of the solve; the sweeps and residuals use other kernels)
+        t_launch.append(lx.last_exec_stats().exec_ms * 1e-3)
         sweeps.append(st.sweeps)
         berrs.append(st.berr)
         fallbacks.append(st.fallback)
@@ -130,12 +178,13 @@ def main():
     A2.free()
     X2.free()
 
+    t_kernel = float(np.mean(t_launch))
     roofline = {
         "bound": "mfma",
-        "achieved": round(flops / float(np.mean(t_fac)) / 1e12, 3),
+        "achieved": round(flops / t_kernel / 1e12, 3),
         "peak": PEAK["fp32"],
         "unit": "TFLOP/s",
-        "frac": round(flops / float(np.mean(t_fac)) / 1e12 / PEAK["fp32"], 4),
+        "frac": round(flops / t_kernel / 1e12 / PEAK["fp32"], 4),
         "traffic": None,
     }
     tf = ROOT / "profiles" / "traffic_fp32_factor_n32768.json"
@@ -176,6 +225,7 @@ def main():
         "sweeps": int(np.max(sweeps)),
         "fallback": int(np.max(fallbacks)),
         "factor_ms": round(float(np.mean(t_fac)) * 1e3, 3),
+        "factor_launch_ms": round(t_kernel * 1e3, 3),
         "fp64_posv_ms": round(t_fp64 * 1e3, 3),
         "fp64_posv_gflops": round(flops / t_fp64 / 1e9, 2),
         "fp64_berr": berr64,

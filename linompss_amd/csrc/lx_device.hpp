@@ -177,23 +177,30 @@ __device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)
 {
     const int lane = lx_lane();
     const int lr = lane & 15, lk = lane >> 4;
+    // one row of tiles at a time, all its loads before any of its stores: a
+    // store may alias a later load as far as the compiler knows, so load/store
+    // pairs written interleaved would each wait a full memory round trip
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        T v[FN][4];
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            int c = c0 + j * 16 + lr;
-            if (c >= N) continue;
+            const int c = c0 + j * 16 + lr, cc = c < N ? c : N - 1;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int r = r0 + i * 16 + lx_mfma<T>::row(lk, e);
-                if (r < M && (!lower || r + gdiag >= c)) {
-                    T *p = C + (long long)c * ldc + r;
-                    *p = *p - acc[i][j][e];
-                }
+                const int r = r0 + i * 16 + lx_mfma<T>::row(lk, e), rr = r < M ? r : M - 1;
+                v[j][e] = C[(long long)cc * ldc + rr];
             }
         }
-        // one row of tiles at a time: keeps the loaded C values from piling up
-        // on top of the accumulators
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int c = c0 + j * 16 + lr;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + i * 16 + lx_mfma<T>::row(lk, e);
+                if (r < M && c < N && (!lower || r + gdiag >= c)) C[(long long)c * ldc + r] = v[j][e] - acc[i][j][e];
+            }
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
 }
