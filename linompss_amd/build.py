@@ -84,6 +84,9 @@ def build_oracle() -> Path:
     """Builds the CPU oracle (test infrastructure) with its Makefile."""
     odir = PKG.parent / "oracle"
     _run(["make", "-s", "-C", str(odir), "-B"])
+    # the reference's own sources, compiled in place (only where they exist)
+    if Path("/root/reference/src/cholesky/dense/chol_rlmain.c").exists():
+        _run(["make", "-s", "-C", str(odir), "-B", "ref"])
     return odir / "_build" / "liblx_oracle.so"
 
 

@@ -41,3 +41,20 @@ def dev():
     if not gpu_available():
         pytest.skip("no GPU")
     return 0
+
+
+REF_LIB = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "libref.so"
+
+
+@pytest.fixture(scope="session")
+def ref():
+    """The reference's own path compiled from its sources (oracle/_ref), when built."""
+    import ctypes
+    if not REF_LIB.exists():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    lib = ctypes.CDLL(str(REF_LIB), mode=ctypes.RTLD_GLOBAL)
+    for nm in ("ompss_dchol_rl", "ompss_dchol_ll", "ompss_schol_rl", "ompss_schol_ll"):
+        getattr(lib, nm).argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p, ctypes.c_int]
+    lib.dspdsolv_main.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+    lib.sspdsolv_main.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+    return lib
