@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--update-prec", type=int, default=0, choices=(0, 1),
+                    help="FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores, 1 split-BF16 matrix cores")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
@@ -137,15 +139,16 @@ def main():
         f"cond2~{(lmax + alpha) / alpha:.3g}")
 
     # ---- mixed-precision refinement (the metric) ----
+    ir_opts = lx.IrOpts(30, 1e-14, 0.5, args.update_prec)
     for w in range(args.warmup):
-        st = lx.dposv_ir_dev(A, B, X, b)
+        st = lx.dposv_ir_dev(A, B, X, b, ir_opts)
         log(f"warmup {w}: sweeps={st.sweeps} fallback={st.fallback} berr={st.berr:.2e} "
             f"factor={st.t_factor_s * 1e3:.1f} ms total={st.t_total_s * 1e3:.1f} ms")
     lx.lib.linompss_dev_sync()
     t_fac, t_launch, sweeps, berrs, fallbacks = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        st = lx.dposv_ir_dev(A, B, X, b)
+        st = lx.dposv_ir_dev(A, B, X, b, ir_opts)
         t_fac.append(st.t_factor_s)
         # HIP events around the FP32 factor launch, on its stream (the last
         # executor launch of the solve; the sweeps and residuals use other kernels)
@@ -215,9 +218,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factor + f64 refinement",
+        "dtype": ("f32 factor (trailing SYRK/GEMM on bf16 matrix cores, split operands hi+mid+lo, f32 "
+                  "accumulation) + f64 refinement") if args.update_prec == 1 else "f32 factor + f64 refinement",
         "data": "synthetic: A = G G^T/n + alpha I, G ~ N(0,1) (Philox4x32-10, seed %d), B ~ U[-1,1)" % args.seed,
         "config": {"workload": "FP32-factor + FP64 iterative refinement, SPD solve to backward error 1e-14",
+                   "update_prec": ["fp32", "split-bf16"][args.update_prec],
                    "n": n, "b": b, "cond2": args.kappa, "nrhs": nr, "alpha": alpha, "lambda_max": lmax},
         "roofline": roofline,
         "cpu_baseline": cpu,

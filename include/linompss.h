@@ -74,7 +74,8 @@ typedef struct {
     int max_sweeps;       /* corrections before falling back (default 30) */
     double tol;           /* backward-error target (default 1e-14) */
     double stall_ratio;   /* ||r_k+1|| > stall_ratio ||r_k|| is a stall (default 0.5) */
-    int update_prec;      /* trailing-update inputs: 0 FP32, 1 BF16, 2 FP16 (default 0) */
+    int update_prec;      /* FP32 factor's trailing updates: 0 FP32 (default), 1 split BF16
+                             (see linompss_exec_opts.update_prec) */
 } linompss_ir_opts;
 
 typedef struct {
@@ -102,6 +103,10 @@ typedef struct {
     double timeout_s;     /* per-dependency wait limit on the device (default 30 s) */
     int fused_solve;      /* posv: 1 = factor and sweeps in one persistent launch (the full
                              task DAG); 0 (default) = factor launch + sweep kernels */
+    int update_prec;      /* FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores (default);
+                             1 BF16 matrix cores on split operands (each FP32 input as
+                             hi + mid + lo BF16, six products, FP32 accumulation: FP32-level
+                             accuracy at the BF16 rate); ignored for FP64 */
 } linompss_exec_opts;
 
 typedef struct {

@@ -58,7 +58,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 class ExecOpts(ctypes.Structure):
     _fields_ = [("policy", _i), ("unit_m", _i), ("unit_n", _i), ("wg_per_cu", _i), ("timeout_s", _d),
-                ("fused_solve", _i)]
+                ("fused_solve", _i), ("update_prec", _i)]
 
 
 class ExecStats(ctypes.Structure):

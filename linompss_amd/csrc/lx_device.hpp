@@ -344,6 +344,179 @@ __device__ __forceinline__ void lx_unit_gemm_lds(T *__restrict__ C, long long ld
 }
 
 // ---------------------------------------------------------------------------
+// Trailing update of the FP32 factor on the BF16 matrix cores, split operands:
+// every FP32 input x is written as hi + mid + lo, three BF16 numbers (8
+// significant bits each, the FP32 exponent range, so no scaling is needed),
+// and A B^T is accumulated in FP32 from the six products whose size is at
+// least 2^-16 of the leading one (lo*hi, hi*lo, mid*mid, mid*hi, hi*mid,
+// hi*hi, smallest first): the dropped terms (mid*lo, lo*mid, lo*lo) are below
+// 2^-24 relative, so the update carries FP32 accuracy at the BF16 matrix rate
+// (2.5 PFLOP/s dense, six products = 16x the MFMA work of one FP32 pass at
+// 1/16 of its cost per flop).
+//
+// Same contract as lx_unit_gemm_lds (M, N <= 128, T = float).  K-slabs of 32:
+// each thread loads 16 consecutive k of one row of A and of B, splits them in
+// registers and writes the three planes to LDS as [row][k] (row stride 40
+// elements: the eight rows of a 128-byte LDS access fall on disjoint banks);
+// the loads of slab s+1 are in flight while slab s is multiplied.  Waves own
+// 64 x 64 of C as 2 x 2 tiles of v_mfma_f32_32x32x16_bf16 (measured gfx950
+// layout: lane l supplies A[l%32][8(l/32)+e], B[8(l/32)+e][l%32], e < 8;
+// result value i of lane l is row 8(i/4) + 4(l/32) + i%4, column l%32).
+typedef __bf16 lx_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float lx_f16v __attribute__((ext_vector_type(16)));
+#define LX_BX_KS 32
+#define LX_BX_STR 40
+struct lx_bx_smem {
+    static constexpr int bytes = 2 * 3 * 128 * LX_BX_STR * 2;  // two operands, three planes
+};
+
+__device__ __forceinline__ void lx_bx_split8(const float (&x)[8], lx_bf16x8 &h, lx_bf16x8 &m, lx_bf16x8 &l)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 bh = (__bf16)x[e];
+        const float r1 = x[e] - (float)bh;
+        const __bf16 bm = (__bf16)r1;
+        const float r2 = r1 - (float)bm;
+        h[e] = bh;
+        m[e] = bm;
+        l[e] = (__bf16)r2;
+    }
+}
+
+__device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long long ldc, const float *__restrict__ A,
+                                                 long long lda, const float *__restrict__ B, long long ldb, int M,
+                                                 int N, int K, bool lower, int gdiag, char *smem)
+{
+    constexpr int KS = LX_BX_KS, STR = LX_BX_STR;
+    // planes: [operand][plane][row][k]
+    __bf16(*sP)[3][128][STR] = reinterpret_cast<__bf16(*)[3][128][STR]>(smem);
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+    const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
+    const int nslab = (K + KS - 1) / KS;
+    // staging: thread -> (row, 16 consecutive k), rows clamped (never stored), k past K zero
+    const int srow = tid & 127, sk = (tid >> 7) * 16;
+    const float *pa = A + (srow < M ? srow : M - 1);
+    const float *pb = B + (srow < N ? srow : N - 1);
+    float xa[16], xb[16];
+    const bool kfull = K % KS == 0;  // uniform
+    auto gload = [&](int k0) {
+        if (kfull) {
+            // plain loads: their results are first used after the MFMAs of the
+            // current slab, so the wait for them lands there
+            const float *qa = pa + (long long)(k0 + sk) * lda, *qb = pb + (long long)(k0 + sk) * ldb;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                xa[j] = qa[(long long)j * lda];
+                xb[j] = qb[(long long)j * ldb];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int k = k0 + sk + j;
+                const bool kv = k < K;
+                const long long kc = kv ? k : K - 1;
+                const float va = pa[kc * lda], vb = pb[kc * ldb];
+                xa[j] = kv ? va : 0.f;
+                xb[j] = kv ? vb : 0.f;
+            }
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            float ta[8], tb[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                ta[e] = xa[half * 8 + e];
+                tb[e] = xb[half * 8 + e];
+            }
+            lx_bf16x8 h, m, l;
+            lx_bx_split8(ta, h, m, l);
+            *(lx_bf16x8 *)&sP[0][0][srow][sk + half * 8] = h;
+            *(lx_bf16x8 *)&sP[0][1][srow][sk + half * 8] = m;
+            *(lx_bf16x8 *)&sP[0][2][srow][sk + half * 8] = l;
+            lx_bx_split8(tb, h, m, l);
+            *(lx_bf16x8 *)&sP[1][0][srow][sk + half * 8] = h;
+            *(lx_bf16x8 *)&sP[1][1][srow][sk + half * 8] = m;
+            *(lx_bf16x8 *)&sP[1][2][srow][sk + half * 8] = l;
+        }
+    };
+    lx_f16v acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+    const int lr = lane & 31, lk = lane >> 5;
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int sl = 0; sl < nslab; ++sl) {
+        if (sl + 1 < nslab) gload((sl + 1) * KS);  // in flight during the MFMAs below
+        if (active) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ks += 16) {
+                lx_bf16x8 a[2][3], b[2][3];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[0][p][wr + i * 32 + lr][ks + 8 * lk];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[1][p][wc + j * 32 + lr][ks + 8 * lk];
+                // product-major: the four tiles' accumulators are independent, so
+                // consecutive MFMAs never wait on each other's result
+                constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][j], 0,
+                                                                                0, 0);
+            }
+        }
+        if (sl + 1 < nslab) {
+            __syncthreads();  // every wave is done with the planes of slab sl
+            lstore();
+            __syncthreads();
+        }
+    }
+    if (active) {
+        // C -= acc, one row of tiles at a time, its loads before its stores
+        const int lane2 = lx_lane(), lr2 = lane2 & 31, lk2 = lane2 >> 5;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            float v[2][16];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int c = wc + j * 32 + lr2, cc = c < N ? c : N - 1;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4), rr = r < M ? r : M - 1;
+                    v[j][e] = C[(long long)cc * ldc + rr];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int c = wc + j * 32 + lr2;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4);
+                    if (r < M && c < N && (!lower || r + gdiag >= c)) C[(long long)c * ldc + r] = v[j][e] - acc[i][j][e];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Wave-level C_acc += A * B^T with A from global memory (rows r0.. of a
 // column-major panel, clamped past M) and B^T from LDS: element (k, c) of B^T
 // is sB[c * ldsb + k] (row c of a row-major 64 x K block).  16 rows x 16*FN

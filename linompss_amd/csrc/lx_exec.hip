@@ -129,6 +129,7 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         P.trsm_rate *= 1.5;
     }
     if (o) P.policy = o->policy;
+    if (o && dtype == 1 && o->update_prec == 1) P.update_prec = 1;
 }
 
 bool build_plan(lx_planner &pl, const lx_shape &sh)
@@ -160,8 +161,8 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
     lx_plan_params P;
     fill_params(P, sh.dtype, workers, o);
     char key[256];
-    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs,
-             P.policy, P.unit_m, P.unit_n, P.workers);
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs,
+             P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec);
     for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
         if (it->key == key) {
             c->plans.splice(c->plans.begin(), c->plans, it);

@@ -25,7 +25,8 @@ template <typename T, int FAM> struct lx_fam_smem {
     // POTRF (2 * 64 * 65 * sizeof(T) + 16; its trailing update reuses the space
     // for the staged GEMM), TRSM (2 * 64 * 65 * sizeof(T)), staged GEMM
     static constexpr int potrf = 2 * 64 * 65 * (int)sizeof(T) + 16;
-    static constexpr int factor = potrf > lx_lgemm_smem<T>::bytes ? potrf : lx_lgemm_smem<T>::bytes;
+    static constexpr int gemm = lx_lgemm_smem<T>::bytes > lx_bx_smem::bytes ? lx_lgemm_smem<T>::bytes : lx_bx_smem::bytes;
+    static constexpr int factor = potrf > gemm ? potrf : gemm;
     static constexpr int solve = 42 * 1024;
     static constexpr int bytes = FAM == LX_FAM_FACTOR ? factor : FAM == LX_FAM_SOLVE ? solve : (factor > solve ? factor : solve);
 };
@@ -73,11 +74,25 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     case LX_OP_SYRK:
     case LX_OP_GEMM_NT:
         // a unit covers up to M = 128 rows and N = 128 * m columns, done as m
-        // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0))
-        if constexpr (F)
-            for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N)  // uniform
-                lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M,
-                                    u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N, u.K, u.aux0 != 0, u.aux1 - c0, smem);
+        // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0));
+        // aux0 bit 0: lower mask (SYRK), bit 1: split-BF16 matrix cores (FP32 only)
+        if constexpr (F) {
+            const bool lower = (u.aux0 & 1) != 0;
+            bool bx = false;
+            if constexpr (sizeof(T) == 4) bx = (u.aux0 & 2) != 0;
+            for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N) {  // uniform
+                const int nc = u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N;
+                if constexpr (sizeof(T) == 4) {
+                    if (bx) {
+                        lx_unit_gemm_bx6(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                                         u.aux1 - c0, smem);
+                        continue;
+                    }
+                }
+                lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                                    u.aux1 - c0, smem);
+            }
+        }
         break;
     case LX_OP_TRSV:
         if constexpr (S) lx_unit_trsv_block<T>(C, ldc, Ap, lda, u.M, u.N, u.aux0, smem);

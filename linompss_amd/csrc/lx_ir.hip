@@ -307,7 +307,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     if (n == 0 || nrhs == 0) return 0;
     linompss_ir_opts o = {30, 1e-14, 0.5, 0};
     if (opts) o = *opts;
-    if (o.update_prec != 0) return -11;  // low-precision trailing updates are not built yet
+    if (o.update_prec != 0 && o.update_prec != 1) return -11;  // 0 FP32, 1 split-BF16 trailing updates
     std::lock_guard<std::mutex> g(g_ir_mu);
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
@@ -330,7 +330,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         hipLaunchKernelGGL(lx_d2s_lower_kernel, grid, dim3(256), 0, s, dA, lda, n, Af);
         LX_HIP_TRY(hipGetLastError());
     }
-    linompss_exec_opts eo = {1, 128, 128, 0, 30.0, 0};
+    linompss_exec_opts eo = {1, 128, 128, 0, 30.0, 0, o.update_prec};
     int info = 0;
     int rc = linompss_chol_dev(1, 0, n, b, Af, n, s, &eo);
     linompss_exec_stats es;

@@ -245,6 +245,7 @@ void lx_planner::factor_rl(int s)
                 for (int c0 = 0; c0 < jbs; c0 += un)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         lx_unit u = proto_unit(LX_OP_GEMM_NT, s);
+                        u.aux0 = upd_flag();
                         u.M = std::min(P.unit_m, ibs - r0);
                         u.N = std::min(un, jbs - c0);
                         u.K = bs;
@@ -265,6 +266,7 @@ void lx_planner::factor_rl(int s)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         if (r0 + P.unit_m - 1 < c0) continue;  // strictly upper
                         lx_unit u = proto_unit(LX_OP_SYRK, s);
+                        u.aux0 = upd_flag();
                         u.M = std::min(P.unit_m, ibs - r0);
                         u.N = std::min(P.unit_n, ibs - c0);
                         u.K = bs;
@@ -272,7 +274,7 @@ void lx_planner::factor_rl(int s)
                         u.c = rc(i * b + r0, i * b + c0);
                         u.a = rc(i * b + r0, k * b);
                         u.b = rc(i * b + c0, k * b);
-                        u.aux0 = 1;
+                        u.aux0 |= 1;
                         u.aux1 = r0 - c0;
                         add_unit(t, u);
                     }
@@ -297,6 +299,7 @@ void lx_planner::factor_ll(int s)
                 for (int r0 = 0; r0 < bs; r0 += P.unit_m) {
                     if (r0 + P.unit_m - 1 < c0) continue;
                     lx_unit u = proto_unit(LX_OP_SYRK, s);
+                    u.aux0 = upd_flag();
                     u.M = std::min(P.unit_m, bs - r0);
                     u.N = std::min(P.unit_n, bs - c0);
                     u.K = b;
@@ -304,7 +307,7 @@ void lx_planner::factor_ll(int s)
                     u.c = rc(k * b + r0, k * b + c0);
                     u.a = rc(k * b + r0, j * b);
                     u.b = rc(k * b + c0, j * b);
-                    u.aux0 = 1;
+                    u.aux0 |= 1;
                     u.aux1 = r0 - c0;
                     add_unit(t, u);
                 }
@@ -327,6 +330,7 @@ void lx_planner::factor_ll(int s)
                 for (int c0 = 0; c0 < bs; c0 += P.unit_n)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         lx_unit u = proto_unit(LX_OP_GEMM_NT, s);
+                        u.aux0 = upd_flag();
                         u.M = std::min(P.unit_m, ibs - r0);
                         u.N = std::min(P.unit_n, bs - c0);
                         u.K = b;

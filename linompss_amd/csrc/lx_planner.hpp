@@ -20,6 +20,7 @@ struct lx_plan_params {
     int unit_n = 128;       // GEMM / SYRK unit cols
     int gemm_wide_n = 512;  // GEMM unit cols off the lookahead column (one fence per 128 x 512)
     int potrf_sub = 128;    // diagonal tiles wider than this factor as a sub-DAG of sub-tiles of this size
+    int update_prec = 0;    // 1: trailing GEMM/SYRK units of the tile loop on split-BF16 matrix cores
     int trsm_rows = 64;     // panel TRSM rows per unit
     int gemv_rows = 128;    // RHS GEMV output rows per unit
     int policy = 1;         // 0: reference priorities + FIFO, 1: critical-path (bottom level)
@@ -87,6 +88,8 @@ class lx_planner {
     // each other, created in a topological order).
     void add_unit_deps(int task, const lx_unit &proto, const std::vector<std::pair<int32_t, int32_t>> &deps);
     int alloc_counters(int count);
+    // aux0 flag of the tile loop's GEMM/SYRK units: 2 = split-BF16 matrix cores
+    int upd_flag() const { return P.update_prec == 1 ? 2 : 0; }
     // Units of the diagonal-tile factorisation POTRF(k) of task t (its output
     // counter outc, global column offset k*b, width bs).
     void potrf_units(int t, int s, int kb, int bs, int outc);

@@ -103,6 +103,39 @@ inline lx_emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, lx_emu_f
     return lx_emu_mfma<lx_emu_f4, float>(a, b, c);
 }
 
+// v_mfma_f32_32x32x16_bf16 with the measured gfx950 layout: lane l supplies
+// A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32]; result value i of lane l is row
+// 8(i/4) + 4(l/32) + i%4, column l%32.  Products are exact in FP32 (8-bit
+// significands), summed in FP64 then rounded, close to the hardware's
+// accumulation.
+typedef __bf16 lx_emu_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float lx_emu_f16v __attribute__((ext_vector_type(16)));
+struct lx_emu_bf_buf {
+    float a[64][8], b[64][8];
+};
+extern lx_emu_bf_buf lx_emu_bfbuf[4];
+inline lx_emu_f16v __builtin_amdgcn_mfma_f32_32x32x16_bf16(lx_emu_bf16x8 a, lx_emu_bf16x8 b, lx_emu_f16v c, int, int,
+                                                           int)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    lx_emu_bf_buf &B = lx_emu_bfbuf[w];
+    for (int e = 0; e < 8; ++e) {
+        B.a[lane][e] = (float)a[e];
+        B.b[lane][e] = (float)b[e];
+    }
+    lx_emu_wave_barrier[w].wait();
+    lx_emu_f16v d = c;
+    const int col = lane & 31;
+    for (int i = 0; i < 16; ++i) {
+        const int row = 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+        double acc = 0;
+        for (int k = 0; k < 16; ++k) acc += (double)B.a[(k / 8) * 32 + row][k % 8] * (double)B.b[(k / 8) * 32 + col][k % 8];
+        d[i] = (float)((double)c[i] + acc);
+    }
+    lx_emu_wave_barrier[w].wait();
+    return d;
+}
+
 template <typename T>
 inline T lx_emu_xchg(T v, int src_lane)
 {

@@ -22,6 +22,11 @@ lx_emu_dim3 blockDim, gridDim;
 lx_emu_barrier lx_emu_wg_barrier;
 lx_emu_barrier lx_emu_wave_barrier[4];
 lx_emu_wave_buf lx_emu_wbuf[4];
+lx_emu_bf_buf lx_emu_bfbuf[4];
+static int g_update_prec = 0;
+
+// trailing-update precision of the next lx_emu_run plans (0 FP32, 1 split BF16)
+extern "C" void lx_emu_set_update_prec(int u) { g_update_prec = u; }
 
 extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, long long lda, void *X, long long ldx,
                           int policy, int *info_out, int *done_out, int *nunits_out)
@@ -30,6 +35,7 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     P.unit_m = LX_UNIT_M;
     P.unit_n = LX_UNIT_N;
     P.policy = policy;
+    P.update_prec = dtype == 1 ? g_update_prec : 0;
     lx_planner pl(P);
     int s = pl.add_system(n, b, nrhs);
     switch (what) {

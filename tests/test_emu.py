@@ -47,6 +47,22 @@ def test_factor_fp64(emu, orc, what, n, b):
     assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
 
 
+def test_factor_fp32_split_bf16_updates(emu, orc):
+    """The FP32 factor with its tile-loop SYRK/GEMM units on the split-BF16
+    unit (lx_unit_gemm_bx6): FP32-level agreement with LAPACK in FP64."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    n, b = 300, 128
+    A0 = orc.genmat_spd(n, 4, diag_mode=1)
+    Af = A0.astype(np.float32, order="F")
+    E.lx_emu_set_update_prec(1)
+    try:
+        assert emu(1, 0, n, b, 0, Af) == 0
+    finally:
+        E.lx_emu_set_update_prec(0)
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(np.tril(Af) - Lref) / np.linalg.norm(Lref) <= 1e-6
+
+
 def test_factor_fp32_and_policy0(emu, orc):
     n, b = 97, 32
     A0 = orc.genmat_spd(n, 2, diag_mode=1)

@@ -103,3 +103,19 @@ def test_device_api_factor(lx, orc, dev):
     assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) <= 1e-12
     st = lx.last_exec_stats()
     assert st.units > 0 and st.exec_ms > 0
+
+
+@pytest.mark.parametrize("n,b", [(700, 128), (2048, 512), (1989, 533)])
+def test_schol_split_bf16_updates(lx, orc, dev, n, b):
+    """FP32 factor with the tile loop's SYRK/GEMM on split-BF16 matrix cores
+    (update_prec = 1): FP32-level agreement with LAPACK (FP64), like the FP32
+    matrix-core path."""
+    print(f"[schol bf16x6] n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=6, diag_mode=1)
+    dA = lx.DeviceMatrix.from_host(A0.astype(np.float32, order="F"))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 1)
+    assert lx.chol_dev(dA, b, "rl", opts) == 0
+    L = np.tril(dA.to_host().astype(np.float64))
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) <= 1e-5
+    dA.free()

@@ -108,3 +108,22 @@ def test_device_posv_single_launch(lx, orc, dev):
     assert lx.posv_dev(dA, dX, b) == 0
     X = dX.to_host()
     assert _berr(A0, X, B0) <= 1e-14
+
+
+@pytest.mark.parametrize("n,nr,b,alpha", [(1000, 1, 256, 1e-3), (2048, 4, 512, 1e-4)])
+def test_dposv_ir_split_bf16_updates(lx, orc, dev, n, nr, b, alpha):
+    """Refinement with the FP32 factor's trailing SYRK/GEMM on split-BF16
+    matrix cores (update_prec = 1): same bar as the FP32 factor."""
+    print(f"[ir bf16x6] n={n} nr={nr} b={b}", flush=True)
+    A = _spd_gauss(n, 3, alpha)
+    rng = np.random.default_rng(4)
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, nr)))
+    X, st = lx.dposv_ir(A, B, b, lx.IrOpts(30, 1e-14, 0.5, 1))
+    assert st.info == 0
+    assert st.fallback == 0
+    assert st.berr <= 1e-14
+    assert _berr(A, X, B) <= 1e-14
+    # two solutions with backward error <= 1e-14 differ by at most ~2 cond(A) 1e-14
+    # (cond ~ (4 + alpha) / alpha here)
+    Xo, sto, info = orc.dposv_ir(A, B, b)
+    assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 2.5 * (4 + alpha) / alpha * 1e-14
