@@ -399,17 +399,18 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
     const int srow = tid & 127, sk = (tid >> 7) * 16;
     const float *pa = A + (srow < M ? srow : M - 1);
     const float *pb = B + (srow < N ? srow : N - 1);
-    float xa[16], xb[16];
+    // two register sets: the loads of slab s+2 are in flight while slab s is
+    // multiplied and slab s+1 waits in the other set
+    float xa[2][16], xb[2][16];
     const bool kfull = K % KS == 0;  // uniform
-    auto gload = [&](int k0) {
+    auto gload = [&](int set, int k0) {
         if (kfull) {
-            // plain loads: their results are first used after the MFMAs of the
-            // current slab, so the wait for them lands there
+            // plain loads: their results are first used two slabs later
             const float *qa = pa + (long long)(k0 + sk) * lda, *qb = pb + (long long)(k0 + sk) * ldb;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                xa[j] = qa[(long long)j * lda];
-                xb[j] = qb[(long long)j * ldb];
+                xa[set][j] = qa[(long long)j * lda];
+                xb[set][j] = qb[(long long)j * ldb];
             }
         } else {
 #pragma unroll
@@ -418,19 +419,19 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
                 const bool kv = k < K;
                 const long long kc = kv ? k : K - 1;
                 const float va = pa[kc * lda], vb = pb[kc * ldb];
-                xa[j] = kv ? va : 0.f;
-                xb[j] = kv ? vb : 0.f;
+                xa[set][j] = kv ? va : 0.f;
+                xb[set][j] = kv ? vb : 0.f;
             }
         }
     };
-    auto lstore = [&]() {
+    auto lstore = [&](int set) {
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             float ta[8], tb[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                ta[e] = xa[half * 8 + e];
-                tb[e] = xb[half * 8 + e];
+                ta[e] = xa[set][half * 8 + e];
+                tb[e] = xb[set][half * 8 + e];
             }
             lx_bf16x8 h, m, l;
             lx_bx_split8(ta, h, m, l);
@@ -451,40 +452,52 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
     const int lr = lane & 31, lk = lane >> 5;
-    gload(0);
-    lstore();
-    __syncthreads();
-    for (int sl = 0; sl < nslab; ++sl) {
-        if (sl + 1 < nslab) gload((sl + 1) * KS);  // in flight during the MFMAs below
-        if (active) {
+    auto compute = [&]() {
+        if (!active) return;
 #pragma unroll
-            for (int ks = 0; ks < KS; ks += 16) {
-                lx_bf16x8 a[2][3], b[2][3];
+        for (int ks = 0; ks < KS; ks += 16) {
+            lx_bf16x8 a[2][3], b[2][3];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[0][p][wr + i * 32 + lr][ks + 8 * lk];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[1][p][wc + j * 32 + lr][ks + 8 * lk];
+            // product-major: the four tiles' accumulators are independent, so
+            // consecutive MFMAs never wait on each other's result
+            constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[0][p][wr + i * 32 + lr][ks + 8 * lk];
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-#pragma unroll
-                    for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[1][p][wc + j * 32 + lr][ks + 8 * lk];
-                // product-major: the four tiles' accumulators are independent, so
-                // consecutive MFMAs never wait on each other's result
-                constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
-#pragma unroll
-                for (int q = 0; q < 6; ++q)
-#pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][j], 0,
-                                                                                0, 0);
-            }
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][j], 0, 0, 0);
         }
+    };
+    gload(0, 0);
+    lstore(0);
+    __syncthreads();
+    if (nslab > 1) gload(0, KS);
+    if (nslab > 2) gload(1, 2 * KS);
+    // the loop is unrolled by two so that the register sets have fixed names:
+    // on entry slab sl is in LDS, slab sl+1 in set 0 and slab sl+2 in set 1
+    for (int sl = 0; sl < nslab; sl += 2) {
+        compute();  // slab sl
         if (sl + 1 < nslab) {
-            __syncthreads();  // every wave is done with the planes of slab sl
-            lstore();
             __syncthreads();
+            lstore(0);  // slab sl+1
+            __syncthreads();
+            if (sl + 3 < nslab) gload(0, (sl + 3) * KS);
+            compute();  // slab sl+1
+            if (sl + 2 < nslab) {
+                __syncthreads();
+                lstore(1);  // slab sl+2
+                __syncthreads();
+                if (sl + 4 < nslab) gload(1, (sl + 4) * KS);
+            }
         }
     }
     if (active) {

@@ -98,15 +98,56 @@ __device__ __forceinline__ void lx_sw_publish(int *p)
     }
 }
 
+// Mailbox of solved values: one slot per element of x, all-ones bits (a NaN
+// payload no arithmetic produces) until the solve unit writes it with an
+// atomic exchange, which is performed at the device's shared level at once.
+// A consumer polls its element until the bits change: the value arrives with
+// its own "ready" signal, one memory round trip and no release fence on the
+// chain of solve units.
+template <typename T> struct lx_sw_bits;
+template <> struct lx_sw_bits<float> { typedef unsigned int U; };
+template <> struct lx_sw_bits<double> { typedef unsigned long long U; };
+
+template <typename T>
+__device__ __forceinline__ void lx_sw_mb_put(T *mb, T v)
+{
+    typedef typename lx_sw_bits<T>::U U;
+    __hip_atomic_exchange((U *)mb, __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-lane wait (a lane loop with no barrier inside); bounded like every wait.
+template <typename T>
+__device__ __forceinline__ T lx_sw_mb_get(const T *mb, int *abort, long long spin_ticks)
+{
+    typedef typename lx_sw_bits<T>::U U;
+    U v = __hip_atomic_load((const U *)mb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v == ~U(0)) {
+        const long long t0 = wall_clock64();
+        for (;;) {
+            v = __hip_atomic_load((const U *)mb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v != ~U(0)) break;
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (wall_clock64() - t0 > spin_ticks) {
+                atomicOr(abort, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __builtin_bit_cast(T, v);
+}
+
 // trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X (n x nrhs,
 // nrhs <= NR <= 16) is overwritten in place.  ctl = [ticket, abort, pad...,
-// flags[nb], pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64
-// values.
+// unused[nb], pcnt[nb]] must be zero on entry; part holds rowbase[nb] * 16 * 64
+// values; mb (leading dimension ldmb >= 64 nb, nrhs columns) must hold
+// all-ones bits on entry.
 template <typename T, int TRANS, int NR>
 __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T *__restrict__ L, int64_t ldl,
                                                        T *__restrict__ X, int64_t ldx, const int *__restrict__ units,
                                                        int nunits, const int *__restrict__ rowbase, int C,
-                                                       T *__restrict__ part, int *ctl, long long spin_ticks)
+                                                       T *__restrict__ part, int *ctl, T *__restrict__ mb,
+                                                       int64_t ldmb, long long spin_ticks)
 {
     constexpr int KC = 128 / sizeof(T);  // backward partial sums: L rows per LDS slab
     constexpr int S_BYTES = 3 * 64 * 65 * (int)sizeof(T);
@@ -115,7 +156,7 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
     __shared__ T svec[2][64][NR + 1];  // S unit: [0] = b, then y; [1] = a solved block's x
     __shared__ int s_unit;
     const int nb = (n + 63) / 64;
-    int *ticket = ctl, *abort = ctl + 1, *flags = ctl + 64, *pcnt = ctl + 64 + nb;
+    int *ticket = ctl, *abort = ctl + 1, *pcnt = ctl + 64 + nb;
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
     for (;;) {
         if (w == 0) {
@@ -191,11 +232,12 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                 const int ip = TRANS ? ib + d : ib - d;
                 const int hp = min(64, n - ip * 64);
                 T(*sB)[65] = d == 2 ? sA : sC;
-                if (w == 0) lx_sw_wait_ge(&flags[s - d], 1, abort, spin_ticks);  // wave-uniform
-                __syncthreads();  // also: svec[1] is free, sB is loaded
+                __syncthreads();  // svec[1] is free, sB is loaded
                 for (int e = tid; e < 64 * NR; e += LX_WG_THREADS) {
                     const int k = e % 64, cc = e / 64;
-                    svec[1][k][cc] = (k < hp && cc < nrhs) ? X[(int64_t)cc * ldx + ip * 64 + k] : T(0);
+                    svec[1][k][cc] = (k < hp && cc < nrhs)
+                                         ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + ip * 64 + k, abort, spin_ticks)
+                                         : T(0);
                 }
                 __syncthreads();
                 for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
@@ -217,12 +259,11 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                     x0 += (TRANS ? sZ[k][lane] : sZ[lane][k]) * svec[0][k][cc];
                     x1 += (TRANS ? sZ[k + 1][lane] : sZ[lane][k + 1]) * svec[0][k + 1][cc];
                 }
-                if (lane < hb) X[(int64_t)cc * ldx + r0 + lane] = x0 + x1;
+                if (lane < hb) {
+                    X[(int64_t)cc * ldx + r0 + lane] = x0 + x1;
+                    lx_sw_mb_put<T>(mb + (int64_t)cc * ldmb + r0 + lane, x0 + x1);
+                }
             }
-            // published with a read-modify-write: it is performed at the shared level
-            // at once, where a plain atomic store can sit in this XCD's L2 for
-            // microseconds before other XCDs see it
-            lx_sw_publish(&flags[s]);
         } else {
             // ---------------- P(s, c) ----------------
             T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + w * 64 * (KC + 1) * sizeof(T));
@@ -256,8 +297,9 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                             lv[m * (64 / CPI) + q] = L[(int64_t)colg * ldl + rowg];
                         }
                 }
-                lx_sw_wait_ge(&flags[t], 1, abort, spin_ticks);
-                for (int cc = 0; cc < nrhs; ++cc) sx[cc][lane] = lane < hj ? X[(int64_t)cc * ldx + c0 + lane] : T(0);
+                for (int cc = 0; cc < nrhs; ++cc)
+                    sx[cc][lane] = lane < hj ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + c0 + lane, abort, spin_ticks)
+                                             : T(0);
                 __builtin_amdgcn_wave_barrier();
                 if (!TRANS) {
 #pragma unroll
@@ -386,10 +428,15 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
     if (!d) return LINOMPSS_ERR_HIP;
     const size_t ctl_ints = 64 + 2 * (size_t)nb;
     const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
-    char *ws = (char *)lx_scratch(3, part_off + sizeof(T) * (size_t)std::max(1, pl->nparts) * LX_SW_NR * 64);
+    const size_t part_bytes = sizeof(T) * (size_t)std::max(1, pl->nparts) * LX_SW_NR * 64;
+    const size_t mb_off = (part_off + part_bytes + 255) / 256 * 256;
+    const int64_t ldmb = (int64_t)nb * 64;
+    const size_t mb_bytes = sizeof(T) * (size_t)ldmb * LX_SW_NR;
+    char *ws = (char *)lx_scratch(3, mb_off + mb_bytes);
     if (!ws) return LINOMPSS_ERR_NOMEM;
     int *ctl = (int *)ws;
     T *part = (T *)(ws + part_off);
+    T *mb = (T *)(ws + mb_off);
     double tmo = 30.0;
     if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
     const long long ticks = (long long)(tmo * d->wall_khz * 1000.0);
@@ -402,9 +449,10 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
             if (!(which & (1 << tr))) continue;
             const int grid = std::max(1, std::min(pl->nunits, d->cus * d->occ[ti][tr][ri]));
             lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
+            lx_fill_i32((int32_t *)mb, -1, (int64_t)(sizeof(T) * ldmb * nc / 4), s);  // all-ones: empty
 #define LX_SW_LAUNCH(TR, NR)                                                                                       \
     hipLaunchKernelGGL((lx_sweep_kernel<T, TR, NR>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,  \
-                       pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, ticks)
+                       pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, mb, ldmb, ticks)
             if (tr == 0) {
                 if (ri == 0) LX_SW_LAUNCH(0, 1);
                 else if (ri == 1) LX_SW_LAUNCH(0, 4);

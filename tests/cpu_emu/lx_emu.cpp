@@ -112,13 +112,15 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     lx_sw_build_units(nb, C, units, rowbase);
     std::vector<int> ctl(64 + 2 * nb, 0);
     std::vector<double> part((size_t)std::max(1, rowbase[nb]) * LX_SW_NR * 64);  // large enough for either type
+    const int64_t ldmb = (int64_t)nb * 64;
+    std::vector<unsigned long long> mbv((size_t)ldmb * LX_SW_NR, ~0ull);  // all-ones: empty (either type)
     const long long ticks = 100000000LL * 20;
     auto go = [&](auto tp, auto tr, auto nr) {
         using T = decltype(tp);
         run_wg([&] {
             lx_sweep_kernel<T, decltype(tr)::value, decltype(nr)::value>(
                 n, nrhs, (const T *)L, ldl, (T *)X, ldx, units.data(), (int)units.size(), rowbase.data(), C,
-                (T *)part.data(), ctl.data(), ticks);
+                (T *)part.data(), ctl.data(), (T *)mbv.data(), ldmb, ticks);
         });
     };
     using I0 = std::integral_constant<int, 0>;
@@ -132,6 +134,6 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     else { if (trans) by_nr(float(), I1()); else by_nr(float(), I0()); }
     if (ctl[1] != 0) return -3;  // a wait timed out
     for (int b = 0; b < nb; ++b)
-        if (ctl[64 + b] != 1 || ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;
+        if (ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;
     return ctl[0] == (int)units.size() + 1 ? 0 : -2;
 }
