@@ -10,10 +10,16 @@ Workload (the refinement configuration of the tiled Cholesky path):
   residual / update / test, FP64-factor fallback if refinement stalls.
   GFLOP/s = (n^3 / 3) / time-to-solution, inputs resident in HBM.
 
+  The factor's trailing SYRK/GEMM run on BF16 matrix cores with each FP32
+  operand split into three BF16 parts (hi + mid + lo carry all 24 bits) and
+  six partial products accumulated in FP32 (--update-prec 1, the default);
+  --update-prec 0 runs them on FP32 matrix cores.
+
 Beside it: the plain FP64 factor + sweeps (one persistent launch), the
-roofline of the dominant kernel (the FP32 factor launch, FP32 matrix peak),
-and the CPU baseline: the oracle's OpenMP-task restatement of the reference
-loop nest (sequential BLAS per tile task) on a bounded sample.
+roofline of the dominant kernel (the FP32 factor launch; with split-BF16
+updates its ceiling is the BF16 peak over the six products per FP32
+multiply-add), and the CPU baseline: the oracle's OpenMP-task restatement of
+the reference loop nest (sequential BLAS per tile task) on a bounded sample.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -52,6 +58,98 @@ def cpu_baseline(n, b, threads):
                       f"1 RHS, {t:.2f} s, correct={ok}"}
 
 
+def cpu_baseline_batched(sizes, mats, rhs, threads, sample=512):
+    """The oracle's refinement solve (dposv_ir, one tile per system) over the
+    first `sample` systems, one system per host thread at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    idx = list(range(min(sample, len(sizes))))
+
+    def one(i):
+        n = int(sizes[i])
+        X, st, info = oracle.dposv_ir(mats[i], rhs[i].reshape(-1, 1), max(n, 1), nthreads=1)
+        return info == 0 and st.berr <= 1e-14
+
+    t0 = oracle.wtime()
+    with ThreadPoolExecutor(threads) as ex:
+        ok = all(ex.map(one, idx))
+    t = oracle.wtime() - t0
+    return {"value": round(len(idx) / t, 1), "unit": "systems/s", "cores": threads, "kind": "port",
+            "sample": f"oracle refinement solve (FP32 LAPACK factor of one tile + FP64 residual/update, "
+                      f"itref.c:999-1101 contract) of the first {len(idx)} systems, one system per thread, "
+                      f"{t:.2f} s, all converged={ok}"}
+
+
+def bench_batched(args):
+    """Batched uneven SPD solves: systems/s for one persistent launch (a
+    replayed hipGraph) over 4096 systems, inputs resident in HBM."""
+    import numpy as np
+    import linompss_amd as lx
+    if lx.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(3)
+    cnt = args.batch_count
+    log(f"generating {cnt} systems, orders uniform in [32, 512], seed 2")
+    sizes, mats, rhs = lx.gen_batch_spd(cnt, 32, 512, seed=2)
+    batch = lx.DeviceBatch(sizes)
+    batch.upload(mats, rhs)
+    t0 = time.perf_counter()
+    plan = lx.BatchPlan(batch)  # autotuned schedule, captured graph
+    pinfo = plan.info()
+    log(f"plan: {pinfo} ({(time.perf_counter() - t0) * 1e3:.0f} ms incl. tuning)")
+    for w in range(args.warmup):
+        plan.run()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = plan.run()
+    t = (time.perf_counter() - t0) / args.steps
+    X = batch.solutions()
+    berr_np = max(float(np.abs(rhs[i] - mats[i] @ X[i]).max() / (np.abs(mats[i]).sum(1).max() * np.abs(X[i]).max()))
+                  for i in range(0, cnt, max(1, cnt // 256)))
+    n64 = sizes.astype(np.float64)
+    flops = float((n64 ** 3).sum() / 3.0)
+    abytes = float((12.0 * n64 * n64).sum())  # FP64 A + its FP32 copy, once
+    log(f"batched: {t * 1e3:.3f} ms per launch, {cnt / t:.0f} systems/s, max berr {st['berr'].max():.2e}, "
+        f"sweeps max {st['sweeps'].max()}, fallbacks {int(st['fallback'].sum())}")
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline_batched(sizes, mats, rhs, threads)
+            log(f"CPU baseline {cpu['value']} systems/s")
+        except Exception as e:
+            log(f"CPU baseline unavailable: {e}")
+    out = {
+        "metric": "batched uneven SPD solves to FP64 backward error 1e-14 (systems/s)",
+        "value": round(cnt / t, 1),
+        "unit": "systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factor + f64 refinement (per system)",
+        "data": "synthetic: A_i = G_i G_i^T / n_i + 0.1 I, G_i ~ N(0,1), b_i ~ U[-1,1) (numpy PCG64, seed 2)",
+        "config": {"workload": "batched uneven SPD solves, one persistent launch (hipGraph replay)",
+                   "count": cnt, "n_min": 32, "n_max": 512, "sum_n": int(sizes.sum()), "nrhs": 1,
+                   "schedule": pinfo},
+        "roofline": {"bound": "hbm", "achieved": round(abytes / t / 1e9, 1), "peak": PEAK["hbm"], "unit": "GB/s",
+                     "frac": round(abytes / t / 1e9 / PEAK["hbm"], 4), "traffic": None,
+                     "basis": "12 n_i^2 bytes per system (FP64 A and its FP32 copy, once)"},
+        "gflops": round(flops / t / 1e9, 1),
+        "cpu_baseline": cpu,
+        "berr_max": float(st["berr"].max()),
+        "berr_max_numpy_sample": berr_np,
+        "sweeps_max": int(st["sweeps"].max()),
+        "fallbacks": int(st["fallback"].sum()),
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    batch.free()
+
+
 def traffic_from(fetch_csv, write_csv, out, n):
     """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
@@ -73,7 +171,8 @@ def traffic_from(fetch_csv, write_csv, out, n):
     fs, ws = known_kib / fc, known_kib / wc
     res = {
         "kernel": "lx_exec_kernel<float, 0> (FP32 tiled factor, one persistent launch)",
-        "workload": f"n={n}, b=512, FP32 factor inside the mixed-precision solve",
+        "workload": f"n={n}, b=512, FP32 factor inside the mixed-precision solve, trailing updates on "
+                    f"split-BF16 matrix cores (bench default)",
         "copy_calibration": {"bytes": n * n * 8, "fetch_kib": fc, "write_kib": wc,
                              "fetch_scale": fs, "write_scale": ws},
         "factor_fetch_kib": ff, "factor_write_kib": wf, "dispatches": [nf, nw],
@@ -105,12 +204,22 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--update-prec", type=int, default=0, choices=(0, 1),
+    ap.add_argument("--update-prec", type=int, default=1, choices=(0, 1),
                     help="FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores, 1 split-BF16 matrix cores")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
+    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched"),
+                    help="mixed: the n=32768 refinement solve (the headline line); batched: 4096 "
+                         "independent systems of order uniform in [32, 512] in one persistent launch")
+    ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
+    if args.workload == "batched":
+        if args.gpus != 1:
+            print("bench.py measures one GPU. Run with --gpus 1.", file=sys.stderr)
+            sys.exit(2)
+        bench_batched(args)
+        return
     if args.traffic_from:
         traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)
         return
@@ -182,16 +291,24 @@ def main():
     X2.free()
 
     t_kernel = float(np.mean(t_launch))
+    achieved = flops / t_kernel / 1e12
+    if args.update_prec == 1:
+        # six BF16 products per FP32 multiply-add of the trailing updates
+        peak, basis = PEAK["bf16"] / 6.0, "bf16 dense 2500 TFLOP/s / 6 split products"
+    else:
+        peak, basis = PEAK["fp32"], "fp32 dense 157.3 TFLOP/s"
     roofline = {
         "bound": "mfma",
-        "achieved": round(flops / t_kernel / 1e12, 3),
-        "peak": PEAK["fp32"],
+        "achieved": round(achieved, 3),
+        "peak": round(peak, 1),
         "unit": "TFLOP/s",
-        "frac": round(flops / t_kernel / 1e12 / PEAK["fp32"], 4),
+        "frac": round(achieved / peak, 4),
         "traffic": None,
+        "peak_basis": basis,
+        "frac_of_fp32_peak": round(achieved / PEAK["fp32"], 4),
     }
     tf = ROOT / "profiles" / "traffic_fp32_factor_n32768.json"
-    if tf.exists():
+    if tf.exists() and (n, b, args.update_prec) == (32768, 512, 1):  # the workload the counters were read on
         try:
             roofline["traffic"] = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
         except Exception:
@@ -208,7 +325,7 @@ def main():
             log(f"CPU baseline unavailable: {e}")
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if n == 32768 else METRIC.replace("n=32768", f"n={n}"),
         "value": round(flops / t_ir / 1e9, 2),
         "unit": "GFLOP/s",
         "n_gpus": 1,
