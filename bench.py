@@ -150,6 +150,105 @@ def bench_batched(args):
     batch.free()
 
 
+def cpu_baseline_gp(P, y, ell, noise, threads):
+    """The oracle's FP64 closed-form gradient (K^-1 through LAPACK) on the host."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import adjoint as adj
+    t0 = time.perf_counter()
+    dK, dell = adj.gp_nll_grad(P, y, ell, noise)
+    t = time.perf_counter() - t0
+    n = len(y)
+    return dell, {"value": round(t * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+                  "sample": f"oracle closed-form GP gradient dK = (K^-1 - a a^T)/2 in FP64 numpy/LAPACK "
+                            f"(explicit inverse), n={n}, one evaluation, {t:.2f} s"}
+
+
+def bench_gp(args):
+    """GP negative log marginal likelihood at n points: forward (FP64 factor,
+    log det, solve) + backward (factor recomputed, solve adjoint, blocked
+    Cholesky adjoint) through torch.autograd, K resident on the device."""
+    import numpy as np
+    import linompss_amd as lx
+    if lx.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(3)
+    import linompss_amd.autograd as ag
+    import torch
+    n, ell0, noise = args.gp_n, 0.3, 1e-2
+    log(f"GP workload: {n} points in [0,1]^3 (seed 3), l={ell0}, noise={noise}")
+    rng = np.random.default_rng(3)
+    P = rng.uniform(0.0, 1.0, (n, 3))
+    pts = torch.from_numpy(P)
+    with torch.no_grad():
+        K0 = ag.gp_kernel(pts, torch.tensor(ell0, dtype=torch.float64), noise).numpy()
+    L0 = np.asfortranarray(K0.copy())
+    lx.ompss_dchol_rl(L0, 256)
+    y = np.tril(L0) @ rng.standard_normal(n)  # y ~ N(0, K)
+    yt = torch.from_numpy(y)
+
+    def step(tm):
+        ell = torch.tensor(ell0, dtype=torch.float64, requires_grad=True)
+        K = ag.gp_kernel(pts, ell, noise)
+        loss = ag.gp_nll_fused(pts, yt, ell, noise, K=K, timing=tm)
+        loss.backward()
+        return loss.item(), ell.grad.item()
+
+    for w in range(args.warmup):
+        step({})
+    dev_s, wall_s = [], []
+    for k in range(args.steps):
+        tm = {}
+        t0 = time.perf_counter()
+        nll, g = step(tm)
+        wall_s.append(time.perf_counter() - t0)
+        dev_s.append(tm["forward_s"] + tm["backward_s"])
+        log(f"step {k}: nll={nll:.10e} dnll/dl={g:.10e} device fwd {tm['forward_s'] * 1e3:.1f} ms "
+            f"bwd {tm['backward_s'] * 1e3:.1f} ms, wall {wall_s[-1] * 1e3:.0f} ms")
+    # central difference (h = 1e-6) of the same forward
+    h = 1e-6
+    with torch.no_grad():
+        fp = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 + h, dtype=torch.float64), noise).item()
+        fm = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 - h, dtype=torch.float64), noise).item()
+    fd = (fp - fm) / (2 * h)
+    t_dev = float(np.median(dev_s))
+    flops = 4.0 * n ** 3 / 3.0  # forward factor n^3/3, recomputed factor n^3/3, adjoint 2n^3/3
+    cpu, dell_ref = None, None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            dell_ref, cpu = cpu_baseline_gp(P, y, ell0, noise, threads)
+            log(f"CPU closed-form gradient {cpu['value']} ms, dnll/dl={dell_ref:.10e}")
+        except Exception as e:
+            log(f"CPU baseline unavailable: {e}")
+    out = {
+        "metric": f"GP negative log marginal likelihood forward+backward at n={n} (device ms)",
+        "value": round(t_dev * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.median(wall_s)) * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: points ~ U[0,1]^3 (numpy PCG64 seed 3), squared-exponential K, l=0.3, "
+                "noise 1e-2, y = L z",
+        "config": {"workload": "GP likelihood + gradient through torch.autograd (Cholesky and solve adjoints)",
+                   "n": n, "tile": 256, "lengthscale": ell0, "noise": noise},
+        "roofline": {"bound": "mfma", "achieved": round(flops / t_dev / 1e12, 3), "peak": PEAK["fp64"],
+                     "unit": "TFLOP/s", "frac": round(flops / t_dev / 1e12 / PEAK["fp64"], 4), "traffic": None,
+                     "basis": "4 n^3 / 3 flops (forward factor, recomputed factor, 2 n^3/3 adjoint)"},
+        "ms_per_step_note": "wall time per step includes building K in torch on the host and the PCIe "
+                            "copies of K and dK",
+        "dnll_dl": g, "dnll_dl_central_difference": fd,
+        "rel_err_vs_central_difference": abs(g - fd) / abs(fd),
+        "rel_err_vs_cpu_closed_form": (abs(g - dell_ref) / abs(dell_ref)) if dell_ref is not None else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def traffic_from(fetch_csv, write_csv, out, n):
     """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
@@ -209,16 +308,18 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
-    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched"),
+    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched", "gp"),
                     help="mixed: the n=32768 refinement solve (the headline line); batched: 4096 "
-                         "independent systems of order uniform in [32, 512] in one persistent launch")
+                         "independent systems of order uniform in [32, 512] in one persistent launch; "
+                         "gp: Gaussian-process likelihood forward + backward (Cholesky adjoint) at 8192 points")
+    ap.add_argument("--gp-n", type=int, default=8192)
     ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
-    if args.workload == "batched":
+    if args.workload in ("batched", "gp"):
         if args.gpus != 1:
             print("bench.py measures one GPU. Run with --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        bench_batched(args)
+        (bench_batched if args.workload == "batched" else bench_gp)(args)
         return
     if args.traffic_from:
         traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)

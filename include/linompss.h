@@ -123,9 +123,99 @@ int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, 
                       void *stream, const linompss_exec_opts *opts);
 int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
                        int64_t ldx, void *stream, const linompss_exec_opts *opts);
+/* One triangular sweep on device memory: X := op(L)^-1 X, op = N (trans 0)
+ * or T (trans 1), L lower (the device twin of linompss_dltrsm). */
+int linompss_trsm_dev(int dtype, int trans, int n, int nrhs, const void *dL, int64_t ldl, void *dX, int64_t ldx,
+                      void *stream);
 int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, int64_t lda, const double *dB,
                           int64_t ldb, double *dX, int64_t ldx, void *stream, const linompss_ir_opts *opts,
                           linompss_ir_status *st);
+
+/* ---- Batched systems ------------------------------------------------------
+ * Many independent SPD systems of uneven order (0 <= n_i <= 512, one
+ * right-hand side each), each solved as linompss_dposv_ir_dev solves one
+ * system (FP32 Cholesky factor, FP64 residual / update / backward-error test,
+ * FP64-factor fallback on a stall; itref.c:999-1101), in ONE persistent
+ * launch: a workgroup takes a whole system at a time from a ticket queue.  The
+ * reference has no batched entry point (its dspdsolv_main, spdsolv_main.c:10-20,
+ * solves one system per call); these calls stand where a caller would loop
+ * over dspdsolv_main.
+ *
+ * A plan fixes the batch (orders, device pointers, leading dimensions), picks
+ * the schedule (workgroups per CU, system order) -- measured on the batch
+ * itself when left to the autotuner, which runs the batch and so writes X --
+ * and captures the launch into a hipGraph.  linompss_batch_run replays it on
+ * a stream and waits; A and B are not modified.  Runs of one plan must not
+ * overlap (they share its queue counter and workspace).                     */
+typedef struct {
+    int max_sweeps;       /* corrections before falling back (default 30) */
+    double tol;           /* backward-error target (default 1e-14) */
+    double stall_ratio;   /* ||r_k+1|| > stall_ratio ||r_k|| is a stall (default 0.5) */
+    int wg_per_cu;        /* resident workgroups per CU: 1 or 2; 0 = autotuned */
+    int order;            /* 0 largest system first, 1 as given; -1 = autotuned */
+    int use_graph;        /* 1: runs replay a captured hipGraph; 0: plain launches */
+} linompss_batch_opts;
+
+typedef struct {
+    int sweeps;           /* corrections applied */
+    int fallback;         /* 1 when the FP64 factor produced the result */
+    int info;             /* info of the factor that produced the result */
+    int reserved;
+    double berr;          /* final normwise backward error */
+} linompss_batch_status;
+
+typedef struct linompss_batch_plan linompss_batch_plan;
+
+/* n, dA, lda, dB, dX: host arrays of count entries (the pointers are device
+ * pointers).  opts NULL = {30, 1e-14, 0.5, 0, -1, 1}.  Returns 0, -i for an
+ * invalid argument i, or LINOMPSS_ERR_*. */
+int linompss_batch_plan_create(int count, const int *n, const double *const *dA, const int64_t *lda,
+                               const double *const *dB, double *const *dX, const linompss_batch_opts *opts,
+                               linompss_batch_plan **plan);
+/* st: count entries, or NULL. */
+int linompss_batch_run(linompss_batch_plan *plan, void *stream, linompss_batch_status *st);
+int linompss_batch_plan_info(const linompss_batch_plan *plan, int *grid, int *wg_per_cu, int *order,
+                             double *tune_ms);
+int linompss_batch_plan_destroy(linompss_batch_plan *plan);
+/* One-shot: plan (opts NULL = one workgroup per CU, largest first, no graph),
+ * run, destroy. */
+int linompss_dposv_ir_batched_dev(int count, const int *n, const double *const *dA, const int64_t *lda,
+                                  const double *const *dB, double *const *dX, void *stream,
+                                  const linompss_batch_opts *opts, linompss_batch_status *st);
+
+/* ---- Adjoints (reverse mode) ----------------------------------------------
+ * No counterpart in the reference (its factor is task_dpotrf,
+ * tasks_potrf.c:20-24, and its solves bblas_dltrsm, bblas_trsm.h:116-160,
+ * with no derivatives); these give the gradients a differentiable caller
+ * needs, in FP64.
+ *
+ * Cholesky adjoint: given L (A = L L^T, lower) and Lbar (the adjoint of L,
+ * lower), overwrites Lbar with Abar in the lower representation
+ * (dLoss = sum_{i>=j} Abar_ij dA_ij; the symmetric gradient is Abar with its
+ * off-diagonal entries halved and mirrored).  Blocked reverse mode over
+ * 64-column blocks (I. Murray, "Differentiation of the Cholesky
+ * decomposition", 2016), ~2n^3/3 flops on the FP64 matrix cores.
+ * logdet_weight w adds the adjoint of w log det A (Lbar_ii += 2 w / L_ii)
+ * before the sweep, the term every likelihood carries.
+ * linompss_dpotrf_adjoint takes A instead of L and recomputes the factor on
+ * the device (tile b) rather than keeping it from the forward pass.       */
+int linompss_dpotrf_adjoint(int n, int b, const double *A, int lda, double *Lbar, int ldlbar, double logdet_weight);
+int linompss_dpotrf_adjoint_dev(int n, const double *dL, int64_t ldl, double *dLbar, int64_t ldlbar,
+                                double logdet_weight, void *stream);
+
+/* Triangular-solve adjoint: X = op(L)^-1 B (op = 'N' or 'T', L lower n x n,
+ * nrhs columns).  Xbar (in: the adjoint of X) is overwritten with
+ * Bbar = op(L)^-T Xbar, and Lbar (when not NULL) is updated with
+ * -tril(Bbar X^T) (op = N) or -tril(X Bbar^T) (op = T); the device entry
+ * point overwrites Lbar's lower triangle with that term when accumulate is 0. */
+int linompss_dtrsm_adjoint(char trans, int n, int nrhs, const double *L, int ldl, const double *X, int ldx,
+                           double *Xbar, int ldxbar, double *Lbar, int ldlbar);
+int linompss_dtrsm_adjoint_dev(int trans, int n, int nrhs, const double *dL, int64_t ldl, const double *dX,
+                               int64_t ldx, double *dXbar, int64_t ldxbar, double *dLbar, int64_t ldlbar,
+                               int accumulate, void *stream);
+
+/* log det A = 2 sum_i log L_ii from the device factor L (host result). */
+int linompss_dlogdet_dev(int n, const double *dL, int64_t ldl, double *logdet, void *stream);
 
 /* Normwise backward error max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf),
  * computed in FP64 on the device (A symmetric, lower triangle read; nrhs <= 16). */

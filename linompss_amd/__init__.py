@@ -9,7 +9,8 @@ The names and argument meanings follow the reference's interface:
 * ``dspdsolv_main(m, n, b, A, B)`` (src/spdsolv/spdsolv_main.h:20),
 * ``bblas_dltrsm`` lower branches (src/blocked_blas/bblas_trsm.h:116-160),
 * an iterative-refinement solve keeping itref's loop contract
-  (src/itref/itref.c:999-1101).
+  (src/itref/itref.c:999-1101), for one system or for a batch of many
+  independent systems of uneven order in one persistent launch.
 
 Host-array functions take column-major (Fortran-ordered) numpy arrays and
 work in place like the reference.  Device functions take :class:`DeviceMatrix`
@@ -30,7 +31,8 @@ __all__ = [
     "lib", "LinompssError", "ompss_dchol_rl", "ompss_dchol_ll", "ompss_schol_rl", "ompss_schol_ll",
     "dspdsolv_main", "sspdsolv_main", "dltrsm", "sltrsm", "dposv_ir", "chol_dev", "posv_dev",
     "potrs_dev", "dposv_ir_dev", "last_exec_stats", "plan_tdg", "plan_info", "ExecOpts", "IrOpts",
-    "IrStatus", "ExecStats", "DeviceMatrix", "device_count",
+    "IrStatus", "ExecStats", "DeviceMatrix", "device_count", "BatchOpts", "BatchStatus", "DeviceBatch",
+    "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -384,3 +386,135 @@ def plan_info(what: int, n: int, b: int, nrhs: int = 0, opts: ExecOpts | None = 
     if rc != 0:
         raise LinompssError("linompss_plan_info", rc)
     return u.value, t.value, s.value
+
+
+# ---- batched systems -----------------------------------------------------------
+
+BATCH_NMAX = 512
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("max_sweeps", _i), ("tol", _d), ("stall_ratio", _d), ("wg_per_cu", _i), ("order", _i),
+                ("use_graph", _i)]
+
+
+class BatchStatus(ctypes.Structure):
+    _fields_ = [("sweeps", _i), ("fallback", _i), ("info", _i), ("reserved", _i), ("berr", _d)]
+
+
+BATCH_STATUS_DTYPE = np.dtype([("sweeps", "i4"), ("fallback", "i4"), ("info", "i4"), ("reserved", "i4"),
+                               ("berr", "f8")])
+
+_pp = ctypes.POINTER(_p)
+_sig("linompss_batch_plan_create", _i, _i, _ip, _pp, ctypes.POINTER(_i64), _pp, _pp, ctypes.POINTER(BatchOpts),
+     _pp)
+_sig("linompss_batch_run", _i, _p, _p, _p)
+_sig("linompss_batch_plan_info", _i, _p, _ip, _ip, _ip, ctypes.POINTER(_d))
+_sig("linompss_batch_plan_destroy", _i, _p)
+_sig("linompss_dposv_ir_batched_dev", _i, _i, _ip, _pp, ctypes.POINTER(_i64), _pp, _pp, _p,
+     ctypes.POINTER(BatchOpts), _p)
+
+
+class DeviceBatch:
+    """Systems packed back to back in device memory: A_i (n_i x n_i, column-major,
+    lda = n_i), b_i and x_i (n_i each)."""
+
+    def __init__(self, sizes):
+        self.sizes = np.asarray(sizes, dtype=np.int32)
+        if self.sizes.ndim != 1 or (self.sizes < 0).any():
+            raise ValueError("sizes must be a 1-D array of non-negative orders")
+        n64 = self.sizes.astype(np.int64)
+        self.offA = np.concatenate([[0], np.cumsum(n64 * n64)])
+        self.offV = np.concatenate([[0], np.cumsum(n64)])
+        self.A = DeviceMatrix(max(1, int(self.offA[-1])), 1)
+        self.B = DeviceMatrix(max(1, int(self.offV[-1])), 1)
+        self.X = DeviceMatrix(max(1, int(self.offV[-1])), 1)
+        cnt = len(self.sizes)
+        self._n = (_i * cnt)(*self.sizes.tolist())
+        self._lda = (_i64 * cnt)(*np.maximum(n64, 1).tolist())
+        self._pa = (_p * cnt)(*(self.A.ptr + 8 * self.offA[:-1]).tolist())
+        self._pb = (_p * cnt)(*(self.B.ptr + 8 * self.offV[:-1]).tolist())
+        self._px = (_p * cnt)(*(self.X.ptr + 8 * self.offV[:-1]).tolist())
+
+    @property
+    def count(self) -> int:
+        return len(self.sizes)
+
+    def upload(self, mats, rhs) -> None:
+        """mats[i]: n_i x n_i (only the lower triangle is read); rhs[i]: n_i."""
+        hA = np.empty(max(1, int(self.offA[-1])))
+        hB = np.empty(max(1, int(self.offV[-1])))
+        for i, n in enumerate(self.sizes):
+            hA[self.offA[i]:self.offA[i + 1]] = np.asarray(mats[i], dtype=np.float64).ravel(order="F")
+            hB[self.offV[i]:self.offV[i + 1]] = np.asarray(rhs[i], dtype=np.float64).ravel()
+        self.A.upload(hA.reshape(-1, 1))
+        self.B.upload(hB.reshape(-1, 1))
+
+    def solutions(self) -> list:
+        h = self.X.to_host().ravel()
+        return [h[self.offV[i]:self.offV[i + 1]].copy() for i in range(self.count)]
+
+    def free(self) -> None:
+        for m in (self.A, self.B, self.X):
+            m.free()
+
+
+class BatchPlan:
+    """A planned batch (linompss_batch_plan_create): schedule chosen (or
+    autotuned on the batch), launch captured as a hipGraph; run() replays it."""
+
+    def __init__(self, batch: DeviceBatch, opts: BatchOpts | None = None):
+        self.batch = batch
+        self._h = _p()
+        rc = lib.linompss_batch_plan_create(batch.count, batch._n, batch._pa, batch._lda, batch._pb, batch._px,
+                                            ctypes.byref(opts) if opts is not None else None,
+                                            ctypes.byref(self._h))
+        _check("linompss_batch_plan_create", rc, False)
+
+    def run(self, stream=None) -> np.ndarray:
+        st = np.zeros(self.batch.count, dtype=BATCH_STATUS_DTYPE)
+        _check("linompss_batch_run", lib.linompss_batch_run(self._h, _s(stream), st.ctypes.data_as(_p)), False)
+        return st
+
+    def info(self) -> dict:
+        g, w, o, t = _i(0), _i(0), _i(0), _d(0.0)
+        _check("linompss_batch_plan_info", lib.linompss_batch_plan_info(self._h, ctypes.byref(g), ctypes.byref(w),
+                                                                        ctypes.byref(o), ctypes.byref(t)), False)
+        return {"grid": g.value, "wg_per_cu": w.value, "order": o.value, "tune_ms": t.value}
+
+    def close(self) -> None:
+        if self._h:
+            lib.linompss_batch_plan_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dposv_ir_batched_dev(batch: DeviceBatch, opts: BatchOpts | None = None, stream=None) -> np.ndarray:
+    """One-shot batched refinement solve (plan, run, destroy); returns the statuses."""
+    st = np.zeros(batch.count, dtype=BATCH_STATUS_DTYPE)
+    rc = lib.linompss_dposv_ir_batched_dev(batch.count, batch._n, batch._pa, batch._lda, batch._pb, batch._px,
+                                           _s(stream), ctypes.byref(opts) if opts is not None else None,
+                                           st.ctypes.data_as(_p))
+    _check("linompss_dposv_ir_batched_dev", rc, False)
+    return st
+
+
+def gen_batch_spd(count: int, nmin: int, nmax: int, seed: int, shift: float = 0.1):
+    """Host workload: orders uniform in [nmin, nmax], A_i = G_i G_i^T / n_i + shift I
+    (G_i i.i.d. N(0, 1)), b_i uniform in [-1, 1); numpy PCG64 stream `seed`."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=count).astype(np.int32)
+    mats, rhs = [], []
+    for n in sizes:
+        G = rng.standard_normal((n, n))
+        A = G @ G.T
+        A /= max(int(n), 1)
+        A[np.diag_indices(n)] += shift
+        mats.append(np.asfortranarray(A))
+        rhs.append(rng.uniform(-1.0, 1.0, n))
+    return sizes, mats, rhs

@@ -1,0 +1,503 @@
+// lx_adjoint.hip -- reverse-mode (adjoint) kernels of the FP64 Cholesky
+// factor and of the triangular solves.
+//
+// Cholesky adjoint (blocked reverse mode of the left-looking block Cholesky,
+// I. Murray, "Differentiation of the Cholesky decomposition", 2016): with
+// A = L L^T read from its lower triangle and Lbar the adjoint of L (lower),
+// Abar (lower representation: dLoss = sum_{i>=j} Abar_ij dA_ij) is computed in
+// place of Lbar.  Forward block step for block column J (rows j0..j0+jb):
+//     D = chol(A_D - R R^T),   C = (A_C - B R^T) D^-T
+// with R = L[J, <J], B = L[>J, <J], C = L[>J, J].  Reverse, J from last to first:
+//     Qbar = Cbar D^-1              (= Abar_C)
+//     Dbar -= tril(Qbar^T C)
+//     Pbar  = level-2 adjoint of D = chol(P) on the diagonal block (= Abar_D)
+//     Bbar -= Qbar R
+//     Rbar -= Qbar^T B + (Pbar + Pbar^T) R
+// The forward factor has no counterpart in the reference beyond task_dpotrf
+// (tasks_potrf.c:20-24); the adjoint is new.  Work ~ 2 n^3 / 3 flops on the
+// FP64 matrix cores through one tiled GEMM kernel (op(A), op(B) staged through
+// LDS, optional deterministic split-K), plus one workgroup per block for the
+// diagonal steps.
+//
+// Triangular-solve adjoint: x = op(L)^-1 b with op = N or T; given xbar,
+//     bbar = op(L)^-T xbar,  Lbar -= tril(bbar x^T)  (op = N)
+//                            Lbar -= tril(x bbar^T)  (op = T).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lx_device.hpp"
+
+#define LX_AG_T 64   // output tile (rows and columns)
+#define LX_AG_KS 16  // k per LDS slab
+#define LX_AG_LD (LX_AG_T + 4)
+
+// op(A)[M x K] element (i, k): TA = 0 -> A[i + k lda], TA = 1 -> A[k + i lda].
+// op(B)[K x N] element (k, j): TB = 0 -> B[k + j ldb], TB = 1 -> B[j + k ldb].
+// ksplit == 1:  C = alpha op(A) op(B) + beta C   (beta == 0: C is not read)
+// ksplit  > 1:  part[(z N + j) M + i] = alpha * (partial over the z-th K chunk)
+// lower_only: entries with i < j are neither read nor written.
+template <int TA, int TB>
+__global__ void __launch_bounds__(LX_WG_THREADS)
+lx_adj_gemm_kernel(int M, int N, int K, double alpha, const double *__restrict__ A, int64_t lda,
+                   const double *__restrict__ B, int64_t ldb, double beta, double *__restrict__ C, int64_t ldc,
+                   int lower_only, int ksplit, int kchunk, double *__restrict__ part)
+{
+    __shared__ double sA[LX_AG_KS][LX_AG_LD], sB[LX_AG_KS][LX_AG_LD];
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    const int m0 = blockIdx.x * LX_AG_T, n0 = blockIdx.y * LX_AG_T, z = blockIdx.z;
+    const int kb = z * kchunk, ke = K < kb + kchunk ? K : kb + kchunk;
+    // a lower_only tile strictly above the diagonal has nothing to do
+    if (lower_only && m0 + LX_AG_T - 1 < n0) return;
+    // staging: 4 elements per operand per thread per slab
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (TA == 0) {  // rows contiguous
+                const int i = m0 + (tid & 63), k = k0 + (tid >> 6) + 4 * e;
+                ra[e] = (i < M && k < ke) ? A[(int64_t)k * lda + i] : 0.0;
+            } else {  // k contiguous
+                const int k = k0 + (tid & 15), i = m0 + (tid >> 4) + 16 * e;
+                ra[e] = (i < M && k < ke) ? A[(int64_t)i * lda + k] : 0.0;
+            }
+            if (TB == 1) {  // columns contiguous
+                const int j = n0 + (tid & 63), k = k0 + (tid >> 6) + 4 * e;
+                rb[e] = (j < N && k < ke) ? B[(int64_t)k * ldb + j] : 0.0;
+            } else {  // k contiguous
+                const int k = k0 + (tid & 15), j = n0 + (tid >> 4) + 16 * e;
+                rb[e] = (j < N && k < ke) ? B[(int64_t)j * ldb + k] : 0.0;
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (TA == 0) sA[(tid >> 6) + 4 * e][tid & 63] = ra[e];
+            else sA[tid & 15][(tid >> 4) + 16 * e] = ra[e];
+            if (TB == 1) sB[(tid >> 6) + 4 * e][tid & 63] = rb[e];
+            else sB[tid & 15][(tid >> 4) + 16 * e] = rb[e];
+        }
+    };
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    lx_d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = lx_d4{0.0, 0.0, 0.0, 0.0};
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += LX_AG_KS) {
+        __syncthreads();  // the previous slab's reads are done
+        store();
+        __syncthreads();
+        if (k0 + LX_AG_KS < ke) load(k0 + LX_AG_KS);  // next slab in flight during the products
+#pragma unroll
+        for (int kk = 0; kk < LX_AG_KS; kk += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = sA[kk + lk][wr + 16 * i + lr];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wc + 16 * j + lr];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = lx_mfma<double>::mac(a[i], b[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = m0 + wr + 16 * i + lx_mfma<double>::row(lk, e), c = n0 + wc + 16 * j + lr;
+                if (r >= M || c >= N || (lower_only && r < c)) continue;
+                const double v = alpha * acc[i][j][e];
+                if (ksplit > 1) {
+                    part[((int64_t)z * N + c) * M + r] = v;
+                } else {
+                    double *p = C + (int64_t)c * ldc + r;
+                    *p = beta == 0.0 ? v : v + beta * *p;
+                }
+            }
+}
+
+// C = beta C + sum_z part[z] (fixed order), entries i >= j only when lower_only.
+__global__ void lx_adj_reduce_kernel(int M, int N, int ksplit, const double *__restrict__ part, double beta,
+                                     double *__restrict__ C, int64_t ldc, int lower_only)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * N) return;
+    const int r = (int)(e % M), c = (int)(e / M);
+    if (lower_only && r < c) return;
+    double s = 0.0;
+    for (int z = 0; z < ksplit; ++z) s += part[((int64_t)z * N + c) * M + r];
+    double *p = C + (int64_t)c * ldc + r;
+    *p = beta == 0.0 ? s : s + beta * *p;
+}
+
+// Diagonal block step of the reverse sweep, one workgroup:
+//   mode 0: Z = D^-1 (D = L[J, J], jb x jb lower) into Zb (64 x 64, ld 64;
+//           identity past jb);
+//   mode 1: Pbar = level-2 adjoint of D = chol(P) applied to Dbar (Abar[J, J],
+//           lower), written back in place; Sb (64 x 64, ld 64) = Pbar + Pbar^T.
+// Level-2 adjoint, columns j = jb-1 .. 0 of the block (the scalar case of the
+// block recurrences above): cbar = Dbar[>j, j] / d_j; Dbar[j, j] -= cbar . c;
+// pbar = Dbar[j, j] / (2 d_j); Dbar[>j, <j] -= cbar L[j, <j];
+// Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j].
+__global__ void __launch_bounds__(LX_WG_THREADS)
+lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl, double *__restrict__ Ab,
+                   int64_t lda, double *__restrict__ Zb, double *__restrict__ Sb)
+{
+    __shared__ double sL[64][65], sX[64][65];
+    __shared__ double s_red[4];
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+        const int r = e & 63, c = e >> 6;
+        const bool in = r < jb && c < jb && r >= c;
+        sL[r][c] = in ? Ld[(int64_t)c * ldl + r] : (r == c ? 1.0 : 0.0);
+        sX[r][c] = (mode == 1 && in) ? Ab[(int64_t)c * lda + r] : 0.0;
+    }
+    __syncthreads();
+    if (mode == 0) {
+        lx_tri_inv64<double>(sL, sX);
+        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+            const int r = e & 63, c = e >> 6;
+            Zb[c * 64 + r] = sX[r][c];
+        }
+        return;
+    }
+    for (int j = jb - 1; j >= 0; --j) {
+        const double d = sL[j][j];
+        // cbar (rows > j of column j), and its dot with c (wave 0, lane = row)
+        if (w == 0) {
+            double cb = 0.0;
+            if (lane > j && lane < jb) {
+                cb = sX[lane][j] / d;
+                sX[lane][j] = cb;
+            }
+            double t = cb * (lane > j && lane < jb ? sL[lane][j] : 0.0);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0) {
+                const double pbar = (sX[j][j] - t) / (2.0 * d);
+                sX[j][j] = pbar;
+                s_red[0] = pbar;
+            }
+        }
+        __syncthreads();
+        const double pbar = s_red[0];
+        // Dbar[>j, <j] -= cbar L[j, <j]   and   Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j]
+        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+            const int r = e & 63, c = e >> 6;
+            if (c >= j) continue;
+            if (r > j && r < jb) {
+                sX[r][c] -= sX[r][j] * sL[j][c];
+            } else if (r == j) {
+                double s = 2.0 * pbar * sL[j][c];
+                for (int i = j + 1; i < jb; ++i) s += sX[i][j] * sL[i][c];
+                sX[j][c] -= s;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+        const int r = e & 63, c = e >> 6;
+        if (r < jb && c < jb && r >= c) Ab[(int64_t)c * lda + r] = sX[r][c];
+        const double v = r == c ? 2.0 * sX[r][r] : (r > c ? sX[r][c] : sX[c][r]);
+        Sb[c * 64 + r] = (r < jb && c < jb) ? v : 0.0;
+    }
+}
+
+// Lbar_ii += 2 w / L_ii (the adjoint of w log det A = 2 w sum_i log L_ii).
+__global__ void lx_adj_logdet_kernel(int n, const double *__restrict__ L, int64_t ldl, double *__restrict__ Lb,
+                                     int64_t ldlb, double w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Lb[(int64_t)i * ldlb + i] += 2.0 * w / L[(int64_t)i * ldl + i];
+}
+
+// out[0] = 2 sum_i log L_ii (log det of A = L L^T), one workgroup, fixed order.
+__global__ void __launch_bounds__(LX_WG_THREADS) lx_adj_logdet_sum_kernel(int n, const double *__restrict__ L,
+                                                                          int64_t ldl, double *__restrict__ out)
+{
+    __shared__ double red[LX_WG_THREADS];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += LX_WG_THREADS) s += log(L[(int64_t)i * ldl + i]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = LX_WG_THREADS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = 2.0 * red[0];
+}
+
+// Y (ld) := X, a rows x cols block copy.
+__global__ void lx_adj_copy_kernel(int rows, int cols, const double *__restrict__ X, int64_t ldx,
+                                   double *__restrict__ Y, int64_t ldy)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    Y[(int64_t)c * ldy + r] = X[(int64_t)c * ldx + r];
+}
+
+#ifndef LX_ADJOINT_KERNELS_ONLY
+#include <algorithm>
+#include <mutex>
+
+#include "lx_internal.hpp"
+
+namespace {
+
+std::mutex lx_adj_mu;  // the adjoint's scratch slot
+
+int lx_adj_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+    }
+    return cus;
+}
+
+// C = alpha op(A) op(B) + beta C on the matrix cores; split-K (fixed-order
+// reduction through part, part_elems doubles) when the output has too few
+// tiles to fill the device.
+int lx_adj_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, int64_t lda, const double *B,
+                int64_t ldb, double beta, double *C, int64_t ldc, int lower, double *part, size_t part_elems,
+                hipStream_t s)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    const int tm = (M + LX_AG_T - 1) / LX_AG_T, tn = (N + LX_AG_T - 1) / LX_AG_T;
+    const int target = 2 * lx_adj_cus();
+    int ksplit = 1;
+    if ((int64_t)tm * tn < target && K >= 512) {
+        ksplit = std::min<int64_t>((target + (int64_t)tm * tn - 1) / ((int64_t)tm * tn), K / 256);
+        ksplit = (int)std::min<int64_t>(ksplit, (int64_t)(part_elems / ((size_t)M * N)));
+        ksplit = std::max(1, ksplit);
+    }
+    int kchunk = (K + ksplit - 1) / ksplit;
+    kchunk = (kchunk + LX_AG_KS - 1) / LX_AG_KS * LX_AG_KS;
+    ksplit = (K + kchunk - 1) / kchunk;
+    const dim3 grid(tm, tn, ksplit);
+#define LX_AG_LAUNCH(TA_, TB_)                                                                                  \
+    hipLaunchKernelGGL((lx_adj_gemm_kernel<TA_, TB_>), grid, dim3(LX_WG_THREADS), 0, s, M, N, K, alpha, A, lda, \
+                       B, ldb, beta, C, ldc, lower, ksplit, kchunk, part)
+    if (ta == 0 && tb == 0) LX_AG_LAUNCH(0, 0);
+    else if (ta == 0) LX_AG_LAUNCH(0, 1);
+    else if (tb == 0) LX_AG_LAUNCH(1, 0);
+    else LX_AG_LAUNCH(1, 1);
+#undef LX_AG_LAUNCH
+    if (ksplit > 1) {
+        const int64_t tot = (int64_t)M * N;
+        hipLaunchKernelGGL(lx_adj_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, M, N, ksplit,
+                           part, beta, C, ldc, lower);
+    }
+    LX_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// The reverse sweep over 64-column blocks (see the header comment).
+int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda, hipStream_t s)
+{
+    const int nb = (n + 63) / 64;
+    const size_t part_elems = (size_t)16 * 64 * std::max(n, 64);
+    double *ws = (double *)lx_scratch(9, sizeof(double) * (2 * 64 * 64 + (size_t)n * 64 + part_elems));
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    double *Zb = ws, *Sb = ws + 64 * 64, *Qb = ws + 2 * 64 * 64, *part = Qb + (size_t)n * 64;
+    int rc = 0;
+    for (int J = nb - 1; J >= 0 && rc == 0; --J) {
+        const int j0 = 64 * J, jb = std::min(64, n - j0), m = n - j0 - jb, p = j0;
+        const double *D = L + (int64_t)j0 * ldl + j0;
+        double *Dbar = Ab + (int64_t)j0 * lda + j0;
+        const int64_t ldq = std::max(m, 1);
+        if (m > 0) {
+            hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 0, jb, D, ldl, Dbar, lda, Zb,
+                               Sb);
+            double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
+            const double *C = L + (int64_t)j0 * ldl + j0 + jb;
+            // Qbar = Cbar D^-1; Abar_C = Qbar
+            if ((rc = lx_adj_gemm(0, 0, m, jb, jb, 1.0, Cbar, lda, Zb, 64, 0.0, Qb, ldq, 0, part, part_elems, s)))
+                break;
+            const int64_t tot = (int64_t)m * jb;
+            hipLaunchKernelGGL(lx_adj_copy_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, jb, Qb,
+                               ldq, Cbar, lda);
+            // Dbar -= tril(Qbar^T C)
+            if ((rc = lx_adj_gemm(1, 0, jb, jb, m, -1.0, Qb, ldq, C, ldl, 1.0, Dbar, lda, 1, part, part_elems, s)))
+                break;
+        }
+        // Pbar (in place of Dbar) and S = Pbar + Pbar^T
+        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda, Zb, Sb);
+        if (p > 0) {
+            const double *R = L + j0;           // L[J, <J]: jb x p, ld ldl
+            double *Rbar = Ab + j0;            // Abar[J, <J]
+            if (m > 0) {
+                const double *Bm = L + j0 + jb;  // L[>J, <J]: m x p
+                double *Bbar = Ab + j0 + jb;
+                // Bbar -= Qbar R ;  Rbar -= Qbar^T B
+                if ((rc = lx_adj_gemm(0, 0, m, p, jb, -1.0, Qb, ldq, R, ldl, 1.0, Bbar, lda, 0, part, part_elems, s)))
+                    break;
+                if ((rc = lx_adj_gemm(1, 0, jb, p, m, -1.0, Qb, ldq, Bm, ldl, 1.0, Rbar, lda, 0, part, part_elems, s)))
+                    break;
+            }
+            // Rbar -= (Pbar + Pbar^T) R
+            if ((rc = lx_adj_gemm(0, 0, jb, p, jb, -1.0, Sb, 64, R, ldl, 1.0, Rbar, lda, 0, part, part_elems, s)))
+                break;
+        }
+    }
+    if (rc == 0) LX_HIP_TRY(hipGetLastError());
+    return rc;
+}
+
+// Lbar = beta Lbar - tril(U V^T) for n x nrhs U, V (the rank-nrhs term of
+// the solve adjoint); beta 0 writes the lower triangle without reading it.
+int lx_trsm_adjoint_update(int n, int nrhs, const double *U, int64_t ldu, const double *V, int64_t ldv,
+                           double *Lb, int64_t ldlb, double beta, hipStream_t s)
+{
+    return lx_adj_gemm(0, 1, n, n, nrhs, -1.0, U, ldu, V, ldv, beta, Lb, ldlb, 1, nullptr, 0, s);
+}
+
+}  // namespace
+
+extern "C" int linompss_dpotrf_adjoint_dev(int n, const double *dL, int64_t ldl, double *dLbar, int64_t ldlbar,
+                                           double logdet_weight, void *stream)
+{
+    if (n < 0) return -1;
+    if (ldl < std::max(1, n)) return -3;
+    if (ldlbar < std::max(1, n)) return -5;
+    if (n == 0) return 0;
+    std::lock_guard<std::mutex> g(lx_adj_mu);
+    hipStream_t s = (hipStream_t)stream;
+    if (logdet_weight != 0.0)
+        hipLaunchKernelGGL(lx_adj_logdet_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dL, ldl, dLbar, ldlbar,
+                           logdet_weight);
+    const int rc = lx_chol_adjoint(n, dL, ldl, dLbar, ldlbar, s);
+    if (rc) return rc;
+    LX_HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int linompss_dtrsm_adjoint_dev(int trans, int n, int nrhs, const double *dL, int64_t ldl,
+                                          const double *dX, int64_t ldx, double *dXbar, int64_t ldxbar,
+                                          double *dLbar, int64_t ldlbar, int accumulate, void *stream)
+{
+    if (trans != 0 && trans != 1) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (ldl < std::max(1, n)) return -5;
+    if (ldx < std::max(1, n)) return -7;
+    if (ldxbar < std::max(1, n)) return -9;
+    if (dLbar && ldlbar < std::max(1, n)) return -11;
+    if (n == 0 || nrhs == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    {
+        // bbar = op(L)^-T xbar: the backward sweep for op = N, the forward one for op = T
+        std::lock_guard<std::mutex> g(lx_dev_mutex());
+        rc = lx_sweeps_dev(0, n, nrhs, dL, ldl, dXbar, ldxbar, trans ? 1 : 2, s);
+    }
+    if (rc) return rc;
+    if (dLbar) {
+        std::lock_guard<std::mutex> g(lx_adj_mu);
+        const double beta = accumulate ? 1.0 : 0.0;
+        rc = trans ? lx_trsm_adjoint_update(n, nrhs, dX, ldx, dXbar, ldxbar, dLbar, ldlbar, beta, s)
+                   : lx_trsm_adjoint_update(n, nrhs, dXbar, ldxbar, dX, ldx, dLbar, ldlbar, beta, s);
+        if (rc) return rc;
+    }
+    LX_HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int linompss_dlogdet_dev(int n, const double *dL, int64_t ldl, double *logdet, void *stream)
+{
+    if (n < 0) return -1;
+    if (ldl < std::max(1, n)) return -3;
+    if (!logdet) return -4;
+    *logdet = 0.0;
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> g(lx_adj_mu);
+    double *ws = (double *)lx_scratch(10, 256);
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    hipLaunchKernelGGL(lx_adj_logdet_sum_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, n, dL, ldl, ws);
+    LX_HIP_TRY(hipGetLastError());
+    LX_HIP_TRY(hipMemcpyAsync(logdet, ws, sizeof(double), hipMemcpyDeviceToHost, s));
+    LX_HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+// Host arrays: Abar from A itself -- the factor is recomputed on the device
+// (tiled FP64 Cholesky, tile b) rather than kept from the forward pass.
+extern "C" int linompss_dpotrf_adjoint(int n, int b, const double *A, int lda, double *Lbar, int ldlbar,
+                                       double logdet_weight)
+{
+    if (n < 0) return -1;
+    if (b <= 0) return -2;
+    if (lda < std::max(1, n)) return -4;
+    if (ldlbar < std::max(1, n)) return -6;
+    if (n == 0) return 0;
+    double *dA = nullptr, *dB = nullptr;
+    const size_t nn = (size_t)n * n;
+    if (hipMalloc(&dA, sizeof(double) * nn) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    if (hipMalloc(&dB, sizeof(double) * nn) != hipSuccess) {
+        (void)hipFree(dA);
+        return LINOMPSS_ERR_NOMEM;
+    }
+    int rc = 0;
+    if (hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * (size_t)lda, sizeof(double) * n, n,
+                    hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy2D(dB, sizeof(double) * n, Lbar, sizeof(double) * (size_t)ldlbar, sizeof(double) * n, n,
+                    hipMemcpyHostToDevice) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc) rc = linompss_chol_dev(0, 0, n, b, dA, n, nullptr, nullptr);
+    if (!rc) rc = linompss_dpotrf_adjoint_dev(n, dA, n, dB, n, logdet_weight, nullptr);
+    if (!rc && hipMemcpy2D(Lbar, sizeof(double) * (size_t)ldlbar, dB, sizeof(double) * n, sizeof(double) * n, n,
+                           hipMemcpyDeviceToHost) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    return rc;
+}
+
+// Host arrays: x = op(L)^-1 b was computed with L (lower, ldl); xbar (in) ->
+// bbar (out), Lbar -= the rank-nrhs term (Lbar may be NULL).
+extern "C" int linompss_dtrsm_adjoint(char trans, int n, int nrhs, const double *L, int ldl, const double *X,
+                                      int ldx, double *Xbar, int ldxbar, double *Lbar, int ldlbar)
+{
+    const int tr = (trans == 'T' || trans == 't') ? 1 : ((trans == 'N' || trans == 'n') ? 0 : -1);
+    if (tr < 0) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (ldl < std::max(1, n)) return -5;
+    if (ldx < std::max(1, n)) return -7;
+    if (ldxbar < std::max(1, n)) return -9;
+    if (Lbar && ldlbar < std::max(1, n)) return -11;
+    if (n == 0 || nrhs == 0) return 0;
+    const size_t nn = (size_t)n * n, nr = (size_t)n * nrhs;
+    double *d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * (nn * (Lbar ? 2 : 1) + 2 * nr)) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    double *dL = d, *dX = d + nn, *dXb = dX + nr, *dLb = Lbar ? dXb + nr : nullptr;
+    auto up = [&](double *dst, const double *src, int ld, int cols) {
+        return hipMemcpy2D(dst, sizeof(double) * n, src, sizeof(double) * (size_t)ld, sizeof(double) * n, cols,
+                           hipMemcpyHostToDevice) == hipSuccess;
+    };
+    int rc = 0;
+    if (!up(dL, L, ldl, n) || !up(dX, X, ldx, nrhs) || !up(dXb, Xbar, ldxbar, nrhs) ||
+        (Lbar && !up(dLb, Lbar, ldlbar, n)))
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc) rc = linompss_dtrsm_adjoint_dev(tr, n, nrhs, dL, n, dX, n, dXb, n, dLb, n, 1, nullptr);
+    if (!rc && hipMemcpy2D(Xbar, sizeof(double) * (size_t)ldxbar, dXb, sizeof(double) * n, sizeof(double) * n, nrhs,
+                           hipMemcpyDeviceToHost) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc && Lbar &&
+        hipMemcpy2D(Lbar, sizeof(double) * (size_t)ldlbar, dLb, sizeof(double) * n, sizeof(double) * n, n,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+#endif  // LX_ADJOINT_KERNELS_ONLY

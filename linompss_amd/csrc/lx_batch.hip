@@ -1,0 +1,622 @@
+// lx_batch.hip -- many independent SPD systems of uneven order, each solved to
+// FP64 accuracy by an FP32 Cholesky factor plus FP64 iterative refinement, in
+// one persistent launch.
+//
+// The refinement contract is the single-system one (itref.c:999-1101, restated
+// by linompss_dposv_ir_dev): x0 = solve(b); r = b - A x in FP64; repeat
+// { berr = ||r||_inf / (||A||_inf ||x||_inf); stop at tol; fall back to the
+// FP64 factor on a stall or after max_sweeps; d = solve(r); x += d }.  The
+// factor is the reference's tile algorithm applied to one tile
+// (task_dpotrf / tasks_potrf.c:20-24 over the whole system), the solves are
+// bblas_dltrsm's forward and backward sweeps (bblas_trsm.h:119-160).
+//
+// Execution model: a workgroup owns one system at a time.  Workgroups take
+// systems from a ticket counter in the order the host chose (largest first by
+// default, a longest-processing-time schedule), so the launch is a persistent
+// work queue with no cross-workgroup waits.  Per system, all 256 threads:
+//   1. copy the lower triangle of A to FP32 (workspace slot of the workgroup);
+//   2. factor it in place with lx_unit_potrf (64-column panels on the matrix
+//      cores, inverses of the 64 x 64 diagonal blocks into W);
+//   3. mirror L^T into the strict upper triangle, so both sweeps read
+//      coalesced columns;
+//   4. sweeps in dot-product form: t = b_q - L[q, <q] y (or L^T[q, >q] x),
+//      then y_q = Z_q t with Z_q the inverse of the diagonal block (from W);
+//   5. FP64 residual r = b - A x over the lower triangle in 64 x 64 tiles
+//      (each tile used twice: as A_IJ and, through LDS, as A_IJ^T), which
+//      also yields ||A||_inf on the first pass.
+// Systems up to LX_BATCH_NMAX rows; one right-hand side each.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lx_device.hpp"
+
+#define LX_BATCH_NMAX 512
+#define LX_BATCH_SMEM 73728
+
+struct lx_bsys {
+    const double *A;  // n x n, lower triangle read, column-major
+    const double *B;  // n
+    double *X;        // n (output)
+    int64_t lda;
+    int n;
+    int pad;
+};
+
+struct lx_bstat {
+    int sweeps;    // corrections applied
+    int fallback;  // 1: the FP64 factor produced x
+    int info;      // info of the factor that produced x (0 = SPD)
+    int pad;
+    double berr;   // final normwise backward error
+};
+
+// Per-workgroup workspace slot.
+struct lx_bslot {
+    char *F;      // factor: ld * n elements of the factor's type (room for FP64)
+    double *W;    // (ldw) x 64: inverses of the diagonal blocks (room for FP64)
+    double *r;    // residual, nmax
+    double *v;    // right-hand side / solution of the inner solve, nmax (FP32 or FP64)
+    int64_t ldw;
+};
+
+__device__ __forceinline__ double lx_b_nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Largest of v over the workgroup (NaN wins); every thread gets the result.
+__device__ __forceinline__ double lx_b_max(double v, double *red)
+{
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = lx_b_nanmax(v, __shfl_xor(v, o));
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    const double m = lx_b_nanmax(lx_b_nanmax(red[0], red[1]), lx_b_nanmax(red[2], red[3]));
+    __syncthreads();
+    return m;
+}
+
+// L (ld) := lower triangle of A, converted to T.
+template <typename T>
+__device__ __forceinline__ void lx_b_lower_to(const double *__restrict__ A, int64_t lda, int n, T *__restrict__ L,
+                                              int64_t ld)
+{
+    const int tid = lx_tid();
+    for (int c = 0; c < n; ++c)
+        for (int r = c + tid; r < n; r += LX_WG_THREADS) L[(int64_t)c * ld + r] = (T)A[(int64_t)c * lda + r];
+    __syncthreads();
+}
+
+// Strict upper triangle of L := (strict lower triangle)^T, through 64 x 64 LDS tiles.
+template <typename T>
+__device__ __forceinline__ void lx_b_mirror(T *__restrict__ L, int64_t ld, int n, char *smem)
+{
+    T(*sT)[65] = reinterpret_cast<T(*)[65]>(smem);
+    const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
+    const int nb = (n + 63) / 64;
+    for (int J = 0; J < nb; ++J)
+        for (int I = J; I < nb; ++I) {
+            // load: thread (row l, columns 16g..16g+15) of tile (I, J)
+            const int row = I * 64 + l;
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+                const int cl = g * 16 + k, col = J * 64 + cl;
+                T a = T(0);
+                if (row < n && col < n && row > col) a = L[(int64_t)col * ld + row];
+                sT[l][cl] = a;
+            }
+            __syncthreads();
+            // store: U[J*64 + l][I*64 + rr] = L[I*64 + rr][J*64 + l] for rows rr of group g
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+                const int rr = g * 16 + k, urow = J * 64 + l, ucol = I * 64 + rr;
+                if (ucol < n && urow < n && ucol > urow) L[(int64_t)ucol * ld + urow] = sT[rr][l];
+            }
+            __syncthreads();
+        }
+}
+
+// One triangular sweep on v (n entries): trans 0 solves L y = v, trans 1
+// solves L^T x = v; the strict upper triangle of L holds L^T (lx_b_mirror) and
+// W the inverses Z_q of the 64 x 64 diagonal blocks (W[c*ldw + 64q + r] = Z_q[r][c]).
+template <typename T>
+__device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, const T *__restrict__ W,
+                                           int64_t ldw, int n, int trans, T *__restrict__ v, char *smem)
+{
+    T(*sZ)[65] = reinterpret_cast<T(*)[65]>(smem);
+    T(*sred)[64] = reinterpret_cast<T(*)[64]>(smem + sizeof(T) * 64 * 65);
+    T *st = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 4 * 64));
+    T *sy = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 5 * 64));  // solved entries, n
+    const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
+    const int nb = (n + 63) / 64;
+    for (int qi = 0; qi < nb; ++qi) {
+        const int q = trans ? nb - 1 - qi : qi, p = q * 64, pb = n - p < 64 ? n - p : 64;
+        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+            const int r = e & 63, c = e >> 6;
+            sZ[r][c] = W[(int64_t)c * ldw + p + r];
+        }
+        // t = v_q - (coupling rows) . (solved entries): columns k of group g
+        T s = T(0);
+        if (l < pb) {
+            const int kb = trans ? p + pb : 0, ke = trans ? n : p;
+            const T *Lr = L + p + l;
+            for (int k = kb + g; k < ke; k += 4) s += Lr[(int64_t)k * ld] * sy[k];
+        }
+        sred[g][l] = s;
+        __syncthreads();
+        if (g == 0) st[l] = l < pb ? v[p + l] - ((sred[0][l] + sred[1][l]) + (sred[2][l] + sred[3][l])) : T(0);
+        __syncthreads();
+        // y_q = Z_q t (trans 0) or Z_q^T t (trans 1)
+        T z = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int c = g * 16 + k;
+            z += (trans ? sZ[c][l] : sZ[l][c]) * st[c];
+        }
+        sred[g][l] = z;
+        __syncthreads();
+        if (g == 0 && l < pb) {
+            const T y = (sred[0][l] + sred[1][l]) + (sred[2][l] + sred[3][l]);
+            sy[p + l] = y;
+            v[p + l] = y;
+        }
+        __syncthreads();
+    }
+}
+
+// r = b - A x (FP64, A symmetric with its lower triangle stored).  Returns
+// max|r|, max|x| and, when want_anorm, max_i sum_j |A_ij| (= ||A||_inf).
+__device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t lda, int n,
+                                           const double *__restrict__ x, const double *__restrict__ b,
+                                           double *__restrict__ r, bool want_anorm, char *smem, double &rmax,
+                                           double &xmax, double &anorm)
+{
+    double(*sT)[65] = reinterpret_cast<double(*)[65]>(smem);                       // 33280 B
+    double(*racc)[LX_BATCH_NMAX] = reinterpret_cast<double(*)[LX_BATCH_NMAX]>(smem + 33280);
+    double(*aacc)[LX_BATCH_NMAX] = reinterpret_cast<double(*)[LX_BATCH_NMAX]>(smem + 33280 + 16384);
+    double *sx = reinterpret_cast<double *>(smem + 33280 + 2 * 16384);           // LX_BATCH_NMAX
+    double *red = sx + LX_BATCH_NMAX;                                             // 4
+    const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
+    const int nb = (n + 63) / 64;
+    for (int i = tid; i < LX_BATCH_NMAX; i += LX_WG_THREADS) {
+        sx[i] = i < n ? x[i] : 0.0;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+            racc[gg][i] = 0.0;
+            aacc[gg][i] = 0.0;
+        }
+    }
+    __syncthreads();
+    // entry (g, i) of racc / aacc is only ever updated by thread (i & 63, g)
+    for (int J = 0; J < nb; ++J)
+        for (int I = J; I < nb; ++I) {
+            const int row = I * 64 + l;
+            double s = 0.0, sa = 0.0;
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+                const int cl = g * 16 + k, col = J * 64 + cl;
+                double a = 0.0;
+                if (row < n && col < n && row >= col) a = A[(int64_t)col * lda + row];
+                s += a * sx[col < n ? col : 0];
+                sa += fabs(a);
+                sT[cl][l] = row > col ? a : 0.0;  // the transposed use leaves the diagonal out
+            }
+            if (row < n) {
+                racc[g][row] += s;
+                if (want_anorm) aacc[g][row] += sa;
+            }
+            __syncthreads();
+            // (A_IJ^T x_I): thread (column l of the tile, rows 16g..16g+15)
+            double t = 0.0, ta = 0.0;
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+                const int rr = g * 16 + k;
+                const double a = sT[l][rr];
+                t += a * sx[I * 64 + rr < n ? I * 64 + rr : 0];
+                ta += fabs(a);
+            }
+            const int col = J * 64 + l;
+            if (col < n) {
+                racc[g][col] += t;
+                if (want_anorm) aacc[g][col] += ta;
+            }
+            __syncthreads();
+        }
+    double rm = 0.0, xm = 0.0, am = 0.0;
+    for (int i = tid; i < n; i += LX_WG_THREADS) {
+        const double ri = b[i] - ((racc[0][i] + racc[1][i]) + (racc[2][i] + racc[3][i]));
+        r[i] = ri;
+        rm = lx_b_nanmax(rm, fabs(ri));
+        xm = lx_b_nanmax(xm, fabs(sx[i]));
+        if (want_anorm) am = lx_b_nanmax(am, (aacc[0][i] + aacc[1][i]) + (aacc[2][i] + aacc[3][i]));
+    }
+    rmax = lx_b_max(rm, red);
+    xmax = lx_b_max(xm, red);
+    if (want_anorm) anorm = lx_b_max(am, red);
+}
+
+// Factor + sweeps + refinement of one system by the whole workgroup.
+__device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot &slot, lx_bstat *stat,
+                                               int max_sweeps, double tol, double stall, char *smem)
+{
+    const int tid = lx_tid();
+    const int n = S.n;
+    const int64_t ld = ((int64_t)n + 15) & ~(int64_t)15;
+    double *red = reinterpret_cast<double *>(smem);
+    lx_bstat out = {0, 0, 0, 0, 0.0};
+    if (n <= 0) {
+        if (tid == 0) *stat = out;
+        return;
+    }
+    // ||b||_inf (the inner solves work on b / ||b|| and r / ||r||)
+    double bm = 0.0;
+    for (int i = tid; i < n; i += LX_WG_THREADS) bm = lx_b_nanmax(bm, fabs(S.B[i]));
+    bm = lx_b_max(bm, red);
+    if (bm == 0.0) {  // x = 0 solves it exactly
+        for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] = 0.0;
+        __syncthreads();
+        if (tid == 0) *stat = out;
+        return;
+    }
+    float *F = reinterpret_cast<float *>(slot.F);
+    float *Wf = reinterpret_cast<float *>(slot.W);
+    float *vf = reinterpret_cast<float *>(slot.v);
+    lx_b_lower_to<float>(S.A, S.lda, n, F, ld);
+    int info = 0;
+    lx_unit_potrf<float>(F, ld, n, Wf, slot.ldw, &info, smem);
+    info = __builtin_amdgcn_readfirstlane(info);
+    bool fb = info != 0;
+    double anorm = 0.0, rmax = 0.0, xmax = 0.0, berr = 0.0;
+    if (!fb) {
+        lx_b_mirror<float>(F, ld, n, smem);
+        for (int i = tid; i < n; i += LX_WG_THREADS) vf[i] = (float)(S.B[i] / bm);
+        __syncthreads();
+        lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 0, vf, smem);
+        lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 1, vf, smem);
+        for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] = bm * (double)vf[i];
+        __syncthreads();
+        double prev = INFINITY;
+        for (int it = 0;; ++it) {
+            lx_b_resid(S.A, S.lda, n, S.X, S.B, slot.r, it == 0, smem, rmax, xmax, anorm);
+            berr = rmax / (anorm * xmax);
+            if (berr <= tol) break;
+            if (it >= max_sweeps || !(rmax <= stall * prev) || rmax != rmax) {
+                fb = true;
+                break;
+            }
+            prev = rmax;
+            for (int i = tid; i < n; i += LX_WG_THREADS) vf[i] = (float)(slot.r[i] / rmax);
+            __syncthreads();
+            lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 0, vf, smem);
+            lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 1, vf, smem);
+            for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] += rmax * (double)vf[i];
+            __syncthreads();
+            out.sweeps = it + 1;
+        }
+    }
+    if (fb) {
+        // FP64 factor of the same system (the FP32 factor's storage is reused)
+        double *D = reinterpret_cast<double *>(slot.F);
+        double *v = slot.v;
+        lx_b_lower_to<double>(S.A, S.lda, n, D, ld);
+        int info64 = 0;
+        lx_unit_potrf<double>(D, ld, n, slot.W, slot.ldw, &info64, smem);
+        info = __builtin_amdgcn_readfirstlane(info64);
+        lx_b_mirror<double>(D, ld, n, smem);
+        for (int i = tid; i < n; i += LX_WG_THREADS) v[i] = S.B[i];
+        __syncthreads();
+        lx_b_sweep<double>(D, ld, slot.W, slot.ldw, n, 0, v, smem);
+        lx_b_sweep<double>(D, ld, slot.W, slot.ldw, n, 1, v, smem);
+        for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] = v[i];
+        __syncthreads();
+        lx_b_resid(S.A, S.lda, n, S.X, S.B, slot.r, true, smem, rmax, xmax, anorm);
+        berr = rmax / (anorm * xmax);
+    }
+    out.fallback = fb ? 1 : 0;
+    out.info = info;
+    out.berr = berr;
+    if (tid == 0) *stat = out;
+    __syncthreads();
+}
+
+__device__ __forceinline__ lx_bslot lx_b_slot(char *ws, int64_t slot_bytes, int nmax)
+{
+    const int64_t nm = ((int64_t)nmax + 15) & ~(int64_t)15;
+    char *base = ws + (int64_t)blockIdx.x * slot_bytes;
+    lx_bslot s;
+    s.F = base;
+    s.ldw = nm + 64;
+    s.W = reinterpret_cast<double *>(base + nm * nm * 8);
+    s.r = s.W + s.ldw * 64;
+    s.v = s.r + nm;
+    return s;
+}
+
+// Bytes of one workspace slot for systems of order <= nmax.
+__host__ __device__ inline int64_t lx_b_slot_bytes(int nmax)
+{
+    const int64_t nm = ((int64_t)nmax + 15) & ~(int64_t)15;
+    return nm * nm * 8 + (nm + 64) * 64 * 8 + 2 * nm * 8;
+}
+
+// Persistent launch: workgroups take systems order[0], order[1], ... from
+// *ticket (zero at launch) until all count are done.
+__global__ void __launch_bounds__(LX_WG_THREADS, 2)
+lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ order, int count, char *ws,
+                   int64_t slot_bytes, int nmax, unsigned *ticket, lx_bstat *stat, int max_sweeps, double tol,
+                   double stall)
+{
+    __shared__ __attribute__((aligned(16))) char smem[LX_BATCH_SMEM];
+    __shared__ int s_t;
+    const lx_bslot slot = lx_b_slot(ws, slot_bytes, nmax);
+    for (;;) {
+        if (lx_tid() == 0) s_t = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_t);
+        __syncthreads();
+        if (t >= count) break;
+        const int k = order[t];
+        const lx_bsys S = sys[k];
+        lx_b_solve_one(S, slot, stat + k, max_sweeps, tol, stall, smem);
+    }
+}
+
+__global__ void lx_batch_reset_kernel(unsigned *ticket) { ticket[0] = 0u; }
+
+#ifndef LX_BATCH_KERNELS_ONLY
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+#include <vector>
+
+#include "lx_internal.hpp"
+
+static_assert(sizeof(lx_bstat) == sizeof(linompss_batch_status), "status layout");
+
+struct linompss_batch_plan {
+    int count = 0, nmax = 0, cus = 0;
+    int wg_per_cu = 1, order = 0, grid = 0, use_graph = 1;
+    int max_sweeps = 30;
+    double tol = 1e-14, stall = 0.5, tune_ms = 0.0;
+    int64_t slot_bytes = 0;
+    lx_bsys *d_sys = nullptr;
+    int *d_order[2] = {nullptr, nullptr};  // [0] largest first, [1] as given
+    lx_bstat *d_stat = nullptr;
+    unsigned *d_ticket = nullptr;
+    char *d_ws = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Autotuned (workgroups per CU, order) by batch signature: device, count,
+// largest order and the total factor work.
+std::mutex lx_bt_mu;
+std::map<std::tuple<int, int, int, long long>, std::pair<int, int>> lx_bt_cache;
+
+int lx_b_launch(linompss_batch_plan *p, int wg, int ord, hipStream_t s)
+{
+    const int grid = std::min(p->count, std::max(1, wg * p->cus));
+    hipLaunchKernelGGL(lx_batch_reset_kernel, dim3(1), dim3(1), 0, s, p->d_ticket);
+    hipLaunchKernelGGL(lx_batch_ir_kernel, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
+                       p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
+                       p->stall);
+    LX_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+void lx_b_release(linompss_batch_plan *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_sys);
+    (void)hipFree(p->d_order[0]);
+    (void)hipFree(p->d_order[1]);
+    (void)hipFree(p->d_stat);
+    (void)hipFree(p->d_ticket);
+    (void)hipFree(p->d_ws);
+    delete p;
+}
+
+// Times each (workgroups per CU, order) candidate on the batch itself (the
+// runs write X) and returns the fastest.
+int lx_b_tune(linompss_batch_plan *p, bool tune_wg, bool tune_order, int *wg_out, int *ord_out)
+{
+    hipStream_t s = nullptr;
+    LX_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    float best = INFINITY;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = LINOMPSS_ERR_HIP;
+    const int wgs[2] = {1, 2}, ords[2] = {0, 1};
+    for (int a = 0; a < 2 && rc == 0; ++a) {
+        const int wg = tune_wg ? wgs[a] : p->wg_per_cu;
+        if (!tune_wg && a > 0) break;
+        for (int o = 0; o < 2 && rc == 0; ++o) {
+            const int ord = tune_order ? ords[o] : p->order;
+            if (!tune_order && o > 0) break;
+            float ms = 0.f;
+            if ((rc = lx_b_launch(p, wg, ord, s)) != 0) break;  // warm-up
+            if (hipEventRecord(e0, s) != hipSuccess || (rc = lx_b_launch(p, wg, ord, s)) != 0 ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                if (rc == 0) rc = LINOMPSS_ERR_HIP;
+                break;
+            }
+            p->tune_ms += ms;
+            if (ms < best) {
+                best = ms;
+                *wg_out = wg;
+                *ord_out = ord;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int linompss_batch_plan_create(int count, const int *n, const double *const *dA, const int64_t *lda,
+                                          const double *const *dB, double *const *dX,
+                                          const linompss_batch_opts *opts, linompss_batch_plan **plan)
+{
+    if (!plan) return -8;
+    *plan = nullptr;
+    if (count < 0) return -1;
+    linompss_batch_opts o = {30, 1e-14, 0.5, 0, -1, 1};
+    if (opts) o = *opts;
+    if (o.wg_per_cu < 0 || o.wg_per_cu > 2 || o.order < -1 || o.order > 1 || o.max_sweeps < 0) return -7;
+    if (count > 0 && (!n || !dA || !lda || !dB || !dX)) return -2;
+    int nmax = 0;
+    double work = 0.0;
+    std::vector<lx_bsys> hs(count);
+    for (int i = 0; i < count; ++i) {
+        if (n[i] < 0 || n[i] > LX_BATCH_NMAX) return -2;
+        if (n[i] > 0 && !dA[i]) return -3;
+        if (lda[i] < std::max(1, n[i])) return -4;
+        if (n[i] > 0 && !dB[i]) return -5;
+        if (n[i] > 0 && !dX[i]) return -6;
+        hs[i] = lx_bsys{dA[i], dB[i], dX[i], lda[i], n[i], 0};
+        nmax = std::max(nmax, n[i]);
+        work += (double)n[i] * n[i] * n[i];
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return LINOMPSS_ERR_NODEV;
+    linompss_batch_plan *p = new linompss_batch_plan();
+    p->count = count;
+    p->nmax = std::max(1, nmax);
+    p->max_sweeps = o.max_sweeps;
+    p->tol = o.tol;
+    p->stall = o.stall_ratio;
+    p->use_graph = o.use_graph ? 1 : 0;
+    p->wg_per_cu = o.wg_per_cu ? o.wg_per_cu : 1;
+    p->order = o.order < 0 ? 0 : o.order;
+    p->slot_bytes = lx_b_slot_bytes(p->nmax);
+    auto fail = [&](int rc) {
+        lx_b_release(p);
+        return rc;
+    };
+    if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || p->cus < 1)
+        return fail(LINOMPSS_ERR_HIP);
+    if (count == 0) {
+        *plan = p;
+        return 0;
+    }
+    // LPT order: largest system first (stable)
+    std::vector<int> ord_desc(count), ord_given(count);
+    std::iota(ord_given.begin(), ord_given.end(), 0);
+    ord_desc = ord_given;
+    std::stable_sort(ord_desc.begin(), ord_desc.end(), [&](int a, int b) { return n[a] > n[b]; });
+    const int gmax = std::min(count, 2 * p->cus);
+    if (hipMalloc(&p->d_sys, sizeof(lx_bsys) * count) != hipSuccess ||
+        hipMalloc(&p->d_order[0], sizeof(int) * count) != hipSuccess ||
+        hipMalloc(&p->d_order[1], sizeof(int) * count) != hipSuccess ||
+        hipMalloc(&p->d_stat, sizeof(lx_bstat) * count) != hipSuccess ||
+        hipMalloc(&p->d_ticket, 256) != hipSuccess ||
+        hipMalloc(&p->d_ws, (size_t)p->slot_bytes * gmax) != hipSuccess)
+        return fail(LINOMPSS_ERR_NOMEM);
+    if (hipMemcpy(p->d_sys, hs.data(), sizeof(lx_bsys) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_order[0], ord_desc.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_order[1], ord_given.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LINOMPSS_ERR_HIP);
+    // schedule: given, cached for this batch signature, or measured now
+    const bool tune_wg = o.wg_per_cu == 0, tune_order = o.order < 0;
+    if (tune_wg || tune_order) {
+        // total work in buckets of about 1 % (log scale)
+        const long long sig = std::llround(std::log2(work + 1.0) * 64.0);
+        const auto key = std::make_tuple(dev, count, nmax, sig);
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> g(lx_bt_mu);
+            auto it = lx_bt_cache.find(key);
+            if (it != lx_bt_cache.end()) {
+                if (tune_wg) p->wg_per_cu = it->second.first;
+                if (tune_order) p->order = it->second.second;
+                hit = true;
+            }
+        }
+        if (!hit) {
+            int wg = p->wg_per_cu, od = p->order;
+            const int rc = lx_b_tune(p, tune_wg, tune_order, &wg, &od);
+            if (rc) return fail(rc);
+            p->wg_per_cu = wg;
+            p->order = od;
+            std::lock_guard<std::mutex> g(lx_bt_mu);
+            lx_bt_cache[key] = {wg, od};
+        }
+    }
+    p->grid = std::min(count, std::max(1, p->wg_per_cu * p->cus));
+    if (p->use_graph) {
+        // the launch pair (ticket reset, persistent kernel) as one graph
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return fail(LINOMPSS_ERR_HIP);
+        int rc = 0;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = LINOMPSS_ERR_HIP;
+        if (rc == 0) {
+            rc = lx_b_launch(p, p->wg_per_cu, p->order, cs);
+            if (hipStreamEndCapture(cs, &p->graph) != hipSuccess && rc == 0) rc = LINOMPSS_ERR_HIP;
+        }
+        if (rc == 0 && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = LINOMPSS_ERR_HIP;
+        (void)hipStreamDestroy(cs);
+        if (rc) return fail(rc);
+    }
+    *plan = p;
+    return 0;
+}
+
+extern "C" int linompss_batch_run(linompss_batch_plan *p, void *stream, linompss_batch_status *st)
+{
+    if (!p) return -1;
+    if (p->count == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->use_graph) {
+        LX_HIP_TRY(hipGraphLaunch(p->exec, s));
+    } else {
+        const int rc = lx_b_launch(p, p->wg_per_cu, p->order, s);
+        if (rc) return rc;
+    }
+    if (st) LX_HIP_TRY(hipMemcpyAsync(st, p->d_stat, sizeof(lx_bstat) * p->count, hipMemcpyDeviceToHost, s));
+    LX_HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int linompss_batch_plan_info(const linompss_batch_plan *p, int *grid, int *wg_per_cu, int *order,
+                                        double *tune_ms)
+{
+    if (!p) return -1;
+    if (grid) *grid = p->grid;
+    if (wg_per_cu) *wg_per_cu = p->wg_per_cu;
+    if (order) *order = p->order;
+    if (tune_ms) *tune_ms = p->tune_ms;
+    return 0;
+}
+
+extern "C" int linompss_batch_plan_destroy(linompss_batch_plan *p)
+{
+    if (p) {
+        (void)hipDeviceSynchronize();
+        lx_b_release(p);
+    }
+    return 0;
+}
+
+extern "C" int linompss_dposv_ir_batched_dev(int count, const int *n, const double *const *dA, const int64_t *lda,
+                                             const double *const *dB, double *const *dX, void *stream,
+                                             const linompss_batch_opts *opts, linompss_batch_status *st)
+{
+    linompss_batch_opts o = {30, 1e-14, 0.5, 1, 0, 0};  // one launch, no tuning, no graph
+    if (opts) o = *opts;
+    linompss_batch_plan *p = nullptr;
+    int rc = linompss_batch_plan_create(count, n, dA, lda, dB, dX, &o, &p);
+    if (rc) return rc;
+    rc = linompss_batch_run(p, stream, st);
+    linompss_batch_plan_destroy(p);
+    return rc;
+}
+#endif  // LX_BATCH_KERNELS_ONLY

@@ -199,6 +199,8 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
 
 }  // namespace
 
+std::mutex &lx_dev_mutex() { return g_mu; }
+
 void *lx_scratch(int slot, size_t bytes)
 {
     lx_dev_ctx *c = ctx_get();
@@ -400,6 +402,23 @@ extern "C" int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void 
         return lx_run_single(sh, const_cast<void *>(dL), lda, dX, ldx, (hipStream_t)stream, opts, &info);
     }
     int rc = lx_sweeps_dev(dtype, n, nrhs, dL, lda, dX, ldx, 3, (hipStream_t)stream);
+    if (rc) return rc;
+    LX_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int linompss_trsm_dev(int dtype, int trans, int n, int nrhs, const void *dL, int64_t ldl, void *dX,
+                                 int64_t ldx, void *stream)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (trans != 0 && trans != 1) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (ldl < std::max(1, n)) return -6;
+    if (ldx < std::max(1, n)) return -8;
+    if (n == 0 || nrhs == 0) return 0;
+    std::lock_guard<std::mutex> g(g_mu);
+    int rc = lx_sweeps_dev(dtype, n, nrhs, dL, ldl, dX, ldx, trans ? 2 : 1, (hipStream_t)stream);
     if (rc) return rc;
     LX_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     return 0;

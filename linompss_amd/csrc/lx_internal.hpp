@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "lx_plan.hpp"
 #include "linompss.h"
 
@@ -32,6 +34,10 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
 // LINOMPSS_ERR_SCHED if a wait inside a sweep timed out.
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
                   hipStream_t s);
+
+// The lock that serialises the executor's plans, its scratch slots and the
+// sweep kernels on a device.
+std::mutex &lx_dev_mutex();
 
 // Grow-only device scratch owned by the library (one per slot and device).
 void *lx_scratch(int slot, size_t bytes);

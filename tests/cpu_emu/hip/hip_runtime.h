@@ -171,6 +171,7 @@ inline long long wall_clock64()
 #define __HIP_MEMORY_SCOPE_AGENT __MEMORY_SCOPE_DEVICE
 #define __HIP_MEMORY_SCOPE_WORKGROUP __MEMORY_SCOPE_WRKGRP
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicAdd(unsigned *p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicOr(int *p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicMin(int *p, int v)
 {

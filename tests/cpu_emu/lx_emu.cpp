@@ -16,6 +16,8 @@
 // needs the HIP runtime; compile only its kernels here)
 #define LX_SWEEP_KERNELS_ONLY
 #include "lx_sweep.hip"
+#define LX_BATCH_KERNELS_ONLY
+#include "lx_batch.hip"
 
 thread_local lx_emu_dim3 threadIdx, blockIdx;
 lx_emu_dim3 blockDim, gridDim;
@@ -136,4 +138,28 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     for (int b = 0; b < nb; ++b)
         if (ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;
     return ctl[0] == (int)units.size() + 1 ? 0 : -2;
+}
+
+// Batched refinement solves through lx_batch_ir_kernel, one workgroup taking
+// every system from the queue.  stat: count x {sweeps, fallback, info, pad, berr}.
+extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, const long long *lda,
+                            const double *const *B, double *const *X, const int *order, void *stat, int max_sweeps,
+                            double tol, double stall)
+{
+    std::vector<lx_bsys> sys(count);
+    int nmax = 1;
+    for (int i = 0; i < count; ++i) {
+        sys[i] = lx_bsys{A[i], B[i], X[i], (int64_t)lda[i], n[i], 0};
+        nmax = std::max(nmax, n[i]);
+    }
+    const int64_t sb = lx_b_slot_bytes(nmax);
+    std::vector<double> ws((size_t)sb / 8 + 1, 0.0);
+    unsigned ticket = 0;
+    gridDim.x = 1;
+    blockDim.x = 256;
+    run_wg([&] {
+        lx_batch_ir_kernel(sys.data(), order, count, (char *)ws.data(), sb, nmax, &ticket, (lx_bstat *)stat,
+                           max_sweeps, tol, stall);
+    });
+    return ticket == (unsigned)count + 1 ? 0 : -1;
 }

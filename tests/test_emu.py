@@ -131,3 +131,63 @@ def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
         orc.ltrsm_lower(trans, np.asfortranarray(L.astype(np.float64)), Xo, 32)
         tol = 1e-13 if dtype == np.float64 else 1e-5
         assert np.abs(X - Xo).max() <= tol * np.abs(Xo).max()
+
+
+def _emu_batch(sizes, mats, rhs, max_sweeps=30, tol=1e-14, stall=0.5):
+    """lx_batch_ir_kernel on the CPU: one workgroup takes every system."""
+    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    cnt = len(sizes)
+    P = ctypes.c_void_p
+    X = [np.zeros(max(int(n), 1)) for n in sizes]
+    st = np.zeros(cnt, dtype=[("sweeps", "i4"), ("fallback", "i4"), ("info", "i4"), ("pad", "i4"), ("berr", "f8")])
+    order = np.argsort(-np.asarray(sizes), kind="stable").astype(np.int32)
+    rc = E.lx_emu_batch(cnt, (ctypes.c_int * cnt)(*[int(n) for n in sizes]),
+                        (P * cnt)(*[m.ctypes.data for m in mats]),
+                        (ctypes.c_longlong * cnt)(*[max(int(n), 1) for n in sizes]),
+                        (P * cnt)(*[b.ctypes.data for b in rhs]), (P * cnt)(*[x.ctypes.data for x in X]),
+                        order.ctypes.data_as(P), st.ctypes.data_as(P), max_sweeps, ctypes.c_double(tol),
+                        ctypes.c_double(stall))
+    assert rc == 0
+    return X, st
+
+
+def test_batch_kernel(orc):
+    """The batched refinement kernel (lx_batch.hip) on uneven orders, including
+    the empty system, one row, ragged 64-row blocks and the 128-row blocks of
+    the in-workgroup factor's trailing update."""
+    rng = np.random.default_rng(21)
+    sizes = [1, 37, 65, 130, 0]
+    mats, rhs = [], []
+    for n in sizes:
+        G = rng.standard_normal((n, n))
+        mats.append(np.asfortranarray(G @ G.T / max(n, 1) + 0.1 * np.eye(n)) if n else np.zeros(1))
+        rhs.append(rng.uniform(-1, 1, n) if n else np.zeros(1))
+    X, st = _emu_batch(sizes, mats, rhs)
+    for i, n in enumerate(sizes):
+        if n == 0:
+            assert tuple(st[i])[:3] == (0, 0, 0)
+            continue
+        A, b, x = mats[i], rhs[i], X[i]
+        be = orc.backward_error(A, x, b)
+        assert st["fallback"][i] == 0 and st["info"][i] == 0
+        assert be <= 1e-14 and st["berr"][i] <= 1e-14
+        xr = np.linalg.solve(A, b)
+        assert np.abs(x - xr).max() / np.abs(xr).max() <= np.linalg.cond(A) * 1e-14
+
+
+def test_batch_kernel_fallback(orc):
+    """Not positive definite: info from the FP64 factor; too ill-conditioned
+    for FP32 refinement: solved by the FP64 factor."""
+    rng = np.random.default_rng(22)
+    n = 70
+    G = rng.standard_normal((n, n))
+    A1 = np.asfortranarray(G @ G.T / n + 0.1 * np.eye(n))
+    A1[3, 3] = -1.0
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A2 = np.asfortranarray((Q * np.logspace(0, -9, n)) @ Q.T)
+    b1, b2 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    X, st = _emu_batch([n, n], [A1, A2], [b1, b2])
+    assert st["fallback"][0] == 1 and st["info"][0] == 4
+    assert st["fallback"][1] == 1 and st["info"][1] == 0
+    assert orc.backward_error(A2, X[1], b2) <= 1e-14
