@@ -166,6 +166,8 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
         }
         return;
     }
+    __shared__ double s_part[4][64];
+    const int cc = tid & 63, part = tid >> 6;  // column of the row update, and a quarter of the rows
     for (int j = jb - 1; j >= 0; --j) {
         const double d = sL[j][j];
         // cbar (rows > j of column j), and its dot with c (wave 0, lane = row)
@@ -185,19 +187,28 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
             }
         }
         __syncthreads();
-        const double pbar = s_red[0];
-        // Dbar[>j, <j] -= cbar L[j, <j]   and   Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j]
-        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
-            const int r = e & 63, c = e >> 6;
-            if (c >= j) continue;
-            if (r > j && r < jb) {
-                sX[r][c] -= sX[r][j] * sL[j][c];
-            } else if (r == j) {
-                double s = 2.0 * pbar * sL[j][c];
-                for (int i = j + 1; i < jb; ++i) s += sX[i][j] * sL[i][c];
-                sX[j][c] -= s;
+        // thread (column cc < j, rows 16 part .. 16 part + 15):
+        //   Dbar[r, cc] -= cbar_r L[j, cc] for rows r > j, and the partial
+        //   sum over those rows of cbar_r L[r, cc] for the row update below
+        double ps = 0.0;
+        if (cc < j) {
+            const double ljc = sL[j][cc];
+#pragma unroll 4
+            for (int q = 0; q < 16; ++q) {
+                const int r = part * 16 + q;
+                if (r > j && r < jb) {
+                    const double cb = sX[r][j];
+                    sX[r][cc] -= cb * ljc;
+                    ps += cb * sL[r][cc];
+                }
             }
         }
+        s_part[part][cc] = ps;
+        __syncthreads();
+        // Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j]
+        if (part == 0 && cc < j)
+            sX[j][cc] -= ((s_part[0][cc] + s_part[1][cc]) + (s_part[2][cc] + s_part[3][cc])) +
+                         2.0 * s_red[0] * sL[j][cc];
         __syncthreads();
     }
     for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {

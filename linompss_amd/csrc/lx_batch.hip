@@ -74,14 +74,29 @@ __device__ __forceinline__ double lx_b_max(double v, double *red)
     return m;
 }
 
-// L (ld) := lower triangle of A, converted to T.
+// L (ld) := lower triangle of A, converted to T, by 64 x 64 tiles: thread
+// (row l, columns 16g..16g+15) issues its 16 loads of a tile together.
 template <typename T>
 __device__ __forceinline__ void lx_b_lower_to(const double *__restrict__ A, int64_t lda, int n, T *__restrict__ L,
                                               int64_t ld)
 {
-    const int tid = lx_tid();
-    for (int c = 0; c < n; ++c)
-        for (int r = c + tid; r < n; r += LX_WG_THREADS) L[(int64_t)c * ld + r] = (T)A[(int64_t)c * lda + r];
+    const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
+    const int nb = (n + 63) / 64;
+    for (int J = 0; J < nb; ++J)
+        for (int I = J; I < nb; ++I) {
+            const int row = I * 64 + l;
+            double a[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int col = J * 64 + g * 16 + k;
+                a[k] = (row < n && col < n && row >= col) ? A[(int64_t)col * lda + row] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int col = J * 64 + g * 16 + k;
+                if (row < n && col < n && row >= col) L[(int64_t)col * ld + row] = (T)a[k];
+            }
+        }
     __syncthreads();
 }
 
@@ -96,7 +111,7 @@ __device__ __forceinline__ void lx_b_mirror(T *__restrict__ L, int64_t ld, int n
         for (int I = J; I < nb; ++I) {
             // load: thread (row l, columns 16g..16g+15) of tile (I, J)
             const int row = I * 64 + l;
-#pragma unroll 4
+#pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int cl = g * 16 + k, col = J * 64 + cl;
                 T a = T(0);
@@ -105,7 +120,7 @@ __device__ __forceinline__ void lx_b_mirror(T *__restrict__ L, int64_t ld, int n
             }
             __syncthreads();
             // store: U[J*64 + l][I*64 + rr] = L[I*64 + rr][J*64 + l] for rows rr of group g
-#pragma unroll 4
+#pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int rr = g * 16 + k, urow = J * 64 + l, ucol = I * 64 + rr;
                 if (ucol < n && urow < n && ucol > urow) L[(int64_t)ucol * ld + urow] = sT[rr][l];
@@ -117,49 +132,64 @@ __device__ __forceinline__ void lx_b_mirror(T *__restrict__ L, int64_t ld, int n
 // One triangular sweep on v (n entries): trans 0 solves L y = v, trans 1
 // solves L^T x = v; the strict upper triangle of L holds L^T (lx_b_mirror) and
 // W the inverses Z_q of the 64 x 64 diagonal blocks (W[c*ldw + 64q + r] = Z_q[r][c]).
+// Right-looking over 64-row blocks, v held in LDS: y_q = Z_q v_q (Z_q^T for
+// trans 1), then every row still to be solved takes v_r -= sum_c M[r][c] y_q[c]
+// with M[r][c] = L[(64q + c) ld + r] in both directions (the lower triangle
+// below the block, or the mirrored upper triangle above it): one thread per
+// row, its 64 loads independent of each other.
 template <typename T>
 __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, const T *__restrict__ W,
                                            int64_t ldw, int n, int trans, T *__restrict__ v, char *smem)
 {
     T(*sZ)[65] = reinterpret_cast<T(*)[65]>(smem);
     T(*sred)[64] = reinterpret_cast<T(*)[64]>(smem + sizeof(T) * 64 * 65);
-    T *st = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 4 * 64));
-    T *sy = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 5 * 64));  // solved entries, n
+    T *sy = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 4 * 64));  // y_q, 64
+    T *sv = reinterpret_cast<T *>(smem + sizeof(T) * (64 * 65 + 5 * 64));  // v, n
     const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
     const int nb = (n + 63) / 64;
+    for (int i = tid; i < n; i += LX_WG_THREADS) sv[i] = v[i];
     for (int qi = 0; qi < nb; ++qi) {
         const int q = trans ? nb - 1 - qi : qi, p = q * 64, pb = n - p < 64 ? n - p : 64;
-        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
-            const int r = e & 63, c = e >> 6;
+#pragma unroll
+        for (int q2 = 0; q2 < 64 * 64 / LX_WG_THREADS; ++q2) {
+            const int e = tid + q2 * LX_WG_THREADS, r = e & 63, c = e >> 6;
             sZ[r][c] = W[(int64_t)c * ldw + p + r];
         }
-        // t = v_q - (coupling rows) . (solved entries): columns k of group g
-        T s = T(0);
-        if (l < pb) {
-            const int kb = trans ? p + pb : 0, ke = trans ? n : p;
-            const T *Lr = L + p + l;
-            for (int k = kb + g; k < ke; k += 4) s += Lr[(int64_t)k * ld] * sy[k];
-        }
-        sred[g][l] = s;
-        __syncthreads();
-        if (g == 0) st[l] = l < pb ? v[p + l] - ((sred[0][l] + sred[1][l]) + (sred[2][l] + sred[3][l])) : T(0);
-        __syncthreads();
-        // y_q = Z_q t (trans 0) or Z_q^T t (trans 1)
+        __syncthreads();  // sZ, and sv after the previous block's updates
         T z = T(0);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int c = g * 16 + k;
-            z += (trans ? sZ[c][l] : sZ[l][c]) * st[c];
+            z += (trans ? sZ[c][l] : sZ[l][c]) * (c < pb ? sv[p + c] : T(0));
         }
         sred[g][l] = z;
         __syncthreads();
-        if (g == 0 && l < pb) {
+        if (g == 0) {
             const T y = (sred[0][l] + sred[1][l]) + (sred[2][l] + sred[3][l]);
-            sy[p + l] = y;
-            v[p + l] = y;
+            sy[l] = l < pb ? y : T(0);
+            if (l < pb) sv[p + l] = y;
         }
         __syncthreads();
+        // rows still to be solved: [p + pb, n) forward, [0, p) backward
+        const int rlo = trans ? 0 : p + pb, rhi = trans ? p : n;
+        for (int r = rlo + tid; r < rhi; r += LX_WG_THREADS) {
+            const T *Lr = L + (int64_t)p * ld + r;
+            T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int c0 = 0; c0 < 64; c0 += 16) {
+                T a[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lr[(int64_t)(c0 + k) * ld] : T(0);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
+                __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
+            }
+            sv[r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        }
     }
+    __syncthreads();
+    for (int i = tid; i < n; i += LX_WG_THREADS) v[i] = sv[i];
+    __syncthreads();
 }
 
 // r = b - A x (FP64, A symmetric with its lower triangle stored).  Returns
@@ -190,7 +220,7 @@ __device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t
         for (int I = J; I < nb; ++I) {
             const int row = I * 64 + l;
             double s = 0.0, sa = 0.0;
-#pragma unroll 4
+#pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int cl = g * 16 + k, col = J * 64 + cl;
                 double a = 0.0;
@@ -206,7 +236,7 @@ __device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t
             __syncthreads();
             // (A_IJ^T x_I): thread (column l of the tile, rows 16g..16g+15)
             double t = 0.0, ta = 0.0;
-#pragma unroll 4
+#pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int rr = g * 16 + k;
                 const double a = sT[l][rr];
@@ -338,8 +368,11 @@ __host__ __device__ inline int64_t lx_b_slot_bytes(int nmax)
 }
 
 // Persistent launch: workgroups take systems order[0], order[1], ... from
-// *ticket (zero at launch) until all count are done.
-__global__ void __launch_bounds__(LX_WG_THREADS, 2)
+// *ticket (zero at launch) until all count are done.  MINB = resident
+// workgroups per CU the registers are budgeted for (1: no spills, 2: twice
+// the latency hiding); the autotuner picks.
+template <int MINB>
+__global__ void __launch_bounds__(LX_WG_THREADS, MINB)
 lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ order, int count, char *ws,
                    int64_t slot_bytes, int nmax, unsigned *ticket, lx_bstat *stat, int max_sweeps, double tol,
                    double stall)
@@ -400,9 +433,14 @@ int lx_b_launch(linompss_batch_plan *p, int wg, int ord, hipStream_t s)
 {
     const int grid = std::min(p->count, std::max(1, wg * p->cus));
     hipLaunchKernelGGL(lx_batch_reset_kernel, dim3(1), dim3(1), 0, s, p->d_ticket);
-    hipLaunchKernelGGL(lx_batch_ir_kernel, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
-                       p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
-                       p->stall);
+    if (wg == 1)
+        hipLaunchKernelGGL(lx_batch_ir_kernel<1>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
+                           p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
+                           p->stall);
+    else
+        hipLaunchKernelGGL(lx_batch_ir_kernel<2>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
+                           p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
+                           p->stall);
     LX_HIP_TRY(hipGetLastError());
     return 0;
 }

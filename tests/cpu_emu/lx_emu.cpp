@@ -18,6 +18,8 @@
 #include "lx_sweep.hip"
 #define LX_BATCH_KERNELS_ONLY
 #include "lx_batch.hip"
+#define LX_ADJOINT_KERNELS_ONLY
+#include "lx_adjoint.hip"
 
 thread_local lx_emu_dim3 threadIdx, blockIdx;
 lx_emu_dim3 blockDim, gridDim;
@@ -158,8 +160,45 @@ extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, con
     gridDim.x = 1;
     blockDim.x = 256;
     run_wg([&] {
-        lx_batch_ir_kernel(sys.data(), order, count, (char *)ws.data(), sb, nmax, &ticket, (lx_bstat *)stat,
+        lx_batch_ir_kernel<2>(sys.data(), order, count, (char *)ws.data(), sb, nmax, &ticket, (lx_bstat *)stat,
                            max_sweeps, tol, stall);
     });
     return ticket == (unsigned)count + 1 ? 0 : -1;
+}
+
+// The adjoint's diagonal-block kernel (mode 0: inverse, mode 1: level-2 adjoint), one workgroup.
+extern "C" void lx_emu_adj_diag(int mode, int jb, const double *L, long long ldl, double *Ab, long long lda,
+                                double *Zb, double *Sb)
+{
+    gridDim.x = 1;
+    blockDim.x = 256;
+    run_wg([&] { lx_adj_diag_kernel(mode, jb, L, ldl, Ab, lda, Zb, Sb); });
+}
+
+// The adjoint's GEMM kernel over its whole grid, one workgroup at a time.
+extern "C" void lx_emu_adj_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, long long lda,
+                                const double *B, long long ldb, double beta, double *C, long long ldc, int lower,
+                                int ksplit, int kchunk, double *part)
+{
+    const int tm = (M + 63) / 64, tn = (N + 63) / 64;
+    gridDim.x = tm;
+    gridDim.y = tn;
+    gridDim.z = ksplit;
+    blockDim.x = 256;
+    for (int z = 0; z < ksplit; ++z)
+        for (int y = 0; y < tn; ++y)
+            for (int x = 0; x < tm; ++x)
+                run_wg([&] {
+                    blockIdx.x = x;
+                    blockIdx.y = y;
+                    blockIdx.z = z;
+                    if (ta == 0 && tb == 0)
+                        lx_adj_gemm_kernel<0, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                    else if (ta == 0)
+                        lx_adj_gemm_kernel<0, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                    else if (tb == 0)
+                        lx_adj_gemm_kernel<1, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                    else
+                        lx_adj_gemm_kernel<1, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                });
 }

@@ -191,3 +191,67 @@ def test_batch_kernel_fallback(orc):
     assert st["fallback"][0] == 1 and st["info"][0] == 4
     assert st["fallback"][1] == 1 and st["info"][1] == 0
     assert orc.backward_error(A2, X[1], b2) <= 1e-14
+
+
+def _emu_lib():
+    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    P, I, L, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+    E.lx_emu_adj_diag.argtypes = [I, I, P, L, P, L, P, P]
+    E.lx_emu_adj_gemm.argtypes = [I, I, I, I, I, D, P, L, P, L, D, P, L, I, I, I, P]
+    return E
+
+
+@pytest.mark.parametrize("jb", [64, 37, 1])
+def test_adjoint_diag_kernel(jb):
+    """lx_adj_diag_kernel: mode 0 = the block inverse, mode 1 = the level-2
+    Cholesky adjoint of the block (oracle chol_level2_adjoint) and P + P^T."""
+    from oracle import adjoint as adj
+    E = _emu_lib()
+    rng = np.random.default_rng(jb)
+    G = rng.standard_normal((jb, jb))
+    Lb = np.asfortranarray(np.linalg.cholesky(G @ G.T / jb + np.eye(jb)))
+    Db = np.asfortranarray(np.tril(rng.standard_normal((jb, jb))))
+    Z = np.zeros((64, 64), order="F")
+    S = np.zeros((64, 64), order="F")
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    E.lx_emu_adj_diag(0, jb, p(Lb), jb, p(Db), jb, p(Z), p(S))
+    assert np.abs(Z[:jb, :jb] - np.linalg.inv(Lb)).max() <= 1e-12 * np.abs(np.linalg.inv(Lb)).max()
+    assert np.array_equal(Z[jb:, jb:], np.eye(64 - jb))
+    ref = adj.chol_level2_adjoint(Lb, Db)
+    out = Db.copy(order="F")
+    E.lx_emu_adj_diag(1, jb, p(Lb), jb, p(out), jb, p(Z), p(S))
+    assert np.abs(np.tril(out) - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(np.triu(out, 1), np.triu(Db, 1))
+    assert np.abs(S[:jb, :jb] - (ref + ref.T)).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_adjoint_gemm_kernel(ta, tb):
+    """lx_adj_gemm_kernel: C = alpha op(A) op(B) + beta C on ragged shapes, the
+    lower-only mask, and split-K partials."""
+    E = _emu_lib()
+    rng = np.random.default_rng(10 * ta + tb)
+    M, N, K = 70, 45, 37
+    op = lambda X, t: X.T if t else X
+    A = np.asfortranarray(rng.standard_normal((K, M) if ta else (M, K)))
+    B = np.asfortranarray(rng.standard_normal((N, K) if tb else (K, N)))
+    C0 = np.asfortranarray(rng.standard_normal((M, N)))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    C = C0.copy(order="F")
+    E.lx_emu_adj_gemm(ta, tb, M, N, K, -1.5, p(A), A.shape[0], p(B), B.shape[0], 0.5, p(C), M, 0, 1, 48, None)
+    ref = -1.5 * op(A, ta) @ op(B, tb) + 0.5 * C0
+    assert np.abs(C - ref).max() <= 1e-12 * np.abs(ref).max()
+    # lower-only: entries above the diagonal untouched
+    C = C0.copy(order="F")
+    E.lx_emu_adj_gemm(ta, tb, M, N, K, 1.0, p(A), A.shape[0], p(B), B.shape[0], 1.0, p(C), M, 1, 1, 48, None)
+    ref = op(A, ta) @ op(B, tb) + C0
+    low = np.tril(np.ones((M, N), dtype=bool))
+    assert np.abs(C[low] - ref[low]).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(C[~low], C0[~low])
+    # split-K: three chunks of 16 (the last ragged), partials summed in order
+    part = np.zeros(3 * M * N)
+    E.lx_emu_adj_gemm(ta, tb, M, N, K, 2.0, p(A), A.shape[0], p(B), B.shape[0], 0.0, p(C), M, 0, 3, 16, p(part))
+    tot = part.reshape(3, N, M).sum(0).T
+    ref = 2.0 * op(A, ta) @ op(B, tb)
+    assert np.abs(tot - ref).max() <= 1e-12 * np.abs(ref).max()
