@@ -300,6 +300,7 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--fp64-b", type=int, default=0, help="tile of the plain FP64 factor (0: same as --b)")
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -381,7 +382,7 @@ def main():
         lx.copy_dev(X2, B)
         lx.lib.linompss_dev_sync()
         t0 = time.perf_counter()
-        info = lx.posv_dev(A2, X2, b)
+        info = lx.posv_dev(A2, X2, args.fp64_b or b)
         lx.lib.linompss_dev_sync()
         t64.append(time.perf_counter() - t0)
         st64 = lx.last_exec_stats()
@@ -450,6 +451,7 @@ def main():
         "factor_ms": round(float(np.mean(t_fac)) * 1e3, 3),
         "factor_launch_ms": round(t_kernel * 1e3, 3),
         "fp64_posv_ms": round(t_fp64 * 1e3, 3),
+        "fp64_b": args.fp64_b or b,
         "fp64_posv_gflops": round(flops / t_fp64 / 1e9, 2),
         "fp64_berr": berr64,
         "speedup_vs_fp64": round(t_fp64 / t_ir, 3),

@@ -642,9 +642,11 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
     int first_bad = 0;
     for (int p = 0; p < M; p += PB) {
         const int pb = M - p < PB ? M - p : PB;
-        // load D (full square, identity pad past pb); clear Z
-        for (int e = tid; e < PB * PB; e += LX_WG_THREADS) {
-            const int r = e % PB, c = e / PB;
+        // load D (full square, identity pad past pb); clear Z.  A constant trip
+        // count, unrolled, so several loads are in flight per thread
+#pragma unroll 4
+        for (int q = 0; q < PB * PB / LX_WG_THREADS; ++q) {
+            const int e = tid + q * LX_WG_THREADS, r = e % PB, c = e / PB;
             sD[r][c] = (r < pb && c < pb) ? A[(long long)(p + c) * lda + p + r] : (r == c ? T(1) : T(0));
             sZ[r][c] = T(0);
         }
@@ -758,9 +760,11 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
     const bool active = r0 < R;  // wave-uniform
     for (int q = 0; q < N; q += QB) {
         const int qb = N - q < QB ? N - q : QB;
-        // stage Z_q (column-major in W: Z[r][c] at W[c * ldw + q + r])
-        for (int e = tid; e < QB * QB; e += LX_WG_THREADS) {
-            const int r = e % QB, c = e / QB;
+        // stage Z_q (column-major in W: Z[r][c] at W[c * ldw + q + r]), several
+        // loads in flight per thread
+#pragma unroll 4
+        for (int q2 = 0; q2 < QB * QB / LX_WG_THREADS; ++q2) {
+            const int e = tid + q2 * LX_WG_THREADS, r = e % QB, c = e / QB;
             sZq[r][c] = W[(long long)c * ldw + q + r];
         }
         // this block's right-hand sides, loaded before the long update
