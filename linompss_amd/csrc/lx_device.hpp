@@ -323,11 +323,13 @@ __device__ __forceinline__ void lx_unit_gemm_lds(T *__restrict__ C, long long ld
         }
     } else {
         // ragged unit (edge tiles): element loads straight into LDS, rows past
-        // M / N clamped (never stored), columns past K zero
+        // M / N clamped (never stored), columns past K zero; a constant trip
+        // count, unrolled, so the loads of a slab are in flight together
         for (int sl = 0; sl < nslab; ++sl) {
             const int k0 = sl * KS;
-            for (int e = tid; e < 128 * KS; e += LX_WG_THREADS) {
-                const int r = e % 128, kk = e / 128;
+#pragma unroll 8
+            for (int q = 0; q < 128 * KS / LX_WG_THREADS; ++q) {
+                const int e = tid + q * LX_WG_THREADS, r = e % 128, kk = e / 128;
                 const bool kv = k0 + kk < K;
                 const int kc = kv ? k0 + kk : K - 1;
                 const T va = A[(long long)kc * lda + (r < M ? r : M - 1)];
@@ -541,6 +543,17 @@ __device__ __forceinline__ void lx_wave_mm_ldsb(typename lx_mfma<T>::acc (&acc)[
     const int lane = lx_lane(), lr = lane & 15, lk = lane >> 4;
     const int r = r0 + lr < M ? r0 + lr : M - 1;
     const T *pa = A + r;
+    if (K == 64) {  // uniform: the POTRF panel; all 16 loads in flight at once
+        T a[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) a[s] = pa[(long long)(4 * s + lk) * lda];
+#pragma unroll
+        for (int s = 0; s < 16; ++s)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[0][j] = lx_mfma<T>::mac(a[s], sB[(j * 16 + lr) * ldsb + 4 * s + lk], acc[0][j]);
+        return;
+    }
 #pragma unroll 2
     for (int k0 = 0; k0 < K; k0 += 4) {
         const T a = pa[(long long)(k0 + lk) * lda];
@@ -735,27 +748,112 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
 }
 
 // ---------------------------------------------------------------------------
+// The TRSM unit's update of one 64-column block:
+//   acc (wave w: rows 16w..16w+15, the block's 64 columns) += X[:, <K] Lq[:, <K]^T
+// with X the unit's R <= 64 rows (ld ldx) and Lq the block's qb <= 64 rows of
+// L (ld ldl), K a multiple of 64.  Both operands go through LDS in K-slabs
+// shared by the four waves ([k][row], double buffered; 16-byte loads of slab
+// s+1 in flight while slab s is multiplied), so each element of L is loaded
+// once per workgroup instead of once per wave, and the slab loads overlap.
+// Ragged blocks (R or qb < 64) take element loads with the rows clamped (the
+// clamped rows are never stored).  LDS: 4 * KS * STR * sizeof(T) bytes
+// (lx_trsm_upd_smem<T>::bytes), any 16-byte aligned arena.
+template <typename T> struct lx_trsm_upd_smem {
+    static constexpr int STR = sizeof(T) == 8 ? 68 : 80;
+    static constexpr int bytes = 4 * lx_lgemm_cfg<T>::KS * STR * (int)sizeof(T);
+};
+template <typename T>
+__device__ __forceinline__ void lx_trsm_update_lds(typename lx_mfma<T>::acc (&acc)[1][4], const T *__restrict__ X,
+                                                   long long ldx, int R, const T *__restrict__ Lq, long long ldl,
+                                                   int qb, int K, char *smem)
+{
+    constexpr int KS = lx_lgemm_cfg<T>::KS, VN = lx_vec16<T>::n, STR = lx_trsm_upd_smem<T>::STR;
+    typedef typename lx_vec16<T>::type V;
+    constexpr int VPC = 64 / VN;                        // vectors per slab column
+    constexpr int NV = 64 * KS / VN / LX_WG_THREADS;    // vectors per thread per operand and slab
+    T(*sX)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem);
+    T(*sL)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem + 2 * KS * STR * sizeof(T));
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    const int nslab = K / KS;
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            const T a = sX[buf][kk + lk][16 * w + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[0][j] = lx_mfma<T>::mac(a, sL[buf][kk + lk][16 * j + lr], acc[0][j]);
+        }
+    };
+    if (R == 64 && qb == 64) {  // uniform
+        V rx[NV], rl[NV];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+                rx[i] = *(const V *)(X + (long long)(k0 + kk) * ldx + r);
+                rl[i] = *(const V *)(Lq + (long long)(k0 + kk) * ldl + r);
+            }
+        };
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    sX[buf][kk][r + e] = rx[i][e];
+                    sL[buf][kk][r + e] = rl[i][e];
+                }
+            }
+        };
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int buf = sl & 1;
+            if (sl + 1 < nslab) gload((sl + 1) * KS);
+            compute(buf);
+            if (sl + 1 < nslab) lstore(buf ^ 1);
+            __syncthreads();
+        }
+    } else {
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int k0 = sl * KS;
+#pragma unroll 4
+            for (int q = 0; q < 64 * KS / LX_WG_THREADS; ++q) {
+                const int e = tid + q * LX_WG_THREADS, r = e % 64, kk = e / 64;
+                sX[0][kk][r] = X[(long long)(k0 + kk) * ldx + (r < R ? r : R - 1)];
+                sL[0][kk][r] = Lq[(long long)(k0 + kk) * ldl + (r < qb ? r : qb - 1)];
+            }
+            __syncthreads();
+            compute(0);
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Panel TRSM unit (reference task_dtrsm(R, L, T, N), chol_rlmain.c:37 ->
 // tasks_trsm.c:22-25):  X[R x N] = B[R x N] * L^{-T}, R <= 64, rows
 // independent.  Wavefront w owns rows 16w..16w+15 and the workgroup walks the
 // 64-column blocks q left to right:
-//   Y = B_q - X_{<q} L[q, <q]^T   (matrix cores, K = 64 q, operands from L2)
+//   Y = B_q - X_{<q} L[q, <q]^T   (matrix cores, K = 64 q, operands staged
+//                                   through LDS by lx_trsm_update_lds)
 //   X_q = Y Z_q^T                  (Z_q = inverse of the diagonal 64 x 64
 //                                   block, written to W by the POTRF unit and
 //                                   staged in LDS once per block)
 // Every wave runs every block (waves past R compute clamped rows and store
-// nothing), so the two barriers per block match in all four.
-// LDS: (64 * 65 + 4 * 16 * 65) * sizeof(T).
+// nothing), so the barriers per block match in all four.
+// LDS: 64 * 65 * sizeof(T) + max(4 * 16 * 65 * sizeof(T), lx_trsm_upd_smem<T>::bytes).
 template <typename T>
 __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L,
                                                   long long ldl, const T *__restrict__ W, long long ldw, int R, int N,
                                                   char *smem)
 {
     constexpr int QB = 64, LD = 65;
-    constexpr int U = sizeof(T) == 8 ? 2 : 4;
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
     T(*sZq)[LD] = reinterpret_cast<T(*)[LD]>(smem);  // sZq[c][k] = Z_q[c][k]
-    T(*sY)[LD] = reinterpret_cast<T(*)[LD]>(smem + sizeof(T) * (QB * LD + 16 * LD * w));
+    // the update's slabs, then (after a barrier) sY, both past sZq
+    char *sU = smem + sizeof(T) * QB * LD;
+    T(*sY)[LD] = reinterpret_cast<T(*)[LD]>(sU + sizeof(T) * 16 * LD * w);
     const int r0 = w * 16;
     const bool active = r0 < R;  // wave-uniform
     for (int q = 0; q < N; q += QB) {
@@ -779,8 +877,8 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
             }
         typename lx_mfma<T>::acc acc[1][4];
         lx_acc_zero<T, 1, 4>(acc);
-        // X_{<q} L[q, <q]^T
-        if (active && q > 0) lx_wave_gemm_nt<T, 1, 4, U>(acc, Bm, ldb, L + q, ldl, r0, 0, R, qb, q);
+        // X_{<q} L[q, <q]^T through LDS (every wave takes part in the staging)
+        if (q > 0) lx_trsm_update_lds<T>(acc, Bm, ldb, R, L + q, ldl, qb, q, sU);  // ends with a barrier
         // Y = B_q - acc -> LDS (row-major, zero past the block)
 #pragma unroll
         for (int j = 0; j < 4; ++j)

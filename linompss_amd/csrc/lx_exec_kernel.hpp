@@ -23,10 +23,14 @@
 enum lx_family : int { LX_FAM_FACTOR = 0, LX_FAM_SOLVE = 1, LX_FAM_ALL = 2 };
 template <typename T, int FAM> struct lx_fam_smem {
     // POTRF (2 * 64 * 65 * sizeof(T) + 16; its trailing update reuses the space
-    // for the staged GEMM), TRSM (2 * 64 * 65 * sizeof(T)), staged GEMM
+    // for the staged GEMM), TRSM (Z_q, then its update slabs or Y), staged GEMM
     static constexpr int potrf = 2 * 64 * 65 * (int)sizeof(T) + 16;
     static constexpr int gemm = lx_lgemm_smem<T>::bytes > lx_bx_smem::bytes ? lx_lgemm_smem<T>::bytes : lx_bx_smem::bytes;
-    static constexpr int factor = potrf > gemm ? potrf : gemm;
+    static constexpr int trsm = 64 * 65 * (int)sizeof(T) + (4 * 16 * 65 * (int)sizeof(T) > lx_trsm_upd_smem<T>::bytes
+                                                                 ? 4 * 16 * 65 * (int)sizeof(T)
+                                                                 : lx_trsm_upd_smem<T>::bytes);
+    static constexpr int factor0 = potrf > gemm ? potrf : gemm;
+    static constexpr int factor = factor0 > trsm ? factor0 : trsm;
     static constexpr int solve = 42 * 1024;
     static constexpr int bytes = FAM == LX_FAM_FACTOR ? factor : FAM == LX_FAM_SOLVE ? solve : (factor > solve ? factor : solve);
 };
