@@ -37,7 +37,7 @@ def emu():
 
 
 @pytest.mark.parametrize("what", [0, 1])
-@pytest.mark.parametrize("n,b", [(1, 1), (8, 4), (37, 8), (200, 256)])
+@pytest.mark.parametrize("n,b", [(1, 1), (8, 4), (37, 16), (200, 256)])
 def test_factor_fp64(emu, orc, what, n, b):
     A0 = orc.genmat_spd(n, 0, diag_mode=1)
     A = A0.copy(order="F")
@@ -72,7 +72,7 @@ def test_factor_fp32_and_policy0(emu, orc):
     assert np.linalg.norm(np.tril(Af) - Lref) / np.linalg.norm(Lref) <= 1e-6
 
 
-@pytest.mark.parametrize("n,b,nr", [(37, 8, 2)])
+@pytest.mark.parametrize("n,b,nr", [(37, 16, 2)])
 def test_posv_single_launch(emu, orc, n, b, nr):
     A0 = orc.genmat_spd(n, 0, diag_mode=1)
     B0 = np.asfortranarray(A0 @ np.ones((n, nr)))
@@ -180,7 +180,7 @@ def test_batch_kernel_fallback(orc):
     """Not positive definite: info from the FP64 factor; too ill-conditioned
     for FP32 refinement: solved by the FP64 factor."""
     rng = np.random.default_rng(22)
-    n = 70
+    n = 50
     G = rng.standard_normal((n, n))
     A1 = np.asfortranarray(G @ G.T / n + 0.1 * np.eye(n))
     A1[3, 3] = -1.0
