@@ -626,6 +626,89 @@ __device__ __forceinline__ void lx_tri_inv64(T (*sL)[65], T (*sZ)[65])
 }
 
 // ---------------------------------------------------------------------------
+// The TRSM unit's update of one 64-column block:
+//   acc (wave w: rows 16w..16w+15, the block's 64 columns) += X[:, <K] Lq[:, <K]^T
+// with X the unit's R <= 64 rows (ld ldx) and Lq the block's qb <= 64 rows of
+// L (ld ldl), K a multiple of 64.  Both operands go through LDS in K-slabs
+// shared by the four waves ([k][row], double buffered; 16-byte loads of slab
+// s+1 in flight while slab s is multiplied), so each element of L is loaded
+// once per workgroup instead of once per wave, and the slab loads overlap.
+// Ragged blocks (R or qb < 64) take element loads with the rows clamped (the
+// clamped rows are never stored).  LDS: 4 * KS * STR * sizeof(T) bytes
+// (lx_trsm_upd_smem<T>::bytes), any 16-byte aligned arena.
+template <typename T> struct lx_trsm_upd_smem {
+    static constexpr int STR = sizeof(T) == 8 ? 68 : 80;
+    static constexpr int bytes = 4 * lx_lgemm_cfg<T>::KS * STR * (int)sizeof(T);
+};
+template <typename T>
+__device__ __forceinline__ void lx_trsm_update_lds(typename lx_mfma<T>::acc (&acc)[1][4], const T *__restrict__ X,
+                                                   long long ldx, int R, const T *__restrict__ Lq, long long ldl,
+                                                   int qb, int K, char *smem)
+{
+    constexpr int KS = lx_lgemm_cfg<T>::KS, VN = lx_vec16<T>::n, STR = lx_trsm_upd_smem<T>::STR;
+    typedef typename lx_vec16<T>::type V;
+    constexpr int VPC = 64 / VN;                        // vectors per slab column
+    constexpr int NV = 64 * KS / VN / LX_WG_THREADS;    // vectors per thread per operand and slab
+    T(*sX)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem);
+    T(*sL)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem + 2 * KS * STR * sizeof(T));
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    const int nslab = K / KS;
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            const T a = sX[buf][kk + lk][16 * w + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[0][j] = lx_mfma<T>::mac(a, sL[buf][kk + lk][16 * j + lr], acc[0][j]);
+        }
+    };
+    if (R == 64 && qb == 64) {  // uniform
+        V rx[NV], rl[NV];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+                rx[i] = *(const V *)(X + (long long)(k0 + kk) * ldx + r);
+                rl[i] = *(const V *)(Lq + (long long)(k0 + kk) * ldl + r);
+            }
+        };
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    sX[buf][kk][r + e] = rx[i][e];
+                    sL[buf][kk][r + e] = rl[i][e];
+                }
+            }
+        };
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int buf = sl & 1;
+            if (sl + 1 < nslab) gload((sl + 1) * KS);
+            compute(buf);
+            if (sl + 1 < nslab) lstore(buf ^ 1);
+            __syncthreads();
+        }
+    } else {
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int k0 = sl * KS;
+#pragma unroll 4
+            for (int q = 0; q < 64 * KS / LX_WG_THREADS; ++q) {
+                const int e = tid + q * LX_WG_THREADS, r = e % 64, kk = e / 64;
+                sX[0][kk][r] = X[(long long)(k0 + kk) * ldx + (r < R ? r : R - 1)];
+                sL[0][kk][r] = Lq[(long long)(k0 + kk) * ldl + (r < qb ? r : qb - 1)];
+            }
+            __syncthreads();
+            compute(0);
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Diagonal-tile Cholesky (reference task_dpotrf -> dpotrf_("L"), tasks_potrf.c
 // :20-24), blocked right-looking inside the tile with 64-wide panels.  For
 // each panel the 64 x 64 diagonal block D sits in LDS (rows past the tile are
@@ -729,7 +812,22 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
         // 4) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel, K = pb,
         // in staged 128 x 128 blocks (D and Z are no longer needed: the blocks
         // take the whole LDS arena)
-        if (nrow > 0) {
+        if (nrow > 0 && nrow <= 64 && pb == 64) {  // uniform: the last panel of a 128-row tile
+            // one 64 x 64 block through the shared LDS slabs of the TRSM update,
+            // then C -= acc on and below the diagonal
+            const T *Pp = A + (long long)p * lda + r_beg;
+            T *Ct = A + (long long)r_beg * lda + r_beg;
+            typename lx_mfma<T>::acc acc[1][4];
+            lx_acc_zero<T, 1, 4>(acc);
+            lx_trsm_update_lds<T>(acc, Pp, lda, nrow, Pp, lda, nrow, PB, smem);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * w + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                    if (r < nrow && c <= r) Ct[(long long)c * lda + r] -= acc[0][j][e];
+                }
+        } else if (nrow > 0) {
             constexpr int BS = 128;
             const T *Pp = A + (long long)p * lda + r_beg;
             T *Ct = A + (long long)r_beg * lda + r_beg;
@@ -745,89 +843,6 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
         __syncthreads();
     }
     *info_local = first_bad;
-}
-
-// ---------------------------------------------------------------------------
-// The TRSM unit's update of one 64-column block:
-//   acc (wave w: rows 16w..16w+15, the block's 64 columns) += X[:, <K] Lq[:, <K]^T
-// with X the unit's R <= 64 rows (ld ldx) and Lq the block's qb <= 64 rows of
-// L (ld ldl), K a multiple of 64.  Both operands go through LDS in K-slabs
-// shared by the four waves ([k][row], double buffered; 16-byte loads of slab
-// s+1 in flight while slab s is multiplied), so each element of L is loaded
-// once per workgroup instead of once per wave, and the slab loads overlap.
-// Ragged blocks (R or qb < 64) take element loads with the rows clamped (the
-// clamped rows are never stored).  LDS: 4 * KS * STR * sizeof(T) bytes
-// (lx_trsm_upd_smem<T>::bytes), any 16-byte aligned arena.
-template <typename T> struct lx_trsm_upd_smem {
-    static constexpr int STR = sizeof(T) == 8 ? 68 : 80;
-    static constexpr int bytes = 4 * lx_lgemm_cfg<T>::KS * STR * (int)sizeof(T);
-};
-template <typename T>
-__device__ __forceinline__ void lx_trsm_update_lds(typename lx_mfma<T>::acc (&acc)[1][4], const T *__restrict__ X,
-                                                   long long ldx, int R, const T *__restrict__ Lq, long long ldl,
-                                                   int qb, int K, char *smem)
-{
-    constexpr int KS = lx_lgemm_cfg<T>::KS, VN = lx_vec16<T>::n, STR = lx_trsm_upd_smem<T>::STR;
-    typedef typename lx_vec16<T>::type V;
-    constexpr int VPC = 64 / VN;                        // vectors per slab column
-    constexpr int NV = 64 * KS / VN / LX_WG_THREADS;    // vectors per thread per operand and slab
-    T(*sX)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem);
-    T(*sL)[KS][STR] = reinterpret_cast<T(*)[KS][STR]>(smem + 2 * KS * STR * sizeof(T));
-    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
-    const int nslab = K / KS;
-    auto compute = [&](int buf) {
-#pragma unroll
-        for (int kk = 0; kk < KS; kk += 4) {
-            const T a = sX[buf][kk + lk][16 * w + lr];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[0][j] = lx_mfma<T>::mac(a, sL[buf][kk + lk][16 * j + lr], acc[0][j]);
-        }
-    };
-    if (R == 64 && qb == 64) {  // uniform
-        V rx[NV], rl[NV];
-        auto gload = [&](int k0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
-                rx[i] = *(const V *)(X + (long long)(k0 + kk) * ldx + r);
-                rl[i] = *(const V *)(Lq + (long long)(k0 + kk) * ldl + r);
-            }
-        };
-        auto lstore = [&](int buf) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                const int q = tid + i * LX_WG_THREADS, kk = q / VPC, r = (q % VPC) * VN;
-#pragma unroll
-                for (int e = 0; e < VN; ++e) {
-                    sX[buf][kk][r + e] = rx[i][e];
-                    sL[buf][kk][r + e] = rl[i][e];
-                }
-            }
-        };
-        gload(0);
-        lstore(0);
-        __syncthreads();
-        for (int sl = 0; sl < nslab; ++sl) {
-            const int buf = sl & 1;
-            if (sl + 1 < nslab) gload((sl + 1) * KS);
-            compute(buf);
-            if (sl + 1 < nslab) lstore(buf ^ 1);
-            __syncthreads();
-        }
-    } else {
-        for (int sl = 0; sl < nslab; ++sl) {
-            const int k0 = sl * KS;
-#pragma unroll 4
-            for (int q = 0; q < 64 * KS / LX_WG_THREADS; ++q) {
-                const int e = tid + q * LX_WG_THREADS, r = e % 64, kk = e / 64;
-                sX[0][kk][r] = X[(long long)(k0 + kk) * ldx + (r < R ? r : R - 1)];
-                sL[0][kk][r] = Lq[(long long)(k0 + kk) * ldl + (r < qb ? r : qb - 1)];
-            }
-            __syncthreads();
-            compute(0);
-            __syncthreads();
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
