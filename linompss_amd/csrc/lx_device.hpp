@@ -873,12 +873,13 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
     const bool active = r0 < R;  // wave-uniform
     for (int q = 0; q < N; q += QB) {
         const int qb = N - q < QB ? N - q : QB;
-        // stage Z_q (column-major in W: Z[r][c] at W[c * ldw + q + r]), several
-        // loads in flight per thread
-#pragma unroll 4
+        // Z_q (column-major in W: Z[r][c] at W[c * ldw + q + r]) into registers
+        // now, into LDS after the update: its loads overlap the update
+        T zr[QB * QB / LX_WG_THREADS];
+#pragma unroll
         for (int q2 = 0; q2 < QB * QB / LX_WG_THREADS; ++q2) {
             const int e = tid + q2 * LX_WG_THREADS, r = e % QB, c = e / QB;
-            sZq[r][c] = W[(long long)c * ldw + q + r];
+            zr[q2] = W[(long long)c * ldw + q + r];
         }
         // this block's right-hand sides, loaded before the long update
         T bq[4][4];
@@ -894,7 +895,7 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
         lx_acc_zero<T, 1, 4>(acc);
         // X_{<q} L[q, <q]^T through LDS (every wave takes part in the staging)
         if (q > 0) lx_trsm_update_lds<T>(acc, Bm, ldb, R, L + q, ldl, qb, q, sU);  // ends with a barrier
-        // Y = B_q - acc -> LDS (row-major, zero past the block)
+        // Y = B_q - acc -> LDS (row-major, zero past the block); Z_q -> LDS
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -902,6 +903,11 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
                 const int rl = lx_mfma<T>::row(lk, e), c = j * 16 + lr;
                 sY[rl][c] = c < qb ? bq[j][e] - acc[0][j][e] : T(0);
             }
+#pragma unroll
+        for (int q2 = 0; q2 < QB * QB / LX_WG_THREADS; ++q2) {
+            const int e = tid + q2 * LX_WG_THREADS, r = e % QB, c = e / QB;
+            sZq[r][c] = zr[q2];
+        }
         __syncthreads();  // sZq staged, sY written
         // X_q = Y Z_q^T
         typename lx_mfma<T>::acc x[1][4];
