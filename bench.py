@@ -189,7 +189,7 @@ def bench_gp(args):
     def step(tm):
         ell = torch.tensor(ell0, dtype=torch.float64, requires_grad=True)
         K = ag.gp_kernel(pts, ell, noise)
-        loss = ag.gp_nll_fused(pts, yt, ell, noise, K=K, timing=tm)
+        loss = ag.gp_nll_fused(pts, yt, ell, noise, b=args.gp_b, K=K, timing=tm)
         loss.backward()
         return loss.item(), ell.grad.item()
 
@@ -207,8 +207,8 @@ def bench_gp(args):
     # central difference (h = 1e-6) of the same forward
     h = 1e-6
     with torch.no_grad():
-        fp = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 + h, dtype=torch.float64), noise).item()
-        fm = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 - h, dtype=torch.float64), noise).item()
+        fp = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 + h, dtype=torch.float64), noise, b=args.gp_b).item()
+        fm = ag.gp_nll_fused(pts, yt, torch.tensor(ell0 - h, dtype=torch.float64), noise, b=args.gp_b).item()
     fd = (fp - fm) / (2 * h)
     t_dev = float(np.median(dev_s))
     flops = 4.0 * n ** 3 / 3.0  # forward factor n^3/3, recomputed factor n^3/3, adjoint 2n^3/3
@@ -235,7 +235,7 @@ def bench_gp(args):
         "data": "synthetic: points ~ U[0,1]^3 (numpy PCG64 seed 3), squared-exponential K, l=0.3, "
                 "noise 1e-2, y = L z",
         "config": {"workload": "GP likelihood + gradient through torch.autograd (Cholesky and solve adjoints)",
-                   "n": n, "tile": 256, "lengthscale": ell0, "noise": noise},
+                   "n": n, "tile": args.gp_b, "lengthscale": ell0, "noise": noise},
         "roofline": {"bound": "mfma", "achieved": round(flops / t_dev / 1e12, 3), "peak": PEAK["fp64"],
                      "unit": "TFLOP/s", "frac": round(flops / t_dev / 1e12 / PEAK["fp64"], 4), "traffic": None,
                      "basis": "4 n^3 / 3 flops (forward factor, recomputed factor, 2 n^3/3 adjoint)"},
@@ -314,6 +314,7 @@ def main():
                          "independent systems of order uniform in [32, 512] in one persistent launch; "
                          "gp: Gaussian-process likelihood forward + backward (Cholesky adjoint) at 8192 points")
     ap.add_argument("--gp-n", type=int, default=8192)
+    ap.add_argument("--gp-b", type=int, default=256, help="tile of the GP workload's FP64 factor")
     ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
     if args.workload in ("batched", "gp"):
