@@ -17,11 +17,10 @@
 //   1. copy the lower triangle of A to FP32 (workspace slot of the workgroup);
 //   2. factor it in place with lx_unit_potrf (64-column panels on the matrix
 //      cores, inverses of the 64 x 64 diagonal blocks into W);
-//   3. mirror L^T into the strict upper triangle, so both sweeps read
-//      coalesced columns;
-//   4. sweeps in dot-product form: t = b_q - L[q, <q] y (or L^T[q, >q] x),
-//      then y_q = Z_q t with Z_q the inverse of the diagonal block (from W);
-//   5. FP64 residual r = b - A x over the lower triangle in 64 x 64 tiles
+//   3. right-looking sweeps over 64-row blocks with the vector in LDS:
+//      y_q = Z_q v_q with Z_q the inverse of the diagonal block (from W), then
+//      the rows still to be solved subtract their products with y_q;
+//   4. FP64 residual r = b - A x over the lower triangle in 64 x 64 tiles
 //      (each tile used twice: as A_IJ and, through LDS, as A_IJ^T), which
 //      also yields ||A||_inf on the first pass.
 // Systems up to LX_BATCH_NMAX rows; one right-hand side each.
@@ -100,43 +99,13 @@ __device__ __forceinline__ void lx_b_lower_to(const double *__restrict__ A, int6
     __syncthreads();
 }
 
-// Strict upper triangle of L := (strict lower triangle)^T, through 64 x 64 LDS tiles.
-template <typename T>
-__device__ __forceinline__ void lx_b_mirror(T *__restrict__ L, int64_t ld, int n, char *smem)
-{
-    T(*sT)[65] = reinterpret_cast<T(*)[65]>(smem);
-    const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
-    const int nb = (n + 63) / 64;
-    for (int J = 0; J < nb; ++J)
-        for (int I = J; I < nb; ++I) {
-            // load: thread (row l, columns 16g..16g+15) of tile (I, J)
-            const int row = I * 64 + l;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int cl = g * 16 + k, col = J * 64 + cl;
-                T a = T(0);
-                if (row < n && col < n && row > col) a = L[(int64_t)col * ld + row];
-                sT[l][cl] = a;
-            }
-            __syncthreads();
-            // store: U[J*64 + l][I*64 + rr] = L[I*64 + rr][J*64 + l] for rows rr of group g
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int rr = g * 16 + k, urow = J * 64 + l, ucol = I * 64 + rr;
-                if (ucol < n && urow < n && ucol > urow) L[(int64_t)ucol * ld + urow] = sT[rr][l];
-            }
-            __syncthreads();
-        }
-}
-
 // One triangular sweep on v (n entries): trans 0 solves L y = v, trans 1
-// solves L^T x = v; the strict upper triangle of L holds L^T (lx_b_mirror) and
-// W the inverses Z_q of the 64 x 64 diagonal blocks (W[c*ldw + 64q + r] = Z_q[r][c]).
-// Right-looking over 64-row blocks, v held in LDS: y_q = Z_q v_q (Z_q^T for
-// trans 1), then every row still to be solved takes v_r -= sum_c M[r][c] y_q[c]
-// with M[r][c] = L[(64q + c) ld + r] in both directions (the lower triangle
-// below the block, or the mirrored upper triangle above it): one thread per
-// row, its 64 loads independent of each other.
+// solves L^T x = v, with W the inverses Z_q of the 64 x 64 diagonal blocks
+// (W[c*ldw + 64q + r] = Z_q[r][c]).  Right-looking over 64-row blocks, v held
+// in LDS: y_q = Z_q v_q (Z_q^T for trans 1), then every row r still to be
+// solved takes v_r -= sum_c M[r][c] y_q[c], one thread per row with its 64
+// loads independent of each other: M[r][c] = L[r][64q + c] (forward, down
+// the block's columns) or L[64q + c][r] (backward, along column r).
 template <typename T>
 __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, const T *__restrict__ W,
                                            int64_t ldw, int n, int trans, T *__restrict__ v, char *smem)
@@ -173,16 +142,49 @@ __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, 
         // rows still to be solved: [p + pb, n) forward, [0, p) backward
         const int rlo = trans ? 0 : p + pb, rhi = trans ? p : n;
         for (int r = rlo + tid; r < rhi; r += LX_WG_THREADS) {
-            const T *Lr = L + (int64_t)p * ld + r;
             T acc[4] = {T(0), T(0), T(0), T(0)};
+            if (!trans) {
+                // forward: L[r][p + c] down the block's columns (coalesced over r)
+                const T *Lr = L + (int64_t)p * ld + r;
 #pragma unroll
-            for (int c0 = 0; c0 < 64; c0 += 16) {
-                T a[16];
+                for (int c0 = 0; c0 < 64; c0 += 16) {
+                    T a[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lr[(int64_t)(c0 + k) * ld] : T(0);
+                    for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lr[(int64_t)(c0 + k) * ld] : T(0);
 #pragma unroll
-                for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
-                __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
+                    for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
+                    __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
+                }
+            } else if (pb == 64) {
+                // backward, full block: L^T[r][p + c] = L[p + c][r], the 64 contiguous
+                // elements of column r from row p, as 16-byte vectors (ld a multiple
+                // of 16 elements, p of 64)
+                typedef typename lx_vec16<T>::type V;
+                constexpr int VN = lx_vec16<T>::n;
+                const T *Lc = L + (int64_t)r * ld + p;
+#pragma unroll
+                for (int c0 = 0; c0 < 64; c0 += 16) {
+                    V a[16 / VN];
+#pragma unroll
+                    for (int i = 0; i < 16 / VN; ++i) a[i] = *(const V *)(Lc + c0 + i * VN);
+#pragma unroll
+                    for (int i = 0; i < 16 / VN; ++i)
+#pragma unroll
+                        for (int e = 0; e < VN; ++e) acc[e & 3] += a[i][e] * sy[c0 + i * VN + e];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+                // backward, the last (partial) block: rows p + c < n only
+                const T *Lc = L + (int64_t)r * ld + p;
+#pragma unroll
+                for (int c0 = 0; c0 < 64; c0 += 16) {
+                    T a[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lc[c0 + k] : T(0);
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
             sv[r] -= (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
@@ -296,7 +298,6 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
     bool fb = info != 0;
     double anorm = 0.0, rmax = 0.0, xmax = 0.0, berr = 0.0;
     if (!fb) {
-        lx_b_mirror<float>(F, ld, n, smem);
         for (int i = tid; i < n; i += LX_WG_THREADS) vf[i] = (float)(S.B[i] / bm);
         __syncthreads();
         lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 0, vf, smem);
@@ -330,7 +331,6 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
         int info64 = 0;
         lx_unit_potrf<double>(D, ld, n, slot.W, slot.ldw, &info64, smem);
         info = __builtin_amdgcn_readfirstlane(info64);
-        lx_b_mirror<double>(D, ld, n, smem);
         for (int i = tid; i < n; i += LX_WG_THREADS) v[i] = S.B[i];
         __syncthreads();
         lx_b_sweep<double>(D, ld, slot.W, slot.ldw, n, 0, v, smem);
