@@ -480,6 +480,8 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
         }
     };
     gload(0, 0);
+    // the previous block of the same unit may still be reading the planes
+    __syncthreads();
     lstore(0);
     __syncthreads();
     if (nslab > 1) gload(0, KS);
@@ -528,6 +530,169 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
             }
             __builtin_amdgcn_sched_barrier(0);
         }
+    }
+}
+
+// The same split-BF16 update on a 128 x 256 block (M <= 128, N <= 256): waves
+// own 64 x 128 of C (2 x 4 tiles of 32 x 32), so each staged element of A
+// feeds twice the MFMAs of the 128 x 128 unit and the staging (global loads,
+// split, LDS stores) per product drops to three quarters.  K-slabs of 16 to
+// keep the planes within 54 KB (two workgroups per CU): planes [plane][row][k]
+// with rows 0..127 = A and 128..383 = B, row stride 24 elements (48 bytes:
+// the eight rows of a 128-byte LDS access fall on disjoint banks).  Each
+// thread stages 8 consecutive k of one row of A and of two rows of B; two
+// register sets keep the loads of slab s+2 in flight while slab s is
+// multiplied.  Same numerical contract as lx_unit_gemm_bx6.
+#define LX_BXW_KS 16
+#define LX_BXW_STR 24
+struct lx_bxw_smem {
+    static constexpr int bytes = 3 * 384 * LX_BXW_STR * 2;
+};
+
+__device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long long ldc, const float *__restrict__ A,
+                                                  long long lda, const float *__restrict__ B, long long ldb, int M,
+                                                  int N, int K, bool lower, int gdiag, char *smem)
+{
+    constexpr int KS = LX_BXW_KS, STR = LX_BXW_STR;
+    __bf16(*sP)[384][STR] = reinterpret_cast<__bf16(*)[384][STR]>(smem);
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 128;
+    const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
+    const int nslab = (K + KS - 1) / KS;
+    // staging: thread -> row srow of A, rows srow and srow + 128 of B, k from sk
+    // (rows clamped, never stored; k past K zero)
+    const int srow = tid & 127, sk = (tid >> 7) * 8;
+    const float *pa = A + (srow < M ? srow : M - 1);
+    const float *pb0 = B + (srow < N ? srow : N - 1);
+    const float *pb1 = B + (srow + 128 < N ? srow + 128 : N - 1);
+    float xa[2][8], xb0[2][8], xb1[2][8];
+    const bool kfull = K % KS == 0;  // uniform
+    auto gload = [&](int set, int k0) {
+        if (kfull) {
+            const long long kk = k0 + sk;
+            const float *qa = pa + kk * lda, *q0 = pb0 + kk * ldb, *q1 = pb1 + kk * ldb;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                xa[set][j] = qa[(long long)j * lda];
+                xb0[set][j] = q0[(long long)j * ldb];
+                xb1[set][j] = q1[(long long)j * ldb];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + sk + j;
+                const bool kv = k < K;
+                const long long kc = kv ? k : K - 1;
+                const float va = pa[kc * lda], v0 = pb0[kc * ldb], v1 = pb1[kc * ldb];
+                xa[set][j] = kv ? va : 0.f;
+                xb0[set][j] = kv ? v0 : 0.f;
+                xb1[set][j] = kv ? v1 : 0.f;
+            }
+        }
+    };
+    auto lstore = [&](int set) {
+        lx_bf16x8 h, m, l;
+        lx_bx_split8(xa[set], h, m, l);
+        *(lx_bf16x8 *)&sP[0][srow][sk] = h;
+        *(lx_bf16x8 *)&sP[1][srow][sk] = m;
+        *(lx_bf16x8 *)&sP[2][srow][sk] = l;
+        lx_bx_split8(xb0[set], h, m, l);
+        *(lx_bf16x8 *)&sP[0][128 + srow][sk] = h;
+        *(lx_bf16x8 *)&sP[1][128 + srow][sk] = m;
+        *(lx_bf16x8 *)&sP[2][128 + srow][sk] = l;
+        lx_bx_split8(xb1[set], h, m, l);
+        *(lx_bf16x8 *)&sP[0][256 + srow][sk] = h;
+        *(lx_bf16x8 *)&sP[1][256 + srow][sk] = m;
+        *(lx_bf16x8 *)&sP[2][256 + srow][sk] = l;
+    };
+    lx_f16v acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+    const int lr = lane & 31, lk = lane >> 5;
+    auto compute = [&]() {
+        if (!active) return;
+        lx_bf16x8 a[2][3];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[p][wr + i * 32 + lr][8 * lk];
+        // two column tiles at a time: four independent accumulators between
+        // consecutive MFMAs on one of them
+#pragma unroll
+        for (int jp = 0; jp < 4; jp += 2) {
+            lx_bf16x8 b[2][3];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[p][128 + wc + (jp + j) * 32 + lr][8 * lk];
+            constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][jp + j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][jp + j], 0, 0, 0);
+        }
+    };
+    gload(0, 0);
+    // the previous block of the same unit may still be reading the planes
+    __syncthreads();
+    lstore(0);
+    __syncthreads();
+    if (nslab > 1) gload(0, KS);
+    if (nslab > 2) gload(1, 2 * KS);
+    // unrolled by two so that the register sets have fixed names: on entry
+    // slab sl is in LDS, slab sl+1 in set 0 and slab sl+2 in set 1
+    for (int sl = 0; sl < nslab; sl += 2) {
+        compute();  // slab sl
+        if (sl + 1 < nslab) {
+            __syncthreads();
+            lstore(0);  // slab sl+1
+            __syncthreads();
+            if (sl + 3 < nslab) gload(0, (sl + 3) * KS);
+            compute();  // slab sl+1
+            if (sl + 2 < nslab) {
+                __syncthreads();
+                lstore(1);  // slab sl+2
+                __syncthreads();
+                if (sl + 4 < nslab) gload(1, (sl + 4) * KS);
+            }
+        }
+    }
+    if (active) {
+        // C -= acc, two tiles at a time, their loads before their stores
+        const int lane2 = lx_lane(), lr2 = lane2 & 31, lk2 = lane2 >> 5;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jp = 0; jp < 4; jp += 2) {
+                float v[2][16];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int c = wc + (jp + j) * 32 + lr2, cc = c < N ? c : N - 1;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4), rr = r < M ? r : M - 1;
+                        v[j][e] = C[(long long)cc * ldc + rr];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int c = wc + (jp + j) * 32 + lr2;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4);
+                        if (r < M && c < N && (!lower || r + gdiag >= c))
+                            C[(long long)c * ldc + r] = v[j][e] - acc[i][jp + j][e];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
     }
 }
 

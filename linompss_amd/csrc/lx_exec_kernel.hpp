@@ -25,7 +25,8 @@ template <typename T, int FAM> struct lx_fam_smem {
     // POTRF (2 * 64 * 65 * sizeof(T) + 16; its trailing update reuses the space
     // for the staged GEMM), TRSM (Z_q, then its update slabs or Y), staged GEMM
     static constexpr int potrf = 2 * 64 * 65 * (int)sizeof(T) + 16;
-    static constexpr int gemm = lx_lgemm_smem<T>::bytes > lx_bx_smem::bytes ? lx_lgemm_smem<T>::bytes : lx_bx_smem::bytes;
+    static constexpr int gemm0 = lx_lgemm_smem<T>::bytes > lx_bx_smem::bytes ? lx_lgemm_smem<T>::bytes : lx_bx_smem::bytes;
+    static constexpr int gemm = gemm0 > lx_bxw_smem::bytes ? gemm0 : lx_bxw_smem::bytes;
     static constexpr int trsm = 64 * 65 * (int)sizeof(T) + (4 * 16 * 65 * (int)sizeof(T) > lx_trsm_upd_smem<T>::bytes
                                                                  ? 4 * 16 * 65 * (int)sizeof(T)
                                                                  : lx_trsm_upd_smem<T>::bytes);
@@ -79,20 +80,33 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     case LX_OP_GEMM_NT:
         // a unit covers up to M = 128 rows and N = 128 * m columns, done as m
         // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0));
-        // aux0 bit 0: lower mask (SYRK), bit 1: split-BF16 matrix cores (FP32 only)
+        // aux0 bit 0: lower mask (SYRK), bit 1: split-BF16 matrix cores (FP32
+        // only), bit 2: split-BF16 in 128 x 256 blocks while more than 128
+        // columns remain
         if constexpr (F) {
             const bool lower = (u.aux0 & 1) != 0;
-            bool bx = false;
-            if constexpr (sizeof(T) == 4) bx = (u.aux0 & 2) != 0;
+            if constexpr (sizeof(T) == 4) {
+                if (u.aux0 & 2) {
+                    const bool wide = (u.aux0 & 4) != 0;
+                    for (int c0 = 0; c0 < u.N;) {  // uniform
+                        const int rem = u.N - c0;
+                        if (wide && rem > LX_UNIT_N) {
+                            const int nc = rem < 2 * LX_UNIT_N ? rem : 2 * LX_UNIT_N;
+                            lx_unit_gemm_bx6w(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K,
+                                              lower, u.aux1 - c0, smem);
+                            c0 += nc;
+                        } else {
+                            const int nc = rem < LX_UNIT_N ? rem : LX_UNIT_N;
+                            lx_unit_gemm_bx6(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                                             u.aux1 - c0, smem);
+                            c0 += nc;
+                        }
+                    }
+                    break;
+                }
+            }
             for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N) {  // uniform
                 const int nc = u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N;
-                if constexpr (sizeof(T) == 4) {
-                    if (bx) {
-                        lx_unit_gemm_bx6(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
-                                         u.aux1 - c0, smem);
-                        continue;
-                    }
-                }
                 lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
                                     u.aux1 - c0, smem);
             }

@@ -105,6 +105,37 @@ static void run_wg(F &&body)
     for (auto &t : th) t.join();
 }
 
+// One FP32 GEMM / SYRK unit through the executor's dispatch (lx_run_unit):
+// C (M x N, ldc) -= A B^T, A = rows 0..M-1 and B = rows M..M+N-1 of the K
+// columns of P (ldp); flags = the unit's aux0 (bit 0 lower mask, bit 1
+// split-BF16 bodies, bit 2 128 x 256 split-BF16 blocks), gdiag = its aux1.
+extern "C" void lx_emu_gemm_unit(int flags, int M, int N, int K, float *C, long long ldc, const float *P,
+                                 long long ldp, int gdiag)
+{
+    lx_sys sys = {};
+    sys.A = C;
+    sys.lda = ldc;
+    sys.X = const_cast<float *>(P);
+    sys.ldx = ldp;
+    lx_unit u = {};
+    u.op = (flags & 1) ? LX_OP_SYRK : LX_OP_GEMM_NT;
+    u.M = M;
+    u.N = N;
+    u.K = K;
+    u.aux0 = flags;
+    u.aux1 = gdiag;
+    u.cbase = LX_BASE_A;
+    u.abase = u.bbase = LX_BASE_X;
+    u.c = 0;
+    u.a = 0;
+    u.b = M;  // row M, column 0
+    int info = INT_MAX;
+    std::vector<double> smem((lx_fam_smem<float, LX_FAM_FACTOR>::bytes + 7) / 8);
+    blockDim.x = 256;
+    gridDim.x = 1;
+    run_wg([&] { lx_run_unit<float, LX_FAM_FACTOR>(u, &sys, &info, reinterpret_cast<char *>(smem.data())); });
+}
+
 // One sweep (trans 0: L Y = X, 1: L^T Z = X) through lx_sweep_kernel, one
 // workgroup at a time in ticket order.
 extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,

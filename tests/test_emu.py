@@ -49,7 +49,7 @@ def test_factor_fp64(emu, orc, what, n, b):
 
 def test_factor_fp32_split_bf16_updates(emu, orc):
     """The FP32 factor with its tile-loop SYRK/GEMM units on the split-BF16
-    unit (lx_unit_gemm_bx6): FP32-level agreement with LAPACK in FP64."""
+    unit bodies: FP32-level agreement with LAPACK in FP64."""
     E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
     n, b = 300, 128
     A0 = orc.genmat_spd(n, 4, diag_mode=1)
@@ -61,6 +61,41 @@ def test_factor_fp32_split_bf16_updates(emu, orc):
         E.lx_emu_set_update_prec(0)
     Lref = np.linalg.cholesky(A0)
     assert np.linalg.norm(np.tril(Af) - Lref) / np.linalg.norm(Lref) <= 1e-6
+
+
+@pytest.mark.parametrize("flags,M,N,K,gdiag", [
+    (6, 128, 256, 64, 0),    # one full 128 x 256 split-BF16 block
+    (6, 100, 512, 40, 0),    # two wide blocks, ragged M and K
+    (6, 128, 384, 48, 0),    # a wide block, then a 128 x 128 block
+    (6, 77, 200, 33, 0),     # one ragged wide block
+    (6, 128, 300, 32, 0),    # a wide block, then a 44-column remainder
+    (7, 128, 256, 32, 100),  # lower mask across the wide block
+    (3, 128, 128, 32, 0),    # SYRK tile on the 128 x 128 body
+    (2, 100, 300, 40, 0),    # 128 x 128 blocks only
+])
+def test_gemm_unit_split_bf16(emu, flags, M, N, K, gdiag):
+    """One GEMM / SYRK unit through the executor's dispatch (lx_run_unit) on the
+    split-BF16 bodies: C -= A B^T within FP32 accumulation error of the FP64
+    product, entries outside the lower mask and rows / columns past M / N
+    untouched."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    F, I, L, P = ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
+    E.lx_emu_gemm_unit.argtypes = [I, I, I, I, P, L, P, L, I]
+    rng = np.random.default_rng(M * 1000 + N + K)
+    Pm = np.asfortranarray(rng.standard_normal((M + N, K)).astype(np.float32))
+    ldc = M + 5
+    C0 = np.asfortranarray(rng.standard_normal((ldc, N)).astype(np.float32))
+    C = C0.copy(order="F")
+    E.lx_emu_gemm_unit(flags, M, N, K, C.ctypes.data_as(P), ldc, Pm.ctypes.data_as(P), M + N, gdiag)
+    A64, B64 = Pm[:M].astype(np.float64), Pm[M:].astype(np.float64)
+    ref = C0[:M].astype(np.float64) - A64 @ B64.T
+    mask = np.ones((M, N), dtype=bool)
+    if flags & 1:
+        mask = np.arange(M)[:, None] + gdiag >= np.arange(N)[None, :]
+    tol = 4 * K * 2.0 ** -24 * (np.abs(A64) @ np.abs(B64).T).max() + 2.0 ** -23 * np.abs(ref).max()
+    assert np.abs(C[:M][mask] - ref[mask]).max() <= tol
+    assert np.array_equal(C[:M][~mask], C0[:M][~mask])
+    assert np.array_equal(C[M:], C0[M:])
 
 
 def test_factor_fp32_and_policy0(emu, orc):
