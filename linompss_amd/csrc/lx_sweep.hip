@@ -33,17 +33,19 @@ __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_re
 // Work units of one sweep over nb 64-row blocks, in "step" order s = 0..nb-1
 // (block s forward, block nb-1-s backward):
 //   S(s)     the solve unit of step s: b_s minus the partial sums of step s,
-//            minus the products with the blocks solved at steps s-2 and s-1
+//            minus the products with the blocks solved at steps s-GAP..s-1
 //            (each subtracted as soon as that step is published), times the
-//            inverse of the diagonal block; publishes flag[s];
+//            inverse of the diagonal block; publishes x_s through the mailbox;
 //   P(s, c)  c >= 1: the partial sum of step s over the steps
-//            [max(0, s-2-cC), s-2-(c-1)C); publishes by counting pcnt[s].
-// A partial sum never reads the two newest steps, so its units run a step
-// ahead of the chain of solve units.  Units are listed by the newest step they
-// read (S(s): s-1, P(s, c): s-3-(c-1)C), S first within a key.  Every wait is
+//            [max(0, s-GAP-cC), s-GAP-(c-1)C); publishes by counting pcnt[s].
+// A partial sum never reads the GAP newest steps, so its units run GAP-1
+// steps ahead of the chain of solve units, and the counter's release and
+// acquire stay off the chain.  Units are listed by the newest step they read
+// (S(s): s-1, P(s, c): s-GAP-1-(c-1)C), S first within a key.  Every wait is
 // on a unit listed earlier, so a persistent grid that takes units in list
 // order cannot deadlock.  Unit code: (s << 16) | c.
-__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s >= 3 ? (s - 2 + C - 1) / C : 0; }
+#define LX_SW_GAP 3  // the solve unit's direct products (s-3, s-2) use sD and sA
+__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s > LX_SW_GAP ? (s - LX_SW_GAP + C - 1) / C : 0; }
 static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase)
 {
     units.clear();
@@ -53,7 +55,7 @@ static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std
         units.push_back((key + 1) << 16);
         if (key < 0) continue;
         for (int c = 1;; ++c) {
-            const long long st = (long long)key + 3 + (long long)(c - 1) * C;
+            const long long st = (long long)key + LX_SW_GAP + 1 + (long long)(c - 1) * C;
             if (st >= nb) break;
             units.push_back((int)(st << 16) | c);
         }
@@ -150,10 +152,10 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                                                        int64_t ldmb, long long spin_ticks)
 {
     constexpr int KC = 128 / sizeof(T);  // backward partial sums: L rows per LDS slab
-    constexpr int S_BYTES = 3 * 64 * 65 * (int)sizeof(T);
+    constexpr int S_BYTES = 4 * 64 * 65 * (int)sizeof(T);
     constexpr int P_BYTES = (4 * 64 * (KC + 1) + 4 * NR * 64) * (int)sizeof(T);
     __shared__ __attribute__((aligned(16))) char smem[S_BYTES > P_BYTES ? S_BYTES : P_BYTES];
-    __shared__ T svec[2][64][NR + 1];  // S unit: [0] = b, then y; [1] = a solved block's x
+    __shared__ T svec[3][64][NR + 1];  // S unit: [0] = b, then y; [1], [2] = solved blocks' x
     __shared__ int s_unit;
     const int nb = (n + 63) / 64;
     int *ticket = ctl, *abort = ctl + 1, *pcnt = ctl + 64 + nb;
@@ -173,10 +175,11 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
         if (c == 0) {
             // ---------------- S(s) ----------------
             // sA: the diagonal block, then the block coupling step s-2;
-            // sZ: its inverse; sC: the block coupling step s-1
+            // sZ: its inverse; sC: the block coupling step s-1; sD: step s-3
             T(*sA)[65] = reinterpret_cast<T(*)[65]>(smem);
             T(*sZ)[65] = reinterpret_cast<T(*)[65]>(smem + 64 * 65 * sizeof(T));
             T(*sC)[65] = reinterpret_cast<T(*)[65]>(smem + 2 * 64 * 65 * sizeof(T));
+            T(*sD)[65] = reinterpret_cast<T(*)[65]>(smem + 3 * 64 * 65 * sizeof(T));
             // coupling block with step s-d into sB: sB[r][k] = L[r0 + r][ip*64 + k]
             // (forward) or L[ip*64 + k][r0 + r] (backward); reads run down columns
             auto load_coupling = [&](T(*sB)[65], int d) {
@@ -201,74 +204,146 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                 svec[0][r][cc] = (r < hb && cc < nrhs) ? X[(int64_t)cc * ldx + r0 + r] : T(0);
             }
             if (s >= 1) load_coupling(sC, 1);
+            if (s >= 3) load_coupling(sD, 3);
             __syncthreads();
             lx_tri_inv64<T>(sA, sZ);  // ends with a barrier
+            // M1 = op(Z) C1 over sC (op = transpose for the backward sweep), so that
+            // once step s-1 is published one product remains: x = op(Z) y - M1 x_{s-1};
+            // thread (row lane, columns 16w..16w+15)
+            if (s >= 1) {
+                T m[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) m[q] = T(0);
+                for (int j = 0; j < 64; ++j) {
+                    const T z = TRANS ? sZ[j][lane] : sZ[lane][j];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) m[q] += z * sC[j][16 * w + q];
+                }
+                __syncthreads();  // every wave has read C1
+#pragma unroll
+                for (int q = 0; q < 16; ++q) sC[lane][16 * w + q] = m[q];
+            }
             if (s >= 2) load_coupling(sA, 2);
-            // b minus the partial sums (in part order): they only read steps <= s-3.
-            // The host picks C so that a step has at most 8 parts; lane = row,
-            // wave w owns rhs w, w+4, ...; all loads of a column are issued together.
+            // b minus the partial sums (in part order; they only read steps <=
+            // s-GAP-1), lane = row, wave w owning rhs w, w+4, ...; the host picks C
+            // so that a step has at most 8 parts, and all loads of a column are
+            // issued together.  Wave 0 waits for the parts' counter and subtracts
+            // its columns at once, while waves 1..3 take x_{s-3} and x_{s-2} from the
+            // mailbox (svec[1], svec[2]): the counter's wait and the two mailbox
+            // reads overlap.
             const int need = lx_sw_nparts(s, C);
-            if (need > 0) {
-                if (w == 0) lx_sw_wait_ge(&pcnt[s], need, abort, spin_ticks);  // wave-uniform
-                __syncthreads();
-                const T *pp = part + (size_t)rowbase[s] * LX_SW_NR * 64;
-                for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
-                    T y = svec[0][lane][cc];
-                    for (int q0 = 0; q0 < need; q0 += 8) {
-                        T v[8];
+            const T *pp = part + (size_t)rowbase[s] * LX_SW_NR * 64;
+            auto sub_parts = [&](int cc) {
+                T y = svec[0][lane][cc];
+                for (int q0 = 0; q0 < need; q0 += 8) {
+                    T v[8];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q)
-                            v[q] = q0 + q < need ? pp[((size_t)(q0 + q) * LX_SW_NR + cc) * 64 + lane] : T(0);
+                    for (int q = 0; q < 8; ++q)
+                        v[q] = q0 + q < need ? pp[((size_t)(q0 + q) * LX_SW_NR + cc) * 64 + lane] : T(0);
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) y -= v[q];
-                    }
-                    svec[0][lane][cc] = y;
+                    for (int q = 0; q < 8; ++q) y -= v[q];
+                }
+                svec[0][lane][cc] = y;
+            };
+            if (w == 0) {  // wave-uniform
+                if (need > 0) {
+                    lx_sw_wait_ge(&pcnt[s], need, abort, spin_ticks);
+                    for (int cc = 0; cc < nrhs; cc += 4) sub_parts(cc);
+                }
+            } else {
+                for (int e = tid - 64; e < 2 * 64 * NR; e += LX_WG_THREADS - 64) {
+                    const int h = e / (64 * NR), e2 = e % (64 * NR), k = e2 % 64, cc = e2 / 64;
+                    const int d = 3 - h;
+                    if (s < d) continue;
+                    const int ip = TRANS ? ib + d : ib - d;
+                    const int hp = min(64, n - ip * 64);
+                    svec[1 + h][k][cc] = (k < hp && cc < nrhs)
+                                             ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + ip * 64 + k, abort, spin_ticks)
+                                             : T(0);
                 }
             }
-            // the products with the blocks solved at steps s-2, then s-1, each
-            // subtracted as soon as its step is published
-            for (int d = 2; d >= 1; --d) {
+            __syncthreads();  // wave 0 past the counter's acquire; svec[1], svec[2] filled
+            if (need > 0 && w > 0)  // wave-uniform
+                for (int cc = w; cc < nrhs; cc += 4) sub_parts(cc);
+            // the products with the blocks solved at steps s-3 and s-2
+            for (int d = LX_SW_GAP; d >= 2; --d) {
                 if (s < d) continue;  // uniform
-                const int ip = TRANS ? ib + d : ib - d;
-                const int hp = min(64, n - ip * 64);
-                T(*sB)[65] = d == 2 ? sA : sC;
-                __syncthreads();  // svec[1] is free, sB is loaded
-                for (int e = tid; e < 64 * NR; e += LX_WG_THREADS) {
-                    const int k = e % 64, cc = e / 64;
-                    svec[1][k][cc] = (k < hp && cc < nrhs)
-                                         ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + ip * 64 + k, abort, spin_ticks)
-                                         : T(0);
-                }
-                __syncthreads();
+                T(*sB)[65] = d == 3 ? sD : sA;
+                const int h = 3 - d;
                 for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform
                     T a0 = T(0), a1 = T(0);
 #pragma unroll 8
                     for (int k = 0; k < 64; k += 2) {
-                        a0 += sB[lane][k] * svec[1][k][cc];
-                        a1 += sB[lane][k + 1] * svec[1][k + 1][cc];
+                        a0 += sB[lane][k] * svec[1 + h][k][cc];
+                        a1 += sB[lane][k + 1] * svec[1 + h][k + 1][cc];
                     }
                     svec[0][lane][cc] -= a0 + a1;
                 }
             }
-            // x = Z y (forward) or Z^T y (backward); lane = row, wave w owns rhs w, w+4, ...
+            // u = op(Z) y for this wave's right-hand sides, kept in registers
+            constexpr int NU = (NR + 3) / 4;
+            T u[NU];
             __builtin_amdgcn_wave_barrier();
-            for (int cc = w; cc < nrhs; cc += 4) {  // wave-uniform; column cc is this wave's own
+#pragma unroll
+            for (int t = 0; t < NU; ++t) {
+                const int cc = w + 4 * t;
                 T x0 = T(0), x1 = T(0);
+                if (cc < nrhs) {  // wave-uniform; column cc is this wave's own
 #pragma unroll 8
-                for (int k = 0; k < 64; k += 2) {
-                    x0 += (TRANS ? sZ[k][lane] : sZ[lane][k]) * svec[0][k][cc];
-                    x1 += (TRANS ? sZ[k + 1][lane] : sZ[lane][k + 1]) * svec[0][k + 1][cc];
+                    for (int k = 0; k < 64; k += 2) {
+                        x0 += (TRANS ? sZ[k][lane] : sZ[lane][k]) * svec[0][k][cc];
+                        x1 += (TRANS ? sZ[k + 1][lane] : sZ[lane][k + 1]) * svec[0][k + 1][cc];
+                    }
                 }
-                if (lane < hb) {
-                    X[(int64_t)cc * ldx + r0 + lane] = x0 + x1;
-                    lx_sw_mb_put<T>(mb + (int64_t)cc * ldmb + r0 + lane, x0 + x1);
+                u[t] = x0 + x1;
+            }
+            if (s >= 1) {  // uniform
+                // the chain: x = u - M1 x_{s-1}.  Wave w takes entries 16w..16w+15 of
+                // x_{s-1} from the mailbox and forms the product over those 16 columns
+                // of M1 for every row; the four partial sums meet in LDS (over sA and
+                // sZ, free once every wave has formed u)
+                const int ip = TRANS ? ib + 1 : ib - 1;
+                const int hp = min(64, n - ip * 64);
+                T(*spart)[64][NR + 1] = reinterpret_cast<T(*)[64][NR + 1]>(smem);
+                __syncthreads();  // u formed in every wave, svec[1] free, M1 in sC
+                for (int e = lane; e < 16 * NR; e += 64) {
+                    const int k = 16 * w + (e & 15), cc = e >> 4;
+                    svec[1][k][cc] = (k < hp && cc < nrhs)
+                                         ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + ip * 64 + k, abort, spin_ticks)
+                                         : T(0);
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (int cc = 0; cc < nrhs; ++cc) {  // uniform
+                    T a0 = T(0), a1 = T(0);
+#pragma unroll
+                    for (int q = 0; q < 16; q += 2) {
+                        a0 += sC[lane][16 * w + q] * svec[1][16 * w + q][cc];
+                        a1 += sC[lane][16 * w + q + 1] * svec[1][16 * w + q + 1][cc];
+                    }
+                    spart[w][lane][cc] = a0 + a1;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int t = 0; t < NU; ++t) {
+                    const int cc = w + 4 * t;
+                    if (cc < nrhs) {  // wave-uniform
+                        u[t] -= (spart[0][lane][cc] + spart[1][lane][cc]) + (spart[2][lane][cc] + spart[3][lane][cc]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NU; ++t) {
+                const int cc = w + 4 * t;
+                if (cc < nrhs && lane < hb) {
+                    X[(int64_t)cc * ldx + r0 + lane] = u[t];
+                    lx_sw_mb_put<T>(mb + (int64_t)cc * ldmb + r0 + lane, u[t]);
                 }
             }
         } else {
             // ---------------- P(s, c) ----------------
             T(*stile)[KC + 1] = reinterpret_cast<T(*)[KC + 1]>(smem + w * 64 * (KC + 1) * sizeof(T));
             T(*sx)[64] = reinterpret_cast<T(*)[64]>(smem + (4 * 64 * (KC + 1) + w * NR * 64) * sizeof(T));
-            const int t_hi = s - 2 - (c - 1) * C, t_lo = max(0, t_hi - C);
+            const int t_hi = s - LX_SW_GAP - (c - 1) * C, t_lo = max(0, t_hi - C);
             const int row = r0 + min(lane, hb - 1);  // clamped: lanes past hb never store
             T acc[NR];
 #pragma unroll
