@@ -67,19 +67,46 @@ __global__ void __launch_bounds__(256) lx_symv_blk_kernel(const double *__restri
         for (int b = 0; b < SB; ++b)
 #pragma unroll
             for (int c = 0; c < NC; ++c) accR[b][c] = accC[b][c] = 0.0;
+        // blocks t = bi * SB + bj of the super-block in order; the 16 values a
+        // thread stages of the next one are loaded while the current one is
+        // multiplied
+        auto blk_ok = [&](int t) {
+            const int I = SI * SB + t / SB, J = SJ * SB + t % SB;
+            return J <= I && I < nb;
+        };
+        double nxt[16];
+        auto gload = [&](int t) {
+            const int I = SI * SB + t / SB, J = SJ * SB + t % SB;
+            const int r0 = I * 64, c0 = J * 64, hr = min(64, n - r0), hc = min(64, n - c0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int e = tid + 256 * i, r = e & 63, c = e >> 6;
+                nxt[i] = (r < hr && c < hc && (I != J || r >= c)) ? A[(int64_t)(c0 + c) * lda + r0 + r] : 0.0;
+            }
+        };
+        {
+            int t = 0;
+            while (t < SB * SB && !blk_ok(t)) ++t;  // uniform
+            if (t < SB * SB) gload(t);
+        }
 #pragma unroll
         for (int bi = 0; bi < SB; ++bi) {
 #pragma unroll
             for (int bj = 0; bj < SB; ++bj) {
                 const int I = SI * SB + bi, J = SJ * SB + bj;
                 if (J > I || I >= nb) continue;  // uniform
-                const int r0 = I * 64, c0 = J * 64, hr = min(64, n - r0), hc = min(64, n - c0);
                 __syncthreads();  // sA is free; also orders the sx writes above
-                for (int e = tid; e < 64 * 64; e += 256) {
-                    const int r = e & 63, c = e >> 6;
-                    sA[c][r] = (r < hr && c < hc && (I != J || r >= c)) ? A[(int64_t)(c0 + c) * lda + r0 + r] : 0.0;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int e = tid + 256 * i;
+                    sA[e >> 6][e & 63] = nxt[i];
                 }
                 __syncthreads();
+                {
+                    int t = bi * SB + bj + 1;
+                    while (t < SB * SB && !blk_ok(t)) ++t;  // uniform
+                    if (t < SB * SB) gload(t);
+                }
                 // row part: row l, columns 16q .. 16q+15
 #pragma unroll 4
                 for (int k = 16 * q; k < 16 * q + 16; ++k) {
@@ -153,27 +180,44 @@ __global__ void __launch_bounds__(256) lx_symv_reduce_kernel(int n, int nrhs, co
     }
 }
 
-// out[c] = max_i |V[i, c]| (NaN-propagating); grid = ncols, one WG per column
-__global__ void __launch_bounds__(256) lx_colmax_kernel(const double *__restrict__ V, int64_t ldv, int n,
-                                                        double *__restrict__ out)
+// NaN-propagating maximum of one value per thread over a 1024-thread
+// workgroup (wave shuffles, then the 16 wave results); the result is valid in
+// thread 0.
+__device__ __forceinline__ double lx_nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ double lx_wg1024_nanmax(double m)
 {
-    __shared__ double red[256];
-    const double *v = V + (int64_t)blockIdx.x * ldv;
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        double a = fabs(v[i]);
-        m = (a > m || a != a) ? a : m;
-    }
-    red[threadIdx.x] = m;
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = lx_nanmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            double a = red[threadIdx.x + s], b = red[threadIdx.x];
-            red[threadIdx.x] = (a > b || a != a) ? a : b;
-        }
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        m = red[0];
+        for (int w = 1; w < 16; ++w) m = lx_nanmax(m, red[w]);
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+    return m;
+}
+
+// out[c] = max_i |V[i, c]| (NaN-propagating); grid = ncols, one 1024-thread
+// workgroup per column with eight loads in flight per thread
+__global__ void __launch_bounds__(1024) lx_colmax_kernel(const double *__restrict__ V, int64_t ldv, int n,
+                                                         double *__restrict__ out)
+{
+    const double *v = V + (int64_t)blockIdx.x * ldv;
+    double m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = 0.0;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 8 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * 1024;
+            if (i < n) m[u] = lx_nanmax(fabs(v[i]), m[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 1; u < 8; ++u) m[0] = lx_nanmax(m[u], m[0]);
+    m[0] = lx_wg1024_nanmax(m[0]);
+    if (threadIdx.x == 0) out[blockIdx.x] = m[0];
 }
 
 // berr[c] = rmax[c] / (anorm * xmax[c])  (the FP64 convergence test)
@@ -226,24 +270,23 @@ __global__ void lx_copy_cols_kernel(int n, int ncols, const double *__restrict__
     }
 }
 
-__global__ void lx_max_kernel(int n, const double *__restrict__ v, double *__restrict__ out)
+// out[0] = max_i v[i] (NaN-propagating, v >= 0); one 1024-thread workgroup
+__global__ void __launch_bounds__(1024) lx_max_kernel(int n, const double *__restrict__ v, double *__restrict__ out)
 {
-    __shared__ double red[256];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        double a = v[i];
-        m = (a > m || a != a) ? a : m;
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            double a = red[threadIdx.x + s], b = red[threadIdx.x];
-            red[threadIdx.x] = (a > b || a != a) ? a : b;
+    double m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = 0.0;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 8 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * 1024;
+            if (i < n) m[u] = lx_nanmax(v[i], m[u]);
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) m[0] = lx_nanmax(m[u], m[0]);
+    m[0] = lx_wg1024_nanmax(m[0]);
+    if (threadIdx.x == 0) out[0] = m[0];
 }
 
 static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
@@ -343,7 +386,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     double h_small[1 + 48];
     if (!fallback) {
         // x0 = solve(b) with per-column scaling by ||b_c||_inf
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, dB, ldb, n, bmax);
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dB, ldb, n, bmax);
         hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, dB, ldb, bmax,
                            Sf);
         rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
@@ -353,9 +396,9 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         for (int it = 0;; ++it) {
             rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, it == 0 ? rowabs : nullptr, s);
             if (rc) return rc;
-            if (it == 0) hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
-            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
-            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, dX, ldx, n, xmax);
+            if (it == 0) hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
+            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
+            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
             hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
             LX_HIP_TRY(hipGetLastError());
             LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
@@ -396,9 +439,9 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         info = rc;
         rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, dX, ldx, n, xmax);
+        hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
         hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
         LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
         LX_HIP_TRY(hipStreamSynchronize(s));
@@ -463,9 +506,9 @@ extern "C" int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda
     hipStream_t s = nullptr;
     int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(256), 0, s, n, rowabs, anorm);
-    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, R, (int64_t)n, n, rmax);
-    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(256), 0, s, dX, ldx, n, xmax);
+    hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
     hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
     double h[16];
     LX_HIP_TRY(hipMemcpy(h, berr, sizeof(double) * nrhs, hipMemcpyDeviceToHost));
