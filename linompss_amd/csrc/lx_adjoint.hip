@@ -135,9 +135,11 @@ __global__ void lx_adj_reduce_kernel(int M, int N, int ksplit, const double *__r
     *p = beta == 0.0 ? s : s + beta * *p;
 }
 
-// Diagonal block step of the reverse sweep, one workgroup:
-//   mode 0: Z = D^-1 (D = L[J, J], jb x jb lower) into Zb (64 x 64, ld 64;
-//           identity past jb);
+// Diagonal block step of the reverse sweep:
+//   mode 0: the inverses of all diagonal blocks of an n-row lower triangle
+//           (n passed as jb), one workgroup per 64-column block J:
+//           Z_J = L[J, J]^-1 into Zb + 4096 J (64 x 64, ld 64; identity past
+//           the block's order);
 //   mode 1: Pbar = level-2 adjoint of D = chol(P) applied to Dbar (Abar[J, J],
 //           lower), written back in place; Sb (64 x 64, ld 64) = Pbar + Pbar^T.
 // Level-2 adjoint, columns j = jb-1 .. 0 of the block (the scalar case of the
@@ -150,6 +152,12 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
 {
     __shared__ double sL[64][65], sX[64][65];
     __shared__ double s_red[4];
+    if (mode == 0) {
+        const int j0 = 64 * (int)blockIdx.x;
+        jb = min(64, jb - j0);
+        Ld += (int64_t)j0 * ldl + j0;
+        Zb += (int64_t)blockIdx.x * 4096;
+    }
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
     for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
         const int r = e & 63, c = e >> 6;
@@ -317,18 +325,19 @@ int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda
 {
     const int nb = (n + 63) / 64;
     const size_t part_elems = (size_t)16 * 64 * std::max(n, 64);
-    double *ws = (double *)lx_scratch(9, sizeof(double) * (2 * 64 * 64 + (size_t)n * 64 + part_elems));
+    double *ws = (double *)lx_scratch(9, sizeof(double) * ((size_t)nb * 4096 + 64 * 64 + (size_t)n * 64 + part_elems));
     if (!ws) return LINOMPSS_ERR_NOMEM;
-    double *Zb = ws, *Sb = ws + 64 * 64, *Qb = ws + 2 * 64 * 64, *part = Qb + (size_t)n * 64;
+    double *Zall = ws, *Sb = Zall + (size_t)nb * 4096, *Qb = Sb + 64 * 64, *part = Qb + (size_t)n * 64;
+    // the inverses of the diagonal blocks do not depend on the sweep: all at once
+    hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(nb), dim3(LX_WG_THREADS), 0, s, 0, n, L, ldl, Ab, lda, Zall, Sb);
     int rc = 0;
     for (int J = nb - 1; J >= 0 && rc == 0; --J) {
         const int j0 = 64 * J, jb = std::min(64, n - j0), m = n - j0 - jb, p = j0;
         const double *D = L + (int64_t)j0 * ldl + j0;
         double *Dbar = Ab + (int64_t)j0 * lda + j0;
         const int64_t ldq = std::max(m, 1);
+        const double *Zb = Zall + (size_t)J * 4096;
         if (m > 0) {
-            hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 0, jb, D, ldl, Dbar, lda, Zb,
-                               Sb);
             double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
             const double *C = L + (int64_t)j0 * ldl + j0 + jb;
             // Qbar = Cbar D^-1; Abar_C = Qbar
@@ -342,7 +351,7 @@ int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda
                 break;
         }
         // Pbar (in place of Dbar) and S = Pbar + Pbar^T
-        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda, Zb, Sb);
+        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda, nullptr, Sb);
         if (p > 0) {
             const double *R = L + j0;           // L[J, <J]: jb x p, ld ldl
             double *Rbar = Ab + j0;            // Abar[J, <J]
