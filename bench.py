@@ -249,6 +249,75 @@ def bench_gp(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_hyper(args):
+    """The hypermatrix factor (ompss_{d,s}chol_hrl, chol_hrlmain.c:16-37) in
+    device memory: 2080 separate 512 x 512 tiles at n = 32768, timed beside the
+    column-major factor of the same matrix (same plan and unit bodies)."""
+    import numpy as np
+    import linompss_amd as lx
+    n, b = args.n, args.b
+    if n % b:
+        print("--n must be a multiple of --b for the hypermatrix workload", file=sys.stderr)
+        sys.exit(2)
+    mt = n // b
+    flops = n ** 3 / 3.0
+    log(f"generating A (n={n}, cond2~{args.kappa:g}, seed {args.seed}); {mt * (mt + 1) // 2} tiles of {b} x {b}")
+    A, alpha, lmax = lx.gen_gauss_spd(n, args.seed, args.kappa)
+    hA = A.to_host()
+    res = {}
+    for dt, name, up in ((np.float64, "fp64", 0), (np.float32, "fp32_split_bf16", 1)):
+        hA_t = hA.astype(dt, order="F") if dt != np.float64 else hA
+        pristine = {(i, j): lx.DeviceMatrix.from_host(np.asfortranarray(hA_t[i * b:(i + 1) * b, j * b:(j + 1) * b]))
+                    for j in range(mt) for i in range(j, mt)}
+        work = {k: lx.DeviceMatrix(b, b, dt) for k in pristine}
+        opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, up)
+        t_tiles = []
+        for k in range(args.warmup + args.steps):
+            for key, m in pristine.items():
+                lx.copy_dev(work[key], m)
+            lx.lib.linompss_dev_sync()
+            info = lx.chol_tiles_dev(work, mt, b, "rl", opts)
+            if k >= args.warmup:
+                t_tiles.append(lx.last_exec_stats().exec_ms * 1e-3)
+            if info != 0:
+                raise RuntimeError(f"tiled factor info {info}")
+        for m in list(pristine.values()) + list(work.values()):
+            m.free()
+        # the column-major factor of the same matrix, same options
+        src = lx.DeviceMatrix.from_host(hA_t)
+        d = lx.DeviceMatrix(n, n, dt)
+        t_col = []
+        for k in range(args.warmup + args.steps):
+            lx.copy_dev(d, src)
+            lx.lib.linompss_dev_sync()
+            info = lx.chol_dev(d, b, "rl", opts)
+            if k >= args.warmup:
+                t_col.append(lx.last_exec_stats().exec_ms * 1e-3)
+            if info != 0:
+                raise RuntimeError(f"column-major factor info {info}")
+        src.free()
+        d.free()
+        tt, tc = float(np.median(t_tiles)), float(np.median(t_col))
+        res[name] = {"tiles_ms": round(tt * 1e3, 3), "colmajor_ms": round(tc * 1e3, 3),
+                     "tiles_tflops": round(flops / tt / 1e12, 3), "tiles_vs_colmajor": round(tc / tt, 4)}
+        log(f"{name}: tiles {tt * 1e3:.1f} ms ({flops / tt / 1e12:.1f} TF), column-major {tc * 1e3:.1f} ms")
+    A.free()
+    t64 = res["fp64"]["tiles_ms"] * 1e-3
+    out = {
+        "metric": f"hypermatrix (tile-contiguous) Cholesky GFLOP/s at n={n}, b={b} (FP64, ompss_dchol_hrl on device)",
+        "value": round(flops / t64 / 1e9, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t64 * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic: A = G G^T/n + alpha I, G ~ N(0,1) (Philox4x32-10, seed {args.seed}), as {mt * (mt + 1) // 2} separate device tiles",
+        "config": {"workload": "hypermatrix factor, tiles as separate device buffers (chol_hrlmain.c:16-37)",
+                   "n": n, "b": b, "mt": mt},
+        "roofline": {"bound": "mfma", "achieved": round(flops / t64 / 1e12, 3), "peak": PEAK["fp64"],
+                     "unit": "TFLOP/s", "frac": round(flops / t64 / 1e12 / PEAK["fp64"], 4), "traffic": None},
+        "results": res,
+    }
+    print(json.dumps(out))
+
+
 def traffic_from(fetch_csv, write_csv, out, n):
     """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
@@ -309,19 +378,20 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
-    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched", "gp"),
+    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched", "gp", "hyper"),
                     help="mixed: the n=32768 refinement solve (the headline line); batched: 4096 "
                          "independent systems of order uniform in [32, 512] in one persistent launch; "
-                         "gp: Gaussian-process likelihood forward + backward (Cholesky adjoint) at 8192 points")
+                         "gp: Gaussian-process likelihood forward + backward (Cholesky adjoint) at 8192 points; "
+                         "hyper: the hypermatrix factor (separate device tiles) beside the column-major one")
     ap.add_argument("--gp-n", type=int, default=8192)
     ap.add_argument("--gp-b", type=int, default=256, help="tile of the GP workload's FP64 factor")
     ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
-    if args.workload in ("batched", "gp"):
+    if args.workload in ("batched", "gp", "hyper"):
         if args.gpus != 1:
             print("bench.py measures one GPU. Run with --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        (bench_batched if args.workload == "batched" else bench_gp)(args)
+        {"batched": bench_batched, "gp": bench_gp, "hyper": bench_hyper}[args.workload](args)
         return
     if args.traffic_from:
         traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)

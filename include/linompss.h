@@ -46,6 +46,29 @@ int ompss_schol_rl(int n, int b, int t, float *A, int lda);
 /* Replaces ompss_schol_ll (ompss_dense_chol.h:32; FP32 build of chol_llmain.c). */
 int ompss_schol_ll(int n, int b, int t, float *A, int lda);
 
+/* Hypermatrix form: Ah is an mt x mt table of host tile pointers, tile (i, j)
+ * at Ah[j * mt + i], each b x b column-major with leading dimension b; the
+ * tiles i >= j are read and overwritten with the factor (the others may be
+ * null).  Returns 0, the first non-positive global column (1-based), or a
+ * negative error; t is accepted and ignored.
+ * Replaces ompss_dchol_hrl (ompss_dense_chol.h:27, ompss_dense_chol.c:20-22 ->
+ * chol_hrlmain.c:16-37). */
+int ompss_dchol_hrl(int mt, int b, int t, double **Ah);
+/* Replaces ompss_dchol_hll (ompss_dense_chol.h:19, ompss_dense_chol.c:12-14 ->
+ * chol_hllmain.c:17-45). */
+int ompss_dchol_hll(int mt, int b, int t, double **Ah);
+/* Replaces ompss_schol_hrl (ompss_dense_chol.h:23; FP32 build of chol_hrlmain.c). */
+int ompss_schol_hrl(int mt, int b, int t, float **Ah);
+/* Replaces ompss_schol_hll (ompss_dense_chol.h:15; FP32 build of chol_hllmain.c). */
+int ompss_schol_hll(int mt, int b, int t, float **Ah);
+/* Replace the nested variants ompss_{d,s}chol_n{hrl,hll} (ompss_dense_chol.h:41-47,
+ * ompss_dense_chol.c:48-62 -> chol_nrlmain.c:11, chol_nllmain.c:12): the same
+ * factor of the same tiles (t only sub-blocks the reference's CPU tasks). */
+int ompss_dchol_nhrl(int mt, int b, int t, double **Ah);
+int ompss_dchol_nhll(int mt, int b, int t, double **Ah);
+int ompss_schol_nhrl(int mt, int b, int t, float **Ah);
+int ompss_schol_nhll(int mt, int b, int t, float **Ah);
+
 /* Replaces dspdsolv_main (src/spdsolv/spdsolv_main.h:20, spdsolv_main.c:10-20):
  * RL factor of the m x m matrix A (lda = m), then the forward and backward
  * sweeps on the m x n right-hand sides B (ldb = m), in place. */
@@ -119,6 +142,14 @@ typedef struct {
 
 int linompss_chol_dev(int dtype, int variant, int n, int b, void *dA, int64_t lda, void *stream,
                       const linompss_exec_opts *opts);
+/* The factor of a hypermatrix in device memory: dTiles is a DEVICE table of
+ * mt * mt device tile pointers, tile (i, j) at dTiles[j * mt + i], b x b with
+ * leading dimension b (tiles i >= j read and written).  Same plans and units
+ * as linompss_chol_dev on the n = mt * b matrix, each unit addressed inside
+ * its tile.  Device form of ompss_{d,s}chol_{hrl,hll} (ompss_dense_chol.h:15-27;
+ * chol_hrlmain.c:16-37, chol_hllmain.c:17-45). */
+int linompss_chol_tiles_dev(int dtype, int variant, int mt, int b, void *const *dTiles, void *stream,
+                            const linompss_exec_opts *opts);
 int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, void *dX, int64_t ldx,
                       void *stream, const linompss_exec_opts *opts);
 int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,

@@ -6,6 +6,8 @@ The names and argument meanings follow the reference's interface:
 
 * ``ompss_dchol_rl(n, b, t, A, lda)`` and its ``ll`` / ``s`` twins
   (src/cholesky/dense/ompss_dense_chol.h:32-38),
+* the hypermatrix forms ``ompss_{d,s}chol_{hrl,hll,nhrl,nhll}(mt, b, t, Ah)``
+  (ompss_dense_chol.h:15-27,41-47) over a dict of b x b tiles,
 * ``dspdsolv_main(m, n, b, A, B)`` (src/spdsolv/spdsolv_main.h:20),
 * ``bblas_dltrsm`` lower branches (src/blocked_blas/bblas_trsm.h:116-160),
 * an iterative-refinement solve keeping itref's loop contract
@@ -32,7 +34,9 @@ __all__ = [
     "dspdsolv_main", "sspdsolv_main", "dltrsm", "sltrsm", "dposv_ir", "chol_dev", "posv_dev",
     "potrs_dev", "dposv_ir_dev", "last_exec_stats", "plan_tdg", "plan_info", "ExecOpts", "IrOpts",
     "IrStatus", "ExecStats", "DeviceMatrix", "device_count", "BatchOpts", "BatchStatus", "DeviceBatch",
-    "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd",
+    "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd", "ompss_dchol_hrl", "ompss_dchol_hll",
+    "ompss_schol_hrl", "ompss_schol_hll", "ompss_dchol_nhrl", "ompss_dchol_nhll", "ompss_schol_nhrl",
+    "ompss_schol_nhll", "chol_tiles_dev",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -87,6 +91,9 @@ _sig("ompss_dchol_rl", _i, _i, _i, _i, _p, _i)
 _sig("ompss_dchol_ll", _i, _i, _i, _i, _p, _i)
 _sig("ompss_schol_rl", _i, _i, _i, _i, _p, _i)
 _sig("ompss_schol_ll", _i, _i, _i, _i, _p, _i)
+for _nm in ("ompss_dchol_hrl", "ompss_dchol_hll", "ompss_schol_hrl", "ompss_schol_hll", "ompss_dchol_nhrl",
+            "ompss_dchol_nhll", "ompss_schol_nhrl", "ompss_schol_nhll"):
+    _sig(_nm, _i, _i, _i, _i, _p)
 _sig("dspdsolv_main", _i, _i, _i, _i, _p, _p)
 _sig("sspdsolv_main", _i, _i, _i, _i, _p, _p)
 _sig("linompss_dltrsm", _i, _c, _c, _c, _i, _i, _i, _i, _d, _p, _i, _p, _i)
@@ -94,6 +101,7 @@ _sig("linompss_sltrsm", _i, _c, _c, _c, _i, _i, _i, _i, _f, _p, _i, _p, _i)
 _sig("linompss_dposv_ir", _i, _i, _i, _i, _p, _i, _p, _i, _p, _i, ctypes.POINTER(IrOpts),
      ctypes.POINTER(IrStatus))
 _sig("linompss_chol_dev", _i, _i, _i, _i, _i, _p, _i64, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_chol_tiles_dev", _i, _i, _i, _i, _i, _p, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_posv_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_potrs_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_dposv_ir_dev", _i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p, ctypes.POINTER(IrOpts),
@@ -158,6 +166,63 @@ def ompss_schol_rl(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
 
 def ompss_schol_ll(A: np.ndarray, b: int, t: int = 0, n=None, lda=None) -> int:
     return _chol("ompss_schol_ll", np.float32, A, b, t, n, lda)
+
+
+def _tile_table(tiles: dict, mt: int, b: int, dtype):
+    """ctypes table of mt*mt tile addresses, entry j*mt + i = tile (i, j);
+    every lower tile must be present, b x b, column-major, of dtype."""
+    tab = (ctypes.c_void_p * max(1, mt * mt))()
+    for j in range(mt):
+        for i in range(j, mt):
+            a = tiles.get((i, j))
+            if a is None:
+                raise ValueError(f"tile ({i}, {j}) missing")
+            _colmajor(a, dtype)
+            if a.shape != (b, b):
+                raise ValueError(f"tile ({i}, {j}) is {a.shape}, expected ({b}, {b})")
+            tab[j * mt + i] = a.ctypes.data
+    return tab
+
+
+def _chol_tiles(name, dtype, tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    tab = _tile_table(tiles, mt, b, dtype) if mt > 0 else None
+    return _check(name, getattr(lib, name)(mt, b, t, ctypes.cast(tab, _p) if tab is not None else None))
+
+
+def ompss_dchol_hrl(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    """Right-looking Cholesky of a hypermatrix in place (chol_hrlmain.c:16-37):
+    tiles[(i, j)], i >= j, are b x b column-major float64 arrays."""
+    return _chol_tiles("ompss_dchol_hrl", np.float64, tiles, mt, b, t)
+
+
+def ompss_dchol_hll(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    """Left-looking Cholesky of a hypermatrix in place (chol_hllmain.c:17-45)."""
+    return _chol_tiles("ompss_dchol_hll", np.float64, tiles, mt, b, t)
+
+
+def ompss_schol_hrl(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    return _chol_tiles("ompss_schol_hrl", np.float32, tiles, mt, b, t)
+
+
+def ompss_schol_hll(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    return _chol_tiles("ompss_schol_hll", np.float32, tiles, mt, b, t)
+
+
+def ompss_dchol_nhrl(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    """The nested variants (ompss_dense_chol.c:48-62): the same factor."""
+    return _chol_tiles("ompss_dchol_nhrl", np.float64, tiles, mt, b, t)
+
+
+def ompss_dchol_nhll(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    return _chol_tiles("ompss_dchol_nhll", np.float64, tiles, mt, b, t)
+
+
+def ompss_schol_nhrl(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    return _chol_tiles("ompss_schol_nhrl", np.float32, tiles, mt, b, t)
+
+
+def ompss_schol_nhll(tiles: dict, mt: int, b: int, t: int = 0) -> int:
+    return _chol_tiles("ompss_schol_nhll", np.float32, tiles, mt, b, t)
 
 
 def dspdsolv_main(A: np.ndarray, B: np.ndarray, b: int) -> int:
@@ -281,6 +346,32 @@ def _dt(m: DeviceMatrix) -> int:
 
 def _s(stream):
     return ctypes.c_void_p(stream) if stream else None
+
+
+def chol_tiles_dev(tiles: dict, mt: int, b: int, variant: str = "rl", opts: ExecOpts | None = None,
+                   stream=None) -> int:
+    """Factor a hypermatrix held in device memory: tiles[(i, j)], i >= j, are
+    b x b DeviceMatrix tiles (ld b) of one dtype; returns info."""
+    ptrs = np.zeros(max(1, mt * mt), dtype=np.uint64)
+    dts = set()
+    for j in range(mt):
+        for i in range(j, mt):
+            t = tiles[(i, j)]
+            if (t.rows, t.cols, t.ld) != (b, b, b):
+                raise ValueError(f"tile ({i}, {j}) must be {b} x {b} with ld {b}")
+            ptrs[j * mt + i] = t.ptr
+            dts.add(t.dtype)
+    if len(dts) > 1:
+        raise TypeError("tiles of mixed dtypes")
+    dt = 0 if dts == {np.dtype(np.float64)} else 1
+    tab = DeviceMatrix(ptrs.size, 1)  # 8-byte entries hold the device addresses
+    try:
+        tab.upload(ptrs.view(np.float64))
+        rc = lib.linompss_chol_tiles_dev(dt, 0 if variant == "rl" else 1, mt, b, tab.ptr, _s(stream),
+                                         ctypes.byref(opts) if opts is not None else None)
+    finally:
+        tab.free()
+    return _check("linompss_chol_tiles_dev", rc)
 
 
 def chol_dev(A: DeviceMatrix, b: int, variant: str = "rl", opts: ExecOpts | None = None, stream=None) -> int:

@@ -55,7 +55,9 @@ struct lx_dplan {
     lx_unit *d_units = nullptr;
     int nunits = 0, ntasks = 0, nctr = 0;
     double sim_s = 0.0;
+    bool in_tiles = false;  // every unit's operands inside single b x b tiles
 };
+
 
 struct lx_dev_ctx {
     int dev = -1;
@@ -195,6 +197,8 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
     d.ntasks = (int)pl.tasks().size();
     d.nctr = pl.num_counters();
     d.sim_s = pl.simulated_time();
+    d.in_tiles = (sh.what == LX_W_FACTOR_RL || sh.what == LX_W_FACTOR_LL) && sh.n % sh.b == 0 &&
+                 lx_units_in_tiles(pl.units(), sh.b);
     if (d.nunits > 0) {
         if (hipMalloc(&d.d_units, sizeof(lx_unit) * d.nunits) != hipSuccess) return nullptr;
         if (hipMemcpy(d.d_units, pl.units().data(), sizeof(lx_unit) * d.nunits, hipMemcpyHostToDevice) !=
@@ -241,7 +245,7 @@ static bool lx_trace_on()
     } while (0)
 
 int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t ldx, hipStream_t stream,
-                  const linompss_exec_opts *opts, int *info_out)
+                  const linompss_exec_opts *opts, int *info_out, void *const *dtiles)
 {
     LX_TRACE("run what=%d dtype=%d n=%d b=%d nrhs=%d", sh.what, sh.dtype, sh.n, sh.b, sh.nrhs);
     lx_dev_ctx *c = ctx_get();
@@ -257,6 +261,7 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     lx_dplan *pl = plan_get(c, sh, workers, opts);
     if (!pl) return LINOMPSS_ERR_PLAN;
     LX_TRACE("plan units=%d tasks=%d ctr=%d", pl->nunits, pl->ntasks, pl->nctr);
+    if (dtiles && !pl->in_tiles) return LINOMPSS_ERR_PLAN;
     if (pl->nunits == 0) {
         *info_out = 0;
         return 0;
@@ -274,6 +279,11 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     hs.n = sh.n;
     hs.nrhs = sh.nrhs;
     hs.info_slot = 0;
+    if (dtiles) {
+        hs.tiles = dtiles;
+        hs.tmt = sh.n / sh.b;
+        hs.tb = sh.b;
+    }
     if (fam != LX_FAM_SOLVE) {
         hs.ldw = (int64_t)sh.n + 64;
         hs.W = scratch_get(c, 1, (size_t)hs.ldw * 64 * (sh.dtype == 0 ? sizeof(double) : sizeof(float)));
@@ -295,11 +305,14 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     long long *prof = nullptr;
     static const bool prof_on = getenv("LINOMPSS_EXEC_PROFILE") != nullptr;
     if (prof_on) prof = (long long *)scratch_get(c, 2, sizeof(long long) * LX_PROF_WORDS * (size_t)grid);
-#define LX_LAUNCH(TT, FF)                                                                                          \
-    hipLaunchKernelGGL((lx_exec_kernel<TT, FF>), dim3(grid), dim3(LX_WG_THREADS), 0, stream, pl->d_units, pl->nunits, \
-                       (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr), (lx_status *)ctl,                   \
-                       (int *)(ctl + off_info), ticks, prof)
-    if (sh.dtype == 0) {
+#define LX_LAUNCH(TT, FF, ...)                                                                                     \
+    hipLaunchKernelGGL((lx_exec_kernel<TT, FF, ##__VA_ARGS__>), dim3(grid), dim3(LX_WG_THREADS), 0, stream,           \
+                       pl->d_units, pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr),             \
+                       (lx_status *)ctl, (int *)(ctl + off_info), ticks, prof)
+    if (dtiles) {  // factor plans only (in_tiles is false otherwise)
+        if (sh.dtype == 0) LX_LAUNCH(double, LX_FAM_FACTOR, true);
+        else LX_LAUNCH(float, LX_FAM_FACTOR, true);
+    } else if (sh.dtype == 0) {
         if (fam == LX_FAM_FACTOR) LX_LAUNCH(double, LX_FAM_FACTOR);
         else if (fam == LX_FAM_SOLVE) LX_LAUNCH(double, LX_FAM_SOLVE);
         else LX_LAUNCH(double, LX_FAM_ALL);
@@ -488,6 +501,114 @@ extern "C" int ompss_schol_ll(int n, int b, int t, float *A, int lda)
 {
     (void)t;
     return host_chol<float>(1, n, b, A, lda);
+}
+
+// Hypermatrix A: the same plans and unit bodies, each unit addressed inside
+// its tile (lx_run_unit); the plan is checked to cut every unit inside a tile.
+extern "C" int linompss_chol_tiles_dev(int dtype, int variant, int mt, int b, void *const *dTiles, void *stream,
+                                       const linompss_exec_opts *opts)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (variant != 0 && variant != 1) return -2;
+    if (mt < 0 || (int64_t)mt * std::max(b, 1) > INT_MAX) return -3;
+    if (b <= 0) return -4;
+    if (mt == 0) return 0;
+    if (!dTiles) return -5;
+    std::lock_guard<std::mutex> g(g_mu);
+    lx_shape sh = {variant == 0 ? LX_W_FACTOR_RL : LX_W_FACTOR_LL, dtype, mt * b, b, 0};
+    int info = 0;
+    int rc = lx_run_single(sh, nullptr, (int64_t)mt * b, nullptr, 0, (hipStream_t)stream, opts, &info, dTiles);
+    return rc ? rc : info;
+}
+
+// Host tiles: the lower tiles are staged into one device buffer, the table of
+// their device addresses uploaded, the factor run, and the tiles copied back.
+template <typename T>
+static int host_chol_tiles(int variant, int mt, int b, T **Ah)
+{
+    if (mt < 0) return -1;
+    if (b <= 0) return -2;
+    if (mt == 0) return 0;
+    if (!Ah) return -4;
+    for (int j = 0; j < mt; ++j)
+        for (int i = j; i < mt; ++i)
+            if (!Ah[(size_t)j * mt + i]) return -4;
+    const int dtype = sizeof(T) == 8 ? 0 : 1;
+    const size_t tsz = (size_t)b * b, nt = (size_t)mt * (mt + 1) / 2;
+    T *dbuf = nullptr;
+    void **dtab = nullptr;
+    if (hipMalloc(&dbuf, sizeof(T) * tsz * nt) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    if (hipMalloc(&dtab, sizeof(void *) * (size_t)mt * mt) != hipSuccess) {
+        (void)hipFree(dbuf);
+        return LINOMPSS_ERR_NOMEM;
+    }
+    std::vector<void *> tab((size_t)mt * mt, nullptr);
+    int rc = 0;
+    size_t q = 0;
+    for (int j = 0; j < mt; ++j)
+        for (int i = j; i < mt; ++i) {
+            T *d = dbuf + tsz * q++;
+            tab[(size_t)j * mt + i] = d;
+            if (!rc && hipMemcpy(d, Ah[(size_t)j * mt + i], sizeof(T) * tsz, hipMemcpyHostToDevice) != hipSuccess)
+                rc = LINOMPSS_ERR_HIP;
+        }
+    if (!rc && hipMemcpy(dtab, tab.data(), sizeof(void *) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    if (!rc) rc = linompss_chol_tiles_dev(dtype, variant, mt, b, dtab, nullptr, nullptr);
+    if (rc >= 0)
+        for (int j = 0; j < mt && rc >= 0; ++j)
+            for (int i = j; i < mt; ++i)
+                if (hipMemcpy(Ah[(size_t)j * mt + i], tab[(size_t)j * mt + i], sizeof(T) * tsz,
+                              hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = LINOMPSS_ERR_HIP;
+                    break;
+                }
+    (void)hipFree(dtab);
+    (void)hipFree(dbuf);
+    return rc;
+}
+
+extern "C" int ompss_dchol_hrl(int mt, int b, int t, double **Ah)
+{
+    (void)t;
+    return host_chol_tiles<double>(0, mt, b, Ah);
+}
+extern "C" int ompss_dchol_hll(int mt, int b, int t, double **Ah)
+{
+    (void)t;
+    return host_chol_tiles<double>(1, mt, b, Ah);
+}
+extern "C" int ompss_schol_hrl(int mt, int b, int t, float **Ah)
+{
+    (void)t;
+    return host_chol_tiles<float>(0, mt, b, Ah);
+}
+extern "C" int ompss_schol_hll(int mt, int b, int t, float **Ah)
+{
+    (void)t;
+    return host_chol_tiles<float>(1, mt, b, Ah);
+}
+// The nested variants sub-block each tile task by t on the CPU; here the
+// executor cuts its own units, so they run the same factor.
+extern "C" int ompss_dchol_nhrl(int mt, int b, int t, double **Ah)
+{
+    (void)t;
+    return host_chol_tiles<double>(0, mt, b, Ah);
+}
+extern "C" int ompss_dchol_nhll(int mt, int b, int t, double **Ah)
+{
+    (void)t;
+    return host_chol_tiles<double>(1, mt, b, Ah);
+}
+extern "C" int ompss_schol_nhrl(int mt, int b, int t, float **Ah)
+{
+    (void)t;
+    return host_chol_tiles<float>(0, mt, b, Ah);
+}
+extern "C" int ompss_schol_nhll(int mt, int b, int t, float **Ah)
+{
+    (void)t;
+    return host_chol_tiles<float>(1, mt, b, Ah);
 }
 
 template <typename T>

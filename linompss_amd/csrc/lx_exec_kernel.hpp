@@ -47,20 +47,40 @@ __device__ __forceinline__ long long lx_off(int64_t rcp, long long ld)
     return (long long)(rcp >> 32) * ld + (long long)(rcp & 0xffffffffll);
 }
 
-template <typename T, int FAM>
+template <typename T, int FAM, bool HY = false>
 __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__restrict__ sys, int *info, char *smem)
 {
     const lx_sys s = sys[u.sys];
     T *A = (T *)s.A, *X = (T *)s.X;
-    T *Cb = u.cbase == LX_BASE_X ? X : A;
-    const T *Ab = u.abase == LX_BASE_X ? X : A;
-    const T *Bb = u.bbase == LX_BASE_X ? X : A;
-    const long long ldc = u.cbase == LX_BASE_X ? s.ldx : s.lda;
-    const long long lda = u.abase == LX_BASE_X ? s.ldx : s.lda;
-    const long long ldb = u.bbase == LX_BASE_X ? s.ldx : s.lda;
-    T *C = Cb + lx_off(u.c, ldc);
-    const T *Ap = Ab + lx_off(u.a, lda);
-    const T *Bp = Bb + lx_off(u.b, ldb);
+    T *C;
+    const T *Ap, *Bp;
+    long long ldc, lda, ldb;
+    if constexpr (HY) {
+        // hypermatrix (factor plans): a unit's operands never straddle a tile
+        // (checked when the plan is built), so each is addressed inside its
+        // own tile, leading dimension tb; an unused operand's offset is 0,
+        // tile (0, 0)
+        auto at = [&](int64_t rcp) {
+            const int r = (int)(rcp & 0xffffffffll), c = (int)(rcp >> 32);
+            return (T *)s.tiles[(int64_t)(c / s.tb) * s.tmt + r / s.tb] + (int64_t)(c % s.tb) * s.tb + r % s.tb;
+        };
+        (void)A;
+        (void)X;
+        ldc = lda = ldb = s.tb;
+        C = at(u.c);
+        Ap = at(u.a);
+        Bp = at(u.b);
+    } else {
+        T *Cb = u.cbase == LX_BASE_X ? X : A;
+        const T *Ab = u.abase == LX_BASE_X ? X : A;
+        const T *Bb = u.bbase == LX_BASE_X ? X : A;
+        ldc = u.cbase == LX_BASE_X ? s.ldx : s.lda;
+        lda = u.abase == LX_BASE_X ? s.ldx : s.lda;
+        ldb = u.bbase == LX_BASE_X ? s.ldx : s.lda;
+        C = Cb + lx_off(u.c, ldc);
+        Ap = Ab + lx_off(u.a, lda);
+        Bp = Bb + lx_off(u.b, ldb);
+    }
     constexpr bool F = FAM != LX_FAM_SOLVE, S = FAM != LX_FAM_FACTOR;
     switch (u.op) {
     case LX_OP_POTRF: {
@@ -162,7 +182,7 @@ __device__ __forceinline__ void lx_wg_publish(int *counter)
 //   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
 #define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
 
-template <typename T, int FAM>
+template <typename T, int FAM, bool HY = false>
 __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
                    lx_status *st, int *info, long long spin_ticks, long long *prof)
@@ -220,7 +240,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
         if (!go) break;
         if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
-        lx_run_unit<T, FAM>(u, sys, info, smem);
+        lx_run_unit<T, FAM, HY>(u, sys, info, smem);
         lx_wg_publish(&ctr[u.ctr]);
         if (lx_wave() == 0) atomicAdd(&st->done, lane == 0 ? 1 : 0);
         if (prof && threadIdx.x == 0 && u.op >= 0 && u.op < LX_OP_NOP) {  // no barrier inside

@@ -26,8 +26,11 @@ struct lx_shape {
 // Runs one plan over nsys systems sharing the same shape family (the plan was
 // built for exactly these systems).  info_out[s] receives each system's info.
 // Returns 0 or a LINOMPSS_ERR_* code.
+// dtiles (factor plans only): a device table of tile pointers for a
+// hypermatrix A (tile (i, j) at dtiles[j * (n / b) + i], b x b, leading
+// dimension b); dA is then unused.
 int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t ldx, hipStream_t stream,
-                  const linompss_exec_opts *opts, int *info_out);
+                  const linompss_exec_opts *opts, int *info_out, void *const *dtiles = nullptr);
 
 // Triangular sweeps with dedicated kernels (lx_sweep.hip): which = 1 forward
 // (L Y = X), 2 backward (L^T Z = Y), 3 both.  Synchronises the stream; returns

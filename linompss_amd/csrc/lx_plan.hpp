@@ -56,6 +56,10 @@ struct lx_sys {
     void *W;               // factor workspace (T*), column-major (n + 64) x 64:
                            // inverses of the 64 x 64 diagonal blocks, rows as in A
     int64_t ldw;
+    void *const *tiles;    // hypermatrix A (factor plans only): element (r, c) in
+                           // tile tiles[(c / tb) * tmt + r / tb] at (c % tb) * tb
+                           // + r % tb; null for a column-major A (lda)
+    int32_t tmt, tb;
 };
 
 // Device-side status words (one per launch).

@@ -539,3 +539,26 @@ void lx_planner::finalize()
     for (int32_t q : order) ordered.push_back(U[q]);
     U.swap(ordered);
 }
+
+// True when each unit's operands (C, and A / B where the op reads them) lie
+// inside single b x b tiles: the condition for running the plan on a
+// hypermatrix, whose tiles are separate buffers.
+bool lx_units_in_tiles(const std::vector<lx_unit> &U, int b)
+{
+    auto inside = [&](int64_t rcp, int rows, int cols) {
+        const int64_t r = rcp & 0xffffffffll, c = rcp >> 32;
+        return rows > 0 && cols > 0 && r / b == (r + rows - 1) / b && c / b == (c + cols - 1) / b;
+    };
+    for (const lx_unit &u : U) {
+        bool ok = false;
+        switch (u.op) {
+        case LX_OP_POTRF: ok = inside(u.c, u.M, u.M); break;
+        case LX_OP_TRSM_RLTN: ok = inside(u.c, u.M, u.N) && inside(u.a, u.N, u.N); break;
+        case LX_OP_SYRK:
+        case LX_OP_GEMM_NT: ok = inside(u.c, u.M, u.N) && inside(u.a, u.M, u.K) && inside(u.b, u.N, u.K); break;
+        default: break;
+        }
+        if (!ok) return false;
+    }
+    return true;
+}

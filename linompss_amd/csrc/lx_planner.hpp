@@ -15,6 +15,10 @@
 
 #include "lx_plan.hpp"
 
+// True when each unit's operands lie inside single b x b tiles (the
+// condition for running a plan on a hypermatrix).
+bool lx_units_in_tiles(const std::vector<lx_unit> &U, int b);
+
 struct lx_plan_params {
     int unit_m = 128;       // GEMM / SYRK unit rows (one workgroup tile)
     int unit_n = 128;       // GEMM / SYRK unit cols

@@ -74,6 +74,8 @@ _sig("lxo_genmat", None, _i, _i, _i, _l, _p)
 _sig("lxo_genmat_sym_full", None, _i, _l, _p)
 for nm in ("lxo_dchol_rl", "lxo_dchol_ll", "lxo_schol_rl", "lxo_schol_ll"):
     _sig(nm, _i, _i, _i, _p, _i, _i)
+for nm in ("lxo_dchol_hrl", "lxo_dchol_hll", "lxo_schol_hrl", "lxo_schol_hll"):
+    _sig(nm, _i, _i, _i, _p, _i)
 _sig("lxo_dltrsm_lower", None, _i, _i, _i, _i, _i, _d, _p, _i, _p, _i, _i)
 _sig("lxo_sltrsm_lower", None, _i, _i, _i, _i, _i, _f, _p, _i, _p, _i, _i)
 _sig("lxo_dspdsolv", _i, _i, _i, _i, _p, _p, _i)
@@ -119,6 +121,40 @@ def chol(A: np.ndarray, b: int, variant: str = "rl", nthreads: int = 0) -> int:
     assert A.flags.f_contiguous
     pre = "lxo_d" if A.dtype == np.float64 else "lxo_s"
     return getattr(lib, f"{pre}chol_{variant}")(A.shape[1], b, _pt(A), A.shape[0], nthreads)
+
+
+def to_tiles(A: np.ndarray, b: int) -> dict:
+    """Hypermatrix form of the lower triangle of A (n a multiple of b): tile
+    (i, j), i >= j, as a contiguous column-major b x b array (chol_hsetup.c)."""
+    n = A.shape[0]
+    assert n % b == 0
+    mt = n // b
+    return {(i, j): np.array(A[i * b:(i + 1) * b, j * b:(j + 1) * b], order="F")
+            for j in range(mt) for i in range(j, mt)}
+
+
+def from_tiles(tiles: dict, mt: int, b: int) -> np.ndarray:
+    """Column-major matrix with the tiles' values in the lower tiles, zeros above."""
+    dt = next(iter(tiles.values())).dtype
+    A = np.zeros((mt * b, mt * b), dtype=dt, order="F")
+    for (i, j), t in tiles.items():
+        A[i * b:(i + 1) * b, j * b:(j + 1) * b] = t
+    return A
+
+
+def tile_table(tiles: dict, mt: int):
+    """ctypes array of mt*mt pointers, entry j*mt + i = tile (i, j) (null above)."""
+    tab = (ctypes.c_void_p * (mt * mt))()
+    for (i, j), t in tiles.items():
+        tab[j * mt + i] = t.ctypes.data
+    return tab
+
+
+def chol_tiles(tiles: dict, mt: int, b: int, variant: str = "hrl", nthreads: int = 0) -> int:
+    """CHOL_HRL / CHOL_HLL in place on the tiles."""
+    pre = "lxo_d" if next(iter(tiles.values())).dtype == np.float64 else "lxo_s"
+    tab = tile_table(tiles, mt)
+    return getattr(lib, f"{pre}chol_{variant}")(mt, b, ctypes.cast(tab, _p), nthreads)
 
 
 def ltrsm_lower(trans: int, A: np.ndarray, B: np.ndarray, bm: int, alpha: float = 1.0, nthreads: int = 0) -> None:

@@ -140,6 +140,10 @@ int lxo_dchol_rl(int n, int b, double *A, int lda, int nthreads) { return lxo_dc
 int lxo_dchol_ll(int n, int b, double *A, int lda, int nthreads) { return lxo_dchol_ll_impl(n, b, A, lda, nthreads); }
 int lxo_schol_rl(int n, int b, float *A, int lda, int nthreads) { return lxo_schol_rl_impl(n, b, A, lda, nthreads); }
 int lxo_schol_ll(int n, int b, float *A, int lda, int nthreads) { return lxo_schol_ll_impl(n, b, A, lda, nthreads); }
+int lxo_dchol_hrl(int mt, int b, double **Ah, int nthreads) { return lxo_dchol_hrl_impl(mt, b, Ah, nthreads); }
+int lxo_dchol_hll(int mt, int b, double **Ah, int nthreads) { return lxo_dchol_hll_impl(mt, b, Ah, nthreads); }
+int lxo_schol_hrl(int mt, int b, float **Ah, int nthreads) { return lxo_schol_hrl_impl(mt, b, Ah, nthreads); }
+int lxo_schol_hll(int mt, int b, float **Ah, int nthreads) { return lxo_schol_hll_impl(mt, b, Ah, nthreads); }
 
 void lxo_dltrsm_lower(int trans, int bm, int bn, int m, int n, double alpha,
                       const double *A, int lda, double *B, int ldb, int nthreads)

@@ -11,6 +11,8 @@
  *                                tests/dense_tests/test_aux.h:114-128
  *   - right-looking CHOL_RL      src/cholesky/dense/chol_rlmain.c:22-53
  *   - left-looking  CHOL_LL      src/cholesky/dense/chol_llmain.c:23-57
+ *   - hypermatrix CHOL_HRL/HLL   src/cholesky/dense/chol_hrlmain.c:16-37,
+ *                                chol_hllmain.c:17-45
  *   - blocked lower TRSM sweeps  src/blocked_blas/bblas_trsm.h:116-160
  *   - dspdsolv_main              src/spdsolv/spdsolv_main.c:10-20
  *   - refinement loop contract   src/itref/itref.c:999-1101 (inner solver is
@@ -48,6 +50,13 @@ int    lxo_dchol_rl(int n, int b, double *A, int lda, int nthreads);
 int    lxo_dchol_ll(int n, int b, double *A, int lda, int nthreads);
 int    lxo_schol_rl(int n, int b, float *A, int lda, int nthreads);
 int    lxo_schol_ll(int n, int b, float *A, int lda, int nthreads);
+/* The hypermatrix variants CHOL_HRL / CHOL_HLL (chol_hrlmain.c:16-37,
+ * chol_hllmain.c:17-45): tile (i, j) is Ah[j * mt + i], b x b with leading
+ * dimension b, tiles i >= j read.  Same return convention (info = global column). */
+int    lxo_dchol_hrl(int mt, int b, double **Ah, int nthreads);
+int    lxo_dchol_hll(int mt, int b, double **Ah, int nthreads);
+int    lxo_schol_hrl(int mt, int b, float **Ah, int nthreads);
+int    lxo_schol_hll(int mt, int b, float **Ah, int nthreads);
 
 /* bblas_dltrsm lower branches: trans 0 = N (L X = alpha B), 1 = T (L^T X = alpha B) */
 void   lxo_dltrsm_lower(int trans, int bm, int bn, int m, int n, double alpha,

@@ -89,6 +89,75 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     return (st.abort || st.timeouts) ? -1 : 0;
 }
 
+// Whether the factor plan of an n x n matrix (n a multiple of b) cuts every
+// unit inside a b x b tile, with the library's tuning parameters given.
+extern "C" int lx_emu_plan_in_tiles(int variant, int n, int b, int update_prec, int potrf_sub, int wide_n)
+{
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    P.update_prec = update_prec;
+    P.potrf_sub = potrf_sub;
+    P.gemm_wide_n = wide_n;
+    lx_planner pl(P);
+    int s = pl.add_system(n, b, 0);
+    if (variant == 0) pl.factor_rl(s);
+    else pl.factor_ll(s);
+    pl.finalize();
+    return lx_units_in_tiles(pl.units(), b) ? 1 : 0;
+}
+
+// The factor of a hypermatrix (tile (i, j) at tiles[j * mt + i], b x b, ld b)
+// through the tiled executor instantiation, one workgroup of 256 host threads.
+extern "C" int lx_emu_run_tiles(int dtype, int variant, int mt, int b, void **tiles, int *info_out)
+{
+    const int n = mt * b;
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    P.update_prec = dtype == 1 ? g_update_prec : 0;
+    lx_planner pl(P);
+    int s = pl.add_system(n, b, 0);
+    if (variant == 0) pl.factor_rl(s);
+    else pl.factor_ll(s);
+    pl.finalize();
+    const std::vector<lx_unit> &U = pl.units();
+    if (!lx_units_in_tiles(U, b)) return -3;
+    std::vector<int> ctr(pl.num_counters(), 0);
+    lx_status st = {};
+    int info = INT_MAX;
+    lx_sys sys = {};
+    sys.lda = n;
+    sys.n = n;
+    sys.tiles = tiles;
+    sys.tmt = mt;
+    sys.tb = b;
+    sys.ldw = (int64_t)n + 64;
+    std::vector<double> wbuf((size_t)sys.ldw * 64);
+    sys.W = wbuf.data();
+    blockDim.x = 256;
+    gridDim.x = 1;
+    lx_emu_wg_barrier.init(256, "workgroup");
+    for (int w = 0; w < 4; ++w) lx_emu_wave_barrier[w].init(64, "wavefront");
+    const long long ticks = 100000000LL * 20;
+    std::vector<std::thread> th;
+    th.reserve(256);
+    for (int t = 0; t < 256; ++t)
+        th.emplace_back([&, t] {
+            threadIdx.x = t;
+            blockIdx.x = 0;
+            if (dtype == 0)
+                lx_exec_kernel<double, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
+                                                            ticks, nullptr);
+            else
+                lx_exec_kernel<float, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
+                                                           ticks, nullptr);
+        });
+    for (auto &t : th) t.join();
+    *info_out = info == INT_MAX ? 0 : info;
+    return (st.abort || st.timeouts || st.done != (int)U.size()) ? -1 : 0;
+}
+
 template <typename F>
 static void run_wg(F &&body)
 {

@@ -290,3 +290,44 @@ def test_adjoint_gemm_kernel(ta, tb):
     tot = part.reshape(3, N, M).sum(0).T
     ref = 2.0 * op(A, ta) @ op(B, tb)
     assert np.abs(tot - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_plans_cut_units_inside_tiles(emu, variant):
+    """The hypermatrix path needs every unit's operands inside one b x b tile:
+    the factor plans of n = mt * b over tile sizes, both diagonal sub-tiles the
+    library uses (FP32 128, FP64 256), both update precisions and GEMM widths."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E.lx_emu_plan_in_tiles.argtypes = [ctypes.c_int] * 6
+    for b in (1, 37, 64, 96, 128, 200, 256, 384, 512, 1024):
+        for mt in (1, 2, 5):
+            for up, sub, wn in ((0, 128, 512), (1, 128, 512), (0, 256, 512), (1, 128, 1024), (0, 64, 256)):
+                assert E.lx_emu_plan_in_tiles(variant, mt * b, b, up, sub, wn) == 1, (b, mt, up, sub, wn)
+
+
+@pytest.mark.parametrize("dtype,variant,mt,b", [(0, 0, 3, 48), (0, 1, 3, 48), (1, 0, 2, 80), (0, 0, 1, 64)])
+def test_factor_hypermatrix(emu, orc, dtype, variant, mt, b):
+    """The tiled executor instantiation on a hypermatrix (separate tile
+    buffers): each tile of the factor against the oracle's CHOL_HRL / CHOL_HLL
+    on the same tiles and against LAPACK on the assembled matrix."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E.lx_emu_run_tiles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_int)]
+    n = mt * b
+    dt = np.float64 if dtype == 0 else np.float32
+    A0 = orc.genmat_spd(n, 11, diag_mode=1)
+    tiles = orc.to_tiles(A0.astype(dt), b)
+    ref = {k: v.copy(order="F") for k, v in tiles.items()}
+    assert orc.chol_tiles(ref, mt, b, "hrl" if variant == 0 else "hll") == 0
+    tab = orc.tile_table(tiles, mt)
+    info = ctypes.c_int(-1)
+    assert E.lx_emu_run_tiles(dtype, variant, mt, b, ctypes.cast(tab, ctypes.c_void_p), ctypes.byref(info)) == 0
+    assert info.value == 0
+    L = np.linalg.cholesky(A0)
+    tol = 1e-12 if dtype == 0 else 2e-6
+    got = np.tril(orc.from_tiles(tiles, mt, b))
+    assert np.abs(got - L).max() <= tol * np.abs(L).max()
+    assert np.abs(got - np.tril(orc.from_tiles(ref, mt, b))).max() <= tol * np.abs(L).max()
+    # the strict upper part of each diagonal tile is left as it was
+    for k in range(mt):
+        assert np.array_equal(np.triu(tiles[(k, k)], 1), np.triu(A0[k * b:(k + 1) * b, k * b:(k + 1) * b].astype(dt), 1))

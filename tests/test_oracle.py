@@ -100,6 +100,36 @@ def test_chol_reports_info(orc):
     assert orc.chol(A.copy(order="F"), 16, "ll") == 41
 
 
+@pytest.mark.parametrize("variant", ["hrl", "hll"])
+@pytest.mark.parametrize("mt,b", [(1, 1), (1, 7), (4, 3), (5, 16), (3, 100)])
+def test_chol_hypermatrix_vs_lapack(orc, mt, b, variant):
+    """CHOL_HRL / CHOL_HLL over separate tile buffers against LAPACK on the
+    assembled matrix (brute force below n = 17), the tiles above the diagonal
+    untouched by construction (they are not passed), the strict upper part of
+    each diagonal tile left as it was; FP32 at FP32 accuracy."""
+    n = mt * b
+    A0 = orc.genmat_spd(n, 5, diag_mode=0)
+    tiles = orc.to_tiles(A0, b)
+    assert orc.chol_tiles(tiles, mt, b, variant) == 0
+    L = np.tril(orc.from_tiles(tiles, mt, b))
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(L - Lref) <= 1e-13 * np.linalg.norm(Lref)
+    if n <= 17:
+        assert np.allclose(L, _brute_chol(A0), rtol=1e-12, atol=1e-14)
+    for k in range(mt):
+        assert np.array_equal(np.triu(tiles[(k, k)], 1), np.triu(A0[k * b:(k + 1) * b, k * b:(k + 1) * b], 1))
+    tf = orc.to_tiles(A0.astype(np.float32), b)
+    assert orc.chol_tiles(tf, mt, b, variant) == 0
+    assert np.linalg.norm(np.tril(orc.from_tiles(tf, mt, b)) - Lref) <= 1e-6 * np.linalg.norm(Lref)
+
+
+def test_chol_hypermatrix_reports_info(orc):
+    A = orc.genmat_spd(100, 2, diag_mode=1)
+    A[40, 40] = -1.0
+    for variant in ("hrl", "hll"):
+        assert orc.chol_tiles(orc.to_tiles(A, 20), 5, 20, variant) == 41
+
+
 @pytest.mark.parametrize("trans", [0, 1])
 @pytest.mark.parametrize("m,nr,bm", [(600, 5, 121), (64, 1, 64), (65, 3, 64), (1000, 7, 256)])
 def test_ltrsm_vs_scipy(orc, trans, m, nr, bm):
