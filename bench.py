@@ -318,6 +318,68 @@ def bench_hyper(args):
     print(json.dumps(out))
 
 
+def bench_blas3(args):
+    """The standalone BLAS-3 entry points on device memory (src/gemm/ompss_mm.h,
+    src/syrk/ompss_syrk.c, src/trsm/ompss_trsm.c): GEMM and SYRK at
+    n = k = 8192, left lower TRSM with 8192 x 512 right-hand sides, FP64 and FP32."""
+    import ctypes
+    import numpy as np
+    import linompss_amd as lx
+    n, nr = 8192, 512
+    rng = np.random.default_rng(5)
+    res = {}
+    for dt, name, dtype in ((np.float64, "fp64", 0), (np.float32, "fp32", 1)):
+        A = lx.DeviceMatrix.from_host(np.asfortranarray(rng.random((n, n)).astype(dt)))
+        B = lx.DeviceMatrix.from_host(np.asfortranarray(rng.random((n, n)).astype(dt)))
+        C = lx.DeviceMatrix.from_host(np.asfortranarray(rng.random((n, n)).astype(dt)))
+        Lh = np.asfortranarray((np.tril(rng.random((n, n))) / n + np.eye(n)).astype(dt))
+        L = lx.DeviceMatrix.from_host(Lh)
+        X = lx.DeviceMatrix.from_host(np.asfortranarray(rng.random((n, nr)).astype(dt)))
+
+        def timed(fn):
+            ts = []
+            for k in range(args.warmup + args.steps):
+                lx.lib.linompss_dev_sync()
+                t0 = time.perf_counter()
+                rc = fn()
+                lx.lib.linompss_dev_sync()
+                if rc != 0:
+                    raise RuntimeError(f"rc {rc}")
+                if k >= args.warmup:
+                    ts.append(time.perf_counter() - t0)
+            return float(np.median(ts))
+        t_gemm = timed(lambda: lx.lib.linompss_gemm_dev(dtype, b"N", b"T", n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5,
+                                                        C.ptr, n, None))
+        t_syrk = timed(lambda: lx.lib.linompss_syrk_dev(dtype, b"L", b"N", n, n, 1.0, A.ptr, n, 0.5, C.ptr, n, None))
+        t_trsm = timed(lambda: lx.lib.linompss_trsm_blas_dev(dtype, b"L", b"L", b"N", b"N", n, nr, 1.0, L.ptr, n,
+                                                             X.ptr, n, None))
+        peak = PEAK[name]
+        res[name] = {
+            "gemm_ms": round(t_gemm * 1e3, 3), "gemm_tflops": round(2.0 * n ** 3 / t_gemm / 1e12, 2),
+            "gemm_frac": round(2.0 * n ** 3 / t_gemm / 1e12 / peak, 4),
+            "syrk_ms": round(t_syrk * 1e3, 3), "syrk_tflops": round(1.0 * n ** 3 / t_syrk / 1e12, 2),
+            "trsm_ms": round(t_trsm * 1e3, 3), "trsm_tflops": round(1.0 * n * n * nr / t_trsm / 1e12, 2),
+        }
+        log(f"{name}: gemm {t_gemm * 1e3:.2f} ms ({res[name]['gemm_tflops']} TF), syrk {t_syrk * 1e3:.2f} ms, "
+            f"trsm {t_trsm * 1e3:.2f} ms")
+        for m in (A, B, C, L, X):
+            m.free()
+    g64 = res["fp64"]["gemm_ms"] * 1e-3
+    out = {
+        "metric": "standalone blocked BLAS-3 on device: DGEMM TFLOP/s at m = n = k = 8192 (NT)",
+        "value": res["fp64"]["gemm_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(g64 * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: U[0,1) matrices (numpy PCG64 seed 5); TRSM on tril(U[0,1))/n + I",
+        "config": {"workload": "ompss_{d,s}gemm / syrk / trsm entry points on device memory", "n": n,
+                   "trsm_nrhs": nr},
+        "roofline": {"bound": "mfma", "achieved": res["fp64"]["gemm_tflops"], "peak": PEAK["fp64"],
+                     "unit": "TFLOP/s", "frac": res["fp64"]["gemm_frac"], "traffic": None},
+        "results": res,
+    }
+    print(json.dumps(out))
+
+
 def traffic_from(fetch_csv, write_csv, out, n):
     """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
@@ -378,20 +440,21 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
-    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched", "gp", "hyper"),
+    ap.add_argument("--workload", default="mixed", choices=("mixed", "batched", "gp", "hyper", "blas3"),
                     help="mixed: the n=32768 refinement solve (the headline line); batched: 4096 "
                          "independent systems of order uniform in [32, 512] in one persistent launch; "
                          "gp: Gaussian-process likelihood forward + backward (Cholesky adjoint) at 8192 points; "
-                         "hyper: the hypermatrix factor (separate device tiles) beside the column-major one")
+                         "hyper: the hypermatrix factor (separate device tiles) beside the column-major one; "
+                         "blas3: the standalone GEMM / SYRK / TRSM entry points at n = 8192")
     ap.add_argument("--gp-n", type=int, default=8192)
     ap.add_argument("--gp-b", type=int, default=256, help="tile of the GP workload's FP64 factor")
     ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
-    if args.workload in ("batched", "gp", "hyper"):
+    if args.workload in ("batched", "gp", "hyper", "blas3"):
         if args.gpus != 1:
             print("bench.py measures one GPU. Run with --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        {"batched": bench_batched, "gp": bench_gp, "hyper": bench_hyper}[args.workload](args)
+        {"batched": bench_batched, "gp": bench_gp, "hyper": bench_hyper, "blas3": bench_blas3}[args.workload](args)
         return
     if args.traffic_from:
         traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)

@@ -69,6 +69,34 @@ int ompss_dchol_nhll(int mt, int b, int t, double **Ah);
 int ompss_schol_nhrl(int mt, int b, int t, float **Ah);
 int ompss_schol_nhll(int mt, int b, int t, float **Ah);
 
+/* ---- Standalone blocked BLAS-3 (host memory, synchronous) ---------------
+ * The reference's flags are ompssblas_t (src/fp/fpmatr.h:14-17: a CBLAS enum
+ * under MKL, a "N"/"T"-style string otherwise); here they are plain chars,
+ * case-insensitive ('N'/'T', 'L'/'U', 'L'/'R' for side, 'N'/'U' for diag).
+ * Block sizes are accepted (> 0) and ignored.  Return 0 or -i for argument i. */
+
+/* C = alpha op(A) op(B) + beta C.  Replaces ompss_dgemm (src/gemm/ompss_mm.h:27,
+ * ompss_mm.c:6-10 -> mm_main.c); linompss_sgemm replaces ompss_sgemm (:23). */
+int linompss_dgemm(char transa, char transb, int b, int d, int c, int m, int n, int k, double alpha,
+                   const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc);
+int linompss_sgemm(char transa, char transb, int b, int d, int c, int m, int n, int k, float alpha,
+                   const float *A, int lda, const float *B, int ldb, float beta, float *C, int ldc);
+/* C = alpha op(A) op(A)^T + beta C on the uplo triangle (the other untouched).
+ * Replaces ompss_dsyrk / ompss_ssyrk (src/syrk/ompss_syrk.h:22 / :20,
+ * ompss_syrk.c:9-12 -> syrk_main.c). */
+int linompss_dsyrk(char uplo, char trans, int b, int n, int k, double alpha, const double *A, int lda,
+                   double beta, double *C, int ldc);
+int linompss_ssyrk(char uplo, char trans, int b, int n, int k, float alpha, const float *A, int lda,
+                   float beta, float *C, int ldc);
+/* op(A) X = alpha B (side 'L', A m x m) or X op(A) = alpha B (side 'R', A n x n),
+ * B m x n overwritten by X.  Replaces ompss_dtrsm / ompss_strsm
+ * (src/trsm/ompss_trsm.h:23 / :20, ompss_trsm.c:4-12 -> trsm_main.c:6-17 ->
+ * bblas_trsm.h:116-201); the reference leaves side 'R' unimplemented. */
+int linompss_dtrsm(char side, char uplo, char trans, char diag, int m, int n, int b, double alpha,
+                   const double *A, int lda, double *B, int ldb);
+int linompss_strsm(char side, char uplo, char trans, char diag, int m, int n, int b, float alpha,
+                   const float *A, int lda, float *B, int ldb);
+
 /* Replaces dspdsolv_main (src/spdsolv/spdsolv_main.h:20, spdsolv_main.c:10-20):
  * RL factor of the m x m matrix A (lda = m), then the forward and backward
  * sweeps on the m x n right-hand sides B (ldb = m), in place. */
@@ -150,6 +178,14 @@ int linompss_chol_dev(int dtype, int variant, int n, int b, void *dA, int64_t ld
  * chol_hrlmain.c:16-37, chol_hllmain.c:17-45). */
 int linompss_chol_tiles_dev(int dtype, int variant, int mt, int b, void *const *dTiles, void *stream,
                             const linompss_exec_opts *opts);
+/* Device forms of the BLAS-3 entry points above (dtype 0 FP64, 1 FP32; alpha
+ * and beta converted to the dtype): the same flags and argument checks. */
+int linompss_gemm_dev(int dtype, char transa, char transb, int m, int n, int k, double alpha, const void *dA,
+                      int64_t lda, const void *dB, int64_t ldb, double beta, void *dC, int64_t ldc, void *stream);
+int linompss_syrk_dev(int dtype, char uplo, char trans, int n, int k, double alpha, const void *dA, int64_t lda,
+                      double beta, void *dC, int64_t ldc, void *stream);
+int linompss_trsm_blas_dev(int dtype, char side, char uplo, char trans, char diag, int m, int n, double alpha,
+                           const void *dA, int64_t lda, void *dB, int64_t ldb, void *stream);
 int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, void *dX, int64_t ldx,
                       void *stream, const linompss_exec_opts *opts);
 int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,

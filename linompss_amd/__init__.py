@@ -36,7 +36,7 @@ __all__ = [
     "IrStatus", "ExecStats", "DeviceMatrix", "device_count", "BatchOpts", "BatchStatus", "DeviceBatch",
     "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd", "ompss_dchol_hrl", "ompss_dchol_hll",
     "ompss_schol_hrl", "ompss_schol_hll", "ompss_dchol_nhrl", "ompss_dchol_nhll", "ompss_schol_nhrl",
-    "ompss_schol_nhll", "chol_tiles_dev",
+    "ompss_schol_nhll", "chol_tiles_dev", "gemm", "syrk", "trsm",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -102,6 +102,15 @@ _sig("linompss_dposv_ir", _i, _i, _i, _i, _p, _i, _p, _i, _p, _i, ctypes.POINTER
      ctypes.POINTER(IrStatus))
 _sig("linompss_chol_dev", _i, _i, _i, _i, _i, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_chol_tiles_dev", _i, _i, _i, _i, _i, _p, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_dgemm", _i, _c, _c, _i, _i, _i, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i)
+_sig("linompss_sgemm", _i, _c, _c, _i, _i, _i, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sig("linompss_dsyrk", _i, _c, _c, _i, _i, _i, _d, _p, _i, _d, _p, _i)
+_sig("linompss_ssyrk", _i, _c, _c, _i, _i, _i, _f, _p, _i, _f, _p, _i)
+_sig("linompss_dtrsm", _i, _c, _c, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i)
+_sig("linompss_strsm", _i, _c, _c, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i)
+_sig("linompss_gemm_dev", _i, _i, _c, _c, _i, _i, _i, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p)
+_sig("linompss_syrk_dev", _i, _i, _c, _c, _i, _i, _d, _p, _i64, _d, _p, _i64, _p)
+_sig("linompss_trsm_blas_dev", _i, _i, _c, _c, _c, _c, _i, _i, _d, _p, _i64, _p, _i64, _p)
 _sig("linompss_posv_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_potrs_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_dposv_ir_dev", _i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p, ctypes.POINTER(IrOpts),
@@ -223,6 +232,48 @@ def ompss_schol_nhrl(tiles: dict, mt: int, b: int, t: int = 0) -> int:
 
 def ompss_schol_nhll(tiles: dict, mt: int, b: int, t: int = 0) -> int:
     return _chol_tiles("ompss_schol_nhll", np.float32, tiles, mt, b, t)
+
+
+def _blas_pre(a: np.ndarray) -> str:
+    if a.dtype == np.float64:
+        return "d"
+    if a.dtype == np.float32:
+        return "s"
+    raise TypeError(f"expected float64 or float32, got {a.dtype}")
+
+
+def gemm(transa: str, transb: str, m: int, n: int, k: int, alpha: float, A: np.ndarray, B: np.ndarray,
+         beta: float, C: np.ndarray, b: int = 128, d: int = 128, c: int = 128) -> None:
+    """C[:m, :n] = alpha op(A) op(B) + beta C in place (ompss_{d,s}gemm, src/gemm/ompss_mm.h:23-27);
+    leading dimensions are the arrays' row counts."""
+    pre = _blas_pre(C)
+    for a in (A, B, C):
+        _colmajor(a, C.dtype)
+    _check(f"linompss_{pre}gemm", getattr(lib, f"linompss_{pre}gemm")(
+        transa.encode(), transb.encode(), b, d, c, m, n, k, alpha, _ptr(A), A.shape[0], _ptr(B), B.shape[0],
+        beta, _ptr(C), C.shape[0]), False)
+
+
+def syrk(uplo: str, trans: str, n: int, k: int, alpha: float, A: np.ndarray, beta: float, C: np.ndarray,
+         b: int = 128) -> None:
+    """The uplo triangle of C[:n, :n] = alpha op(A) op(A)^T + beta C (ompss_{d,s}syrk, syrk_main.c)."""
+    pre = _blas_pre(C)
+    for a in (A, C):
+        _colmajor(a, C.dtype)
+    _check(f"linompss_{pre}syrk", getattr(lib, f"linompss_{pre}syrk")(
+        uplo.encode(), trans.encode(), b, n, k, alpha, _ptr(A), A.shape[0], beta, _ptr(C), C.shape[0]), False)
+
+
+def trsm(side: str, uplo: str, trans: str, diag: str, m: int, n: int, alpha: float, A: np.ndarray,
+         B: np.ndarray, b: int = 128) -> None:
+    """B[:m, :n] := X with op(A) X = alpha B (side L) or X op(A) = alpha B (side R)
+    (ompss_{d,s}trsm, trsm_main.c:6-17 -> bblas_trsm.h:116-201)."""
+    pre = _blas_pre(B)
+    for a in (A, B):
+        _colmajor(a, B.dtype)
+    _check(f"linompss_{pre}trsm", getattr(lib, f"linompss_{pre}trsm")(
+        side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n, b, alpha, _ptr(A), A.shape[0],
+        _ptr(B), B.shape[0]), False)
 
 
 def dspdsolv_main(A: np.ndarray, B: np.ndarray, b: int) -> int:

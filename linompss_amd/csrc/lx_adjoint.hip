@@ -26,114 +26,10 @@
 #include <stdint.h>
 
 #include "lx_device.hpp"
-
-#define LX_AG_T 64   // output tile (rows and columns)
-#define LX_AG_KS 16  // k per LDS slab
-#define LX_AG_LD (LX_AG_T + 4)
-
-// op(A)[M x K] element (i, k): TA = 0 -> A[i + k lda], TA = 1 -> A[k + i lda].
-// op(B)[K x N] element (k, j): TB = 0 -> B[k + j ldb], TB = 1 -> B[j + k ldb].
-// ksplit == 1:  C = alpha op(A) op(B) + beta C   (beta == 0: C is not read)
-// ksplit  > 1:  part[(z N + j) M + i] = alpha * (partial over the z-th K chunk)
-// lower_only: entries with i < j are neither read nor written.
-template <int TA, int TB>
-__global__ void __launch_bounds__(LX_WG_THREADS)
-lx_adj_gemm_kernel(int M, int N, int K, double alpha, const double *__restrict__ A, int64_t lda,
-                   const double *__restrict__ B, int64_t ldb, double beta, double *__restrict__ C, int64_t ldc,
-                   int lower_only, int ksplit, int kchunk, double *__restrict__ part)
-{
-    __shared__ double sA[LX_AG_KS][LX_AG_LD], sB[LX_AG_KS][LX_AG_LD];
-    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
-    const int m0 = blockIdx.x * LX_AG_T, n0 = blockIdx.y * LX_AG_T, z = blockIdx.z;
-    const int kb = z * kchunk, ke = K < kb + kchunk ? K : kb + kchunk;
-    // a lower_only tile strictly above the diagonal has nothing to do
-    if (lower_only && m0 + LX_AG_T - 1 < n0) return;
-    // staging: 4 elements per operand per thread per slab
-    double ra[4], rb[4];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (TA == 0) {  // rows contiguous
-                const int i = m0 + (tid & 63), k = k0 + (tid >> 6) + 4 * e;
-                ra[e] = (i < M && k < ke) ? A[(int64_t)k * lda + i] : 0.0;
-            } else {  // k contiguous
-                const int k = k0 + (tid & 15), i = m0 + (tid >> 4) + 16 * e;
-                ra[e] = (i < M && k < ke) ? A[(int64_t)i * lda + k] : 0.0;
-            }
-            if (TB == 1) {  // columns contiguous
-                const int j = n0 + (tid & 63), k = k0 + (tid >> 6) + 4 * e;
-                rb[e] = (j < N && k < ke) ? B[(int64_t)k * ldb + j] : 0.0;
-            } else {  // k contiguous
-                const int k = k0 + (tid & 15), j = n0 + (tid >> 4) + 16 * e;
-                rb[e] = (j < N && k < ke) ? B[(int64_t)j * ldb + k] : 0.0;
-            }
-        }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (TA == 0) sA[(tid >> 6) + 4 * e][tid & 63] = ra[e];
-            else sA[tid & 15][(tid >> 4) + 16 * e] = ra[e];
-            if (TB == 1) sB[(tid >> 6) + 4 * e][tid & 63] = rb[e];
-            else sB[tid & 15][(tid >> 4) + 16 * e] = rb[e];
-        }
-    };
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-    lx_d4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = lx_d4{0.0, 0.0, 0.0, 0.0};
-    if (kb < ke) load(kb);
-    for (int k0 = kb; k0 < ke; k0 += LX_AG_KS) {
-        __syncthreads();  // the previous slab's reads are done
-        store();
-        __syncthreads();
-        if (k0 + LX_AG_KS < ke) load(k0 + LX_AG_KS);  // next slab in flight during the products
-#pragma unroll
-        for (int kk = 0; kk < LX_AG_KS; kk += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = sA[kk + lk][wr + 16 * i + lr];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wc + 16 * j + lr];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = lx_mfma<double>::mac(a[i], b[j], acc[i][j]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = m0 + wr + 16 * i + lx_mfma<double>::row(lk, e), c = n0 + wc + 16 * j + lr;
-                if (r >= M || c >= N || (lower_only && r < c)) continue;
-                const double v = alpha * acc[i][j][e];
-                if (ksplit > 1) {
-                    part[((int64_t)z * N + c) * M + r] = v;
-                } else {
-                    double *p = C + (int64_t)c * ldc + r;
-                    *p = beta == 0.0 ? v : v + beta * *p;
-                }
-            }
-}
-
-// C = beta C + sum_z part[z] (fixed order), entries i >= j only when lower_only.
-__global__ void lx_adj_reduce_kernel(int M, int N, int ksplit, const double *__restrict__ part, double beta,
-                                     double *__restrict__ C, int64_t ldc, int lower_only)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)M * N) return;
-    const int r = (int)(e % M), c = (int)(e / M);
-    if (lower_only && r < c) return;
-    double s = 0.0;
-    for (int z = 0; z < ksplit; ++z) s += part[((int64_t)z * N + c) * M + r];
-    double *p = C + (int64_t)c * ldc + r;
-    *p = beta == 0.0 ? s : s + beta * *p;
-}
+#ifdef LX_ADJOINT_KERNELS_ONLY
+#define LX_GEMM64_KERNELS_ONLY
+#endif
+#include "lx_gemm64.hpp"
 
 // Diagonal block step of the reverse sweep:
 //   mode 0: the inverses of all diagonal blocks of an n-row lower triangle
@@ -283,41 +179,14 @@ int lx_adj_cus()
     return cus;
 }
 
-// C = alpha op(A) op(B) + beta C on the matrix cores; split-K (fixed-order
-// reduction through part, part_elems doubles) when the output has too few
-// tiles to fill the device.
+// C = alpha op(A) op(B) + beta C (lx_gemm64, FP64), split-K through part.
 int lx_adj_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, int64_t lda, const double *B,
                 int64_t ldb, double beta, double *C, int64_t ldc, int lower, double *part, size_t part_elems,
                 hipStream_t s)
 {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
-    const int tm = (M + LX_AG_T - 1) / LX_AG_T, tn = (N + LX_AG_T - 1) / LX_AG_T;
-    const int target = 2 * lx_adj_cus();
-    int ksplit = 1;
-    if ((int64_t)tm * tn < target && K >= 512) {
-        ksplit = std::min<int64_t>((target + (int64_t)tm * tn - 1) / ((int64_t)tm * tn), K / 256);
-        ksplit = (int)std::min<int64_t>(ksplit, (int64_t)(part_elems / ((size_t)M * N)));
-        ksplit = std::max(1, ksplit);
-    }
-    int kchunk = (K + ksplit - 1) / ksplit;
-    kchunk = (kchunk + LX_AG_KS - 1) / LX_AG_KS * LX_AG_KS;
-    ksplit = (K + kchunk - 1) / kchunk;
-    const dim3 grid(tm, tn, ksplit);
-#define LX_AG_LAUNCH(TA_, TB_)                                                                                  \
-    hipLaunchKernelGGL((lx_adj_gemm_kernel<TA_, TB_>), grid, dim3(LX_WG_THREADS), 0, s, M, N, K, alpha, A, lda, \
-                       B, ldb, beta, C, ldc, lower, ksplit, kchunk, part)
-    if (ta == 0 && tb == 0) LX_AG_LAUNCH(0, 0);
-    else if (ta == 0) LX_AG_LAUNCH(0, 1);
-    else if (tb == 0) LX_AG_LAUNCH(1, 0);
-    else LX_AG_LAUNCH(1, 1);
-#undef LX_AG_LAUNCH
-    if (ksplit > 1) {
-        const int64_t tot = (int64_t)M * N;
-        hipLaunchKernelGGL(lx_adj_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, M, N, ksplit,
-                           part, beta, C, ldc, lower);
-    }
-    LX_HIP_TRY(hipGetLastError());
-    return 0;
+    return lx_gemm64<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower ? 1 : 0, part, part_elems,
+                             lx_adj_cus(), s);
 }
 
 // The reverse sweep over 64-column blocks (see the header comment).

@@ -174,6 +174,33 @@ static void run_wg(F &&body)
     for (auto &t : th) t.join();
 }
 
+// C = alpha op(A) op(B) + beta C through lx_gemm64_kernel<float> (no split-K),
+// uplo 0 full, 1 lower, 2 upper.
+extern "C" void lx_emu_gemm64s(int ta, int tb, int M, int N, int K, float alpha, const float *A, long long lda,
+                               const float *B, long long ldb, float beta, float *C, long long ldc, int uplo)
+{
+    const int tm = (M + 63) / 64, tn = (N + 63) / 64, kchunk = (K + 15) / 16 * 16;
+    gridDim.x = tm;
+    gridDim.y = tn;
+    gridDim.z = 1;
+    blockDim.x = 256;
+    for (int y = 0; y < tn; ++y)
+        for (int x = 0; x < tm; ++x)
+            run_wg([&] {
+                blockIdx.x = x;
+                blockIdx.y = y;
+                blockIdx.z = 0;
+                if (ta == 0 && tb == 0)
+                    lx_gemm64_kernel<float, 0, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, uplo, 1, kchunk, nullptr);
+                else if (ta == 0)
+                    lx_gemm64_kernel<float, 0, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, uplo, 1, kchunk, nullptr);
+                else if (tb == 0)
+                    lx_gemm64_kernel<float, 1, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, uplo, 1, kchunk, nullptr);
+                else
+                    lx_gemm64_kernel<float, 1, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, uplo, 1, kchunk, nullptr);
+            });
+}
+
 // One FP32 GEMM / SYRK unit through the executor's dispatch (lx_run_unit):
 // C (M x N, ldc) -= A B^T, A = rows 0..M-1 and B = rows M..M+N-1 of the K
 // columns of P (ldp); flags = the unit's aux0 (bit 0 lower mask, bit 1
@@ -293,12 +320,12 @@ extern "C" void lx_emu_adj_gemm(int ta, int tb, int M, int N, int K, double alph
                     blockIdx.y = y;
                     blockIdx.z = z;
                     if (ta == 0 && tb == 0)
-                        lx_adj_gemm_kernel<0, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                        lx_gemm64_kernel<double, 0, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
                     else if (ta == 0)
-                        lx_adj_gemm_kernel<0, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                        lx_gemm64_kernel<double, 0, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
                     else if (tb == 0)
-                        lx_adj_gemm_kernel<1, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                        lx_gemm64_kernel<double, 1, 0>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
                     else
-                        lx_adj_gemm_kernel<1, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
+                        lx_gemm64_kernel<double, 1, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
                 });
 }

@@ -331,3 +331,28 @@ def test_factor_hypermatrix(emu, orc, dtype, variant, mt, b):
     # the strict upper part of each diagonal tile is left as it was
     for k in range(mt):
         assert np.array_equal(np.triu(tiles[(k, k)], 1), np.triu(A0[k * b:(k + 1) * b, k * b:(k + 1) * b].astype(dt), 1))
+
+
+@pytest.mark.parametrize("ta,tb,uplo", [(0, 0, 0), (1, 0, 2), (0, 1, 1), (1, 1, 2)])
+def test_gemm64_float_and_upper(emu, ta, tb, uplo):
+    """The shared 64 x 64-tile GEMM kernel in FP32 (v_mfma_f32_16x16x4f32),
+    every operand orientation, with the lower and upper masks of SYRK."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    F, I, L, P = ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
+    E.lx_emu_gemm64s.argtypes = [I, I, I, I, I, F, P, L, P, L, F, P, L, I]
+    M, N, K = 70, 90, 37
+    rng = np.random.default_rng(ta * 4 + tb * 2 + uplo)
+    A = np.asfortranarray(rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32))
+    B = np.asfortranarray(rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32))
+    C0 = np.asfortranarray(rng.standard_normal((M + 3, N)).astype(np.float32))
+    C = C0.copy(order="F")
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    E.lx_emu_gemm64s(ta, tb, M, N, K, 1.5, p(A), A.shape[0], p(B), B.shape[0], -0.5, p(C), C.shape[0], uplo)
+    opA = (A.T if ta else A).astype(np.float64)
+    opB = (B.T if tb else B).astype(np.float64)
+    ref = 1.5 * opA @ opB - 0.5 * C0[:M].astype(np.float64)
+    r, c = np.indices((M, N))
+    mask = np.ones((M, N), bool) if uplo == 0 else (r >= c if uplo == 1 else r <= c)
+    assert np.abs(C[:M][mask] - ref[mask]).max() <= 1e-5 * np.abs(ref).max()
+    assert np.array_equal(C[:M][~mask], C0[:M][~mask])
+    assert np.array_equal(C[M:], C0[M:])
