@@ -32,6 +32,7 @@ extern "C" {
 #define LINOMPSS_ERR_NOMEM  (-1003)  /* device allocation failed */
 #define LINOMPSS_ERR_SCHED  (-1004)  /* a work unit waited past the scheduler timeout */
 #define LINOMPSS_ERR_PLAN   (-1005)  /* the planner rejected the shape */
+#define LINOMPSS_ERR_IO     (-1006)  /* a matrix file could not be opened, written or parsed */
 
 /* ---- Drop-in entry points (host memory, synchronous) -------------------- */
 
@@ -96,6 +97,28 @@ int linompss_dtrsm(char side, char uplo, char trans, char diag, int m, int n, in
                    const double *A, int lda, double *B, int ldb);
 int linompss_strsm(char side, char uplo, char trans, char diag, int m, int n, int b, float alpha,
                    const float *A, int lda, float *B, int ldb);
+
+/* ---- Dense matrices in the Octave text format (A.dat / RHS.dat of the
+ * refinement app, src/itref/itref_setup.c:98-121, 155-199) ------------------
+ * Writer: replaces fprint_dense2mm / print_dense2mm (src/matutil/matfprint.c:
+ * 94-124): "# name:", "# type: matrix", "# rows:", "# columns:" header, then
+ * the m x n values row-major, one "%.16e \n" per line.  Returns 0, -i for
+ * argument i, or LINOMPSS_ERR_IO. */
+int linompss_fprint_ddense2mm(const char *fname, const char *name, int m, int n, const double *A, int lda);
+int linompss_fprint_sdense2mm(const char *fname, const char *name, int m, int n, const float *A, int lda);
+/* Raw reader: replaces read_mm2ddense / read_mm2sdense (src/matutil/matfread.c:
+ * 13-27): up to count values in file order, '#' lines skipped, a line that
+ * does not parse repeating the last value.  Returns the number of values
+ * stored, -i for argument i, or LINOMPSS_ERR_IO. */
+int64_t linompss_read_mm2ddense(const char *fname, int64_t count, double *out);
+int64_t linompss_read_mm2sdense(const char *fname, int64_t count, float *out);
+/* The header's rows and columns; then the values into a column-major m x n
+ * array (ld lda) in the layout the writer used (the reference's setup reads
+ * the row-major values straight into column-major storage).  No counterpart
+ * of its own in the reference. */
+int linompss_dense2mm_dims(const char *fname, int *m, int *n);
+int linompss_read_ddense2mm(const char *fname, int m, int n, double *A, int lda);
+int linompss_read_sdense2mm(const char *fname, int m, int n, float *A, int lda);
 
 /* Replaces dspdsolv_main (src/spdsolv/spdsolv_main.h:20, spdsolv_main.c:10-20):
  * RL factor of the m x m matrix A (lda = m), then the forward and backward

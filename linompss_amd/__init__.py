@@ -36,7 +36,7 @@ __all__ = [
     "IrStatus", "ExecStats", "DeviceMatrix", "device_count", "BatchOpts", "BatchStatus", "DeviceBatch",
     "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd", "ompss_dchol_hrl", "ompss_dchol_hll",
     "ompss_schol_hrl", "ompss_schol_hll", "ompss_dchol_nhrl", "ompss_dchol_nhll", "ompss_schol_nhrl",
-    "ompss_schol_nhll", "chol_tiles_dev", "gemm", "syrk", "trsm",
+    "ompss_schol_nhll", "chol_tiles_dev", "gemm", "syrk", "trsm", "write_dense", "read_dense", "read_values",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -44,7 +44,7 @@ LIB_PATH = Path(os.environ.get("LINOMPSS_LIB", _PKG / "liblinompss.so"))
 
 if not LIB_PATH.exists():
     raise ImportError(
-        f"{LIB_PATH} is missing: build it with `python -m linompss_amd.build` (hipcc, gfx950). "
+        f"{LIB_PATH} is missing: build it with `python linompss_amd/build.py` (hipcc, gfx950). "
         "There is no CPU fallback.")
 # The library binds to the HIP runtime of the ROCm it was built with
 # (/opt/rocm).  PyTorch bundles an older libamdhip64 with the same soname; a
@@ -108,6 +108,13 @@ _sig("linompss_dsyrk", _i, _c, _c, _i, _i, _i, _d, _p, _i, _d, _p, _i)
 _sig("linompss_ssyrk", _i, _c, _c, _i, _i, _i, _f, _p, _i, _f, _p, _i)
 _sig("linompss_dtrsm", _i, _c, _c, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i)
 _sig("linompss_strsm", _i, _c, _c, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i)
+_sig("linompss_fprint_ddense2mm", _i, ctypes.c_char_p, ctypes.c_char_p, _i, _i, _p, _i)
+_sig("linompss_fprint_sdense2mm", _i, ctypes.c_char_p, ctypes.c_char_p, _i, _i, _p, _i)
+_sig("linompss_read_mm2ddense", _i64, ctypes.c_char_p, _i64, _p)
+_sig("linompss_read_mm2sdense", _i64, ctypes.c_char_p, _i64, _p)
+_sig("linompss_dense2mm_dims", _i, ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i))
+_sig("linompss_read_ddense2mm", _i, ctypes.c_char_p, _i, _i, _p, _i)
+_sig("linompss_read_sdense2mm", _i, ctypes.c_char_p, _i, _i, _p, _i)
 _sig("linompss_gemm_dev", _i, _i, _c, _c, _i, _i, _i, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p)
 _sig("linompss_syrk_dev", _i, _i, _c, _c, _i, _i, _d, _p, _i64, _d, _p, _i64, _p)
 _sig("linompss_trsm_blas_dev", _i, _i, _c, _c, _c, _c, _i, _i, _d, _p, _i64, _p, _i64, _p)
@@ -122,7 +129,8 @@ _sig("linompss_version", ctypes.c_char_p)
 
 ERR_BASE = -1000
 ERR_NAMES = {-1001: "no usable HIP device", -1002: "HIP call failed", -1003: "device allocation failed",
-             -1004: "device scheduler timed out", -1005: "planner rejected the shape"}
+             -1004: "device scheduler timed out", -1005: "planner rejected the shape",
+             -1006: "matrix file could not be opened, written or parsed"}
 
 
 class LinompssError(RuntimeError):
@@ -274,6 +282,37 @@ def trsm(side: str, uplo: str, trans: str, diag: str, m: int, n: int, alpha: flo
     _check(f"linompss_{pre}trsm", getattr(lib, f"linompss_{pre}trsm")(
         side.encode(), uplo.encode(), trans.encode(), diag.encode(), m, n, b, alpha, _ptr(A), A.shape[0],
         _ptr(B), B.shape[0]), False)
+
+
+def write_dense(fname, A: np.ndarray, name: str = "A") -> None:
+    """A (2-D, float64 or float32) to fname in the Octave text format of
+    print_dense2mm (src/matutil/matfprint.c:104-124)."""
+    A = np.asfortranarray(A)
+    pre = _blas_pre(A)
+    m, n = A.shape
+    _check(f"linompss_fprint_{pre}dense2mm", getattr(lib, f"linompss_fprint_{pre}dense2mm")(
+        os.fsencode(fname), name.encode(), m, n, _ptr(A), max(1, m)), False)
+
+
+def read_dense(fname, dtype=np.float64) -> np.ndarray:
+    """The matrix of an Octave text file, column-major, in the layout it was written."""
+    m, n = _i(), _i()
+    _check("linompss_dense2mm_dims", lib.linompss_dense2mm_dims(os.fsencode(fname), ctypes.byref(m), ctypes.byref(n)),
+           False)
+    A = np.zeros((m.value, n.value), dtype=dtype, order="F")
+    pre = _blas_pre(A)
+    _check(f"linompss_read_{pre}dense2mm", getattr(lib, f"linompss_read_{pre}dense2mm")(
+        os.fsencode(fname), m.value, n.value, _ptr(A), max(1, m.value)), False)
+    return A
+
+
+def read_values(fname, count: int, dtype=np.float64) -> np.ndarray:
+    """Up to count values in file order (read_mm2{d,s}dense, src/matutil/matfread.c:13-27)."""
+    out = np.zeros(max(1, count), dtype=dtype)
+    pre = _blas_pre(out)
+    got = getattr(lib, f"linompss_read_mm2{pre}dense")(os.fsencode(fname), count, _ptr(out))
+    _check(f"linompss_read_mm2{pre}dense", int(got) if got < 0 else 0, False)
+    return out[:got]
 
 
 def dspdsolv_main(A: np.ndarray, B: np.ndarray, b: int) -> int:

@@ -127,3 +127,20 @@ def test_dposv_ir_split_bf16_updates(lx, orc, dev, n, nr, b, alpha):
     # (cond ~ (4 + alpha) / alpha here)
     Xo, sto, info = orc.dposv_ir(A, B, b)
     assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 2.5 * (4 + alpha) / alpha * 1e-14
+
+
+def test_dposv_ir_from_files(lx, dev, tmp_path):
+    """The refinement app's file path (src/itref/itref_setup.c:98-121,
+    155-199): A.dat and RHS.dat in the Octave text format drive the solve.
+    The FP64 values read back exactly, so the solve sees the same system."""
+    n, s, b = 700, 3, 128
+    print(f"[itref files] n={n} s={s} b={b}", flush=True)
+    A0 = _spd_gauss(n, 11, 1.0)
+    B0 = np.asfortranarray(np.random.default_rng(12).uniform(-1, 1, (n, s)))
+    lx.write_dense(tmp_path / "A.dat", A0, "A")
+    lx.write_dense(tmp_path / "RHS.dat", B0, "RHS")
+    A = lx.read_dense(tmp_path / "A.dat")
+    B = lx.read_dense(tmp_path / "RHS.dat")
+    assert np.array_equal(A, A0) and np.array_equal(B, B0)
+    X, st = lx.dposv_ir(A, B, b)
+    assert _berr(A0, X, B0) <= 1e-14
