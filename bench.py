@@ -380,7 +380,7 @@ def bench_blas3(args):
     print(json.dumps(out))
 
 
-def traffic_from(fetch_csv, write_csv, out, n):
+def traffic_from(fetch_csv, write_csv, out, n, update_prec):
     """HBM traffic of the FP32 factor launch from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what the
     same counter reads for the device copy of A (lx_copy_kernel: n*n*8 bytes
@@ -391,7 +391,9 @@ def traffic_from(fetch_csv, write_csv, out, n):
     def read(path, counter):
         rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
         copy = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("lx_copy_kernel")]
-        fac = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("void lx_exec_kernel<float, 0>")]
+        # the column-major FP32 factor instantiation ("<float, 0>" or "<float, 0, false>")
+        fac = [float(r["Counter_Value"]) for r in rows
+               if r["Kernel_Name"].startswith("void lx_exec_kernel<float, 0") and "true" not in r["Kernel_Name"]]
         if not copy or not fac:
             raise SystemExit(f"{path}: no lx_copy_kernel / FP32 factor dispatch with {counter}")
         return max(copy), sum(fac) / len(fac), len(fac)
@@ -402,7 +404,8 @@ def traffic_from(fetch_csv, write_csv, out, n):
     res = {
         "kernel": "lx_exec_kernel<float, 0> (FP32 tiled factor, one persistent launch)",
         "workload": f"n={n}, b=512, FP32 factor inside the mixed-precision solve, trailing updates on "
-                    f"split-BF16 matrix cores (bench default)",
+                    f"{['FP32', 'split-BF16', 'split-FP16'][update_prec]} matrix cores",
+        "update_prec": update_prec,
         "copy_calibration": {"bytes": n * n * 8, "fetch_kib": fc, "write_kib": wc,
                              "fetch_scale": fs, "write_scale": ws},
         "factor_fetch_kib": ff, "factor_write_kib": wf, "dispatches": [nf, nw],
@@ -410,11 +413,12 @@ def traffic_from(fetch_csv, write_csv, out, n):
         "hbm_bytes_per_launch": round((ff * fs + wf * ws) * 1024),
         "commands": [
             "rocprofv3 --pmc FETCH_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
-            "--warmup 0 --fp64-steps 1 --no-cpu",
+            f"--warmup 0 --fp64-steps 1 --no-cpu --update-prec {update_prec}",
             "rocprofv3 --pmc WRITE_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
-            "--warmup 0 --fp64-steps 1 --no-cpu",
+            f"--warmup 0 --fp64-steps 1 --no-cpu --update-prec {update_prec}",
             "python3 bench.py --traffic-from <fetch>/run_counter_collection.csv "
-            "<write>/run_counter_collection.csv --traffic-out profiles/traffic_fp32_factor_n32768.json"],
+            f"<write>/run_counter_collection.csv --update-prec {update_prec} "
+            "--traffic-out profiles/traffic_fp32_factor_n32768.json"],
     }
     Path(out).write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res))
@@ -435,8 +439,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--update-prec", type=int, default=1, choices=(0, 1),
-                    help="FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores, 1 split-BF16 matrix cores")
+    ap.add_argument("--update-prec", type=int, default=2, choices=(0, 1, 2),
+                    help="FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores, 1 split-BF16 matrix cores "
+                         "(six products), 2 split-FP16 matrix cores (three products, equilibrated A)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="derive the factor launch's HBM traffic from two rocprofv3 --pmc CSVs and exit")
     ap.add_argument("--traffic-out", default=str(ROOT / "profiles" / "traffic_fp32_factor_n32768.json"))
@@ -457,7 +462,7 @@ def main():
         {"batched": bench_batched, "gp": bench_gp, "hyper": bench_hyper, "blas3": bench_blas3}[args.workload](args)
         return
     if args.traffic_from:
-        traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n)
+        traffic_from(args.traffic_from[0], args.traffic_from[1], args.traffic_out, args.n, args.update_prec)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -531,6 +536,9 @@ def main():
     if args.update_prec == 1:
         # six BF16 products per FP32 multiply-add of the trailing updates
         peak, basis = PEAK["bf16"] / 6.0, "bf16 dense 2500 TFLOP/s / 6 split products"
+    elif args.update_prec == 2:
+        # three FP16 products per FP32 multiply-add (FP16 and BF16 share the dense rate)
+        peak, basis = PEAK["bf16"] / 3.0, "fp16 dense 2500 TFLOP/s / 3 split products"
     else:
         peak, basis = PEAK["fp32"], "fp32 dense 157.3 TFLOP/s"
     roofline = {
@@ -544,9 +552,11 @@ def main():
         "frac_of_fp32_peak": round(achieved / PEAK["fp32"], 4),
     }
     tf = ROOT / "profiles" / "traffic_fp32_factor_n32768.json"
-    if tf.exists() and (n, b, args.update_prec) == (32768, 512, 1):  # the workload the counters were read on
+    if tf.exists() and (n, b) == (32768, 512):
         try:
-            roofline["traffic"] = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+            rec = json.loads(tf.read_text())
+            if rec.get("update_prec", 1) == args.update_prec:  # the workload the counters were read on
+                roofline["traffic"] = rec.get("hbm_bytes_per_launch")
         except Exception:
             pass
 
@@ -571,11 +581,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 factor (trailing SYRK/GEMM on bf16 matrix cores, split operands hi+mid+lo, f32 "
-                  "accumulation) + f64 refinement") if args.update_prec == 1 else "f32 factor + f64 refinement",
+        "dtype": ["f32 factor + f64 refinement",
+                  "f32 factor (trailing SYRK/GEMM on bf16 matrix cores, split operands hi+mid+lo, f32 "
+                  "accumulation) + f64 refinement",
+                  "f32 factor of the equilibrated matrix (trailing SYRK/GEMM on fp16 matrix cores, split "
+                  "operands hi+lo of x*2^12, f32 accumulation) + f64 refinement"][args.update_prec],
         "data": "synthetic: A = G G^T/n + alpha I, G ~ N(0,1) (Philox4x32-10, seed %d), B ~ U[-1,1)" % args.seed,
         "config": {"workload": "FP32-factor + FP64 iterative refinement, SPD solve to backward error 1e-14",
-                   "update_prec": ["fp32", "split-bf16"][args.update_prec],
+                   "update_prec": ["fp32", "split-bf16", "split-fp16"][args.update_prec],
                    "n": n, "b": b, "cond2": args.kappa, "nrhs": nr, "alpha": alpha, "lambda_max": lmax},
         "roofline": roofline,
         "cpu_baseline": cpu,

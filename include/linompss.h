@@ -148,8 +148,11 @@ typedef struct {
     int max_sweeps;       /* corrections before falling back (default 30) */
     double tol;           /* backward-error target (default 1e-14) */
     double stall_ratio;   /* ||r_k+1|| > stall_ratio ||r_k|| is a stall (default 0.5) */
-    int update_prec;      /* FP32 factor's trailing updates: 0 FP32 (default), 1 split BF16
-                             (see linompss_exec_opts.update_prec) */
+    int update_prec;      /* FP32 factor's trailing updates: 0 FP32 (default), 1 split BF16,
+                             2 split FP16 (see linompss_exec_opts.update_prec); with 2 the
+                             FP32 copy of A is equilibrated first, D A D with D = diag of
+                             powers of two making every diagonal entry lie in [1/4, 1), and
+                             each correction solves with it: d = D (D A D)^-1 D r */
 } linompss_ir_opts;
 
 typedef struct {
@@ -180,7 +183,12 @@ typedef struct {
     int update_prec;      /* FP32 factor's trailing SYRK/GEMM: 0 FP32 matrix cores (default);
                              1 BF16 matrix cores on split operands (each FP32 input as
                              hi + mid + lo BF16, six products, FP32 accumulation: FP32-level
-                             accuracy at the BF16 rate); ignored for FP64 */
+                             accuracy at the BF16 rate); 2 FP16 matrix cores on split
+                             operands (x * 2^12 as hi + lo FP16, three products, half the
+                             matrix-core work of 1 at about 4x its rounding error per
+                             product; needs every entry of L within 16 in magnitude, which
+                             a diagonal of at most 256 guarantees: past it the factor
+                             reports a non-positive pivot); ignored for FP64 */
 } linompss_exec_opts;
 
 typedef struct {

@@ -386,13 +386,68 @@ __device__ __forceinline__ void lx_bx_split8(const float (&x)[8], lx_bf16x8 &h, 
     }
 }
 
+// The two split kinds of the bodies below.
+//   H16 = false: three BF16 planes (hi, mid, lo), six products (above).
+//   H16 = true:  two FP16 planes of x * 2^12: hi = fp16(x'), lo = fp16(x' - hi)
+//   (x' - hi is exact in FP32), so x' = hi + lo within 2^-22 |x'|; three
+//   products (lo*hi, hi*lo, hi*hi; the dropped lo*lo is below 2^-22), each
+//   exact in FP32, accumulated in FP32 and scaled back by 2^-24.  Half the
+//   matrix-core work and two thirds of the LDS planes of the BF16 split, at
+//   about 4x its rounding error per product (still below FP32's own
+//   accumulation error over a 512-deep K).  FP16's range needs |x| <= 16
+//   (x' <= 65504); the refinement driver equilibrates A to a unit diagonal
+//   (powers of two) so every entry of L is at most 1.  Past the range the hi
+//   part is infinite and the factor reports a non-positive pivot.  The lo part
+//   of |x| < 2^-15 is subnormal: its absolute error stays below 2^-37.
+typedef _Float16 lx_f16x8 __attribute__((ext_vector_type(8)));
+template <bool H16> struct lx_split;
+template <> struct lx_split<false> {
+    typedef __bf16 e;
+    typedef lx_bf16x8 v;
+    static constexpr int np = 3, nq = 6;
+    static constexpr int pa(int q) { return q == 0 ? 2 : q == 1 ? 0 : q < 4 ? 1 : 0; }  // 2 0 1 1 0 0
+    static constexpr int pb(int q) { return q == 0 ? 0 : q == 1 ? 2 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 1 : 0; }
+    __device__ static __forceinline__ void split(const float (&x)[8], v (&p)[3]) { lx_bx_split8(x, p[0], p[1], p[2]); }
+    __device__ static __forceinline__ lx_f16v mma(v a, v b, lx_f16v c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ float unscale(float a) { return a; }
+};
+template <> struct lx_split<true> {
+    typedef _Float16 e;
+    typedef lx_f16x8 v;
+    static constexpr int np = 2, nq = 3;
+    static constexpr int pa(int q) { return q == 0 ? 1 : 0; }  // 1 0 0
+    static constexpr int pb(int q) { return q == 1 ? 1 : 0; }  // 0 1 0
+    __device__ static __forceinline__ void split(const float (&x)[8], v (&p)[3])
+    {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float xs = x[k] * 4096.0f;
+            const _Float16 h = (_Float16)xs;
+            p[0][k] = h;
+            p[1][k] = (_Float16)(xs - (float)h);
+        }
+    }
+    __device__ static __forceinline__ lx_f16v mma(v a, v b, lx_f16v c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ float unscale(float a) { return a * 0x1p-24f; }
+};
+
+template <bool H16 = false>
 __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long long ldc, const float *__restrict__ A,
                                                  long long lda, const float *__restrict__ B, long long ldb, int M,
                                                  int N, int K, bool lower, int gdiag, char *smem)
 {
+    typedef lx_split<H16> S;
+    typedef typename S::e E;
+    typedef typename S::v V;
     constexpr int KS = LX_BX_KS, STR = LX_BX_STR;
     // planes: [operand][plane][row][k]
-    __bf16(*sP)[3][128][STR] = reinterpret_cast<__bf16(*)[3][128][STR]>(smem);
+    E(*sP)[3][128][STR] = reinterpret_cast<E(*)[3][128][STR]>(smem);
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
     const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
     const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
@@ -435,15 +490,13 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
                 ta[e] = xa[set][half * 8 + e];
                 tb[e] = xb[set][half * 8 + e];
             }
-            lx_bf16x8 h, m, l;
-            lx_bx_split8(ta, h, m, l);
-            *(lx_bf16x8 *)&sP[0][0][srow][sk + half * 8] = h;
-            *(lx_bf16x8 *)&sP[0][1][srow][sk + half * 8] = m;
-            *(lx_bf16x8 *)&sP[0][2][srow][sk + half * 8] = l;
-            lx_bx_split8(tb, h, m, l);
-            *(lx_bf16x8 *)&sP[1][0][srow][sk + half * 8] = h;
-            *(lx_bf16x8 *)&sP[1][1][srow][sk + half * 8] = m;
-            *(lx_bf16x8 *)&sP[1][2][srow][sk + half * 8] = l;
+            V pl[3];
+            S::split(ta, pl);
+#pragma unroll
+            for (int q = 0; q < S::np; ++q) *(V *)&sP[0][q][srow][sk + half * 8] = pl[q];
+            S::split(tb, pl);
+#pragma unroll
+            for (int q = 0; q < S::np; ++q) *(V *)&sP[1][q][srow][sk + half * 8] = pl[q];
         }
     };
     lx_f16v acc[2][2];
@@ -458,25 +511,23 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
         if (!active) return;
 #pragma unroll
         for (int ks = 0; ks < KS; ks += 16) {
-            lx_bf16x8 a[2][3], b[2][3];
+            V a[2][S::np], b[2][S::np];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[0][p][wr + i * 32 + lr][ks + 8 * lk];
+                for (int p = 0; p < S::np; ++p) a[i][p] = *(const V *)&sP[0][p][wr + i * 32 + lr][ks + 8 * lk];
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[1][p][wc + j * 32 + lr][ks + 8 * lk];
+                for (int p = 0; p < S::np; ++p) b[j][p] = *(const V *)&sP[1][p][wc + j * 32 + lr][ks + 8 * lk];
             // product-major: the four tiles' accumulators are independent, so
             // consecutive MFMAs never wait on each other's result
-            constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
 #pragma unroll
-            for (int q = 0; q < 6; ++q)
+            for (int q = 0; q < S::nq; ++q)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < 2; ++j) acc[i][j] = S::mma(a[i][S::pa(q)], b[j][S::pb(q)], acc[i][j]);
         }
     };
     gload(0, 0);
@@ -525,7 +576,8 @@ __device__ __forceinline__ void lx_unit_gemm_bx6(float *__restrict__ C, long lon
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4);
-                    if (r < M && c < N && (!lower || r + gdiag >= c)) C[(long long)c * ldc + r] = v[j][e] - acc[i][j][e];
+                    if (r < M && c < N && (!lower || r + gdiag >= c))
+                        C[(long long)c * ldc + r] = v[j][e] - S::unscale(acc[i][j][e]);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -549,12 +601,16 @@ struct lx_bxw_smem {
     static constexpr int bytes = 3 * 384 * LX_BXW_STR * 2;
 };
 
+template <bool H16 = false>
 __device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long long ldc, const float *__restrict__ A,
                                                   long long lda, const float *__restrict__ B, long long ldb, int M,
                                                   int N, int K, bool lower, int gdiag, char *smem)
 {
+    typedef lx_split<H16> S;
+    typedef typename S::e E;
+    typedef typename S::v V;
     constexpr int KS = LX_BXW_KS, STR = LX_BXW_STR;
-    __bf16(*sP)[384][STR] = reinterpret_cast<__bf16(*)[384][STR]>(smem);
+    E(*sP)[384][STR] = reinterpret_cast<E(*)[384][STR]>(smem);
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
     const int wr = (w >> 1) * 64, wc = (w & 1) * 128;
     const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
@@ -591,19 +647,16 @@ __device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long lo
         }
     };
     auto lstore = [&](int set) {
-        lx_bf16x8 h, m, l;
-        lx_bx_split8(xa[set], h, m, l);
-        *(lx_bf16x8 *)&sP[0][srow][sk] = h;
-        *(lx_bf16x8 *)&sP[1][srow][sk] = m;
-        *(lx_bf16x8 *)&sP[2][srow][sk] = l;
-        lx_bx_split8(xb0[set], h, m, l);
-        *(lx_bf16x8 *)&sP[0][128 + srow][sk] = h;
-        *(lx_bf16x8 *)&sP[1][128 + srow][sk] = m;
-        *(lx_bf16x8 *)&sP[2][128 + srow][sk] = l;
-        lx_bx_split8(xb1[set], h, m, l);
-        *(lx_bf16x8 *)&sP[0][256 + srow][sk] = h;
-        *(lx_bf16x8 *)&sP[1][256 + srow][sk] = m;
-        *(lx_bf16x8 *)&sP[2][256 + srow][sk] = l;
+        V pl[3];
+        S::split(xa[set], pl);
+#pragma unroll
+        for (int q = 0; q < S::np; ++q) *(V *)&sP[q][srow][sk] = pl[q];
+        S::split(xb0[set], pl);
+#pragma unroll
+        for (int q = 0; q < S::np; ++q) *(V *)&sP[q][128 + srow][sk] = pl[q];
+        S::split(xb1[set], pl);
+#pragma unroll
+        for (int q = 0; q < S::np; ++q) *(V *)&sP[q][256 + srow][sk] = pl[q];
     };
     lx_f16v acc[2][4];
 #pragma unroll
@@ -615,28 +668,27 @@ __device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long lo
     const int lr = lane & 31, lk = lane >> 5;
     auto compute = [&]() {
         if (!active) return;
-        lx_bf16x8 a[2][3];
+        V a[2][S::np];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int p = 0; p < 3; ++p) a[i][p] = *(const lx_bf16x8 *)&sP[p][wr + i * 32 + lr][8 * lk];
+            for (int p = 0; p < S::np; ++p) a[i][p] = *(const V *)&sP[p][wr + i * 32 + lr][8 * lk];
         // two column tiles at a time: four independent accumulators between
         // consecutive MFMAs on one of them
 #pragma unroll
         for (int jp = 0; jp < 4; jp += 2) {
-            lx_bf16x8 b[2][3];
+            V b[2][S::np];
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int p = 0; p < 3; ++p) b[j][p] = *(const lx_bf16x8 *)&sP[p][128 + wc + (jp + j) * 32 + lr][8 * lk];
-            constexpr int PA[6] = {2, 0, 1, 1, 0, 0}, PB[6] = {0, 2, 1, 0, 1, 0};
+                for (int p = 0; p < S::np; ++p) b[j][p] = *(const V *)&sP[p][128 + wc + (jp + j) * 32 + lr][8 * lk];
 #pragma unroll
-            for (int q = 0; q < 6; ++q)
+            for (int q = 0; q < S::nq; ++q)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
-                        acc[i][jp + j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[q]], b[j][PB[q]], acc[i][jp + j], 0, 0, 0);
+                        acc[i][jp + j] = S::mma(a[i][S::pa(q)], b[j][S::pb(q)], acc[i][jp + j]);
         }
     };
     gload(0, 0);
@@ -688,7 +740,7 @@ __device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long lo
                     for (int e = 0; e < 16; ++e) {
                         const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4);
                         if (r < M && c < N && (!lower || r + gdiag >= c))
-                            C[(long long)c * ldc + r] = v[j][e] - acc[i][jp + j][e];
+                            C[(long long)c * ldc + r] = v[j][e] - S::unscale(acc[i][jp + j][e]);
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);

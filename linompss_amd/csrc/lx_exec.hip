@@ -131,7 +131,7 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         P.trsm_rate *= 1.5;
     }
     if (o) P.policy = o->policy;
-    if (o && dtype == 1 && o->update_prec == 1) P.update_prec = 1;
+    if (o && dtype == 1 && (o->update_prec == 1 || o->update_prec == 2)) P.update_prec = o->update_prec;
     // FP64: 256-wide diagonal sub-tiles (fewer, longer steps on the critical
     // path; measured at n = 32768, b = 512: 235 ms against 255 ms with 128)
     if (dtype == 0) P.potrf_sub = 256;

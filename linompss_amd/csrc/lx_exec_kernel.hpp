@@ -42,6 +42,29 @@ template <> struct lx_gemm_cfg<double> { static constexpr int FM = 4, FN = 4, U 
 template <> struct lx_gemm_cfg<float> { static constexpr int FM = 4, FN = 4, U = 4; };
 static constexpr int LX_UNIT_M = 128, LX_UNIT_N = 128;  // 2 x 2 waves of 16*FM x 16*FN
 
+// The columns of a split-body GEMM / SYRK unit: 128 x 256 blocks while more
+// than 128 columns remain (aux0 bit 2), then 128 x 128.
+template <bool H16>
+__device__ __forceinline__ void lx_split_blocks(const lx_unit &u, float *C, long long ldc, const float *Ap,
+                                                long long lda, const float *Bp, long long ldb, bool lower, char *smem)
+{
+    const bool wide = (u.aux0 & 4) != 0;
+    for (int c0 = 0; c0 < u.N;) {  // uniform
+        const int rem = u.N - c0;
+        if (wide && rem > LX_UNIT_N) {
+            const int nc = rem < 2 * LX_UNIT_N ? rem : 2 * LX_UNIT_N;
+            lx_unit_gemm_bx6w<H16>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                                   u.aux1 - c0, smem);
+            c0 += nc;
+        } else {
+            const int nc = rem < LX_UNIT_N ? rem : LX_UNIT_N;
+            lx_unit_gemm_bx6<H16>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                                  u.aux1 - c0, smem);
+            c0 += nc;
+        }
+    }
+}
+
 __device__ __forceinline__ long long lx_off(int64_t rcp, long long ld)
 {
     return (long long)(rcp >> 32) * ld + (long long)(rcp & 0xffffffffll);
@@ -100,28 +123,17 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
     case LX_OP_GEMM_NT:
         // a unit covers up to M = 128 rows and N = 128 * m columns, done as m
         // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0));
-        // aux0 bit 0: lower mask (SYRK), bit 1: split-BF16 matrix cores (FP32
-        // only), bit 2: split-BF16 in 128 x 256 blocks while more than 128
-        // columns remain
+        // aux0 bit 0: lower mask (SYRK), bit 1: split matrix-core bodies (FP32
+        // only), bit 2: split bodies in 128 x 256 blocks while more than 128
+        // columns remain, bit 3: the FP16 split instead of the BF16 one
         if constexpr (F) {
             const bool lower = (u.aux0 & 1) != 0;
             if constexpr (sizeof(T) == 4) {
                 if (u.aux0 & 2) {
-                    const bool wide = (u.aux0 & 4) != 0;
-                    for (int c0 = 0; c0 < u.N;) {  // uniform
-                        const int rem = u.N - c0;
-                        if (wide && rem > LX_UNIT_N) {
-                            const int nc = rem < 2 * LX_UNIT_N ? rem : 2 * LX_UNIT_N;
-                            lx_unit_gemm_bx6w(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K,
-                                              lower, u.aux1 - c0, smem);
-                            c0 += nc;
-                        } else {
-                            const int nc = rem < LX_UNIT_N ? rem : LX_UNIT_N;
-                            lx_unit_gemm_bx6(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
-                                             u.aux1 - c0, smem);
-                            c0 += nc;
-                        }
-                    }
+                    if (u.aux0 & 8)
+                        lx_split_blocks<true>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
+                    else
+                        lx_split_blocks<false>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
                     break;
                 }
             }

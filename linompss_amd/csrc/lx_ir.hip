@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <mutex>
@@ -229,33 +230,60 @@ __global__ void lx_berr_kernel(int nrhs, const double *__restrict__ rmax, const 
 }
 
 // lower triangle of A (FP64) -> Af (FP32), ld n
-__global__ void lx_d2s_lower_kernel(const double *__restrict__ A, int64_t lda, int n, float *__restrict__ Af)
+// Equilibration by powers of two: Deq[i] = 2^-e, e = ceil(p / 2) for
+// a_ii = m 2^p (m in [1/2, 1)), so that Deq[i]^2 a_ii lies in [1/4, 1); 1 where
+// a_ii is not a positive finite number (the factor then reports it).  For an
+// SPD matrix |a_ij| <= sqrt(a_ii a_jj), so every entry of D A D and of its
+// Cholesky factor is below 1 in magnitude.
+__global__ void lx_equil_kernel(int n, const double *__restrict__ A, int64_t lda, double *__restrict__ Deq)
 {
-    for (int j = blockIdx.y; j < n; j += gridDim.y)
-        for (int i = j + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-            Af[(int64_t)j * n + i] = (float)A[(int64_t)j * lda + i];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = A[(int64_t)i * lda + i];
+    double d = 1.0;
+    if (a > 0.0 && a <= DBL_MAX) {
+        int p = 0;
+        (void)frexp(a, &p);
+        const int e = p >= 0 ? (p + 1) / 2 : -((-p) / 2);
+        d = ldexp(1.0, -e);
+    }
+    Deq[i] = d;
 }
 
-// Sf[:, c] = (float)(V[:, c] * inv[c])
+// FP32 copy of the lower triangle, of D A D when Deq is given (exact scaling)
+__global__ void lx_d2s_lower_kernel(const double *__restrict__ A, int64_t lda, int n, const double *__restrict__ Deq,
+                                    float *__restrict__ Af)
+{
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        const double dj = Deq ? Deq[j] : 1.0;
+        for (int i = j + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            Af[(int64_t)j * n + i] = (float)(Deq ? A[(int64_t)j * lda + i] * (Deq[i] * dj) : A[(int64_t)j * lda + i]);
+    }
+}
+
+// Sf[:, c] = (float)(D V[:, c] / scale[c])   (D = I without Deq)
 __global__ void lx_d2s_scaled_kernel(int n, int nrhs, const double *__restrict__ V, int64_t ldv,
-                                     const double *__restrict__ scale, float *__restrict__ Sf)
+                                     const double *__restrict__ scale, const double *__restrict__ Deq,
+                                     float *__restrict__ Sf)
 {
     int64_t tot = (int64_t)n * nrhs;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = e / n, i = e % n;
         double s = scale[c];
-        Sf[c * n + i] = (float)(s > 0.0 ? V[c * ldv + i] / s : 0.0);
+        const double v = Deq ? V[c * ldv + i] * Deq[i] : V[c * ldv + i];
+        Sf[c * n + i] = (float)(s > 0.0 ? v / s : 0.0);
     }
 }
 
-// X[:, c] (+)= scale[c] * Sf[:, c]   (FP64 update, itref.c:1084)
+// X[:, c] (+)= scale[c] * D Sf[:, c]   (FP64 update, itref.c:1084)
 __global__ void lx_update_kernel(int n, int nrhs, const float *__restrict__ Sf, const double *__restrict__ scale,
-                                 double *__restrict__ X, int64_t ldx, int accumulate)
+                                 const double *__restrict__ Deq, double *__restrict__ X, int64_t ldx, int accumulate)
 {
     int64_t tot = (int64_t)n * nrhs;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = e / n, i = e % n;
         double d = scale[c] * (double)Sf[c * n + i];
+        if (Deq) d *= Deq[i];
         X[c * ldx + i] = accumulate ? X[c * ldx + i] + d : d;
     }
 }
@@ -350,17 +378,19 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     if (n == 0 || nrhs == 0) return 0;
     linompss_ir_opts o = {30, 1e-14, 0.5, 0};
     if (opts) o = *opts;
-    if (o.update_prec != 0 && o.update_prec != 1) return -11;  // 0 FP32, 1 split-BF16 trailing updates
+    if (o.update_prec < 0 || o.update_prec > 2) return -11;  // 0 FP32, 1 split-BF16, 2 split-FP16 updates
     std::lock_guard<std::mutex> g(g_ir_mu);
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
     const size_t nn = (size_t)n * n, nr = (size_t)n * nrhs;
     float *Af = (float *)lx_scratch(4, sizeof(float) * nn);
-    char *ws = (char *)lx_scratch(5, sizeof(double) * (3 * nr + 2 * (size_t)n + 64) + sizeof(float) * nr + 1024);
+    char *ws = (char *)lx_scratch(5, sizeof(double) * (3 * nr + 3 * (size_t)n + 64) + sizeof(float) * nr + 1024);
     if (!Af || !ws) return LINOMPSS_ERR_NOMEM;
     double *R = (double *)ws;
     double *rowabs = R + nr;
-    double *small = rowabs + n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
+    // the FP16 split needs |L| <= 16: factor D A D (unit-range diagonal)
+    double *Deq = o.update_prec == 2 ? rowabs + n : nullptr;
+    double *small = rowabs + 2 * (size_t)n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
     float *Sf = (float *)(small + 128);
     double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32, *bmax = small + 1 + 48;
     if (nrhs > 16) {
@@ -370,7 +400,8 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     // 1) FP32 copy of the lower triangle, FP32 factor
     {
         dim3 grid(std::max(1, std::min(64, (n + 255) / 256)), std::min(n, 4096));
-        hipLaunchKernelGGL(lx_d2s_lower_kernel, grid, dim3(256), 0, s, dA, lda, n, Af);
+        if (Deq) hipLaunchKernelGGL(lx_equil_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, dA, lda, Deq);
+        hipLaunchKernelGGL(lx_d2s_lower_kernel, grid, dim3(256), 0, s, dA, lda, n, Deq, Af);
         LX_HIP_TRY(hipGetLastError());
     }
     linompss_exec_opts eo = {1, 128, 128, 0, 30.0, 0, o.update_prec};
@@ -388,11 +419,11 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         // x0 = solve(b) with per-column scaling by ||b_c||_inf
         hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dB, ldb, n, bmax);
         hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, dB, ldb, bmax,
-                           Sf);
+                           Deq, Sf);
         rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
         if (rc) return rc;
-        hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, bmax, dX, ldx,
-                           0);
+        hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, bmax, Deq, dX,
+                           ldx, 0);
         for (int it = 0;; ++it) {
             rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, it == 0 ? rowabs : nullptr, s);
             if (rc) return rc;
@@ -418,11 +449,11 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
             prev = rn;
             // d = solve(r / ||r||); x += ||r|| d
             hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, R,
-                               (int64_t)n, rmax, Sf);
+                               (int64_t)n, rmax, Deq, Sf);
             rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
             if (rc) return rc;
-            hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, rmax, dX,
-                               ldx, 1);
+            hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, rmax, Deq,
+                               dX, ldx, 1);
             st->sweeps = it + 1;
         }
     }

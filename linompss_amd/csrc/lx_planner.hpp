@@ -24,7 +24,7 @@ struct lx_plan_params {
     int unit_n = 128;       // GEMM / SYRK unit cols
     int gemm_wide_n = 512;  // GEMM unit cols off the lookahead column (one fence per 128 x 512)
     int potrf_sub = 128;    // diagonal tiles wider than this factor as a sub-DAG of sub-tiles of this size
-    int update_prec = 0;    // 1: trailing GEMM/SYRK units of the tile loop on split-BF16 matrix cores
+    int update_prec = 0;    // trailing GEMM/SYRK units of the tile loop: 1 split-BF16, 2 split-FP16 matrix cores
     int bx_wide = 1;        // split-BF16 units: 128 x 256 blocks (0: 128 x 128 blocks only)
     int trsm_rows = 64;     // panel TRSM rows per unit
     int gemv_rows = 128;    // RHS GEMV output rows per unit
@@ -94,7 +94,7 @@ class lx_planner {
     void add_unit_deps(int task, const lx_unit &proto, const std::vector<std::pair<int32_t, int32_t>> &deps);
     int alloc_counters(int count);
     // aux0 flag of the tile loop's GEMM/SYRK units: 2 = split-BF16 matrix cores
-    int upd_flag() const { return P.update_prec == 1 ? (P.bx_wide ? 6 : 2) : 0; }
+    int upd_flag() const { return P.update_prec >= 1 ? (P.bx_wide ? 6 : 2) | (P.update_prec == 2 ? 8 : 0) : 0; }
     // Units of the diagonal-tile factorisation POTRF(k) of task t (its output
     // counter outc, global column offset k*b, width bs).
     void potrf_units(int t, int s, int kb, int bs, int outc);

@@ -136,6 +136,30 @@ inline lx_emu_f16v __builtin_amdgcn_mfma_f32_32x32x16_bf16(lx_emu_bf16x8 a, lx_e
     return d;
 }
 
+// v_mfma_f32_32x32x16_f16: the same layout as the BF16 form; products of two
+// FP16 numbers (11-bit significands) are exact in FP32.
+typedef _Float16 lx_emu_f16x8 __attribute__((ext_vector_type(8)));
+inline lx_emu_f16v __builtin_amdgcn_mfma_f32_32x32x16_f16(lx_emu_f16x8 a, lx_emu_f16x8 b, lx_emu_f16v c, int, int, int)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    lx_emu_bf_buf &B = lx_emu_bfbuf[w];
+    for (int e = 0; e < 8; ++e) {
+        B.a[lane][e] = (float)a[e];
+        B.b[lane][e] = (float)b[e];
+    }
+    lx_emu_wave_barrier[w].wait();
+    lx_emu_f16v d = c;
+    const int col = lane & 31;
+    for (int i = 0; i < 16; ++i) {
+        const int row = 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+        double acc = 0;
+        for (int k = 0; k < 16; ++k) acc += (double)B.a[(k / 8) * 32 + row][k % 8] * (double)B.b[(k / 8) * 32 + col][k % 8];
+        d[i] = (float)((double)c[i] + acc);
+    }
+    lx_emu_wave_barrier[w].wait();
+    return d;
+}
+
 template <typename T>
 inline T lx_emu_xchg(T v, int src_lane)
 {

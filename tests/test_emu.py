@@ -72,10 +72,17 @@ def test_factor_fp32_split_bf16_updates(emu, orc):
     (7, 128, 256, 32, 100),  # lower mask across the wide block
     (3, 128, 128, 32, 0),    # SYRK tile on the 128 x 128 body
     (2, 100, 300, 40, 0),    # 128 x 128 blocks only
+    (14, 128, 256, 64, 0),   # the FP16 split: one full wide block
+    (14, 77, 200, 33, 0),    # ragged wide block
+    (14, 128, 384, 48, 0),   # wide block, then a 128 x 128 block
+    (15, 128, 256, 32, 100), # lower mask
+    (11, 128, 128, 32, 0),   # SYRK tile on the 128 x 128 body
+    (10, 100, 300, 40, 0),   # 128 x 128 blocks only
 ])
 def test_gemm_unit_split_bf16(emu, flags, M, N, K, gdiag):
     """One GEMM / SYRK unit through the executor's dispatch (lx_run_unit) on the
-    split-BF16 bodies: C -= A B^T within FP32 accumulation error of the FP64
+    split bodies (BF16, or FP16 with bit 3): C -= A B^T within FP32
+    accumulation error (plus the split's error per product) of the FP64
     product, entries outside the lower mask and rows / columns past M / N
     untouched."""
     E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
@@ -92,10 +99,37 @@ def test_gemm_unit_split_bf16(emu, flags, M, N, K, gdiag):
     mask = np.ones((M, N), dtype=bool)
     if flags & 1:
         mask = np.arange(M)[:, None] + gdiag >= np.arange(N)[None, :]
-    tol = 4 * K * 2.0 ** -24 * (np.abs(A64) @ np.abs(B64).T).max() + 2.0 ** -23 * np.abs(ref).max()
+    # FP32 accumulation, plus the split's own error per product: below 2^-24
+    # for the BF16 split, 3 * 2^-22 for the FP16 one (bit 3)
+    per = 3 * 2.0 ** -22 if flags & 8 else 0.0
+    tol = (4 * K * 2.0 ** -24 + per) * (np.abs(A64) @ np.abs(B64).T).max() + 2.0 ** -23 * np.abs(ref).max()
     assert np.abs(C[:M][mask] - ref[mask]).max() <= tol
     assert np.array_equal(C[:M][~mask], C0[:M][~mask])
     assert np.array_equal(C[M:], C0[M:])
+
+
+def test_factor_fp32_split_fp16_updates(emu, orc):
+    """The FP32 factor on the split-FP16 bodies: on a matrix with its diagonal
+    below 1 (as the refinement driver equilibrates it) FP32-level agreement
+    with LAPACK; with entries of the panels the trailing units read past
+    FP16's range (x 2^12) the factor reports a non-positive pivot instead of
+    a wrong factor."""
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    n, b = 300, 128
+    A0 = orc.genmat_spd(n, 4, diag_mode=1) / 512.0  # diagonal 300/512
+    Af = A0.astype(np.float32, order="F")
+    E.lx_emu_set_update_prec(2)
+    try:
+        assert emu(1, 0, n, b, 0, Af) == 0
+        Lref = np.linalg.cholesky(A0)
+        assert np.linalg.norm(np.tril(Af) - Lref) / np.linalg.norm(Lref) <= 2e-6
+        # rows past the first tile scaled by 2^16 (S A S, still SPD): their
+        # panel entries, the trailing units' operands, reach ~100
+        S = np.where(np.arange(n) >= b, 2.0 ** 16, 1.0)
+        Big = (S[:, None] * A0 * S[None, :]).astype(np.float32, order="F")
+        assert emu(1, 0, n, b, 0, Big) > 0
+    finally:
+        E.lx_emu_set_update_prec(0)
 
 
 def test_factor_fp32_and_policy0(emu, orc):

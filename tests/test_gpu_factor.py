@@ -119,3 +119,31 @@ def test_schol_split_bf16_updates(lx, orc, dev, n, b):
     Lref = np.linalg.cholesky(A0)
     assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) <= 1e-5
     dA.free()
+
+
+@pytest.mark.parametrize("n,b", [(700, 128), (2048, 512), (1989, 533)])
+def test_schol_split_fp16_updates(lx, orc, dev, n, b):
+    """FP32 factor with the tile loop's SYRK/GEMM on split-FP16 matrix cores
+    (update_prec = 2) on a matrix whose diagonal is below 1, as the
+    refinement driver equilibrates it: FP32-level agreement with LAPACK."""
+    print(f"[schol fp16x3] n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=6, diag_mode=1) / 2.0 ** np.ceil(np.log2(n))
+    dA = lx.DeviceMatrix.from_host(A0.astype(np.float32, order="F"))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2)
+    assert lx.chol_dev(dA, b, "rl", opts) == 0
+    L = np.tril(dA.to_host().astype(np.float64))
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) <= 1e-5
+    dA.free()
+
+
+def test_schol_split_fp16_out_of_range_reports(lx, orc, dev):
+    """Panel entries past FP16's range (x 2^12) end in a non-positive pivot
+    (info > 0), never in a silently wrong factor."""
+    n, b = 700, 128
+    A0 = orc.genmat_spd(n, seed=6, diag_mode=1) / 1024.0
+    S = np.where(np.arange(n) >= b, 2.0 ** 16, 1.0)
+    dA = lx.DeviceMatrix.from_host((S[:, None] * A0 * S[None, :]).astype(np.float32, order="F"))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2)
+    assert lx.chol_dev(dA, b, "rl", opts) > 0
+    dA.free()

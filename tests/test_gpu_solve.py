@@ -129,6 +129,30 @@ def test_dposv_ir_split_bf16_updates(lx, orc, dev, n, nr, b, alpha):
     assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 2.5 * (4 + alpha) / alpha * 1e-14
 
 
+@pytest.mark.parametrize("n,nr,b,alpha,scale", [(1000, 1, 256, 1e-3, 0), (2048, 4, 512, 1e-4, 0),
+                                                (1500, 2, 256, 1e-3, 12)])
+def test_dposv_ir_split_fp16_updates(lx, orc, dev, n, nr, b, alpha, scale):
+    """Refinement with the trailing SYRK/GEMM on split-FP16 matrix cores
+    (update_prec = 2): the driver factors D A D (powers of two, diagonal in
+    [1/4, 1)), so a badly scaled A (rows scaled by up to 2^scale) converges
+    like a well scaled one, without the FP64 fallback."""
+    print(f"[ir fp16x3] n={n} nr={nr} b={b} scale=2^{scale}", flush=True)
+    A = _spd_gauss(n, 3, alpha)
+    if scale:
+        S = 2.0 ** np.random.default_rng(5).integers(-scale, scale + 1, n)
+        A = np.asfortranarray(S[:, None] * A * S[None, :])
+    rng = np.random.default_rng(4)
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, nr)))
+    X, st = lx.dposv_ir(A, B, b, lx.IrOpts(30, 1e-14, 0.5, 2))
+    assert st.info == 0
+    assert st.fallback == 0
+    assert st.berr <= 1e-14
+    assert _berr(A, X, B) <= 1e-14
+    if not scale:
+        Xo, sto, info = orc.dposv_ir(A, B, b)
+        assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 2.5 * (4 + alpha) / alpha * 1e-14
+
+
 def test_dposv_ir_from_files(lx, dev, tmp_path):
     """The refinement app's file path (src/itref/itref_setup.c:98-121,
     155-199): A.dat and RHS.dat in the Octave text format drive the solve.
