@@ -201,8 +201,9 @@ __device__ __forceinline__ double lx_wg1024_nanmax(double m)
 
 // out[c] = max_i |V[i, c]| (NaN-propagating); grid = ncols, one 1024-thread
 // workgroup per column with eight loads in flight per thread
+// out[c] = max_i |D_ii V_ic| (D = I without Deq); one workgroup per column
 __global__ void __launch_bounds__(1024) lx_colmax_kernel(const double *__restrict__ V, int64_t ldv, int n,
-                                                         double *__restrict__ out)
+                                                         const double *__restrict__ Deq, double *__restrict__ out)
 {
     const double *v = V + (int64_t)blockIdx.x * ldv;
     double m[8];
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(1024) lx_colmax_kernel(const double *__restric
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int i = i0 + u * 1024;
-            if (i < n) m[u] = lx_nanmax(fabs(v[i]), m[u]);
+            if (i < n) m[u] = lx_nanmax(fabs(Deq ? v[i] * Deq[i] : v[i]), m[u]);
         }
     }
 #pragma unroll
@@ -393,6 +394,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     double *small = rowabs + 2 * (size_t)n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
     float *Sf = (float *)(small + 128);
     double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32, *bmax = small + 1 + 48;
+    double *smax = small + 1 + 64;  // max |D r| per column: the scale of the equilibrated corrections
     if (nrhs > 16) {
         // small per-column arrays sized for 16 columns
         return -2;
@@ -417,7 +419,8 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     double h_small[1 + 48];
     if (!fallback) {
         // x0 = solve(b) with per-column scaling by ||b_c||_inf
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dB, ldb, n, bmax);
+        // the FP32 right-hand side D b / max|D b| lies in [-1, 1] whatever D is
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dB, ldb, n, Deq, bmax);
         hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, dB, ldb, bmax,
                            Deq, Sf);
         rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
@@ -428,9 +431,10 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
             rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, it == 0 ? rowabs : nullptr, s);
             if (rc) return rc;
             if (it == 0) hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
-            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
-            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
+            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
+            hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
             hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
+            if (Deq) hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, Deq, smax);
             LX_HIP_TRY(hipGetLastError());
             LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
             LX_HIP_TRY(hipStreamSynchronize(s));
@@ -449,11 +453,11 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
             prev = rn;
             // d = solve(r / ||r||); x += ||r|| d
             hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, R,
-                               (int64_t)n, rmax, Deq, Sf);
+                               (int64_t)n, Deq ? smax : rmax, Deq, Sf);
             rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
             if (rc) return rc;
-            hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, rmax, Deq,
-                               dX, ldx, 1);
+            hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf,
+                               Deq ? smax : rmax, Deq, dX, ldx, 1);
             st->sweeps = it + 1;
         }
     }
@@ -471,8 +475,8 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
         if (rc) return rc;
         hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
-        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
+        hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
         hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
         LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
         LX_HIP_TRY(hipStreamSynchronize(s));
@@ -538,8 +542,8 @@ extern "C" int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda
     int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
     if (rc) return rc;
     hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
-    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, rmax);
-    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, xmax);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
+    hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
     hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
     double h[16];
     LX_HIP_TRY(hipMemcpy(h, berr, sizeof(double) * nrhs, hipMemcpyDeviceToHost));

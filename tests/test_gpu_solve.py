@@ -130,12 +130,14 @@ def test_dposv_ir_split_bf16_updates(lx, orc, dev, n, nr, b, alpha):
 
 
 @pytest.mark.parametrize("n,nr,b,alpha,scale", [(1000, 1, 256, 1e-3, 0), (2048, 4, 512, 1e-4, 0),
-                                                (1500, 2, 256, 1e-3, 12)])
+                                                (1500, 2, 256, 1e-3, 12), (1200, 2, 256, 1e-3, 150)])
 def test_dposv_ir_split_fp16_updates(lx, orc, dev, n, nr, b, alpha, scale):
     """Refinement with the trailing SYRK/GEMM on split-FP16 matrix cores
     (update_prec = 2): the driver factors D A D (powers of two, diagonal in
     [1/4, 1)), so a badly scaled A (rows scaled by up to 2^scale) converges
-    like a well scaled one, without the FP64 fallback."""
+    like a well scaled one, without the FP64 fallback.  At 2^150 the FP32
+    right-hand sides must be scaled by max |D r|, not max |r| (D reaches 2^150,
+    past FP32's range)."""
     print(f"[ir fp16x3] n={n} nr={nr} b={b} scale=2^{scale}", flush=True)
     A = _spd_gauss(n, 3, alpha)
     if scale:
