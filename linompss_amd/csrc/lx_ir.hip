@@ -251,14 +251,22 @@ __global__ void lx_equil_kernel(int n, const double *__restrict__ A, int64_t lda
     Deq[i] = d;
 }
 
-// FP32 copy of the lower triangle, of D A D when Deq is given (exact scaling)
+// FP32 copy of the lower triangle, of D A D when Deq is given (exact scaling).
+// A thread owns rows i = x + k (gridDim.x blockDim.x) and walks the columns
+// j = blockIdx.y + m gridDim.y <= i: for each column the rows of a wave are
+// consecutive (coalesced), D_i stays in a register and D_j is one address for
+// the whole wave.
 __global__ void lx_d2s_lower_kernel(const double *__restrict__ A, int64_t lda, int n, const double *__restrict__ Deq,
                                     float *__restrict__ Af)
 {
-    for (int j = blockIdx.y; j < n; j += gridDim.y) {
-        const double dj = Deq ? Deq[j] : 1.0;
-        for (int i = j + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-            Af[(int64_t)j * n + i] = (float)(Deq ? A[(int64_t)j * lda + i] * (Deq[i] * dj) : A[(int64_t)j * lda + i]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (Deq) {
+            const double di = Deq[i];
+            for (int j = blockIdx.y; j <= i; j += gridDim.y)
+                Af[(int64_t)j * n + i] = (float)(A[(int64_t)j * lda + i] * (di * Deq[j]));
+        } else {
+            for (int j = blockIdx.y; j <= i; j += gridDim.y) Af[(int64_t)j * n + i] = (float)A[(int64_t)j * lda + i];
+        }
     }
 }
 
