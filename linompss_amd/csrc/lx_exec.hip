@@ -135,6 +135,10 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
     // FP64: 256-wide diagonal sub-tiles (fewer, longer steps on the critical
     // path; measured at n = 32768, b = 512: 235 ms against 255 ms with 128)
     if (dtype == 0) P.potrf_sub = 256;
+    // FP64: 128-column GEMM units in the second half of the panels (more units
+    // while the trailing matrix shrinks; 229.0 against 232.1 ms at n = 32768,
+    // b = 512; the FP32 factor measured no gain at any tail length)
+    if (dtype == 0) P.wide_tail = -1;
     // tuning knobs: the sub-tile of diagonal-tile factorisations, the width
     // of GEMM units off the lookahead column, 128 x 256 split-BF16 blocks
     if (const char *ev = getenv("LINOMPSS_POTRF_SUB")) {
@@ -146,6 +150,10 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         if (v >= 128 && v % 128 == 0) P.gemm_wide_n = v;
     }
     if (const char *ev = getenv("LINOMPSS_BX_WIDE")) P.bx_wide = atoi(ev) != 0;
+    if (const char *ev = getenv("LINOMPSS_WIDE_TAIL")) {
+        const int v = atoi(ev);
+        if (v >= -1) P.wide_tail = v;
+    }
 }
 
 bool build_plan(lx_planner &pl, const lx_shape &sh)
@@ -177,9 +185,9 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
     lx_plan_params P;
     fill_params(P, sh.dtype, workers, o);
     char key[256];
-    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d bw%d", sh.what, sh.dtype, sh.n,
-             sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec, P.potrf_sub, P.gemm_wide_n,
-             P.bx_wide);
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d bw%d wt%d", sh.what, sh.dtype,
+             sh.n, sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec, P.potrf_sub, P.gemm_wide_n,
+             P.bx_wide, P.wide_tail);
     for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
         if (it->key == key) {
             c->plans.splice(c->plans.begin(), c->plans, it);

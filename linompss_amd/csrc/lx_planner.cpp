@@ -241,7 +241,10 @@ void lx_planner::factor_rl(int s)
                 int t = new_task(LX_T_GEMM, s, i, j, k, PRIO_LO, outc, in, 2);
                 // the next panel's column (j = k+1) feeds the critical path: fine units;
                 // elsewhere wide units, so that one release fence covers more work
-                const int un = j == k + 1 ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
+                // (and in the last wide_tail panels, whose trailing matrix no longer
+                // gives every workgroup a wide unit)
+                const int tail = P.wide_tail < 0 ? mt / 2 : P.wide_tail;
+                const int un = (j == k + 1 || mt - k <= tail) ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
                 for (int c0 = 0; c0 < jbs; c0 += un)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         lx_unit u = proto_unit(LX_OP_GEMM_NT, s);

@@ -23,6 +23,7 @@ struct lx_plan_params {
     int unit_m = 128;       // GEMM / SYRK unit rows (one workgroup tile)
     int unit_n = 128;       // GEMM / SYRK unit cols
     int gemm_wide_n = 512;  // GEMM unit cols off the lookahead column (one fence per 128 x 512)
+    int wide_tail = 0;      // the last wide_tail panels (-1: half of them) use unit_n-column GEMM units throughout
     int potrf_sub = 128;    // diagonal tiles wider than this factor as a sub-DAG of sub-tiles of this size
     int update_prec = 0;    // trailing GEMM/SYRK units of the tile loop: 1 split-BF16, 2 split-FP16 matrix cores
     int bx_wide = 1;        // split-BF16 units: 128 x 256 blocks (0: 128 x 128 blocks only)
