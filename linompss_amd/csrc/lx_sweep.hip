@@ -348,9 +348,43 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
             T acc[NR];
 #pragma unroll
             for (int cc = 0; cc < NR; ++cc) acc[cc] = T(0);
+            // several right-hand sides: the block products on the matrix cores
+            // (four 16-row tiles of 16 columns, 16 MFMAs of K = 4 each per block);
+            // lane l supplies op(L)[16 tt + l%16][4 kk + l/16] (forward: from
+            // global memory; backward: from the slab transposed in LDS) and
+            // x[4 kk + l/16][l%16]
+            constexpr bool MM = NR > 1;
+            typename lx_mfma<T>::acc macc[4];
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) macc[tt][i] = T(0);
             for (int t = t_lo + w; t < t_hi; t += 4) {  // wave-uniform
                 const int jb = TRANS ? nb - 1 - t : t;
                 const int c0 = jb * 64, hj = min(64, n - c0);
+                if constexpr (MM && !TRANS) {
+                    // the block of L before the wait (block jb < ib is full; rows past hb clamped)
+                    const int lr = lane & 15, lk = lane >> 4;
+                    T a[4][16];
+#pragma unroll
+                    for (int tt = 0; tt < 4; ++tt) {
+                        const T *Lb = L + (int64_t)(c0 + lk) * ldl + r0 + min(16 * tt + lr, hb - 1);
+#pragma unroll
+                        for (int kk = 0; kk < 16; ++kk) a[tt][kk] = Lb[(int64_t)(4 * kk) * ldl];
+                    }
+                    for (int cc = 0; cc < nrhs; ++cc)
+                        sx[cc][lane] = lane < hj ? lx_sw_mb_get<T>(mb + (int64_t)cc * ldmb + c0 + lane, abort, spin_ticks)
+                                                 : T(0);
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int kk = 0; kk < 16; ++kk) {
+                        const T bv = lr < nrhs ? sx[lr][4 * kk + lk] : T(0);
+#pragma unroll
+                        for (int tt = 0; tt < 4; ++tt) macc[tt] = lx_mfma<T>::mac(a[tt][kk], bv, macc[tt]);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    continue;
+                }
                 // the block of L is read before the wait: only x_j depends on step t
                 T lv[64];
                 if (!TRANS) {
@@ -392,12 +426,23 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                         for (int q = 0; q < 64 / CPI; ++q) stile[q * CPI + cq][kk] = lv[m * (64 / CPI) + q];
                         __builtin_amdgcn_wave_barrier();
                         // rows past hj: sx is zero there
-#pragma unroll 8
-                        for (int k = 0; k < KC; ++k) {
-                            const T l = stile[lane][k];
+                        if constexpr (MM) {
+                            const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-                            for (int cc = 0; cc < NR; ++cc)
-                                if (cc < nrhs) acc[cc] += l * sx[cc][m * KC + k];
+                            for (int k4 = 0; k4 < KC; k4 += 4) {
+                                const T bv = lr < nrhs ? sx[lr][m * KC + k4 + lk] : T(0);
+#pragma unroll
+                                for (int tt = 0; tt < 4; ++tt)
+                                    macc[tt] = lx_mfma<T>::mac(stile[16 * tt + lr][k4 + lk], bv, macc[tt]);
+                            }
+                        } else {
+#pragma unroll 8
+                            for (int k = 0; k < KC; ++k) {
+                                const T l = stile[lane][k];
+#pragma unroll
+                                for (int cc = 0; cc < NR; ++cc)
+                                    if (cc < nrhs) acc[cc] += l * sx[cc][m * KC + k];
+                            }
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -405,8 +450,17 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
                 __builtin_amdgcn_wave_barrier();
             }
             // reduce the four waves in a fixed order through sx (each wave's own buffer)
+            if constexpr (MM) {
+                const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-            for (int cc = 0; cc < NR; ++cc) sx[cc][lane] = acc[cc];
+                for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (lr < NR) sx[lr][16 * tt + lx_mfma<T>::row(lk, i)] = macc[tt][i];
+            } else {
+#pragma unroll
+                for (int cc = 0; cc < NR; ++cc) sx[cc][lane] = acc[cc];
+            }
             __syncthreads();
             T(*sx0)[NR][64] = reinterpret_cast<T(*)[NR][64]>(smem + 4 * 64 * (KC + 1) * sizeof(T));
             T *pp = part + ((size_t)rowbase[s] + (c - 1)) * LX_SW_NR * 64;

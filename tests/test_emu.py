@@ -187,7 +187,7 @@ def emu_sweep(emu):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("n,nr", [(1, 1), (100, 3), (200, 16), (700, 2), (300, 7)])
+@pytest.mark.parametrize("n,nr", [(1, 1), (100, 3), (200, 16), (700, 2), (300, 7), (520, 16)])
 def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
     """lx_sweep.hip's forward and backward kernels against the oracle's blocked sweeps."""
     rng = np.random.default_rng(n + nr)
