@@ -316,7 +316,9 @@ int linompss_dtrsm_adjoint_dev(int trans, int n, int nrhs, const double *dL, int
 int linompss_dlogdet_dev(int n, const double *dL, int64_t ldl, double *logdet, void *stream);
 
 /* Normwise backward error max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf),
- * computed in FP64 on the device (A symmetric, lower triangle read; nrhs <= 16). */
+ * computed in FP64 on the device (A symmetric, lower triangle read; any nrhs).
+ * A column with a zero residual counts as backward error 0 (b_c = 0 gives
+ * x_c = 0).  Returns 0 or -i for argument i. */
 int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
                        const double *dB, int64_t ldb, double *berr);
 
@@ -341,7 +343,7 @@ const char *linompss_version(void);
 /* ---- Workload generators (device memory; used by the benchmark and tests).
  * linompss_gen_gauss_spd: A = G G^T / n + alpha I (both triangles stored),
  * G i.i.d. N(0,1) from Philox4x32-10 keyed by seed, alpha = lambda_max /
- * (kappa - 1) with lambda_max(G G^T / n) from power iteration, so that
+ * (kappa - 1) with lambda_max(G G^T / n) from Lanczos (largest Ritz value), so that
  * cond2(A) ~ kappa.  Needs n*n*8 bytes of temporary device memory.
  * linompss_gen_uniform: X (m x n, dtype 0 FP64 / 1 FP32) uniform in [lo, hi).
  * linompss_gen_gauss: G (m x n, FP64) i.i.d. N(0,1), element e = j*m + i
@@ -350,6 +352,13 @@ int linompss_gen_gauss_spd(int n, uint32_t seed, double kappa, double *dA, int64
                            double *lambda_max);
 int linompss_gen_uniform(int dtype, int m, int n, uint32_t seed, double lo, double hi, void *dX, int64_t ldx);
 int linompss_gen_gauss(int m, int n, uint32_t seed, double *dG, int64_t ldg);
+/* The refinement app's default "COVAR" matrix: GENMAT_COVARIANT
+ * (src/matutil/genmat.c:263-302, reached through dgenmat_config,
+ * genmat_config.c:34-133, with p = k = 2 by default): a_ij = |i - j|^-p off
+ * the diagonal, a_jj = 1 + (j + 1)^(1/k), both triangles, n x n with leading
+ * dimension lda, in device memory.  Returns 0, -i for argument i (p <= 0 is
+ * refused, as the reference returns NULL), or LINOMPSS_ERR_HIP. */
+int linompss_gen_covariant(int n, double p, double k, double *dA, int64_t lda);
 
 /* ---- Device-memory plumbing (so C and ctypes callers need no other HIP
  * binding).  kind: 1 host->device, 2 device->host, 3 device->device.       */

@@ -414,6 +414,16 @@ class DeviceMatrix:
         _check("linompss_memcpy", lib.linompss_memcpy(out.ctypes.data_as(_p), self.ptr, out.nbytes, 2), False)
         return out[: self.rows]
 
+    def cols_to_host(self, c0: int, c1: int) -> np.ndarray:
+        """Columns c0 .. c1-1 (all ld rows read, the first rows returned): a
+        panel of a matrix too large to bring to the host at once."""
+        if not (0 <= c0 <= c1 <= self.cols):
+            raise ValueError(f"columns {c0}..{c1} outside 0..{self.cols}")
+        out = np.empty((self.ld, c1 - c0), dtype=self.dtype, order="F")
+        src = ctypes.c_void_p(self.ptr + c0 * self.ld * self.dtype.itemsize)
+        _check("linompss_memcpy", lib.linompss_memcpy(out.ctypes.data_as(_p), src, out.nbytes, 2), False)
+        return out[: self.rows]
+
     def free(self) -> None:
         if getattr(self, "ptr", None):
             lib.linompss_dev_free(self.ptr)
@@ -496,6 +506,7 @@ def dposv_ir_dev(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, b: int, opts
 _sig("linompss_gen_gauss_spd", _i, _i, ctypes.c_uint32, _d, _p, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d))
 _sig("linompss_gen_uniform", _i, _i, _i, _i, ctypes.c_uint32, _d, _d, _p, _i64)
 _sig("linompss_gen_gauss", _i, _i, _i, ctypes.c_uint32, _p, _i64)
+_sig("linompss_gen_covariant", _i, _i, _d, _d, _p, _i64)
 
 
 def gen_gauss_spd(n: int, seed: int, kappa: float) -> tuple[DeviceMatrix, float, float]:
@@ -512,6 +523,14 @@ def gen_uniform(m: int, n: int, seed: int, lo: float = -1.0, hi: float = 1.0, dt
     _check("linompss_gen_uniform", lib.linompss_gen_uniform(0 if np.dtype(dtype) == np.float64 else 1, m, n, seed,
                                                              lo, hi, X.ptr, X.ld), False)
     return X
+
+
+def gen_covariant(n: int, p: float = 2.0, k: float = 2.0, ld: int | None = None) -> DeviceMatrix:
+    """The refinement app's default matrix GENMAT_COVARIANT (genmat.c:263-302)
+    in device memory: |i - j|^-p off the diagonal, 1 + (j + 1)^(1/k) on it."""
+    A = DeviceMatrix(n, n, ld=ld)
+    _check("linompss_gen_covariant", lib.linompss_gen_covariant(n, p, k, A.ptr, A.ld), False)
+    return A
 
 
 def gen_gauss(m: int, n: int, seed: int) -> DeviceMatrix:

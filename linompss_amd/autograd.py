@@ -179,7 +179,9 @@ class GaussianNLL(torch.autograd.Function):
             ctx.timing["backward_s"] = time.perf_counter() - t0
         Ab = dLb.to_host()
         ybar = dbb.to_host()[:, 0]
-        for m in (dL, dLb, dbb, dK, da):
+        # K and a stay on the device until ctx is released, so a second
+        # backward (retain_graph=True) recomputes the same gradient
+        for m in (dL, dLb, dbb):
             m.free()
         low = np.tril(Ab, -1)
         S = 0.5 * (low + low.T) + np.diag(np.diag(Ab))

@@ -7,7 +7,7 @@
 //   * G from a counter-based generator, Philox4x32-10 (Salmon et al., SC'11),
 //     keyed by the seed and counted by element, then Box-Muller;
 //   * A = G G^T / n on the FP64 matrix cores (lower tiles, mirrored);
-//   * lambda_max(G G^T / n) by power iteration, alpha = lambda_max / (kappa - 1)
+//   * lambda_max(G G^T / n) by Lanczos, alpha = lambda_max / (kappa - 1)
 //     so that cond2(A) = (lambda_max + alpha) / (lambda_min + alpha) ~ kappa
 //     (lambda_min of a square Wishart matrix is ~1/n^2, negligible);
 //   * right-hand sides uniform in [-1, 1) from the same generator (another key).
@@ -72,7 +72,7 @@ __global__ void lx_gen_uniform_kernel(int m, int n, T *X, int64_t ldx, uint32_t 
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         lx_u4 r = lx_philox4x32_10(lx_u4{(uint32_t)e, (uint32_t)(e >> 32), 1u, 0u}, seed, 0x89abcdefu);
         const double u = (double)((((uint64_t)r.x) << 21) ^ (r.y >> 11)) * 0x1p-53;
-        X[(e / m) * ldx + e % m] = (T)(lo + (hi - lo) * u);
+        X[(e / m) * ldx + e % m] = (T)fma(hi - lo, u, lo);  // one rounding, then the element type
     }
 }
 
@@ -115,7 +115,7 @@ __global__ void lx_add_diag_kernel(int n, double *A, int64_t lda, double alpha)
         A[(int64_t)i * lda + i] += alpha;
 }
 
-// y = A x for symmetric A stored full (power iteration on the generated matrix)
+// y = A x for symmetric A stored full (Lanczos on the generated matrix)
 __global__ void __launch_bounds__(256) lx_gemv_full_kernel(int n, const double *__restrict__ A, int64_t lda,
                                                            const double *__restrict__ x, double *__restrict__ y)
 {
@@ -145,6 +145,30 @@ __global__ void __launch_bounds__(256) lx_norm2_kernel(int n, const double *__re
     if (threadIdx.x == 0) out[0] = sqrt(red[0]);
 }
 
+// out[0] = x . y (one workgroup, fixed-order tree: deterministic)
+__global__ void __launch_bounds__(256) lx_dot_kernel(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                     double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += x[i] * y[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Lanczos three-term step: w -= a v + b u
+__global__ void lx_lanczos_axpy_kernel(int n, double *__restrict__ w, const double *__restrict__ v,
+                                       const double *__restrict__ u, double a, double b)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        w[i] -= a * v[i] + b * u[i];
+}
+
 __global__ void lx_scale_vec_kernel(int n, double *v, const double *nrm)
 {
     const double s = nrm[0] > 0 ? 1.0 / nrm[0] : 0.0;
@@ -160,7 +184,9 @@ extern "C" int linompss_gen_gauss_spd(int n, uint32_t seed, double kappa, double
     if (!(kappa > 1.0)) return -3;
     if (lda < n) return -5;
     double *G = nullptr;
-    if (hipMalloc(&G, sizeof(double) * (size_t)n * n) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+    // G, then four Lanczos vectors in its storage
+    if (hipMalloc(&G, sizeof(double) * std::max((size_t)n * n, 4 * (size_t)n)) != hipSuccess)
+        return LINOMPSS_ERR_NOMEM;
     hipLaunchKernelGGL(lx_gen_gauss_kernel, dim3(grid_for((int64_t)n * n / 2)), dim3(256), 0, 0, n, n, G, (int64_t)n,
                        seed);
     const int nt = (n + 127) / 128;
@@ -170,28 +196,79 @@ extern "C" int linompss_gen_gauss_spd(int n, uint32_t seed, double kappa, double
         (void)hipFree(G);
         return LINOMPSS_ERR_HIP;
     }
-    // power iteration for lambda_max (start from a positive vector: the
-    // Perron-like direction of a Wishart matrix is close to it)
-    double *v = G, *y = G + n, *nrm = G + 2 * (size_t)n;  // reuse G's storage
+    // lambda_max by Lanczos from the vector of ones: the largest Ritz value
+    // (from below) converges with the square root of the top spectral gap,
+    // which for a Wishart matrix is too small for the power method (60
+    // power steps left it 0.75 % low at n = 600).  The Ritz value is the
+    // largest eigenvalue of the tridiagonal T_k, found by bisection on its
+    // Sturm sequence; iterate until it moves by less than 1e-13 relative.
+    double *v = G, *u = G + n, *w = G + 2 * (size_t)n, *sc = G + 3 * (size_t)n;  // reuse G's storage
     (void)hipDeviceSynchronize();
-    std::vector<double> ones(n, 1.0);
-    (void)hipMemcpy(v, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-    double lam = 0.0;
-    for (int it = 0; it < 60; ++it) {
-        hipLaunchKernelGGL(lx_norm2_kernel, dim3(1), dim3(256), 0, 0, n, v, nrm);
-        hipLaunchKernelGGL(lx_scale_vec_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, v, nrm);
-        hipLaunchKernelGGL(lx_gemv_full_kernel, dim3((n + 63) / 64), dim3(256), 0, 0, n, dA, lda, v, y);
-        hipLaunchKernelGGL(lx_norm2_kernel, dim3(1), dim3(256), 0, 0, n, y, nrm);
-        double h = 0.0;
-        if (hipMemcpy(&h, nrm, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
+    {
+        std::vector<double> ones(n, 1.0 / std::sqrt((double)n));
+        if (hipMemcpy(v, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(u, 0, sizeof(double) * n) != hipSuccess) {
             (void)hipFree(G);
             return LINOMPSS_ERR_HIP;
         }
-        const double prev = lam;
-        lam = h;
-        std::swap(v, y);
-        if (it > 5 && std::fabs(lam - prev) <= 1e-10 * lam) break;
     }
+    std::vector<double> al, be;  // T_k: diagonal al[0..k-1], off-diagonal be[1..k-1]
+    be.push_back(0.0);
+    auto ritz_max = [&]() {
+        // number of eigenvalues of T below x (Sturm count), then bisection
+        const int k = (int)al.size();
+        double lo = al[0], hi = al[0];
+        for (int i = 0; i < k; ++i) {
+            const double r = (i > 0 ? std::fabs(be[i]) : 0.0) + (i + 1 < k ? std::fabs(be[i + 1]) : 0.0);
+            lo = std::min(lo, al[i] - r);
+            hi = std::max(hi, al[i] + r);
+        }
+        auto below = [&](double x) {
+            int c = 0;
+            double d = 1.0;
+            for (int i = 0; i < k; ++i) {
+                d = al[i] - x - (i > 0 ? be[i] * be[i] / d : 0.0);
+                if (d == 0.0) d = -1e-300;
+                if (d < 0.0) ++c;
+            }
+            return c;
+        };
+        for (int it = 0; it < 200 && hi - lo > 1e-15 * std::fabs(hi); ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (below(mid) >= k) hi = mid;  // every eigenvalue below mid
+            else lo = mid;
+        }
+        return hi;
+    };
+    double lam = 0.0;
+    for (int j = 0; j < 400; ++j) {
+        hipLaunchKernelGGL(lx_gemv_full_kernel, dim3((n + 63) / 64), dim3(256), 0, 0, n, dA, lda, v, w);
+        hipLaunchKernelGGL(lx_dot_kernel, dim3(1), dim3(256), 0, 0, n, w, v, sc);
+        double a = 0.0;
+        if (hipMemcpy(&a, sc, sizeof a, hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipFree(G);
+            return LINOMPSS_ERR_HIP;
+        }
+        hipLaunchKernelGGL(lx_lanczos_axpy_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, w, v, u, a, be.back());
+        hipLaunchKernelGGL(lx_norm2_kernel, dim3(1), dim3(256), 0, 0, n, w, sc);
+        double bn = 0.0;
+        if (hipMemcpy(&bn, sc, sizeof bn, hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipFree(G);
+            return LINOMPSS_ERR_HIP;
+        }
+        al.push_back(a);
+        const double prev = lam;
+        if (j % 5 == 4 || !(bn > 0.0)) {
+            lam = ritz_max();
+            if (j >= 9 && std::fabs(lam - prev) <= 1e-13 * lam) break;
+        }
+        if (!(bn > 1e-300 * std::fabs(a))) break;  // an invariant subspace: T_k is exact
+        be.push_back(bn);
+        hipLaunchKernelGGL(lx_scale_vec_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, w, sc);
+        std::swap(u, v);  // u = v_j
+        std::swap(v, w);  // v = v_{j+1}
+    }
+    if (al.size() % 5 != 0) lam = std::max(lam, ritz_max());
     const double alpha = lam / (kappa - 1.0);
     hipLaunchKernelGGL(lx_add_diag_kernel, dim3(grid_for(n)), dim3(256), 0, 0, n, dA, lda, alpha);
     const hipError_t e = hipDeviceSynchronize();
@@ -223,5 +300,31 @@ extern "C" int linompss_gen_gauss(int m, int n, uint32_t seed, double *dG, int64
     if ((int64_t)m * n == 0) return 0;
     hipLaunchKernelGGL(lx_gen_gauss_kernel, dim3(grid_for((int64_t)m * n / 2 + 1)), dim3(256), 0, 0, m, n, dG, ldg,
                        seed);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+// The refinement app's default matrix, GENMAT_COVARIANT (src/matutil/
+// genmat.c:263-302, called from itref_setup.c:115 through dgenmat_config with
+// p = k = 2, genmat_config.c:37): a_ij = |i - j|^-p off the diagonal,
+// a_jj = 1 + (j + 1)^(1/k); both triangles written.  One thread per element,
+// rows of a column on consecutive lanes.
+__global__ void lx_gen_covariant_kernel(int n, double mp, double kinv, double *__restrict__ A, int64_t lda)
+{
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e / n), i = (int)(e % n);
+        A[(int64_t)j * lda + i] = i == j ? 1.0 + pow((double)(j + 1), kinv) : pow((double)(i > j ? i - j : j - i), mp);
+    }
+}
+
+extern "C" int linompss_gen_covariant(int n, double p, double k, double *dA, int64_t lda)
+{
+    if (n < 0) return -1;
+    if (!(p > 0.0)) return -2;
+    if (!(k != 0.0)) return -3;
+    if (lda < std::max(1, n)) return -5;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(lx_gen_covariant_kernel, dim3(grid_for((int64_t)n * n)), dim3(256), 0, 0, n, -p, 1.0 / k, dA,
+                       lda);
     return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
 }

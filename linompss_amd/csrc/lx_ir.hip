@@ -222,12 +222,13 @@ __global__ void __launch_bounds__(1024) lx_colmax_kernel(const double *__restric
     if (threadIdx.x == 0) out[blockIdx.x] = m[0];
 }
 
-// berr[c] = rmax[c] / (anorm * xmax[c])  (the FP64 convergence test)
+// berr[c] = rmax[c] / (anorm * xmax[c])  (the FP64 convergence test); a zero
+// residual is an exact solution (b_c = 0 gives x_c = 0 and r_c = 0), backward
+// error 0 rather than 0 / 0
 __global__ void lx_berr_kernel(int nrhs, const double *__restrict__ rmax, const double *__restrict__ xmax,
                                const double *__restrict__ anorm, double *__restrict__ berr)
 {
-    int c = threadIdx.x;
-    if (c < nrhs) berr[c] = rmax[c] / (anorm[0] * xmax[c]);
+    for (int c = threadIdx.x; c < nrhs; c += blockDim.x) berr[c] = rmax[c] == 0.0 ? 0.0 : rmax[c] / (anorm[0] * xmax[c]);
 }
 
 // lower triangle of A (FP64) -> Af (FP32), ld n
@@ -392,21 +393,21 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     hipStream_t s = (hipStream_t)stream;
     auto t0 = std::chrono::steady_clock::now();
     const size_t nn = (size_t)n * n, nr = (size_t)n * nrhs;
+    // per-column values: [0] anorm, then rmax, xmax, berr, bmax, smax (nrhs each)
+    const size_t nsmall = (1 + 5 * (size_t)nrhs + 63) / 64 * 64;
     float *Af = (float *)lx_scratch(4, sizeof(float) * nn);
-    char *ws = (char *)lx_scratch(5, sizeof(double) * (3 * nr + 3 * (size_t)n + 64) + sizeof(float) * nr + 1024);
+    char *ws = (char *)lx_scratch(5, sizeof(double) * (nr + 2 * (size_t)n + nsmall) + sizeof(float) * nr + 1024);
     if (!Af || !ws) return LINOMPSS_ERR_NOMEM;
     double *R = (double *)ws;
     double *rowabs = R + nr;
     // the FP16 split needs |L| <= 16: factor D A D (unit-range diagonal)
     double *Deq = o.update_prec == 2 ? rowabs + n : nullptr;
-    double *small = rowabs + 2 * (size_t)n;  // [0] anorm, [1..16] rmax, [17..32] xmax, [33..48] berr, [49..] bmax
-    float *Sf = (float *)(small + 128);
-    double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32, *bmax = small + 1 + 48;
-    double *smax = small + 1 + 64;  // max |D r| per column: the scale of the equilibrated corrections
-    if (nrhs > 16) {
-        // small per-column arrays sized for 16 columns
-        return -2;
-    }
+    double *small = rowabs + 2 * (size_t)n;
+    float *Sf = (float *)(small + nsmall);
+    double *anorm = small, *rmax = small + 1, *xmax = rmax + nrhs, *berr = xmax + nrhs, *bmax = berr + nrhs;
+    double *smax = bmax + nrhs;  // max |D r| per column: the scale of the equilibrated corrections
+    const size_t nhost = 1 + 3 * (size_t)nrhs;  // anorm, rmax, xmax, berr
+    std::vector<double> h_small(nhost);
     // 1) FP32 copy of the lower triangle, FP32 factor
     {
         dim3 grid(std::max(1, std::min(64, (n + 255) / 256)), std::min(n, 4096));
@@ -424,7 +425,6 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     info = rc;
     bool fallback = info != 0;
     double prev = INFINITY, cur_berr = INFINITY;
-    double h_small[1 + 48];
     if (!fallback) {
         // x0 = solve(b) with per-column scaling by ||b_c||_inf
         // the FP32 right-hand side D b / max|D b| lies in [-1, 1] whatever D is
@@ -441,14 +441,14 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
             if (it == 0) hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
             hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
             hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
-            hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
+            hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(256), 0, s, nrhs, rmax, xmax, anorm, berr);
             if (Deq) hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, Deq, smax);
             LX_HIP_TRY(hipGetLastError());
-            LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
+            LX_HIP_TRY(hipMemcpyAsync(h_small.data(), small, sizeof(double) * nhost, hipMemcpyDeviceToHost, s));
             LX_HIP_TRY(hipStreamSynchronize(s));
             double worst = 0.0, rn = 0.0;
             for (int c = 0; c < nrhs; ++c) {
-                double e = h_small[1 + 32 + c], r = h_small[1 + c];
+                double e = h_small[1 + 2 * (size_t)nrhs + c], r = h_small[1 + c];
                 if (!(e <= worst)) worst = e;
                 if (!(r <= rn)) rn = r;
             }
@@ -485,12 +485,12 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
         hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
         hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
         hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
-        hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
-        LX_HIP_TRY(hipMemcpyAsync(h_small, small, sizeof(double) * (1 + 48), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(256), 0, s, nrhs, rmax, xmax, anorm, berr);
+        LX_HIP_TRY(hipMemcpyAsync(h_small.data(), small, sizeof(double) * nhost, hipMemcpyDeviceToHost, s));
         LX_HIP_TRY(hipStreamSynchronize(s));
         cur_berr = 0.0;
         for (int c = 0; c < nrhs; ++c)
-            if (!(h_small[1 + 32 + c] <= cur_berr)) cur_berr = h_small[1 + 32 + c];
+            if (!(h_small[1 + 2 * (size_t)nrhs + c] <= cur_berr)) cur_berr = h_small[1 + 2 * (size_t)nrhs + c];
     }
     st->fallback = fallback ? 1 : 0;
     st->info = info;
@@ -538,23 +538,27 @@ extern "C" int linompss_dposv_ir(int n, int nrhs, int b, const double *A, int ld
 extern "C" int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
                                   const double *dB, int64_t ldb, double *berr_out)
 {
-    if (n <= 0 || nrhs <= 0 || nrhs > 16) return -1;
+    if (n <= 0) return -1;
+    if (nrhs <= 0) return -2;
+    if (lda < n) return -4;
+    if (ldx < n) return -6;
+    if (ldb < n) return -8;
     std::lock_guard<std::mutex> g(g_ir_mu);
     const size_t nr = (size_t)n * nrhs;
-    char *ws = (char *)lx_scratch(7, sizeof(double) * (3 * nr + 2 * (size_t)n + 128));
+    char *ws = (char *)lx_scratch(7, sizeof(double) * (nr + (size_t)n + 1 + 3 * (size_t)nrhs + 64));
     if (!ws) return LINOMPSS_ERR_NOMEM;
     double *R = (double *)ws, *rowabs = R + nr;
     double *small = rowabs + n;
-    double *anorm = small, *rmax = small + 1, *xmax = small + 1 + 16, *berr = small + 1 + 32;
+    double *anorm = small, *rmax = small + 1, *xmax = rmax + nrhs, *berr = xmax + nrhs;
     hipStream_t s = nullptr;
     int rc = residual(n, nrhs, dA, lda, dX, ldx, dB, ldb, R, rowabs, s);
     if (rc) return rc;
     hipLaunchKernelGGL(lx_max_kernel, dim3(1), dim3(1024), 0, s, n, rowabs, anorm);
     hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, R, (int64_t)n, n, nullptr, rmax);
     hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dX, ldx, n, nullptr, xmax);
-    hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(64), 0, s, nrhs, rmax, xmax, anorm, berr);
-    double h[16];
-    LX_HIP_TRY(hipMemcpy(h, berr, sizeof(double) * nrhs, hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(lx_berr_kernel, dim3(1), dim3(256), 0, s, nrhs, rmax, xmax, anorm, berr);
+    std::vector<double> h(nrhs);
+    LX_HIP_TRY(hipMemcpy(h.data(), berr, sizeof(double) * nrhs, hipMemcpyDeviceToHost));
     double worst = 0.0;
     for (int c = 0; c < nrhs; ++c)
         if (!(h[c] <= worst)) worst = h[c];

@@ -45,7 +45,7 @@ __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_re
 // on a unit listed earlier, so a persistent grid that takes units in list
 // order cannot deadlock.  Unit code: (s << 16) | c.
 #define LX_SW_GAP 3  // the solve unit's direct products (s-3, s-2) use sD and sA
-__host__ __device__ inline int lx_sw_nparts(int s, int C) { return s > LX_SW_GAP ? (s - LX_SW_GAP + C - 1) / C : 0; }
+__host__ __device__ constexpr int lx_sw_nparts(int s, int C) { return s > LX_SW_GAP ? (s - LX_SW_GAP + C - 1) / C : 0; }
 static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase)
 {
     units.clear();
@@ -100,21 +100,32 @@ __device__ __forceinline__ void lx_sw_publish(int *p)
     }
 }
 
-// Mailbox of solved values: one slot per element of x, all-ones bits (a NaN
-// payload no arithmetic produces) until the solve unit writes it with an
-// atomic exchange, which is performed at the device's shared level at once.
+// Mailbox of solved values: one slot per element of x, all-ones bits (a
+// negative quiet NaN with a full payload) until the solve unit writes it with
+// an atomic exchange, which is performed at the device's shared level at once.
 // A consumer polls its element until the bits change: the value arrives with
 // its own "ready" signal, one memory round trip and no release fence on the
-// chain of solve units.
+// chain of solve units.  Arithmetic keeps an input NaN's payload, so a NaN in
+// B or L can reach the mailbox with exactly those bits: the writer replaces
+// that one pattern by the canonical quiet NaN, and the sweep propagates NaN
+// instead of leaving its readers waiting for a value that looks unwritten.
 template <typename T> struct lx_sw_bits;
-template <> struct lx_sw_bits<float> { typedef unsigned int U; };
-template <> struct lx_sw_bits<double> { typedef unsigned long long U; };
+template <> struct lx_sw_bits<float> {
+    typedef unsigned int U;
+    static constexpr U qnan = 0x7fc00000u;
+};
+template <> struct lx_sw_bits<double> {
+    typedef unsigned long long U;
+    static constexpr U qnan = 0x7ff8000000000000ull;
+};
 
 template <typename T>
 __device__ __forceinline__ void lx_sw_mb_put(T *mb, T v)
 {
     typedef typename lx_sw_bits<T>::U U;
-    __hip_atomic_exchange((U *)mb, __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    U bits = __builtin_bit_cast(U, v);
+    if (bits == ~U(0)) bits = lx_sw_bits<T>::qnan;
+    __hip_atomic_exchange((U *)mb, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Per-lane wait (a lane loop with no barrier inside); bounded like every wait.
@@ -480,18 +491,22 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
 namespace {
 
 // 64-row blocks per partial-sum unit: at least 8, and at most 8 parts per step
-int lx_sw_chunk(int nb) { return std::max(8, (nb - 1 + 7) / 8); }
+constexpr int lx_sw_chunk(int nb) { return (nb - 1 + 7) / 8 > 8 ? (nb - 1 + 7) / 8 : 8; }
 
 struct lx_sw_plan {
     int *d_units = nullptr, *d_rowbase = nullptr;
     int nunits = 0, nparts = 0;
 };
-struct lx_sw_dev {
-    int cus = 0, wall_khz = 0, occ[2][2][3] = {};  // [type][trans][NR 1, 4, 16]
-    std::map<int, lx_sw_plan> plans;  // by number of blocks
+// Per-device properties: CUs, wall clock, resident workgroups of each sweep
+// kernel ([type][trans][NR 1, 4, 16]).  Filled once per device, in a fixed
+// table (no allocation), so the scratch-free entry point can use it too.
+struct lx_sw_props {
+    int cus = 0, wall_khz = 0, occ[2][2][3] = {};
 };
+constexpr int LX_SW_MAXDEV = 64;
 std::mutex lx_sw_mu;
-std::map<int, lx_sw_dev> lx_sw_devs;
+lx_sw_props lx_sw_prop_tab[LX_SW_MAXDEV];
+std::map<int, std::map<int, lx_sw_plan>> lx_sw_plans;  // [device][number of blocks]
 
 template <typename T, int TR, int NR>
 int lx_sw_occ()
@@ -512,22 +527,35 @@ void lx_sw_occ_all(int (&o)[2][3])
     o[1][2] = lx_sw_occ<T, 1, 16>();
 }
 
-// Device properties, occupancy and the unit list for nb blocks (built once).
-lx_sw_dev *lx_sw_get(int nb, const lx_sw_plan **plan)
+// Properties of the current device (caller holds lx_sw_mu).
+const lx_sw_props *lx_sw_props_get(int *dev_out)
 {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    std::lock_guard<std::mutex> g(lx_sw_mu);
-    lx_sw_dev &d = lx_sw_devs[dev];
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= LX_SW_MAXDEV) return nullptr;
+    lx_sw_props &d = lx_sw_prop_tab[dev];
     if (d.cus == 0) {
-        if (hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return nullptr;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return nullptr;
         if (hipDeviceGetAttribute(&d.wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || d.wall_khz <= 0)
             d.wall_khz = 100000;
         lx_sw_occ_all<double>(d.occ[0]);
         lx_sw_occ_all<float>(d.occ[1]);
+        d.cus = cus;
     }
-    auto it = d.plans.find(nb);
-    if (it == d.plans.end()) {
+    if (dev_out) *dev_out = dev;
+    return &d;
+}
+
+// Properties and the unit list for nb blocks (built once per device and nb).
+const lx_sw_props *lx_sw_get(int nb, const lx_sw_plan **plan)
+{
+    std::lock_guard<std::mutex> g(lx_sw_mu);
+    int dev = 0;
+    const lx_sw_props *d = lx_sw_props_get(&dev);
+    if (!d) return nullptr;
+    auto &plans = lx_sw_plans[dev];
+    auto it = plans.find(nb);
+    if (it == plans.end()) {
         std::vector<int> units, rowbase;
         lx_sw_build_units(nb, lx_sw_chunk(nb), units, rowbase);
         lx_sw_plan pl;
@@ -541,31 +569,43 @@ lx_sw_dev *lx_sw_get(int nb, const lx_sw_plan **plan)
         if (hipMemcpy(pl.d_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(pl.d_rowbase, rowbase.data(), sizeof(int) * rowbase.size(), hipMemcpyHostToDevice) != hipSuccess)
             return nullptr;
-        it = d.plans.emplace(nb, pl).first;
+        it = plans.emplace(nb, pl).first;
     }
     *plan = &it->second;
-    return &d;
+    return d;
+}
+
+// Workspace of one sweep launch over nb blocks with nparts partial sums:
+// ctl (ticket, abort, flags, part counters) | part | mailbox, each 256-byte
+// aligned.  Offsets in bytes for element size es.
+struct lx_sw_ws {
+    size_t ctl_ints, part_off, mb_off, bytes;
+    int64_t ldmb;
+};
+constexpr lx_sw_ws lx_sw_layout(int nb, int nparts, size_t es)
+{
+    const size_t ctl_ints = 64 + 2 * (size_t)nb;
+    const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
+    const size_t part_bytes = es * (size_t)(nparts > 1 ? nparts : 1) * LX_SW_NR * 64;
+    const size_t mb_off = (part_off + part_bytes + 255) / 256 * 256;
+    const int64_t ldmb = (int64_t)nb * 64;
+    return lx_sw_ws{ctl_ints, part_off, mb_off, (mb_off + es * (size_t)ldmb * LX_SW_NR + 255) / 256 * 256, ldmb};
 }
 }  // namespace
 
+// The sweeps of X := op(L)^-1 X (which: 1 forward, 2 backward, 3 both) over
+// the given unit list and workspace, 16 right-hand sides per launch; every
+// launch is waited for and checked for a wait that gave up.
 template <typename T>
-static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, int which, hipStream_t s)
+static int sweeps_launch(const lx_sw_props *d, int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, int which,
+                         hipStream_t s, const int *d_units, int nunits, const int *d_rowbase, int nparts, char *ws)
 {
     const int nb = (n + 63) / 64;
-    const lx_sw_plan *pl = nullptr;
-    lx_sw_dev *d = lx_sw_get(nb, &pl);
-    if (!d) return LINOMPSS_ERR_HIP;
-    const size_t ctl_ints = 64 + 2 * (size_t)nb;
-    const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
-    const size_t part_bytes = sizeof(T) * (size_t)std::max(1, pl->nparts) * LX_SW_NR * 64;
-    const size_t mb_off = (part_off + part_bytes + 255) / 256 * 256;
-    const int64_t ldmb = (int64_t)nb * 64;
-    const size_t mb_bytes = sizeof(T) * (size_t)ldmb * LX_SW_NR;
-    char *ws = (char *)lx_scratch(3, mb_off + mb_bytes);
-    if (!ws) return LINOMPSS_ERR_NOMEM;
+    const lx_sw_ws lay = lx_sw_layout(nb, nparts, sizeof(T));
     int *ctl = (int *)ws;
-    T *part = (T *)(ws + part_off);
-    T *mb = (T *)(ws + mb_off);
+    T *part = (T *)(ws + lay.part_off);
+    T *mb = (T *)(ws + lay.mb_off);
+    const int64_t ldmb = lay.ldmb;
     double tmo = 30.0;
     if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
     const long long ticks = (long long)(tmo * d->wall_khz * 1000.0);
@@ -576,12 +616,12 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
         const int ri = nc <= 1 ? 0 : nc <= 4 ? 1 : 2;
         for (int tr = 0; tr < 2; ++tr) {
             if (!(which & (1 << tr))) continue;
-            const int grid = std::max(1, std::min(pl->nunits, d->cus * d->occ[ti][tr][ri]));
-            lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
+            const int grid = std::max(1, std::min(nunits, d->cus * d->occ[ti][tr][ri]));
+            lx_fill_i32(ctl, 0, (int64_t)lay.ctl_ints, s);
             lx_fill_i32((int32_t *)mb, -1, (int64_t)(sizeof(T) * ldmb * nc / 4), s);  // all-ones: empty
 #define LX_SW_LAUNCH(TR, NR)                                                                                       \
     hipLaunchKernelGGL((lx_sweep_kernel<T, TR, NR>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, X + c0 * ldx, ldx,  \
-                       pl->d_units, pl->nunits, pl->d_rowbase, lx_sw_chunk(nb), part, ctl, mb, ldmb, ticks)
+                       d_units, nunits, d_rowbase, lx_sw_chunk(nb), part, ctl, mb, ldmb, ticks)
             if (tr == 0) {
                 if (ri == 0) LX_SW_LAUNCH(0, 1);
                 else if (ri == 1) LX_SW_LAUNCH(0, 4);
@@ -602,6 +642,19 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
     return aborted ? LINOMPSS_ERR_SCHED : 0;
 }
 
+template <typename T>
+static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, int which, hipStream_t s)
+{
+    const int nb = (n + 63) / 64;
+    const lx_sw_plan *pl = nullptr;
+    const lx_sw_props *d = lx_sw_get(nb, &pl);
+    if (!d) return LINOMPSS_ERR_HIP;
+    char *ws = (char *)lx_scratch(3, lx_sw_layout(nb, pl->nparts, sizeof(T)).bytes);
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    return sweeps_launch<T>(d, n, nrhs, L, ldl, X, ldx, which, s, pl->d_units, pl->nunits, pl->d_rowbase, pl->nparts,
+                            ws);
+}
+
 // which: 1 forward, 2 backward, 3 both.  Returns after the sweeps finished
 // (each launch is checked for a timed-out wait).
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
@@ -611,4 +664,5 @@ int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *
     if (dtype == 0) return sweeps<double>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, which, s);
     return sweeps<float>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, s);
 }
+
 #endif  // LX_SWEEP_KERNELS_ONLY

@@ -21,6 +21,13 @@ Parity pins (each function is checked by ``tests/test_oracle.py``):
   that the FP32 factor cannot refine
 * ``relerr_lower``, ``potrf_resid``, ``backward_error`` -- closed forms in
   numpy on random inputs and hand-made cases
+* ``genmat_covariant``, ``genmat_costas``, ``dgenmat_config`` -- exact entries
+  fixed by the definition (integer powers, square roots of perfect squares),
+  the Toeplitz structure, the SPD bound of the Toeplitz symbol, and the
+  drand48 stream
+* ``philox4x32_10`` -- Random123's published known-answer vectors;
+  ``gen_uniform`` / ``gen_gauss`` -- the stream's moments and the Box-Muller
+  identity g0^2 + g1^2 = -2 ln u0
 """
 from __future__ import annotations
 
@@ -83,6 +90,12 @@ _sig("lxo_dposv_ir", _i, _i, _i, _i, _p, _i, _p, _i, _p, _i, ctypes.POINTER(IrOp
 _sig("lxo_relerr_lower", _d, _i, _p, _i, _p, _i)
 _sig("lxo_potrf_resid", _d, _i, _p, _i, _p, _i)
 _sig("lxo_backward_error", _d, _i, _i, _p, _i, _p, _i, _p, _i)
+_sig("lxo_genmat_covariant", _i, _i, _d, _d, _p)
+_sig("lxo_genmat_costas", None, _i, _i, _l, _p)
+_sig("lxo_dgenmat_config", _i, _i, _i, _i, _i, _p, _l, _p)
+_sig("lxo_philox4x32_10", None, _p, _p, _p)
+_sig("lxo_gen_uniform", None, _i, _i, ctypes.c_uint32, _d, _d, _p, _i)
+_sig("lxo_gen_gauss", None, _i, _i, ctypes.c_uint32, _p, _i)
 _sig("lxo_wtime", _d)
 _sig("lxo_max_threads", _i)
 
@@ -114,6 +127,55 @@ def genmat_sym_full(n: int, seed: int) -> np.ndarray:
     A = np.zeros((n, n), dtype=np.float64, order="F")
     lib.lxo_genmat_sym_full(n, seed, _pt(A))
     return A
+
+
+def genmat_covariant(n: int, p: float = 2.0, k: float = 2.0) -> np.ndarray | None:
+    """GENMAT_COVARIANT (genmat.c:263-302); None for p <= 0 (the reference's NULL)."""
+    A = np.zeros((n, n), dtype=np.float64, order="F")
+    return A if lib.lxo_genmat_covariant(n, p, k, _pt(A)) == 0 else None
+
+
+def genmat_costas(m: int, n: int, seed: int) -> np.ndarray:
+    A = np.zeros((m, n), dtype=np.float64, order="F")
+    lib.lxo_genmat_costas(m, n, seed, _pt(A))
+    return A
+
+
+# ompssmat_t (src/matutil/ompss_mat.h:5-14)
+GENMATCONF = {"spd": 0, "covar": 1, "rand": 2, "eye": 3, "diagdom": 4, "ones": 5, "zero": 6, "costas": 7}
+
+
+def dgenmat_config(kind, m: int, n: int, ldim: int, pars=None, seed: int = 0) -> np.ndarray | None:
+    """dgenmat_config (genmat_config.c:34-133) with explicit parameters and seed;
+    None where the reference returns NULL.  The array is ldim x n (ld m for
+    COVAR and COSTAS, which the reference allocates at m x m / m x n)."""
+    k = GENMATCONF[kind] if isinstance(kind, str) else int(kind)
+    rows = m if k in (1, 7) else ldim
+    cols = m if k in (0, 1, 3, 6) else n
+    A = np.zeros((rows, cols), dtype=np.float64, order="F")
+    pa = None if pars is None else (ctypes.c_double * 4)(*(list(pars) + [2.0] * (4 - len(pars))))
+    rc = lib.lxo_dgenmat_config(k, m, n, ldim, pa, seed, _pt(A))
+    return A if rc == 0 else None
+
+
+def philox4x32_10(ctr, key) -> tuple:
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib.lxo_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def gen_uniform(m: int, n: int, seed: int, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    X = np.zeros((m, n), dtype=np.float64, order="F")
+    lib.lxo_gen_uniform(m, n, seed, lo, hi, _pt(X), m)
+    return X
+
+
+def gen_gauss(m: int, n: int, seed: int) -> np.ndarray:
+    G = np.zeros((m, n), dtype=np.float64, order="F")
+    lib.lxo_gen_gauss(m, n, seed, _pt(G), m)
+    return G
 
 
 def chol(A: np.ndarray, b: int, variant: str = "rl", nthreads: int = 0) -> int:

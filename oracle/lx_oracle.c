@@ -247,11 +247,118 @@ double lxo_backward_error(int n, int nrhs, const double *A, int lda, const doubl
     residual(n, nrhs, A, lda, X, ldx, B, ldb, R);
     double an = mat_inf_norm(n, A, lda), worst = 0.0;
     for (int c = 0; c < nrhs; ++c) {
-        double e = col_inf(n, &R[(size_t)c * n]) / (an * col_inf(n, &X[(size_t)c * ldx]));
+        const double r = col_inf(n, &R[(size_t)c * n]);
+        /* a zero residual is an exact solution: backward error 0 (b_c = 0, x_c = 0
+         * would otherwise give 0 / 0) */
+        double e = r == 0.0 ? 0.0 : r / (an * col_inf(n, &X[(size_t)c * ldx]));
         if (!(e <= worst)) worst = e;
     }
     free(R);
     return worst;
+}
+
+/* ------------------------------------------------------------------ */
+/* GENMAT_COVARIANT (genmat.c:263-302): column j holds |j - i|^-p above the
+ * diagonal, 1 + (j + 1)^(1/k) on it, |i - j|^-p below, in storage order. */
+int lxo_genmat_covariant(int n, double p, double k, double *A)
+{
+    if (p <= 0.0) return -1;
+    const double mp = -p, kinv = 1.0 / k;
+    for (int j = 0; j < n; ++j) {
+        double *ap = &A[(size_t)j * n];
+        for (int i = 0; i < j; ++i) *ap++ = pow((double)(j - i), mp);
+        *ap++ = 1.0 + pow((double)(j + 1), kinv);
+        for (int i = j + 1; i < n; ++i) *ap++ = pow((double)(i - j), mp);
+    }
+    return 0;
+}
+
+/* GENMAT_COSTAS (genmat.c:199-220): one draw per element, storage order. */
+void lxo_genmat_costas(int m, int n, long seed, double *A)
+{
+    srand48(seed);
+    const size_t s = (size_t)m * n;
+    for (size_t c = 0; c < s; ++c) A[c] = drand48() > 0.5 ? 1.0 : -1.0;
+}
+
+/* dgenmat_config (genmat_config.c:34-133): the switch on ompssmat_t with the
+ * default parameters of :37 when no configuration file is read. */
+int lxo_dgenmat_config(int kind, int m, int n, int ldim, const double *pars, long seed, double *A)
+{
+    const double def[4] = {2.0, 2.0, 2.0, 2.0};
+    if (!pars) pars = def;
+    switch (kind) {
+    case 0: /* GENMATCONF_SPD: GENMAT_SPD(m, ldim) */
+        lxo_genmat_spd(m, ldim, seed, 0, A);
+        return 0;
+    case 1: /* GENMATCONF_COVAR: GENMAT_COVARIANT(m, pars[0], pars[1]) */
+        return lxo_genmat_covariant(m, pars[0], pars[1], A);
+    case 2: /* GENMATCONF_RAND: GENMAT(m, n, ldim) */
+        lxo_genmat(m, n, ldim, seed, A);
+        return 0;
+    case 3: /* GENMATCONF_EYE: GENMAT_EYE(m, ldim) */
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < ldim; ++i) A[(size_t)j * ldim + i] = i == j ? 1.0 : 0.0;
+        return 0;
+    case 6: /* GENMATCONF_ZERO: GENMAT_ZERO(m, ldim) */
+        memset(A, 0, sizeof(double) * (size_t)m * ldim);
+        return 0;
+    case 7: /* GENMATCONF_COSTAS: GENMAT_COSTAS(m, n) */
+        lxo_genmat_costas(m, n, seed, A);
+        return 0;
+    default: /* DIAGDOM, ONES and unknown kinds: the reference returns NULL */
+        return -1;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon et al., SC'11): ten rounds; each round multiplies
+ * ctr[0] and ctr[2] by the round multipliers, swaps the halves across and
+ * xors in the key, whose words are bumped by the Weyl constants between
+ * rounds. */
+void lxo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double lxo_u53(uint32_t hi, uint32_t lo) { return (double)((((uint64_t)hi) << 21) ^ (lo >> 11)) * 0x1p-53; }
+
+void lxo_gen_uniform(int m, int n, uint32_t seed, double lo, double hi, double *X, int ldx)
+{
+    const uint32_t key[2] = {seed, 0x89abcdefu};
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e = 0; e < tot; ++e) {
+        const uint32_t ctr[4] = {(uint32_t)e, (uint32_t)(e >> 32), 1u, 0u};
+        uint32_t r[4];
+        lxo_philox4x32_10(ctr, key, r);
+        /* one rounding: the device evaluates lo + (hi - lo) u as a fused multiply-add */
+        X[(e / m) * ldx + e % m] = fma(hi - lo, lxo_u53(r[0], r[1]), lo);
+    }
+}
+
+void lxo_gen_gauss(int m, int n, uint32_t seed, double *G, int ldg)
+{
+    const uint32_t key[2] = {seed, 0x1234567u};
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e2 = 0; e2 < tot; e2 += 2) {
+        const uint64_t q = (uint64_t)(e2 >> 1);
+        const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), 0u, 0u};
+        uint32_t r[4];
+        lxo_philox4x32_10(ctr, key, r);
+        const double u0 = lxo_u53(r[0], r[1]) + 0x1p-53, u1 = lxo_u53(r[2], r[3]) + 0x1p-53;
+        const double rad = sqrt(-2.0 * log(u0)), th = 6.283185307179586 * u1;
+        G[(e2 / m) * ldg + e2 % m] = rad * cos(th);
+        if (e2 + 1 < tot) G[((e2 + 1) / m) * ldg + (e2 + 1) % m] = rad * sin(th);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -287,7 +394,7 @@ int lxo_dposv_ir(int n, int nrhs, int b, const double *A, int lda, const double 
             double worst = 0.0, rn = 0.0;
             for (int c = 0; c < nrhs; ++c) {
                 double r = col_inf(n, &R[(size_t)c * n]);
-                double e = r / (anorm * col_inf(n, &X[(size_t)c * ldx]));
+                double e = r == 0.0 ? 0.0 : r / (anorm * col_inf(n, &X[(size_t)c * ldx]));
                 if (!(e <= worst)) worst = e;
                 if (!(r <= rn)) rn = r;
             }

@@ -7,8 +7,12 @@
  *
  * It restates, in plain C over host LAPACK/BLAS (one sequential BLAS call per
  * tile task, as the reference does), the reference algorithms:
- *   - drand48 generators         src/matutil/genmat.c:49-70, 106-128;
- *                                tests/dense_tests/test_aux.h:114-128
+ *   - drand48 generators         src/matutil/genmat.c:49-70, 106-128,
+ *                                199-220; tests/dense_tests/test_aux.h:114-128
+ *   - covariance generator       src/matutil/genmat.c:263-302
+ *   - generator dispatch         src/matutil/genmat_config.c:34-133
+ *   - the benchmark's Philox4x32-10 streams (no reference counterpart;
+ *     Salmon et al., SC'11; pinned by Random123's published known answers)
  *   - right-looking CHOL_RL      src/cholesky/dense/chol_rlmain.c:22-53
  *   - left-looking  CHOL_LL      src/cholesky/dense/chol_llmain.c:23-57
  *   - hypermatrix CHOL_HRL/HLL   src/cholesky/dense/chol_hrlmain.c:16-37,
@@ -43,6 +47,38 @@ void   lxo_genmat_spd(int n, int lda, long seed, int diag_mode, double *A);
 void   lxo_genmat(int m, int n, int lda, long seed, double *A);
 /* GENMAT_SYM_FULL of the unit tests (test_aux.h:114-128): symmetric U[0,1) + 10n I */
 void   lxo_genmat_sym_full(int n, long seed, double *A);
+
+/* GENMAT_COVARIANT (genmat.c:263-302): n x n, lda = n, a_ij = |i - j|^-p off
+ * the diagonal, a_jj = 1 + (j + 1)^(1/k).  Returns -1 (and writes nothing)
+ * for p <= 0, as the reference returns NULL. */
+int    lxo_genmat_covariant(int n, double p, double k, double *A);
+/* GENMAT_COSTAS (genmat.c:199-220): m x n (ld m), each drand48() draw in
+ * element order mapped to +1 when > 0.5, else -1; explicit seed. */
+void   lxo_genmat_costas(int m, int n, long seed, double *A);
+/* dgenmat_config (genmat_config.c:34-133) without the libconfig file (absent
+ * here; the reference also skips it when built without libconfig): kind is
+ * the ompssmat_t value (ompss_mat.h:5-14: 0 SPD, 1 COVAR, 2 RAND, 3 EYE,
+ * 4 DIAGDOM, 5 ONES, 6 ZERO, 7 COSTAS); pars = the config's parameters
+ * (NULL: the defaults {2, 2, 2, 2}); seed replaces time(NULL).  Writes into A
+ * (m x n, ld ldim; COVAR and COSTAS use ld m as the reference allocates them)
+ * and returns 0, or -1 where the reference returns NULL (DIAGDOM, ONES,
+ * unknown kinds, COVAR with p <= 0).  EYE writes zeros off the diagonal
+ * (the reference leaves them uninitialised, genmat.c:147-158). */
+int    lxo_dgenmat_config(int kind, int m, int n, int ldim, const double *pars, long seed, double *A);
+
+/* Workload generators of the benchmark (no counterpart in the reference):
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11), ctr[4] and key[2] in, out[4]. */
+void   lxo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* X (m x n, ld ldx) uniform in [lo, hi): element e = j m + i from block
+ * {e_lo, e_hi, 1, 0} under key {seed, 0x89abcdef}, u = ((x << 21) ^ (y >> 11)) 2^-53,
+ * X = fma(hi - lo, u, lo) (one rounding). */
+void   lxo_gen_uniform(int m, int n, uint32_t seed, double lo, double hi, double *X, int ldx);
+/* G (m x n, ld ldg) N(0,1): elements 2q and 2q+1 (order e = j m + i) from
+ * block {q_lo, q_hi, 0, 0} under key {seed, 0x1234567}: u0, u1 in (0, 1]
+ * from (x, y) and (z, w) as above plus 2^-53, then Box-Muller
+ * sqrt(-2 ln u0) (cos, sin)(2 pi u1) with the host libm. */
+void   lxo_gen_gauss(int m, int n, uint32_t seed, double *G, int ldg);
 
 /* Tiled Cholesky, lower, in place.  Returns 0 or the LAPACK-style info of the
  * first failing global column (1-based).  nthreads <= 0 means all cores. */

@@ -174,6 +174,54 @@ static void run_wg(F &&body)
     for (auto &t : th) t.join();
 }
 
+// The RL factor plan run with the first dependency counter of unit `stuck`
+// (a unit with dependencies) started far below any threshold, so that its
+// wait can only end through the bounded-wait path: spin_ticks later the wave
+// sets the abort flag and every wave leaves the persistent loop.  Returns the
+// unit's index, or -1 when no unit from `stuck` on has a dependency;
+// st_out = {abort, timeouts, done, nunits}.
+extern "C" int lx_emu_run_stuck(int dtype, int n, int b, int stuck, long long spin_ticks, void *A, long long lda,
+                                int *st_out)
+{
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    lx_planner pl(P);
+    int s = pl.add_system(n, b, 0);
+    pl.factor_rl(s);
+    pl.finalize();
+    const std::vector<lx_unit> &U = pl.units();
+    int u = stuck;
+    while (u < (int)U.size() && U[u].nd == 0) ++u;
+    if (u >= (int)U.size()) return -1;
+    std::vector<int> ctr(pl.num_counters(), 0);
+    ctr[U[u].dc[0]] = -1000000000;
+    lx_status st = {};
+    int info = INT_MAX;
+    lx_sys sys = {};
+    sys.A = A;
+    sys.lda = lda;
+    sys.n = n;
+    sys.ldw = (int64_t)n + 64;
+    std::vector<double> wbuf((size_t)sys.ldw * 64);
+    sys.W = wbuf.data();
+    blockDim.x = 256;
+    gridDim.x = 1;
+    run_wg([&] {
+        if (dtype == 0)
+            lx_exec_kernel<double, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
+                                                  nullptr);
+        else
+            lx_exec_kernel<float, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
+                                                 nullptr);
+    });
+    st_out[0] = st.abort;
+    st_out[1] = st.timeouts;
+    st_out[2] = st.done;
+    st_out[3] = (int)U.size();
+    return u;
+}
+
 // C = alpha op(A) op(B) + beta C through lx_gemm64_kernel<float> (no split-K),
 // uplo 0 full, 1 lower, 2 upper.
 extern "C" void lx_emu_gemm64s(int ta, int tb, int M, int N, int K, float alpha, const float *A, long long lda,
@@ -234,18 +282,24 @@ extern "C" void lx_emu_gemm_unit(int flags, int M, int N, int K, float *C, long 
 
 // One sweep (trans 0: L Y = X, 1: L^T Z = X) through lx_sweep_kernel, one
 // workgroup at a time in ticket order.
-extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,
-                            long long ldx)
+// One sweep through lx_sweep_kernel.  stuck >= 0 starts the partial-sum
+// counter of that step far below its count, so the step's solve unit can only
+// leave its wait through the bounded-wait path (spin_ticks); returns -3 when
+// a wait gave up (the abort flag), -1 / -2 when the counters or the ticket do
+// not add up.
+extern "C" int lx_emu_sweep_ex(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,
+                               long long ldx, int stuck, long long spin_ticks)
 {
     // one emulated workgroup takes every unit in list order
     const int nb = (n + 63) / 64, C = 3;  // small chunks: several partial-sum units per step
     std::vector<int> units, rowbase;
     lx_sw_build_units(nb, C, units, rowbase);
     std::vector<int> ctl(64 + 2 * nb, 0);
+    if (stuck >= 0 && stuck < nb) ctl[64 + nb + stuck] = -1000000000;
     std::vector<double> part((size_t)std::max(1, rowbase[nb]) * LX_SW_NR * 64);  // large enough for either type
     const int64_t ldmb = (int64_t)nb * 64;
     std::vector<unsigned long long> mbv((size_t)ldmb * LX_SW_NR, ~0ull);  // all-ones: empty (either type)
-    const long long ticks = 100000000LL * 20;
+    const long long ticks = spin_ticks;
     auto go = [&](auto tp, auto tr, auto nr) {
         using T = decltype(tp);
         run_wg([&] {
@@ -267,6 +321,12 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     for (int b = 0; b < nb; ++b)
         if (ctl[64 + nb + b] != lx_sw_nparts(b, C)) return -1;
     return ctl[0] == (int)units.size() + 1 ? 0 : -2;
+}
+
+extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L, long long ldl, void *X,
+                            long long ldx)
+{
+    return lx_emu_sweep_ex(dtype, trans, n, nrhs, L, ldl, X, ldx, -1, 100000000LL * 20);
 }
 
 // Batched refinement solves through lx_batch_ir_kernel, one workgroup taking

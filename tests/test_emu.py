@@ -390,3 +390,77 @@ def test_gemm64_float_and_upper(emu, ta, tb, uplo):
     assert np.abs(C[:M][mask] - ref[mask]).max() <= 1e-5 * np.abs(ref).max()
     assert np.array_equal(C[:M][~mask], C0[:M][~mask])
     assert np.array_equal(C[M:], C0[M:])
+
+
+def _emu_raw():
+    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
+    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E.lx_emu_run_stuck.restype = ctypes.c_int
+    E.lx_emu_run_stuck.argtypes = [ctypes.c_int] * 4 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
+                                                        ctypes.POINTER(ctypes.c_int)]
+    E.lx_emu_sweep_ex.restype = ctypes.c_int
+    E.lx_emu_sweep_ex.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+                                                       ctypes.c_longlong, ctypes.c_int, ctypes.c_longlong]
+    return E
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_executor_bounded_wait_aborts(orc, dtype):
+    """A dependency that is never met ends through the executor's bounded wait:
+    the wave that gave up sets the abort flag, counts one timeout, and every
+    wave leaves the persistent loop (the call returns, all 256 emulated
+    threads joined) with fewer units done than listed -- the condition the
+    host turns into LINOMPSS_ERR_SCHED."""
+    E = _emu_raw()
+    n, b = 300, 64
+    A = orc.genmat_spd(n, 3, diag_mode=1).astype(dtype, order="F")
+    st = (ctypes.c_int * 4)()
+    u = E.lx_emu_run_stuck(0 if dtype == np.float64 else 1, n, b, 5, 100000, A.ctypes.data_as(ctypes.c_void_p),
+                           n, st)
+    assert u >= 5
+    abort, timeouts, done, nunits = list(st)
+    assert abort == 1 and timeouts == 1
+    assert done == u < nunits
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+def test_sweep_bounded_wait_aborts(trans):
+    """A step whose partial sums never arrive: its solve unit gives up after
+    spin_ticks, sets the abort flag, and the remaining waits end at once, so
+    the launch drains (rc -3 = the abort flag, which the host reports as
+    LINOMPSS_ERR_SCHED)."""
+    E = _emu_raw()
+    n, nr = 700, 2
+    rng = np.random.default_rng(9)
+    L = np.asfortranarray(np.tril(rng.uniform(0, 1, (n, n))) + n * np.eye(n))
+    X = np.asfortranarray(rng.uniform(-1, 1, (n, nr)))
+    rc = E.lx_emu_sweep_ex(0, trans, n, nr, L.ctypes.data_as(ctypes.c_void_p), n, X.ctypes.data_as(ctypes.c_void_p),
+                           n, 6, 100000)
+    assert rc == -3
+
+
+@pytest.mark.parametrize("dtype,bits", [(np.float64, np.uint64), (np.float32, np.uint32)])
+@pytest.mark.parametrize("trans", [0, 1])
+def test_sweep_nan_with_mailbox_bits(orc, dtype, bits, trans):
+    """An input NaN whose bits equal the mailbox's 'empty' mark (all ones)
+    propagates as NaN: the solve unit publishes the canonical quiet NaN in its
+    place, so no reader waits for a value that looks unwritten (rc 0, no
+    abort)."""
+    E = _emu_raw()
+    n, nr, row = 300, 2, 70
+    rng = np.random.default_rng(10)
+    L = np.asfortranarray(np.tril(rng.uniform(0.5, 1, (n, n))) + n * np.eye(n)).astype(dtype, order="F")
+    X = np.asfortranarray(rng.uniform(-1, 1, (n, nr))).astype(dtype, order="F")
+    X.view(bits)[row, 0] = np.iinfo(bits).max
+    assert np.isnan(X[row, 0])
+    rc = E.lx_emu_sweep_ex(0 if dtype == np.float64 else 1, trans, n, nr, L.ctypes.data_as(ctypes.c_void_p), n,
+                           X.ctypes.data_as(ctypes.c_void_p), n, -1, 100000000 * 20)
+    assert rc == 0
+    # forward: rows from `row` on depend on x[row]; backward: rows up to `row`.
+    # The solve multiplies by the inverse of the 64 x 64 diagonal block, whose
+    # zeros times NaN are NaN, so the whole 64-row block of `row` turns NaN
+    blk = row // 64 * 64
+    dep = np.arange(n) >= blk if trans == 0 else np.arange(n) < blk + 64
+    assert np.isnan(X[dep, 0]).all()
+    assert np.isfinite(X[~dep, 0]).all()
+    assert np.isfinite(X[:, 1]).all()

@@ -145,3 +145,24 @@ def test_gp_fused(lx, dev, n):
     assert np.abs(K.grad.numpy() - dK_ref).max() <= 1e-9 * np.abs(dK_ref).max()
     ybar = np.linalg.solve(adj.gp_kernel(P, ell0, noise), y)
     assert np.abs(yt.grad.numpy() - ybar).max() <= 1e-9 * np.abs(ybar).max()
+
+
+def test_gp_fused_second_backward(lx, dev):
+    """backward twice on one graph (retain_graph=True): K and a stay on the
+    device until the graph is released, so the second pass recomputes the
+    same gradient instead of reading freed buffers."""
+    import torch
+    from oracle import adjoint as adj
+    import linompss_amd.autograd as ag
+    n = 500
+    print(f"[gp second backward] n={n}", flush=True)
+    rng = np.random.default_rng(5)
+    P = rng.uniform(0, 1, (n, 3))
+    y = np.linalg.cholesky(adj.gp_kernel(P, 0.3, 1e-2)) @ rng.standard_normal(n)
+    ell = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    loss = ag.gp_nll_fused(torch.from_numpy(P), torch.from_numpy(y), ell, 1e-2)
+    loss.backward(retain_graph=True)
+    g1 = ell.grad.item()
+    ell.grad = None
+    loss.backward()
+    assert ell.grad.item() == g1

@@ -78,9 +78,59 @@ def test_dposv_ir(lx, orc, dev, n, nr, b, alpha):
     assert st.info == 0
     assert st.fallback == 0
     assert st.berr <= 1e-14
+    assert st.sweeps <= 4  # an FP32 factor at cond ~1e4 needs about two corrections
     assert _berr(A, X, B) <= 1e-14
     Xo, sto, info = orc.dposv_ir(A, B, b)
     assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("update_prec", [0, 2])
+def test_dposv_ir_zero_column(lx, orc, dev, update_prec):
+    """A zero right-hand side among several: x = 0 and r = 0 there, which is
+    an exact solution (backward error 0, not 0 / 0), so the loop converges on
+    the other columns without the FP64 fallback."""
+    n, b = 800, 128
+    print(f"[ir zero column] update_prec={update_prec}", flush=True)
+    A = _spd_gauss(n, 6, 1e-2)
+    B = np.asfortranarray(np.random.default_rng(7).uniform(-1, 1, (n, 4)))
+    B[:, 2] = 0.0
+    X, st = lx.dposv_ir(A, B, b, lx.IrOpts(30, 1e-14, 0.5, update_prec))
+    assert st.info == 0 and st.fallback == 0
+    assert np.isfinite(st.berr) and st.berr <= 1e-14
+    assert not X[:, 2].any()
+    assert _berr(A, X[:, [0, 1, 3]], B[:, [0, 1, 3]]) <= 1e-14
+
+
+def test_dposv_ir_many_rhs(lx, orc, dev):
+    """More than 16 right-hand sides (the sweeps take 16 at a time)."""
+    n, nr, b = 1100, 37, 256
+    print(f"[ir many rhs] n={n} nr={nr}", flush=True)
+    A = _spd_gauss(n, 8, 1e-3)
+    B = np.asfortranarray(np.random.default_rng(9).uniform(-1, 1, (n, nr)))
+    X, st = lx.dposv_ir(A, B, b, lx.IrOpts(30, 1e-14, 0.5, 2))
+    assert st.info == 0 and st.fallback == 0 and st.berr <= 1e-14
+    assert _berr(A, X, B) <= 1e-14
+    Xo, sto, info = orc.dposv_ir(A, B, b)
+    assert np.abs(X - Xo).max() / np.abs(Xo).max() <= 2.5 * (4 + 1e-3) / 1e-3 * 1e-14
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("nr", [8, 16])
+def test_sltrsm_multi_rhs(lx, orc, dev, trans, nr):
+    """FP32 sweeps with several right-hand sides (partial sums on the matrix
+    cores) against the oracle's blocked sweeps in FP64 on the same FP32
+    inputs, on a ragged order (1000 = 3 x 256 + 232)."""
+    m, bm = 1000, 256
+    print(f"[sltrsm {trans}] m={m} nr={nr}", flush=True)
+    rng = np.random.default_rng(11)
+    L = np.asfortranarray(np.tril(rng.uniform(0, 1, (m, m))) + 0.1 * m * np.eye(m)).astype(np.float32)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, nr))).astype(np.float32)
+    B = B0.copy(order="F")
+    lx.sltrsm(trans, L, B, bm, 1.5)
+    Bo = B0.astype(np.float64, order="F")
+    orc.ltrsm_lower(0 if trans == "N" else 1, np.asfortranarray(L.astype(np.float64)), Bo, bm, 1.5)
+    rel = np.abs(B - Bo).sum(axis=0).max() / np.abs(Bo).sum(axis=0).max()
+    assert rel <= 2e-5, rel
 
 
 def test_dposv_ir_fallback(lx, dev):

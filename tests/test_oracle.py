@@ -197,3 +197,88 @@ def test_dposv_ir_converges_and_falls_back(orc):
     X2, st2, info2 = orc.dposv_ir(A2, B, 64)
     assert st2.fallback == 1
     assert orc.backward_error(A2, X2, B) <= 1e-14
+
+
+def test_genmat_covariant_entries(orc):
+    """GENMAT_COVARIANT (genmat.c:263-302) with the itref app's p = k = 2
+    (genmat_config.c:37): exact entries (integer powers, square roots of
+    perfect squares), symmetric Toeplitz off the diagonal, and SPD with
+    lambda_min >= min diag - pi^2/6 (the Toeplitz part's symbol
+    2 sum cos(d t) / d^2 has minimum -pi^2/6 at t = pi)."""
+    n = 40
+    A = orc.genmat_covariant(n, 2.0, 2.0)
+    assert A[0, 0] == 2.0 and A[3, 3] == 3.0 and A[8, 8] == 4.0 and A[24, 24] == 6.0
+    assert A[0, 1] == A[1, 0] == 1.0 and A[0, 2] == 0.25 and A[7, 3] == 1.0 / 16 and A[2, 12] == 0.01
+    assert np.array_equal(A, A.T)
+    off = A - np.diag(np.diag(A))
+    assert all(np.all(np.diag(off, d) == np.diag(off, d)[0]) for d in range(1, n))
+    lam = np.linalg.eigvalsh(A)
+    assert lam.min() >= 2.0 - np.pi ** 2 / 6 - 1e-12
+    # p = 3, k = 3: cubes and cube roots of cubes
+    B = orc.genmat_covariant(30, 3.0, 3.0)
+    assert B[0, 2] == 0.125 and B[26, 26] == 4.0 and B[7, 7] == 3.0
+    assert orc.genmat_covariant(5, 0.0, 2.0) is None and orc.genmat_covariant(5, -1.0, 2.0) is None
+
+
+def test_genmat_costas_and_config_dispatch(orc):
+    """dgenmat_config (genmat_config.c:34-133): each ompssmat_t kind reaches
+    its generator (ompss_mat.h:5-14), with the default parameters {2, 2, 2, 2}
+    when no configuration file is read; DIAGDOM and ONES return NULL in the
+    reference.  GENMAT_COSTAS (genmat.c:199-220) maps each drand48 draw to +-1."""
+    m, n = 12, 5
+    r = orc.drand48_fill(4, m * n)
+    C = orc.genmat_costas(m, n, 4)
+    assert set(np.unique(C)) <= {-1.0, 1.0}
+    assert np.array_equal(C.ravel(order="F") > 0, r > 0.5)
+    assert np.array_equal(orc.dgenmat_config("costas", m, n, m, seed=4), C)
+    assert np.array_equal(orc.dgenmat_config("covar", 9, 9, 9), orc.genmat_covariant(9, 2.0, 2.0))
+    assert np.array_equal(orc.dgenmat_config("covar", 9, 9, 9, pars=[3.0, 3.0]), orc.genmat_covariant(9, 3.0, 3.0))
+    assert np.array_equal(orc.dgenmat_config("spd", 7, 7, 9, seed=3)[:7], orc.genmat_spd(7, 3, diag_mode=0, lda=9)[:7])
+    assert np.array_equal(orc.dgenmat_config("rand", 6, 4, 8, seed=5)[:6], orc.genmat(6, 4, 5))
+    assert np.array_equal(orc.dgenmat_config("eye", 6, 6, 6), np.eye(6))
+    assert not orc.dgenmat_config("zero", 6, 6, 7).any()
+    for k in ("diagdom", "ones", 9):
+        assert orc.dgenmat_config(k, 4, 4, 4) is None
+
+
+def test_philox_known_answers(orc):
+    """Philox4x32-10 against the known-answer vectors published with Random123
+    (Salmon et al., SC'11): counter, key -> output."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, out in kat:
+        assert orc.philox4x32_10(ctr, key) == out
+
+
+def test_philox_streams_distribution(orc):
+    """The benchmark's uniform and Gaussian streams: range, moments and a
+    Kolmogorov-Smirnov test against U[lo, hi) and N(0, 1); the two Gaussians
+    of one Box-Muller pair are uncorrelated."""
+    from scipy import stats
+    U = orc.gen_uniform(1000, 200, 7, -1.0, 1.0).ravel()
+    assert U.min() >= -1.0 and U.max() < 1.0
+    assert abs(U.mean()) < 4 * np.sqrt(1 / 3 / U.size)
+    assert stats.kstest(U, stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    G = orc.gen_gauss(1000, 200, 7).ravel(order="F")
+    assert abs(G.mean()) < 4 / np.sqrt(G.size) and abs(G.var() - 1) < 4 * np.sqrt(2 / G.size)
+    assert stats.kstest(G, "norm").pvalue > 1e-3
+    assert abs(np.corrcoef(G[0::2], G[1::2])[0, 1]) < 4 / np.sqrt(G.size / 2)
+    # another seed gives another stream; the same seed the same
+    assert not np.array_equal(orc.gen_gauss(10, 10, 8), orc.gen_gauss(10, 10, 7))
+    assert np.array_equal(orc.gen_gauss(10, 10, 7), orc.gen_gauss(10, 10, 7))
+
+
+def test_backward_error_zero_residual(orc):
+    """A zero residual is an exact solution: backward error 0, also for the
+    column b = 0, x = 0 (no 0 / 0), and the refinement loop stops at once."""
+    A = orc.genmat_spd(50, 1, diag_mode=1)
+    X = np.zeros((50, 2), order="F")
+    X[3, 1] = 1.0  # A x = column 3 of A, exact in any summation order
+    B = np.asfortranarray(A @ X)
+    assert orc.backward_error(A, X, B) == 0.0
+    B2 = np.asfortranarray(np.c_[np.zeros(50), np.random.default_rng(0).uniform(-1, 1, 50)])
+    Xo, st, info = orc.dposv_ir(A, B2, 16)
+    assert info == 0 and st.fallback == 0 and st.berr <= 1e-14
+    assert not Xo[:, 0].any()
