@@ -33,6 +33,7 @@ extern "C" {
 #define LINOMPSS_ERR_SCHED  (-1004)  /* a work unit waited past the scheduler timeout */
 #define LINOMPSS_ERR_PLAN   (-1005)  /* the planner rejected the shape */
 #define LINOMPSS_ERR_IO     (-1006)  /* a matrix file could not be opened, written or parsed */
+#define LINOMPSS_ERR_LIMIT  (-1007)  /* the size exceeds the fixed limit of a scratch-free entry point */
 
 /* ---- Drop-in entry points (host memory, synchronous) -------------------- */
 
@@ -221,6 +222,16 @@ int linompss_posv_dev(int dtype, int n, int nrhs, int b, void *dA, int64_t lda, 
                       void *stream, const linompss_exec_opts *opts);
 int linompss_potrs_dev(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
                        int64_t ldx, void *stream, const linompss_exec_opts *opts);
+/* linompss_potrs_dev that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same sweep kernels run
+ * on a workspace in a fixed device array of the library (under 24 MiB per
+ * device, held until the process ends), so n <= 32768 (LINOMPSS_ERR_LIMIT
+ * beyond), any nrhs, opts->fused_solve must be 0 (-10 otherwise).  Every call
+ * on a device shares that array: calls that overlap on one device race (the
+ * library serialises its own calls).  Results equal linompss_potrs_dev's. */
+int linompss_potrs_dev_noalloc(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
+                               int64_t ldx, void *stream, const linompss_exec_opts *opts);
 /* One triangular sweep on device memory: X := op(L)^-1 X, op = N (trans 0)
  * or T (trans 1), L lower (the device twin of linompss_dltrsm). */
 int linompss_trsm_dev(int dtype, int trans, int n, int nrhs, const void *dL, int64_t ldl, void *dX, int64_t ldx,

@@ -120,6 +120,7 @@ _sig("linompss_syrk_dev", _i, _i, _c, _c, _i, _i, _d, _p, _i64, _d, _p, _i64, _p
 _sig("linompss_trsm_blas_dev", _i, _i, _c, _c, _c, _c, _i, _i, _d, _p, _i64, _p, _i64, _p)
 _sig("linompss_posv_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_potrs_dev", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
+_sig("linompss_potrs_dev_noalloc", _i, _i, _i, _i, _i, _p, _i64, _p, _i64, _p, ctypes.POINTER(ExecOpts))
 _sig("linompss_dposv_ir_dev", _i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p, ctypes.POINTER(IrOpts),
      ctypes.POINTER(IrStatus))
 _sig("linompss_last_exec_stats", _i, ctypes.POINTER(ExecStats))
@@ -130,7 +131,8 @@ _sig("linompss_version", ctypes.c_char_p)
 ERR_BASE = -1000
 ERR_NAMES = {-1001: "no usable HIP device", -1002: "HIP call failed", -1003: "device allocation failed",
              -1004: "device scheduler timed out", -1005: "planner rejected the shape",
-             -1006: "matrix file could not be opened, written or parsed"}
+             -1006: "matrix file could not be opened, written or parsed",
+             -1007: "size beyond the fixed limit of a scratch-free entry point"}
 
 
 class LinompssError(RuntimeError):
@@ -492,6 +494,13 @@ def potrs_dev(L: DeviceMatrix, X: DeviceMatrix, b: int, opts: ExecOpts | None = 
     rc = lib.linompss_potrs_dev(_dt(L), L.cols, X.cols, b, L.ptr, L.ld, X.ptr, X.ld, _s(stream),
                                 ctypes.byref(opts) if opts is not None else None)
     _check("linompss_potrs_dev", rc, allow_info=False)
+
+
+def potrs_dev_noalloc(L: DeviceMatrix, X: DeviceMatrix, b: int, stream=None) -> None:
+    """potrs_dev on the library's fixed device array instead of allocated
+    scratch (n <= 32768): for callers that cannot allocate."""
+    rc = lib.linompss_potrs_dev_noalloc(_dt(L), L.cols, X.cols, b, L.ptr, L.ld, X.ptr, X.ld, _s(stream), None)
+    _check("linompss_potrs_dev_noalloc", rc, allow_info=False)
 
 
 def dposv_ir_dev(A: DeviceMatrix, B: DeviceMatrix, X: DeviceMatrix, b: int, opts: IrOpts | None = None,

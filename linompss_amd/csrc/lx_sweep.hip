@@ -665,4 +665,110 @@ int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *
     return sweeps<float>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, s);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch-free solve with an existing factor (linompss_potrs_dev_noalloc).
+//
+// For callers that cannot allocate device memory (code under stream capture,
+// serving loops, harnesses that forbid hipMalloc): the sweeps' workspace and
+// their unit list live in one file-scope device array, sized for n <= 32768
+// and either element type, that each device holds until the process ends.
+// Host code never passes the array's name to a kernel: on the first call on a
+// device the array's device address is looked up with hipGetSymbolAddress
+// (with that device current) and kept per device.  Every call on a device
+// shares the array and overwrites it (counters and mailbox are cleared by
+// fill kernels at each launch, the unit list is rebuilt by a one-thread
+// kernel at each call), so calls that overlap on one device race; within
+// this library they are serialised by the device lock.
+#define LX_NA_MAX_N 32768
+namespace {
+// partial sums of a sweep over nb blocks: sum over s = 4 .. nb-1 of
+// ceil((s - 3) / C), in closed form (q full groups of C, then r more)
+constexpr int lx_na_nparts(int nb)
+{
+    const int C = lx_sw_chunk(nb), m = nb - 1 - LX_SW_GAP;
+    if (m <= 0) return 0;
+    const int q = m / C, r = m % C;
+    return C * q * (q + 1) / 2 + r * (q + 1);
+}
+// bytes of the workspace plus the unit list (nb + nparts codes) and the
+// part offsets (nb + 1), for nb blocks
+constexpr size_t lx_na_bytes(int nb)
+{
+    const int np = lx_na_nparts(nb);
+    return lx_sw_layout(nb, np, sizeof(double)).bytes + ((size_t)(nb + np) + (size_t)(nb + 1)) * sizeof(int) + 256;
+}
+constexpr size_t lx_na_max_bytes()
+{
+    size_t m = 0;
+    for (int nb = 1; nb <= LX_NA_MAX_N / 64; ++nb) m = lx_na_bytes(nb) > m ? lx_na_bytes(nb) : m;
+    return m;
+}
+constexpr size_t LX_NA_BYTES = (lx_na_max_bytes() + 255) / 256 * 256;
+static_assert(LX_NA_BYTES <= (size_t)24 << 20, "the scratch-free solve keeps its array under 24 MiB");
+}  // namespace
+
+static __device__ __align__(256) char scratch[LX_NA_BYTES];
+
+// The unit list of lx_sw_build_units, written by one thread on the device.
+__global__ void lx_sw_units_kernel(int nb, int C, int *__restrict__ units, int *__restrict__ rowbase)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int acc = 0;
+    rowbase[0] = 0;
+    for (int s2 = 0; s2 < nb; ++s2) {
+        acc += lx_sw_nparts(s2, C);
+        rowbase[s2 + 1] = acc;
+    }
+    int u = 0;
+    for (int key = -1; key <= nb - 2; ++key) {
+        units[u++] = (key + 1) << 16;
+        if (key < 0) continue;
+        for (int c = 1;; ++c) {
+            const long long st = (long long)key + LX_SW_GAP + 1 + (long long)(c - 1) * C;
+            if (st >= nb) break;
+            units[u++] = (int)(st << 16) | c;
+        }
+    }
+}
+
+static void *lx_na_ptr[LX_SW_MAXDEV];
+
+extern "C" int linompss_potrs_dev_noalloc(int dtype, int n, int nrhs, int b, const void *dL, int64_t lda, void *dX,
+                                          int64_t ldx, void *stream, const linompss_exec_opts *opts)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (b <= 0) return -4;
+    if (lda < std::max(1, n)) return -6;
+    if (ldx < std::max(1, n)) return -8;
+    if (opts && opts->fused_solve) return -10;  // the fused form runs on the executor's own workspace
+    if (n > LX_NA_MAX_N) return LINOMPSS_ERR_LIMIT;
+    if (n == 0 || nrhs == 0) return 0;
+    std::lock_guard<std::mutex> g(lx_dev_mutex());
+    std::lock_guard<std::mutex> g2(lx_sw_mu);
+    int dev = 0;
+    const lx_sw_props *d = lx_sw_props_get(&dev);
+    if (!d) return LINOMPSS_ERR_NODEV;
+    if (!lx_na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LINOMPSS_ERR_HIP;
+        lx_na_ptr[dev] = p;
+    }
+    char *base = (char *)lx_na_ptr[dev];
+    const int nb = (n + 63) / 64, C = lx_sw_chunk(nb), np = lx_na_nparts(nb);
+    const size_t es = dtype == 0 ? sizeof(double) : sizeof(float);
+    const lx_sw_ws lay = lx_sw_layout(nb, np, es);
+    int *units = (int *)(base + lay.bytes);
+    int *rowbase = units + (nb + np);
+    if ((size_t)((char *)(rowbase + nb + 1) - base) > LX_NA_BYTES) return LINOMPSS_ERR_LIMIT;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(lx_sw_units_kernel, dim3(1), dim3(64), 0, s, nb, C, units, rowbase);
+    if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
+    int rc = dtype == 0 ? sweeps_launch<double>(d, n, nrhs, (const double *)dL, lda, (double *)dX, ldx, 3, s, units,
+                                                nb + np, rowbase, np, base)
+                        : sweeps_launch<float>(d, n, nrhs, (const float *)dL, lda, (float *)dX, ldx, 3, s, units,
+                                               nb + np, rowbase, np, base);
+    return rc;
+}
 #endif  // LX_SWEEP_KERNELS_ONLY
