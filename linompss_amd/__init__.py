@@ -68,7 +68,8 @@ class ExecOpts(ctypes.Structure):
 
 
 class ExecStats(ctypes.Structure):
-    _fields_ = [("exec_ms", _d), ("units", _i), ("tasks", _i), ("grid", _i), ("sim_ms", _d)]
+    _fields_ = [("exec_ms", _d), ("units", _i), ("tasks", _i), ("grid", _i), ("sim_ms", _d), ("xcd_lists", _i),
+                ("reserved", _i)]
 
 
 class IrOpts(ctypes.Structure):

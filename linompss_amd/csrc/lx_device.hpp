@@ -749,6 +749,147 @@ __device__ __forceinline__ void lx_unit_gemm_bx6w(float *__restrict__ C, long lo
 }
 
 // ---------------------------------------------------------------------------
+// The split-FP16 trailing update from pre-split operands (lx_sys::planes):
+// C[M x N] -= A[M x K] B[N x K]^T with M <= 128, N <= 256, K a multiple of 16
+// and the operands' columns starting on a 16-column slab boundary.  The hi and
+// lo FP16 planes of 2^12 x were written once, by the TRSM unit that produced
+// x (lx_trsm_write_planes), in slabs of 16 columns:
+//   plane p, slab g, row r -> halves [((p nsig + g) ldp + r) 16, + 16)
+// so 32 rows of one slab of one plane are a contiguous KB whose 64 chunks of
+// 16 bytes are, in order, the v_mfma_f32_32x32x16_f16 operand of lanes 0..63
+// (lane l: row l % 32, columns 8 (l / 32) .. + 7).  A slab of the block's
+// operands (A: 4 tiles of 32 rows, B: 8, two planes each: 24 blocks of 1 KB)
+// is staged into LDS in that order by 16-byte loads, six per thread, and each
+// wave reads an operand tile back with one 16-byte LDS read per lane at 16 l:
+// no split and no conversion in the loop, and no LDS bank conflicts.  Two LDS
+// buffers and two register sets: slab s is multiplied from one buffer while
+// slab s+1 is stored to the other and slabs s+2 and s+3 are in flight.  The
+// products (lo hi, hi lo, hi hi), their order and the FP32 accumulation are
+// those of lx_unit_gemm_bx6w<true>, from the same FP32 values of L, so the
+// result is the same bit for bit.
+#define LX_P16_BLK 24
+struct lx_p16_smem {
+    static constexpr int bytes = 2 * LX_P16_BLK * 1024;
+};
+
+__device__ __forceinline__ void lx_unit_gemm_p16(float *__restrict__ C, long long ldc, const _Float16 *__restrict__ P,
+                                                 long long ldp, int nsig, int arow, int brow, int g0, int M, int N,
+                                                 int K, bool lower, int gdiag, char *smem)
+{
+    typedef lx_split<true> S;
+    typedef S::v V;
+    const int tid = lx_tid(), lane = tid & 63, w = __builtin_amdgcn_readfirstlane(lx_wave());
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 128;
+    const bool active = wr < M && wc < N && !(lower && wr + 63 + gdiag < wc);
+    const int nslab = K / 16;
+    // chunk m of this thread: operand block 4 m + w (blocks 2 t + p: A tile t
+    // of 4, then B tile t of 8, plane p), lane `lane` of it.  The block's
+    // address is wave-uniform (scalar registers); the lane's offset inside it,
+    // row lane % 32 and half lane / 32, is the same for every block
+    const long long sstride = ldp * 16;
+    const int loff = (lane & 31) * 16 + 8 * (lane >> 5);
+    auto blk_base = [&](int m) {
+        const int blk = 4 * m + w, isb = blk >= 8, t = (isb ? blk - 8 : blk) >> 1, p = blk & 1;
+        return P + ((long long)(p * nsig + g0) * ldp + (isb ? brow : arow) + 32 * t) * 16;
+    };
+    V xs[2][6];
+    auto gload = [&](int set, int sl) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) xs[set][m] = *(const V *)(blk_base(m) + sl * sstride + loff);
+    };
+    auto lstore = [&](int buf, int set) {
+        char *b = smem + buf * LX_P16_BLK * 1024 + 16 * lane;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) *(V *)(b + (4 * m + w) * 1024) = xs[set][m];
+    };
+    lx_f16v acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+    auto compute = [&](int buf) {
+        if (!active) return;
+        const char *b0 = smem + buf * LX_P16_BLK * 1024 + 16 * lane;
+        V a[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) a[i][p] = *(const V *)(b0 + (2 * (wr / 32 + i) + p) * 1024);
+        // two column tiles at a time: four independent accumulators between
+        // consecutive MFMAs on one of them
+#pragma unroll
+        for (int jp = 0; jp < 4; jp += 2) {
+            V b[2][2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) b[j][p] = *(const V *)(b0 + (8 + 2 * (wc / 32 + jp + j) + p) * 1024);
+#pragma unroll
+            for (int q = 0; q < S::nq; ++q)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][jp + j] = S::mma(a[i][S::pa(q)], b[j][S::pb(q)], acc[i][jp + j]);
+        }
+    };
+    gload(0, 0);
+    if (nslab > 1) gload(1, 1);
+    // the previous block of the same unit may still be reading the buffers
+    __syncthreads();
+    lstore(0, 0);
+    __syncthreads();
+    if (nslab > 2) gload(0, 2);
+    // on entry: buffer 0 holds slab sl, set 1 slab sl+1, set 0 slab sl+2
+    for (int sl = 0; sl < nslab; sl += 2) {
+        compute(0);
+        if (sl + 1 < nslab) {
+            lstore(1, 1);
+            __syncthreads();  // slab sl+1 visible; every wave is past compute(0)
+            if (sl + 3 < nslab) gload(1, sl + 3);
+            compute(1);
+            if (sl + 2 < nslab) {
+                lstore(0, 0);
+                __syncthreads();  // slab sl+2 visible; every wave is past compute(1)
+                if (sl + 4 < nslab) gload(0, sl + 4);
+            }
+        }
+    }
+    if (active) {
+        // C -= acc, two tiles at a time, their loads before their stores
+        const int lane2 = lx_lane(), lr2 = lane2 & 31, lk2 = lane2 >> 5;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jp = 0; jp < 4; jp += 2) {
+                float v[2][16];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int c = wc + (jp + j) * 32 + lr2, cc = c < N ? c : N - 1;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4), rr = r < M ? r : M - 1;
+                        v[j][e] = C[(long long)cc * ldc + rr];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int c = wc + (jp + j) * 32 + lr2;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int r = wr + i * 32 + 8 * (e / 4) + 4 * lk2 + (e % 4);
+                        if (r < M && c < N && (!lower || r + gdiag >= c))
+                            C[(long long)c * ldc + r] = v[j][e] - S::unscale(acc[i][jp + j][e]);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Wave-level C_acc += A * B^T with A from global memory (rows r0.. of a
 // column-major panel, clamped past M) and B^T from LDS: element (k, c) of B^T
 // is sB[c * ldsb + k] (row c of a row-major 64 x K block).  16 rows x 16*FN
@@ -1075,10 +1216,14 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
 // Every wave runs every block (waves past R compute clamped rows and store
 // nothing), so the barriers per block match in all four.
 // LDS: 64 * 65 * sizeof(T) + max(4 * 16 * 65 * sizeof(T), lx_trsm_upd_smem<T>::bytes).
+// P (FP32 factor with split-FP16 updates, else null): the plane buffer of
+// lx_unit_gemm_p16; the unit then also writes hi and lo of 2^12 X there,
+// X being rows grow.. and columns gcol.. of L (gcol a multiple of 16).
 template <typename T>
 __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long ldb, const T *__restrict__ L,
                                                   long long ldl, const T *__restrict__ W, long long ldw, int R, int N,
-                                                  char *smem)
+                                                  char *smem, _Float16 *__restrict__ P = nullptr, long long ldp = 0,
+                                                  int nsig = 0, int grow = 0, int gcol = 0)
 {
     constexpr int QB = 64, LD = 65;
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
@@ -1143,6 +1288,34 @@ __device__ __forceinline__ void lx_unit_trsm_rltn(T *__restrict__ Bm, long long 
                     const int r = r0 + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
                     if (r < R && c < qb) Bm[(long long)(q + c) * ldb + r] = x[0][j][e];
                 }
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (P != nullptr) {  // uniform
+                // this wave's 16 rows of X_q, row-major through its own sY, then
+                // 8 columns of one row per lane and chunk: split into hi and lo
+                // of 2^12 x (as lx_unit_gemm_bx6w<true> splits the same values)
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) sY[lx_mfma<T>::row(lk, e)][j * 16 + lr] = x[0][j][e];
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int ch = lane + 64 * h, rl = ch >> 3, c = (ch & 7) * 8;
+                    if (active && r0 + rl < R && c + 8 <= qb) {
+                        float xv[8];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) xv[e] = (float)sY[rl][c + e];
+                        lx_split<true>::v pl[3];
+                        lx_split<true>::split(xv, pl);
+                        const int gc = gcol + q + c;
+                        _Float16 *dst = P + ((long long)(gc >> 4) * ldp + grow + r0 + rl) * 16 + 8 * ((gc >> 3) & 1);
+                        *(lx_split<true>::v *)dst = pl[0];
+                        *(lx_split<true>::v *)(dst + (long long)nsig * ldp * 16) = pl[1];
+                    }
+                }
+            }
         }
         __syncthreads();  // sZq and sY are rewritten by the next block
     }

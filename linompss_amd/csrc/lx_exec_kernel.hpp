@@ -26,7 +26,8 @@ template <typename T, int FAM> struct lx_fam_smem {
     // for the staged GEMM), TRSM (Z_q, then its update slabs or Y), staged GEMM
     static constexpr int potrf = 2 * 64 * 65 * (int)sizeof(T) + 16;
     static constexpr int gemm0 = lx_lgemm_smem<T>::bytes > lx_bx_smem::bytes ? lx_lgemm_smem<T>::bytes : lx_bx_smem::bytes;
-    static constexpr int gemm = gemm0 > lx_bxw_smem::bytes ? gemm0 : lx_bxw_smem::bytes;
+    static constexpr int gemm1 = gemm0 > lx_bxw_smem::bytes ? gemm0 : lx_bxw_smem::bytes;
+    static constexpr int gemm = gemm1 > lx_p16_smem::bytes ? gemm1 : lx_p16_smem::bytes;
     static constexpr int trsm = 64 * 65 * (int)sizeof(T) + (4 * 16 * 65 * (int)sizeof(T) > lx_trsm_upd_smem<T>::bytes
                                                                  ? 4 * 16 * 65 * (int)sizeof(T)
                                                                  : lx_trsm_upd_smem<T>::bytes);
@@ -68,6 +69,46 @@ __device__ __forceinline__ void lx_split_blocks(const lx_unit &u, float *C, long
 __device__ __forceinline__ long long lx_off(int64_t rcp, long long ld)
 {
     return (long long)(rcp >> 32) * ld + (long long)(rcp & 0xffffffffll);
+}
+
+// The trailing update of rows r0 .. r0 + M - 1 of a GEMM / SYRK unit (C, A
+// and the unit's global offsets advanced by r0; M <= 128).
+template <typename T, bool HY>
+__device__ __forceinline__ void lx_run_update_rows(const lx_unit &u0, const lx_sys &s, T *C, long long ldc,
+                                                   const T *Ap, long long lda, const T *Bp, long long ldb, int r0,
+                                                   int M, char *smem)
+{
+    lx_unit u = u0;
+    u.M = M;
+    u.aux1 += r0;
+    u.a += r0;
+    const bool lower = (u.aux0 & 1) != 0;
+    if constexpr (sizeof(T) == 4) {
+        // split-FP16 units whose operands are whole 16-column slabs of
+        // one panel read the planes the TRSM units wrote
+        const int ca = (int)(u.a >> 32);
+        if (s.planes && (u.aux0 & 10) == 10 && u.K % 16 == 0 && (ca & 15) == 0 && ca == (int)(u.b >> 32)) {
+            const int ra = (int)(u.a & 0xffffffffll), rb = (int)(u.b & 0xffffffffll);
+            for (int c0 = 0; c0 < u.N; c0 += 2 * LX_UNIT_N) {  // uniform
+                const int nc = u.N - c0 < 2 * LX_UNIT_N ? u.N - c0 : 2 * LX_UNIT_N;
+                lx_unit_gemm_p16(C + (long long)c0 * ldc, ldc, (const _Float16 *)s.planes, s.ldp, s.nsig, ra,
+                                 rb + c0, ca >> 4, u.M, nc, u.K, lower, u.aux1 - c0, smem);
+            }
+            return;
+        }
+        if (u.aux0 & 2) {
+            if (u.aux0 & 8)
+                lx_split_blocks<true>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
+            else
+                lx_split_blocks<false>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
+            return;
+        }
+    }
+    for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N) {  // uniform
+        const int nc = u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N;
+        lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
+                            u.aux1 - c0, smem);
+    }
 }
 
 template <typename T, int FAM, bool HY = false>
@@ -115,33 +156,25 @@ __device__ __forceinline__ void lx_run_unit(const lx_unit &u, const lx_sys *__re
         break;
     }
     case LX_OP_TRSM_RLTN:
-        // the diagonal tile's inverse blocks start at W row = its row in A
+        // the diagonal tile's inverse blocks start at W row = its row in A; a
+        // unit of more than 64 rows runs as 64-row passes
         if constexpr (F)
-            lx_unit_trsm_rltn<T>(C, ldc, Ap, lda, (const T *)s.W + (u.a & 0xffffffffll), s.ldw, u.M, u.N, smem);
+            for (int r0 = 0; r0 < u.M; r0 += 64)  // uniform
+                lx_unit_trsm_rltn<T>(C + r0, ldc, Ap, lda, (const T *)s.W + (u.a & 0xffffffffll), s.ldw,
+                                     u.M - r0 < 64 ? u.M - r0 : 64, u.N, smem, (_Float16 *)s.planes, s.ldp, s.nsig,
+                                     (int)(u.c & 0xffffffffll) + r0, (int)(u.c >> 32));
         break;
     case LX_OP_SYRK:
     case LX_OP_GEMM_NT:
-        // a unit covers up to M = 128 rows and N = 128 * m columns, done as m
+        // a unit covers up to 128 m' rows and N = 128 m columns, done as m' x m
         // staged 128 x 128 blocks (aux1 = global row - global col of C(0, 0));
         // aux0 bit 0: lower mask (SYRK), bit 1: split matrix-core bodies (FP32
         // only), bit 2: split bodies in 128 x 256 blocks while more than 128
         // columns remain, bit 3: the FP16 split instead of the BF16 one
         if constexpr (F) {
-            const bool lower = (u.aux0 & 1) != 0;
-            if constexpr (sizeof(T) == 4) {
-                if (u.aux0 & 2) {
-                    if (u.aux0 & 8)
-                        lx_split_blocks<true>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
-                    else
-                        lx_split_blocks<false>(u, C, ldc, Ap, lda, Bp, ldb, lower, smem);
-                    break;
-                }
-            }
-            for (int c0 = 0; c0 < u.N; c0 += LX_UNIT_N) {  // uniform
-                const int nc = u.N - c0 < LX_UNIT_N ? u.N - c0 : LX_UNIT_N;
-                lx_unit_gemm_lds<T>(C + (long long)c0 * ldc, ldc, Ap, lda, Bp + c0, ldb, u.M, nc, u.K, lower,
-                                    u.aux1 - c0, smem);
-            }
+            for (int r0 = 0; r0 < u.M; r0 += LX_UNIT_M)  // uniform
+                lx_run_update_rows<T, HY>(u, s, C + r0, ldc, Ap + r0, lda, Bp, ldb, r0,
+                                          u.M - r0 < LX_UNIT_M ? u.M - r0 : LX_UNIT_M, smem);
         }
         break;
     case LX_OP_TRSV:
@@ -188,16 +221,50 @@ __device__ __forceinline__ void lx_wg_publish(int *counter)
     }
 }
 
+// Publishes a unit whose output is read next only through this XCD's L2
+// (LX_AUX_LOCAL): every wave waits for its stores to reach the L2, then the
+// counter is bumped, with no L2 write-back.
+__device__ __forceinline__ void lx_wg_publish_local(int *counter)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (lx_wave() == 0)
+        __hip_atomic_fetch_add(counter, (threadIdx.x & 63) == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Acquire for a unit of an XCD's own list: invalidate this CU's vector L1
+// only.  The L2 needs no invalidation: the unit reads tiles its XCD owns
+// (their every earlier version went through this L2) and panels other XCDs
+// produced, written back to memory when they were published and never read
+// by this XCD before (a tile is read off its XCD only once it is final).
+__device__ __forceinline__ void lx_acquire_l1()
+{
+#if defined(__AMDGCN__)
+    __asm__ volatile("buffer_inv sc0" ::: "memory");
+#else
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the CPU emulation's fence
+#endif
+}
+
+// The XCD this workgroup runs on (hardware register XCC_ID, bits 0..3).
+__device__ __forceinline__ int lx_xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)); }
+
 // Optional profile (prof != null): per workgroup, wave 0 accumulates
 // wall-clock ticks spent waiting for dependencies and running each op type,
 // written at exit as LX_PROF_WORDS int64 words per workgroup:
 //   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
 #define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
 
+// lists (null: one shared list in units[0..nunits)): lists[0] = nx, then
+// nx + 2 offsets; list q < nx = units [lists[1 + q], lists[2 + q]) for the
+// workgroups of XCD q, list nx = the shared list, taken once a workgroup's own
+// list is empty.  heads[q] = next index of list q (zero on entry).
 template <typename T, int FAM, bool HY = false>
 __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
-                   lx_status *st, int *info, long long spin_ticks, long long *prof)
+                   lx_status *st, int *info, long long spin_ticks, long long *prof, const int *__restrict__ lists,
+                   int *heads)
 {
     __shared__ __attribute__((aligned(16))) char smem[lx_fam_smem<T, FAM>::bytes];
     __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
@@ -205,11 +272,30 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
     __shared__ long long s_prof[LX_PROF_WORDS];  // in LDS: keeps the profile out of the registers
     if (threadIdx.x < LX_PROF_WORDS) s_prof[threadIdx.x] = 0;  // ordered by the loop's first barrier
     long long t_grab = 0, t_ready = 0;
+    const int nx = lists ? lx_uniform(lists[0]) : 0;
+    int cur = nx > 0 ? lx_uniform(lx_xcc_id() % nx) : 0;  // the list being taken (nx: the shared one)
     for (;;) {
         if (prof) t_grab = wall_clock64();
         if (lx_wave() == 0) {  // wave-uniform branch
             // one atomic per wave: lane 0 adds 1, the others add 0; lane 0's old value is the index
-            const int idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
+            int idx;
+            if (nx == 0) {
+                idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
+            } else {
+                for (;;) {  // uniform: this XCD's list, then the shared one
+                    const int i = lx_uniform(atomicAdd(&heads[cur], lane == 0 ? 1 : 0));
+                    const int lo = lx_uniform(lists[1 + cur]), hi = lx_uniform(lists[2 + cur]);
+                    if (lo + i < hi) {
+                        idx = lo + i;
+                        break;
+                    }
+                    if (cur == nx) {
+                        idx = nunits;
+                        break;
+                    }
+                    cur = nx;
+                }
+            }
             int go = idx < nunits ? 1 : 0;
             if (go && lx_uniform(__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
                 go = 0;
@@ -237,7 +323,10 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
                         __builtin_amdgcn_s_sleep(2);
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (cur < nx)  // uniform: a unit of this XCD's own list
+                    lx_acquire_l1();
+                else
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             // every lane of wave 0 stores the same (uniform) values: no lane branch
             s_sched[0] = go;
@@ -253,7 +342,10 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
         if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
         lx_run_unit<T, FAM, HY>(u, sys, info, smem);
-        lx_wg_publish(&ctr[u.ctr]);
+        if (u.aux0 & LX_AUX_LOCAL)  // uniform
+            lx_wg_publish_local(&ctr[u.ctr]);
+        else
+            lx_wg_publish(&ctr[u.ctr]);
         if (lx_wave() == 0) atomicAdd(&st->done, lane == 0 ? 1 : 0);
         if (prof && threadIdx.x == 0 && u.op >= 0 && u.op < LX_OP_NOP) {  // no barrier inside
             const long long t_done = wall_clock64();

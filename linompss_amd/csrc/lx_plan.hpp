@@ -31,6 +31,12 @@ enum lx_base : int32_t { LX_BASE_A = 0, LX_BASE_X = 1 };
 
 #define LX_MAX_DEPS 3
 
+// aux0 flag (factor plans with per-XCD unit lists): the unit's output is read
+// only through its own XCD's L2 until a unit of the same XCD publishes with a
+// write-back of that L2, so it publishes with its stores in the L2 and no
+// write-back of its own (see lx_planner::split_xcd).
+#define LX_AUX_LOCAL (1 << 16)
+
 struct lx_unit {
     int32_t op;
     int32_t sys;           // system index (batched plans hold many systems)
@@ -60,6 +66,15 @@ struct lx_sys {
                            // tile tiles[(c / tb) * tmt + r / tb] at (c % tb) * tb
                            // + r % tb; null for a column-major A (lda)
     int32_t tmt, tb;
+    void *planes;          // FP32 factor with split-FP16 updates (else null): the
+                           // FP16 planes hi, lo of 2^12 L, written once by the
+                           // unit that produces each element of L (TRSM units) and
+                           // read by the trailing GEMM / SYRK units.  Layout
+                           // [plane][slab c/16][row][16]: the 16 columns of a slab
+                           // of one row are 32 contiguous bytes, so 32 rows of a
+                           // slab (one MFMA operand tile) are 1 KB contiguous
+    int64_t ldp;           // rows per slab (>= n + 256, a multiple of 32)
+    int32_t nsig, pad2_;   // slabs per plane (ceil(n / 16))
 };
 
 // Device-side status words (one per launch).

@@ -2,6 +2,8 @@
 #include "lx_planner.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <queue>
 #include <stdexcept>
 
@@ -220,16 +222,19 @@ void lx_planner::factor_rl(int s)
             const int ibs = std::min(b, n - i * b);
             int outc = tile_ctr(s, i, k), in[1] = {tile_ctr(s, k, k)};
             int t = new_task(LX_T_TRSM, s, i, k, k, PRIO_HI, outc, in, 1);
-            for (int r0 = 0; r0 < ibs; r0 += P.trsm_rows) {
+            // the lookahead tile's panel solve feeds the next diagonal factor:
+            // short units; elsewhere taller ones (fewer publications)
+            const int tr = i == k + 1 ? P.trsm_rows : std::max(P.trsm_rows, P.trsm_rows_wide);
+            for (int r0 = 0; r0 < ibs; r0 += tr) {
                 lx_unit u = proto_unit(LX_OP_TRSM_RLTN, s);
-                u.M = std::min(P.trsm_rows, ibs - r0);
+                u.M = std::min(tr, ibs - r0);
                 u.N = bs;
                 u.ctr = outc;
                 u.c = rc(i * b + r0, k * b);
                 u.a = rc(k * b, k * b);
                 add_unit(t, u);
             }
-            Tk[t].unit_cost = (double)std::min(P.trsm_rows, ibs) * bs * bs / P.trsm_rate + P.unit_overhead;
+            Tk[t].unit_cost = (double)std::min(tr, ibs) * bs * bs / P.trsm_rate + P.unit_overhead;
             issued[outc] += Tk[t].nunits;
             last_writer[outc] = t;
         }
@@ -244,12 +249,27 @@ void lx_planner::factor_rl(int s)
                 // (and in the last wide_tail panels, whose trailing matrix no longer
                 // gives every workgroup a wide unit)
                 const int tail = P.wide_tail < 0 ? mt / 2 : P.wide_tail;
-                const int un = (j == k + 1 || mt - k <= tail) ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
+                const bool fine = j == k + 1 || mt - k <= tail;
+                const int un = fine ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
+                // each unit's publication writes its XCD's L2 back (measured at
+                // about 100 us of unit time when units are 128 rows), so units
+                // are as tall as the step's parallelism allows: the tallest
+                // height (gemm_wide_m, halved down to unit_m) that still leaves
+                // wide_units units per workgroup among the step's trailing tiles
+                int um = P.unit_m;
+                if (!fine) {
+                    const double tiles = 0.5 * (mt - k - 1) * (mt - k - 2);
+                    um = std::max(P.unit_m, P.gemm_wide_m);
+                    while (um > P.unit_m &&
+                           tiles * ((b + um - 1) / um) * ((b + un - 1) / un) < P.wide_units * P.workers)
+                        um /= 2;
+                    um = std::max(um, P.unit_m) / P.unit_m * P.unit_m;
+                }
                 for (int c0 = 0; c0 < jbs; c0 += un)
-                    for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
+                    for (int r0 = 0; r0 < ibs; r0 += um) {
                         lx_unit u = proto_unit(LX_OP_GEMM_NT, s);
                         u.aux0 = upd_flag();
-                        u.M = std::min(P.unit_m, ibs - r0);
+                        u.M = std::min(um, ibs - r0);
                         u.N = std::min(un, jbs - c0);
                         u.K = bs;
                         u.ctr = outc;
@@ -258,7 +278,7 @@ void lx_planner::factor_rl(int s)
                         u.b = rc(j * b + c0, k * b);
                         add_unit(t, u);
                     }
-                Tk[t].unit_cost = 2.0 * P.unit_m * un * bs / P.flop_rate + P.unit_overhead;
+                Tk[t].unit_cost = 2.0 * um * un * bs / P.flop_rate + P.unit_overhead;
                 issued[outc] += Tk[t].nunits;
                 last_writer[outc] = t;
             }
@@ -537,9 +557,86 @@ void lx_planner::finalize()
     }
     if (order.size() != U.size()) throw std::logic_error("plan simulation did not schedule every unit");
     sim_time = now;
+    if (P.xcd > 0) {
+        split_xcd(order);
+        return;
+    }
     std::vector<lx_unit> ordered;
     ordered.reserve(U.size());
     for (int32_t q : order) ordered.push_back(U[q]);
+    U.swap(ordered);
+}
+
+// Per-XCD lists.  A GEMM / SYRK unit's release costs a write-back of its
+// XCD's whole L2 (about 15 % of the FP32 factor).  It is needed only for data
+// another XCD reads next.  So every unit of tile (i, j) -- its trailing
+// updates, its diagonal factorisation or panel solve -- runs on the XCD that
+// owns the tile, (i + j) % xcd, which spreads each step's panel solves over
+// all XCDs; the units of the last xcd_tail steps, where the trailing matrix no
+// longer gives every XCD enough tiles, go to a shared list that workgroups
+// take from once their own list is empty.  Each list is the simulated order
+// restricted to its units, so it is topological; units before the tail
+// depend only on units before the tail, and every XCD runs resident
+// workgroups, so the launch cannot deadlock (the minimal unfinished unit of
+// the lists always has a workgroup that will take it).
+//
+// A unit may publish without the write-back (LX_AUX_LOCAL) when every unit
+// on another list that waits on its counter waits at least for all the bumps
+// of the counter's last task on this list: that task's units run on the same
+// XCD after this unit (they wait for it, write-after-write) and each writes
+// the whole L2 back when it publishes, so what this unit stored is in memory
+// before any other XCD can read it.  Those last tasks (a tile's panel solve
+// or diagonal factorisation, whose results every XCD reads, or its last
+// update before the shared tail) publish with the write-back.
+void lx_planner::split_xcd(const std::vector<int32_t> &order)
+{
+    const int nx = P.xcd, nU = (int)U.size(), G = nx;
+    std::vector<int32_t> ulist(nU, G);
+    for (int u = 0; u < nU; ++u) {
+        const lx_unit &x = U[u];
+        const sysinfo &sy = S[x.sys];
+        const int ci = (int)((x.c & 0xffffffffll) / sy.b), cj = (int)((x.c >> 32) / sy.b);
+        const int64_t sc = (x.op == LX_OP_GEMM_NT || x.op == LX_OP_SYRK) ? (x.a >> 32) : (x.c >> 32);
+        const int step = (int)(sc / sy.b);
+        const bool factor_op = x.op == LX_OP_POTRF || x.op == LX_OP_TRSM_RLTN || x.op == LX_OP_SYRK ||
+                               x.op == LX_OP_GEMM_NT;
+        if (factor_op && step < sy.mt - P.xcd_tail) ulist[u] = (ci + cj) % nx;
+    }
+    // per counter: the list of its bumpers before the tail (-1 none, -2 several),
+    // its last task there, and the bumps through that task
+    std::vector<int32_t> cl(nctr, -1), lastL(nctr, -1), lastLb(nctr, 0), bumps(nctr, 0);
+    for (int u = 0; u < nU; ++u) {
+        const int c = U[u].ctr;
+        bumps[c] += 1;
+        if (ulist[u] == G) continue;
+        cl[c] = cl[c] == -1 ? ulist[u] : (cl[c] == ulist[u] ? cl[c] : -2);
+        if (unit_task[u] >= lastL[c]) {
+            lastL[c] = unit_task[u];
+            lastLb[c] = bumps[c];
+        }
+    }
+    // the smallest threshold a unit on another list waits for
+    std::vector<int32_t> xmin(nctr, INT32_MAX);
+    for (int u = 0; u < nU; ++u)
+        for (int d = 0; d < U[u].nd; ++d) {
+            const int c = U[u].dc[d];
+            if (ulist[u] != cl[c]) xmin[c] = std::min(xmin[c], U[u].dt[d]);
+        }
+    for (int u = 0; u < nU; ++u) {
+        const int c = U[u].ctr;
+        bool local = ulist[u] != G && cl[c] >= 0;
+        if (local && xmin[c] != INT32_MAX) local = xmin[c] >= lastLb[c] && unit_task[u] != lastL[c];
+        if (local) U[u].aux0 |= LX_AUX_LOCAL;
+    }
+    std::vector<lx_unit> ordered;
+    ordered.reserve(nU);
+    list_off.assign(nx + 2, 0);
+    for (int q = 0; q <= nx; ++q) {
+        list_off[q] = (int32_t)ordered.size();
+        for (int32_t u : order)
+            if (ulist[u] == q) ordered.push_back(U[u]);
+    }
+    list_off[nx + 1] = (int32_t)ordered.size();
     U.swap(ordered);
 }
 

@@ -28,9 +28,26 @@ lx_emu_barrier lx_emu_wave_barrier[4];
 lx_emu_wave_buf lx_emu_wbuf[4];
 lx_emu_bf_buf lx_emu_bfbuf[4];
 static int g_update_prec = 0;
+static int g_planes = 0;
+static int g_wide_m = 128, g_wide_n = 512, g_trsm_wide = 64, g_workers = 512;
+
+// unit shapes of the next lx_emu_run plans: GEMM units up to wide_m x wide_n
+// off the lookahead column, panel solves of trsm_wide rows off the lookahead
+// tile, the workgroup count the unit heights are chosen for
+extern "C" void lx_emu_set_units(int wide_m, int wide_n, int trsm_wide, int workers)
+{
+    g_wide_m = wide_m;
+    g_wide_n = wide_n;
+    g_trsm_wide = trsm_wide;
+    g_workers = workers;
+}
 
 // trailing-update precision of the next lx_emu_run plans (0 FP32, 1 split BF16)
 extern "C" void lx_emu_set_update_prec(int u) { g_update_prec = u; }
+// 1: FP32 factors with split-FP16 updates get the plane buffer (TRSM units
+// write the FP16 planes, eligible GEMM / SYRK units read them), as the
+// library sets it up
+extern "C" void lx_emu_set_planes(int on) { g_planes = on; }
 
 extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, long long lda, void *X, long long ldx,
                           int policy, int *info_out, int *done_out, int *nunits_out)
@@ -40,6 +57,10 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     P.unit_n = LX_UNIT_N;
     P.policy = policy;
     P.update_prec = dtype == 1 ? g_update_prec : 0;
+    P.gemm_wide_m = g_wide_m;
+    P.gemm_wide_n = g_wide_n;
+    P.trsm_rows_wide = g_trsm_wide;
+    P.workers = g_workers;
     lx_planner pl(P);
     int s = pl.add_system(n, b, nrhs);
     switch (what) {
@@ -66,6 +87,13 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     sys.ldw = (int64_t)n + 64;
     std::vector<double> wbuf((size_t)sys.ldw * 64);  // large enough for either precision
     sys.W = wbuf.data();
+    std::vector<_Float16> planes;
+    if (g_planes && dtype == 1 && P.update_prec == 2 && b % 16 == 0 && what <= 1) {
+        sys.ldp = ((int64_t)n + 256 + 31) / 32 * 32;
+        sys.nsig = (n + 15) / 16;
+        planes.assign((size_t)2 * sys.nsig * sys.ldp * 16, (_Float16)0.0f);
+        sys.planes = planes.data();
+    }
     blockDim.x = 256;
     gridDim.x = 1;
     lx_emu_wg_barrier.init(256, "workgroup");
@@ -78,9 +106,9 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
             threadIdx.x = t;
             blockIdx.x = 0;
             if (dtype == 0)
-                lx_exec_kernel<double, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
+                lx_exec_kernel<double, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr, nullptr, nullptr);
             else
-                lx_exec_kernel<float, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
+                lx_exec_kernel<float, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr, nullptr, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
@@ -105,6 +133,50 @@ extern "C" int lx_emu_plan_in_tiles(int variant, int n, int b, int update_prec, 
     else pl.factor_ll(s);
     pl.finalize();
     return lx_units_in_tiles(pl.units(), b) ? 1 : 0;
+}
+
+// A factor plan split into per-XCD lists (what 0 RL, 1 LL): per unit, in
+// list order, {list, local, op, ctr, nd, dc[3], dt[3], c row, c col, a col};
+// offs = the nx + 2 list offsets.  Returns the unit count (or -1 when max is
+// too small).
+extern "C" int lx_emu_plan_xcd(int what, int n, int b, int dtype, int update_prec, int nx, int tail, int max,
+                               int *out, int *offs)
+{
+    lx_plan_params P;
+    P.unit_m = LX_UNIT_M;
+    P.unit_n = LX_UNIT_N;
+    P.elem_bytes = dtype == 0 ? 8 : 4;
+    P.update_prec = update_prec;
+    if (dtype == 0) P.potrf_sub = 256;
+    P.xcd = nx;
+    P.xcd_tail = tail;
+    lx_planner pl(P);
+    int s = pl.add_system(n, b, 0);
+    if (what == 0) pl.factor_rl(s);
+    else pl.factor_ll(s);
+    pl.finalize();
+    const std::vector<lx_unit> &U = pl.units();
+    if ((int)U.size() > max) return -1;
+    const std::vector<int32_t> &lo = pl.list_offsets();
+    for (int q = 0; q < nx + 2; ++q) offs[q] = lo[q];
+    for (size_t u = 0; u < U.size(); ++u) {
+        int list = 0;
+        while (list < nx && (int)u >= lo[list + 1]) ++list;
+        int *o = out + 14 * u;
+        o[0] = list;
+        o[1] = (U[u].aux0 & LX_AUX_LOCAL) ? 1 : 0;
+        o[2] = U[u].op;
+        o[3] = U[u].ctr;
+        o[4] = U[u].nd;
+        for (int d = 0; d < 3; ++d) {
+            o[5 + d] = U[u].dc[d];
+            o[8 + d] = U[u].dt[d];
+        }
+        o[11] = (int)(U[u].c & 0xffffffffll);
+        o[12] = (int)(U[u].c >> 32);
+        o[13] = (int)(U[u].a >> 32);
+    }
+    return (int)U.size();
 }
 
 // The factor of a hypermatrix (tile (i, j) at tiles[j * mt + i], b x b, ld b)
@@ -148,10 +220,10 @@ extern "C" int lx_emu_run_tiles(int dtype, int variant, int mt, int b, void **ti
             blockIdx.x = 0;
             if (dtype == 0)
                 lx_exec_kernel<double, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
-                                                            ticks, nullptr);
+                                                            ticks, nullptr, nullptr, nullptr);
             else
                 lx_exec_kernel<float, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
-                                                           ticks, nullptr);
+                                                           ticks, nullptr, nullptr, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
@@ -210,10 +282,10 @@ extern "C" int lx_emu_run_stuck(int dtype, int n, int b, int stuck, long long sp
     run_wg([&] {
         if (dtype == 0)
             lx_exec_kernel<double, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
-                                                  nullptr);
+                                                  nullptr, nullptr, nullptr);
         else
             lx_exec_kernel<float, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
-                                                 nullptr);
+                                                 nullptr, nullptr, nullptr);
     });
     st_out[0] = st.abort;
     st_out[1] = st.timeouts;
@@ -261,6 +333,51 @@ extern "C" void lx_emu_gemm_unit(int flags, int M, int N, int K, float *C, long 
     sys.lda = ldc;
     sys.X = const_cast<float *>(P);
     sys.ldx = ldp;
+    lx_unit u = {};
+    u.op = (flags & 1) ? LX_OP_SYRK : LX_OP_GEMM_NT;
+    u.M = M;
+    u.N = N;
+    u.K = K;
+    u.aux0 = flags;
+    u.aux1 = gdiag;
+    u.cbase = LX_BASE_A;
+    u.abase = u.bbase = LX_BASE_X;
+    u.c = 0;
+    u.a = 0;
+    u.b = M;  // row M, column 0
+    int info = INT_MAX;
+    std::vector<double> smem((lx_fam_smem<float, LX_FAM_FACTOR>::bytes + 7) / 8);
+    blockDim.x = 256;
+    gridDim.x = 1;
+    run_wg([&] { lx_run_unit<float, LX_FAM_FACTOR>(u, &sys, &info, reinterpret_cast<char *>(smem.data())); });
+}
+
+// The same unit with the plane buffer set: the operand rows P (rows 0..M-1 =
+// A, M..M+N-1 = B, K columns, K a multiple of 16) are given only as FP16
+// planes (hi, lo of 2^12 x, the split the TRSM units write), while the FP32
+// operands the unit could also read are zeros: C changes only if the unit
+// took its operands from the planes.
+extern "C" void lx_emu_gemm_unit_planes(int flags, int M, int N, int K, float *C, long long ldc, const float *P,
+                                        long long ldpm, int gdiag)
+{
+    lx_sys sys = {};
+    std::vector<float> zeros((size_t)(M + N) * K, 0.0f);
+    sys.A = C;
+    sys.lda = ldc;
+    sys.X = zeros.data();
+    sys.ldx = M + N;
+    sys.ldp = ((int64_t)(M + N) + 256 + 31) / 32 * 32;
+    sys.nsig = K / 16;
+    std::vector<_Float16> planes((size_t)2 * sys.nsig * sys.ldp * 16, (_Float16)0.0f);
+    for (int r = 0; r < M + N; ++r)
+        for (int c = 0; c < K; ++c) {
+            const float xs = P[(size_t)c * ldpm + r] * 4096.0f;
+            const _Float16 h = (_Float16)xs;
+            const size_t o = ((size_t)(c / 16) * sys.ldp + r) * 16 + c % 16;
+            planes[o] = h;
+            planes[o + (size_t)sys.nsig * sys.ldp * 16] = (_Float16)(xs - (float)h);
+        }
+    sys.planes = planes.data();
     lx_unit u = {};
     u.op = (flags & 1) ? LX_OP_SYRK : LX_OP_GEMM_NT;
     u.M = M;

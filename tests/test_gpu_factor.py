@@ -147,3 +147,89 @@ def test_schol_split_fp16_out_of_range_reports(lx, orc, dev):
     opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2)
     assert lx.chol_dev(dA, b, "rl", opts) > 0
     dA.free()
+
+
+@pytest.mark.parametrize("variant", ["rl", "ll"])
+@pytest.mark.parametrize("n,b", [(700, 128), (4100, 512), (1989, 533), (3000, 256)])
+def test_schol_fp16_planes_bit_identical(lx, orc, dev, monkeypatch, variant, n, b):
+    """The split-FP16 factor with its GEMM / SYRK operands read from the FP16
+    planes the TRSM units write equals the factor that splits the FP32
+    operands in place (LINOMPSS_PLANES=0) bit for bit: the same values are
+    split the same way and multiplied in the same order.  b = 533 is not a
+    multiple of 16, so no unit can use the planes there; n = 4100 ends on a
+    ragged panel of 4 columns."""
+    print(f"[schol planes] n={n} b={b} {variant}", flush=True)
+    A0 = orc.genmat_spd(n, seed=7, diag_mode=1) / 2.0 ** np.ceil(np.log2(n))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2)
+    out = []
+    for planes in ("1", "0"):
+        monkeypatch.setenv("LINOMPSS_PLANES", planes)
+        dA = lx.DeviceMatrix.from_host(A0.astype(np.float32, order="F"))
+        assert lx.chol_dev(dA, b, variant, opts) == 0
+        out.append(np.tril(dA.to_host()))
+        dA.free()
+    assert np.array_equal(out[0], out[1])
+    Lref = np.linalg.cholesky(A0)
+    assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= 1e-5
+
+
+@pytest.mark.parametrize("dtype,variant,n,b,ld", [
+    (np.float32, "rl", 16384, 512, 16384), (np.float32, "ll", 10000, 256, 10016),
+    (np.float64, "rl", 10000, 256, 10016), (np.float64, "ll", 16384, 512, 16384)])
+def test_chol_xcd_lists_bit_identical(lx, orc, dev, monkeypatch, dtype, variant, n, b, ld):
+    """The factor on per-XCD unit lists (each tile's units on the XCD that owns
+    it, most units publishing without an L2 write-back) equals the factor on
+    one shared list (LINOMPSS_XCD=0) bit for bit: the same units, each with
+    the same arithmetic; a unit that read stale data from another XCD's cache
+    would change the result.  FP32 with the split-FP16 updates and planes."""
+    print(f"[xcd lists] {np.dtype(dtype).name} {variant} n={n} b={b} ld={ld}", flush=True)
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, 256))
+    A0 = G @ G.T / 256 + np.eye(n)
+    A0 /= 2.0 ** np.ceil(np.log2(np.abs(np.diag(A0)).max()))
+    Ah = np.zeros((ld, n), dtype=dtype, order="F")
+    Ah[:n] = A0
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2 if dtype == np.float32 else 0)
+    out = []
+    for xcd in ("1", "0"):
+        monkeypatch.setenv("LINOMPSS_XCD", xcd)
+        dA = lx.DeviceMatrix(n, n, dtype, ld=ld)
+        lx.lib.linompss_memcpy(dA.ptr, Ah.ctypes.data, Ah.nbytes, 1)
+        assert lx.chol_dev(dA, b, variant, opts) == 0
+        st = lx.last_exec_stats()
+        assert st.xcd_lists == (8 if xcd == "1" else 0)
+        out.append(np.tril(dA.to_host()))
+        dA.free()
+    assert np.array_equal(out[0], out[1])
+    Lref = np.linalg.cholesky(A0)
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= tol
+
+
+@pytest.mark.parametrize("dtype,variant,n,b", [(np.float64, "rl", 4096, 512), (np.float64, "rl", 3000, 256),
+                                               (np.float32, "rl", 4096, 512), (np.float64, "ll", 4096, 512)])
+def test_chol_tall_units(lx, orc, dev, monkeypatch, dtype, variant, n, b):
+    """GEMM units of up to a whole tile's rows (128-row blocks under one
+    publication) and 128-row panel solves, forced at every step
+    (LINOMPSS_WIDE_UNITS=0; by default only steps with two units per
+    workgroup left use them, which small orders never reach): the factor
+    equals the factor of 128-row units bit for bit (the same blocks with the
+    same arithmetic) and agrees with LAPACK; FP32 with the split-FP16 updates."""
+    print(f"[tall units] {np.dtype(dtype).name} {variant} n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=9, diag_mode=1) / 2.0 ** np.ceil(np.log2(n))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2 if dtype == np.float32 else 0)
+    out = []
+    for env in ({"LINOMPSS_WIDE_UNITS": "0", "LINOMPSS_WIDE_M": "512", "LINOMPSS_TRSM_ROWS": "128"},
+                {"LINOMPSS_WIDE_M": "128", "LINOMPSS_TRSM_ROWS": "64"}):
+        for k in ("LINOMPSS_WIDE_UNITS", "LINOMPSS_WIDE_M", "LINOMPSS_TRSM_ROWS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        dA = lx.DeviceMatrix.from_host(A0.astype(dtype, order="F"))
+        assert lx.chol_dev(dA, b, variant, opts) == 0
+        out.append(np.tril(dA.to_host()))
+        dA.free()
+    assert np.array_equal(out[0], out[1])
+    Lref = np.linalg.cholesky(A0)
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= tol
