@@ -198,8 +198,6 @@ typedef struct {
     int tasks;            /* reference-level tile tasks in that launch */
     int grid;             /* persistent workgroups launched */
     double sim_ms;        /* planner's simulated makespan */
-    int xcd_lists;        /* per-XCD unit lists of that launch (0: one shared list) */
-    int reserved;
 } linompss_exec_stats;
 
 int linompss_chol_dev(int dtype, int variant, int n, int b, void *dA, int64_t lda, void *stream,

@@ -68,8 +68,7 @@ class ExecOpts(ctypes.Structure):
 
 
 class ExecStats(ctypes.Structure):
-    _fields_ = [("exec_ms", _d), ("units", _i), ("tasks", _i), ("grid", _i), ("sim_ms", _d), ("xcd_lists", _i),
-                ("reserved", _i)]
+    _fields_ = [("exec_ms", _d), ("units", _i), ("tasks", _i), ("grid", _i), ("sim_ms", _d)]
 
 
 class IrOpts(ctypes.Structure):

@@ -32,12 +32,6 @@
 #include "lx_internal.hpp"
 #include "lx_planner.hpp"
 
-__global__ void lx_xcc_probe_kernel(int *out, int n)
-{
-    const int v = lx_xcc_id();
-    if (threadIdx.x == 0 && (int)blockIdx.x < n) out[blockIdx.x] = v;
-}
-
 __global__ void lx_fill_i32_kernel(int32_t *p, int32_t v, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,7 +53,6 @@ namespace {
 struct lx_dplan {
     std::string key;
     lx_unit *d_units = nullptr;
-    int *d_lists = nullptr;  // per-XCD list offsets (null: one shared list)
     int nunits = 0, ntasks = 0, nctr = 0;
     double sim_s = 0.0;
     bool in_tiles = false;  // every unit's operands inside single b x b tiles
@@ -70,7 +63,6 @@ struct lx_dev_ctx {
     int dev = -1;
     int cus = 0;
     int wall_khz = 0;
-    int nx = 0;          // XCDs seen by a probe launch (0: per-XCD lists off)
     int occ[2][3] = {};  // resident WGs per CU: [FP64 / FP32][op family]
     std::list<lx_dplan> plans;  // most recent first
     void *scratch[12] = {};
@@ -80,31 +72,6 @@ struct lx_dev_ctx {
 std::mutex g_mu;
 std::vector<std::unique_ptr<lx_dev_ctx>> g_ctx;
 
-// Number of XCDs the executor's workgroups are spread over: a launch of 8
-// workgroups per CU reads the XCC_ID register of each; the per-XCD lists are
-// used only when the IDs are exactly 0..nx-1 with the same number of
-// workgroups on each (else 0: one shared list).
-int probe_xcds(int cus)
-{
-    const int G = 8 * std::max(1, cus);
-    int *d = nullptr;
-    if (hipMalloc(&d, sizeof(int) * G) != hipSuccess) return 0;
-    hipLaunchKernelGGL(lx_xcc_probe_kernel, dim3(G), dim3(64), 0, 0, d, G);
-    std::vector<int> h(G, -1);
-    const bool ok = hipGetLastError() == hipSuccess &&
-                    hipMemcpy(h.data(), d, sizeof(int) * G, hipMemcpyDeviceToHost) == hipSuccess;
-    (void)hipFree(d);
-    if (!ok) return 0;
-    int cnt[16] = {}, nx = 0;
-    for (int v : h) {
-        if (v < 0 || v >= 16) return 0;
-        cnt[v] += 1;
-        nx = std::max(nx, v + 1);
-    }
-    for (int q = 0; q < nx; ++q)
-        if (cnt[q] != G / nx) return 0;
-    return nx >= 2 && G % nx == 0 ? nx : 0;
-}
 thread_local linompss_exec_stats g_last_stats = {};
 
 lx_dev_ctx *ctx_get()
@@ -129,7 +96,6 @@ lx_dev_ctx *ctx_get()
         e = e ? e : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb[5], lx_exec_kernel<float, LX_FAM_ALL>, LX_WG_THREADS, 0);
         if (e != hipSuccess) return nullptr;
         for (int q = 0; q < 6; ++q) c->occ[q / 3][q % 3] = std::max(1, nb[q]);
-        c->nx = probe_xcds(c->cus);
         g_ctx[dev] = std::move(c);
     }
     return g_ctx[dev].get();
@@ -210,10 +176,6 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         const double v = atof(ev);
         if (v >= 0.0) P.wide_units = v;
     }
-    if (const char *ev = getenv("LINOMPSS_XCD_TAIL")) {
-        const int v = atoi(ev);
-        if (v >= 0) P.xcd_tail = v;
-    }
 }
 
 bool build_plan(lx_planner &pl, const lx_shape &sh)
@@ -240,16 +202,14 @@ bool build_plan(lx_planner &pl, const lx_shape &sh)
 }
 
 // Returns a cached device plan (or builds and uploads one).
-lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linompss_exec_opts *o, int xcd)
+lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linompss_exec_opts *o)
 {
     lx_plan_params P;
     fill_params(P, sh.dtype, workers, o);
-    P.xcd = xcd;
     char key[256];
-    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d x%d xt%d",
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d",
              sh.what, sh.dtype, sh.n, sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec,
-             P.potrf_sub, P.gemm_wide_n, P.gemm_wide_m, P.wide_units, P.trsm_rows_wide, P.bx_wide, P.wide_tail,
-             P.xcd, P.xcd_tail);
+             P.potrf_sub, P.gemm_wide_n, P.gemm_wide_m, P.wide_units, P.trsm_rows_wide, P.bx_wide, P.wide_tail);
     for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
         if (it->key == key) {
             c->plans.splice(c->plans.begin(), c->plans, it);
@@ -277,21 +237,10 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
             return nullptr;
         }
     }
-    if (P.xcd > 0) {
-        std::vector<int> li(1, P.xcd);
-        li.insert(li.end(), pl.list_offsets().begin(), pl.list_offsets().end());
-        if (hipMalloc(&d.d_lists, sizeof(int) * li.size()) != hipSuccess ||
-            hipMemcpy(d.d_lists, li.data(), sizeof(int) * li.size(), hipMemcpyHostToDevice) != hipSuccess) {
-            if (d.d_units) (void)hipFree(d.d_units);
-            if (d.d_lists) (void)hipFree(d.d_lists);
-            return nullptr;
-        }
-    }
     c->plans.push_front(d);
     while (c->plans.size() > 8) {
         (void)hipDeviceSynchronize();
         if (c->plans.back().d_units) (void)hipFree(c->plans.back().d_units);
-        if (c->plans.back().d_lists) (void)hipFree(c->plans.back().d_lists);
         c->plans.pop_back();
     }
     return &c->plans.front();
@@ -339,16 +288,7 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     int wpc = occ;
     if (opts && opts->wg_per_cu > 0) wpc = std::min(opts->wg_per_cu, occ);
     const int workers = c->cus * wpc;
-    // Per-XCD unit lists (factor plans on a column-major matrix): the units of
-    // a tile stay on one XCD and most publish without an L2 write-back.  Tiles
-    // must not share 128-byte cache lines: A, its columns and the tile
-    // boundaries on line boundaries.  Opt-in (LINOMPSS_XCD=1).
-    const size_t es = sh.dtype == 0 ? sizeof(double) : sizeof(float);
-    const char *xe = getenv("LINOMPSS_XCD");
-    const bool xcd_on = c->nx > 0 && fam == LX_FAM_FACTOR && !dtiles && xe && atoi(xe) == 1 &&
-                        ((uintptr_t)dA % 128) == 0 && (lda * es) % 128 == 0 && (sh.b * es) % 128 == 0 &&
-                        workers % c->nx == 0;
-    lx_dplan *pl = plan_get(c, sh, workers, opts, xcd_on ? c->nx : 0);
+    lx_dplan *pl = plan_get(c, sh, workers, opts);
     if (!pl) return LINOMPSS_ERR_PLAN;
     LX_TRACE("plan units=%d tasks=%d ctr=%d", pl->nunits, pl->ntasks, pl->nctr);
     if (dtiles && !pl->in_tiles) return LINOMPSS_ERR_PLAN;
@@ -356,8 +296,8 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
         *info_out = 0;
         return 0;
     }
-    // control block: status | info | sys | list heads | counters
-    const size_t off_info = 64, off_sys = 128, off_heads = 256, off_ctr = 384;
+    // control block: status | info | sys | counters
+    const size_t off_info = 64, off_sys = 128, off_ctr = 256;
     const size_t bytes = off_ctr + sizeof(int32_t) * (size_t)pl->nctr;
     char *ctl = (char *)scratch_get(c, 0, bytes);
     if (!ctl) return LINOMPSS_ERR_NOMEM;
@@ -386,7 +326,7 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
         if (!hs.planes) return LINOMPSS_ERR_NOMEM;
     }
     if (fam != LX_FAM_SOLVE) {
-        hs.ldw = ((int64_t)sh.n + 64 + 31) / 32 * 32;  // columns of W on cache-line boundaries
+        hs.ldw = (int64_t)sh.n + 64;
         hs.W = scratch_get(c, 1, (size_t)hs.ldw * 64 * (sh.dtype == 0 ? sizeof(double) : sizeof(float)));
         if (!hs.W) return LINOMPSS_ERR_NOMEM;
     }
@@ -394,13 +334,11 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     lx_fill_i32((int32_t *)ctl, 0, 16, stream);
     lx_fill_i32((int32_t *)(ctl + off_info), INT_MAX, 1, stream);
     lx_fill_i32((int32_t *)(ctl + off_ctr), 0, pl->nctr, stream);
-    if (pl->d_lists) lx_fill_i32((int32_t *)(ctl + off_heads), 0, 32, stream);
     double tmo = 30.0;
     if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
     if (opts && opts->timeout_s > 0) tmo = opts->timeout_s;
     const long long ticks = (long long)(tmo * c->wall_khz * 1000.0);
-    // with per-XCD lists every XCD must run workgroups: the full grid
-    const int grid = pl->d_lists ? workers : std::max(1, std::min(pl->nunits, workers));
+    const int grid = std::max(1, std::min(pl->nunits, workers));
     hipEvent_t e0, e1;
     LX_HIP_TRY(hipEventCreate(&e0));
     LX_HIP_TRY(hipEventCreate(&e1));
@@ -411,8 +349,7 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
 #define LX_LAUNCH(TT, FF, ...)                                                                                     \
     hipLaunchKernelGGL((lx_exec_kernel<TT, FF, ##__VA_ARGS__>), dim3(grid), dim3(LX_WG_THREADS), 0, stream,           \
                        pl->d_units, pl->nunits, (const lx_sys *)(ctl + off_sys), (int *)(ctl + off_ctr),             \
-                       (lx_status *)ctl, (int *)(ctl + off_info), ticks, prof, pl->d_lists,                     \
-                       (int *)(ctl + off_heads))
+                       (lx_status *)ctl, (int *)(ctl + off_info), ticks, prof)
     if (dtiles) {  // factor plans only (in_tiles is false otherwise)
         if (sh.dtype == 0) LX_LAUNCH(double, LX_FAM_FACTOR, true);
         else LX_LAUNCH(float, LX_FAM_FACTOR, true);
@@ -444,7 +381,6 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     g_last_stats.tasks = pl->ntasks;
     g_last_stats.grid = grid;
     g_last_stats.sim_ms = pl->sim_s * 1e3;
-    g_last_stats.xcd_lists = pl->d_lists ? c->nx : 0;
     if (prof) {
         std::vector<long long> h((size_t)LX_PROF_WORDS * grid);
         if (hipMemcpy(h.data(), prof, sizeof(long long) * h.size(), hipMemcpyDeviceToHost) == hipSuccess) {

@@ -221,50 +221,16 @@ __device__ __forceinline__ void lx_wg_publish(int *counter)
     }
 }
 
-// Publishes a unit whose output is read next only through this XCD's L2
-// (LX_AUX_LOCAL): every wave waits for its stores to reach the L2, then the
-// counter is bumped, with no L2 write-back.
-__device__ __forceinline__ void lx_wg_publish_local(int *counter)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (lx_wave() == 0)
-        __hip_atomic_fetch_add(counter, (threadIdx.x & 63) == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Acquire for a unit of an XCD's own list: invalidate this CU's vector L1
-// only.  The L2 needs no invalidation: the unit reads tiles its XCD owns
-// (their every earlier version went through this L2) and panels other XCDs
-// produced, written back to memory when they were published and never read
-// by this XCD before (a tile is read off its XCD only once it is final).
-__device__ __forceinline__ void lx_acquire_l1()
-{
-#if defined(__AMDGCN__)
-    __asm__ volatile("buffer_inv sc0" ::: "memory");
-#else
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the CPU emulation's fence
-#endif
-}
-
-// The XCD this workgroup runs on (hardware register XCC_ID, bits 0..3).
-__device__ __forceinline__ int lx_xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)); }
-
 // Optional profile (prof != null): per workgroup, wave 0 accumulates
 // wall-clock ticks spent waiting for dependencies and running each op type,
 // written at exit as LX_PROF_WORDS int64 words per workgroup:
 //   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
 #define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
 
-// lists (null: one shared list in units[0..nunits)): lists[0] = nx, then
-// nx + 2 offsets; list q < nx = units [lists[1 + q], lists[2 + q]) for the
-// workgroups of XCD q, list nx = the shared list, taken once a workgroup's own
-// list is empty.  heads[q] = next index of list q (zero on entry).
 template <typename T, int FAM, bool HY = false>
 __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd)
     lx_exec_kernel(const lx_unit *__restrict__ units, int nunits, const lx_sys *__restrict__ sys, int *ctr,
-                   lx_status *st, int *info, long long spin_ticks, long long *prof, const int *__restrict__ lists,
-                   int *heads)
+                   lx_status *st, int *info, long long spin_ticks, long long *prof)
 {
     __shared__ __attribute__((aligned(16))) char smem[lx_fam_smem<T, FAM>::bytes];
     __shared__ int s_sched[2];  // [0] = go flag, [1] = unit index
@@ -272,30 +238,11 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
     __shared__ long long s_prof[LX_PROF_WORDS];  // in LDS: keeps the profile out of the registers
     if (threadIdx.x < LX_PROF_WORDS) s_prof[threadIdx.x] = 0;  // ordered by the loop's first barrier
     long long t_grab = 0, t_ready = 0;
-    const int nx = lists ? lx_uniform(lists[0]) : 0;
-    int cur = nx > 0 ? lx_uniform(lx_xcc_id() % nx) : 0;  // the list being taken (nx: the shared one)
     for (;;) {
         if (prof) t_grab = wall_clock64();
         if (lx_wave() == 0) {  // wave-uniform branch
             // one atomic per wave: lane 0 adds 1, the others add 0; lane 0's old value is the index
-            int idx;
-            if (nx == 0) {
-                idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
-            } else {
-                for (;;) {  // uniform: this XCD's list, then the shared one
-                    const int i = lx_uniform(atomicAdd(&heads[cur], lane == 0 ? 1 : 0));
-                    const int lo = lx_uniform(lists[1 + cur]), hi = lx_uniform(lists[2 + cur]);
-                    if (lo + i < hi) {
-                        idx = lo + i;
-                        break;
-                    }
-                    if (cur == nx) {
-                        idx = nunits;
-                        break;
-                    }
-                    cur = nx;
-                }
-            }
+            const int idx = lx_uniform(atomicAdd(&st->head, lane == 0 ? 1 : 0));
             int go = idx < nunits ? 1 : 0;
             if (go && lx_uniform(__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
                 go = 0;
@@ -323,10 +270,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
                         __builtin_amdgcn_s_sleep(2);
                     }
                 }
-                if (cur < nx)  // uniform: a unit of this XCD's own list
-                    lx_acquire_l1();
-                else
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             // every lane of wave 0 stores the same (uniform) values: no lane branch
             s_sched[0] = go;
@@ -342,10 +286,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
         if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
         lx_run_unit<T, FAM, HY>(u, sys, info, smem);
-        if (u.aux0 & LX_AUX_LOCAL)  // uniform
-            lx_wg_publish_local(&ctr[u.ctr]);
-        else
-            lx_wg_publish(&ctr[u.ctr]);
+        lx_wg_publish(&ctr[u.ctr]);
         if (lx_wave() == 0) atomicAdd(&st->done, lane == 0 ? 1 : 0);
         if (prof && threadIdx.x == 0 && u.op >= 0 && u.op < LX_OP_NOP) {  // no barrier inside
             const long long t_done = wall_clock64();

@@ -31,12 +31,6 @@ enum lx_base : int32_t { LX_BASE_A = 0, LX_BASE_X = 1 };
 
 #define LX_MAX_DEPS 3
 
-// aux0 flag (factor plans with per-XCD unit lists): the unit's output is read
-// only through its own XCD's L2 until a unit of the same XCD publishes with a
-// write-back of that L2, so it publishes with its stores in the L2 and no
-// write-back of its own (see lx_planner::split_xcd).
-#define LX_AUX_LOCAL (1 << 16)
-
 struct lx_unit {
     int32_t op;
     int32_t sys;           // system index (batched plans hold many systems)

@@ -2,8 +2,6 @@
 #include "lx_planner.hpp"
 
 #include <algorithm>
-#include <climits>
-#include <cstdint>
 #include <queue>
 #include <stdexcept>
 
@@ -557,86 +555,9 @@ void lx_planner::finalize()
     }
     if (order.size() != U.size()) throw std::logic_error("plan simulation did not schedule every unit");
     sim_time = now;
-    if (P.xcd > 0) {
-        split_xcd(order);
-        return;
-    }
     std::vector<lx_unit> ordered;
     ordered.reserve(U.size());
     for (int32_t q : order) ordered.push_back(U[q]);
-    U.swap(ordered);
-}
-
-// Per-XCD lists.  A GEMM / SYRK unit's release costs a write-back of its
-// XCD's whole L2 (about 15 % of the FP32 factor).  It is needed only for data
-// another XCD reads next.  So every unit of tile (i, j) -- its trailing
-// updates, its diagonal factorisation or panel solve -- runs on the XCD that
-// owns the tile, (i + j) % xcd, which spreads each step's panel solves over
-// all XCDs; the units of the last xcd_tail steps, where the trailing matrix no
-// longer gives every XCD enough tiles, go to a shared list that workgroups
-// take from once their own list is empty.  Each list is the simulated order
-// restricted to its units, so it is topological; units before the tail
-// depend only on units before the tail, and every XCD runs resident
-// workgroups, so the launch cannot deadlock (the minimal unfinished unit of
-// the lists always has a workgroup that will take it).
-//
-// A unit may publish without the write-back (LX_AUX_LOCAL) when every unit
-// on another list that waits on its counter waits at least for all the bumps
-// of the counter's last task on this list: that task's units run on the same
-// XCD after this unit (they wait for it, write-after-write) and each writes
-// the whole L2 back when it publishes, so what this unit stored is in memory
-// before any other XCD can read it.  Those last tasks (a tile's panel solve
-// or diagonal factorisation, whose results every XCD reads, or its last
-// update before the shared tail) publish with the write-back.
-void lx_planner::split_xcd(const std::vector<int32_t> &order)
-{
-    const int nx = P.xcd, nU = (int)U.size(), G = nx;
-    std::vector<int32_t> ulist(nU, G);
-    for (int u = 0; u < nU; ++u) {
-        const lx_unit &x = U[u];
-        const sysinfo &sy = S[x.sys];
-        const int ci = (int)((x.c & 0xffffffffll) / sy.b), cj = (int)((x.c >> 32) / sy.b);
-        const int64_t sc = (x.op == LX_OP_GEMM_NT || x.op == LX_OP_SYRK) ? (x.a >> 32) : (x.c >> 32);
-        const int step = (int)(sc / sy.b);
-        const bool factor_op = x.op == LX_OP_POTRF || x.op == LX_OP_TRSM_RLTN || x.op == LX_OP_SYRK ||
-                               x.op == LX_OP_GEMM_NT;
-        if (factor_op && step < sy.mt - P.xcd_tail) ulist[u] = (ci + cj) % nx;
-    }
-    // per counter: the list of its bumpers before the tail (-1 none, -2 several),
-    // its last task there, and the bumps through that task
-    std::vector<int32_t> cl(nctr, -1), lastL(nctr, -1), lastLb(nctr, 0), bumps(nctr, 0);
-    for (int u = 0; u < nU; ++u) {
-        const int c = U[u].ctr;
-        bumps[c] += 1;
-        if (ulist[u] == G) continue;
-        cl[c] = cl[c] == -1 ? ulist[u] : (cl[c] == ulist[u] ? cl[c] : -2);
-        if (unit_task[u] >= lastL[c]) {
-            lastL[c] = unit_task[u];
-            lastLb[c] = bumps[c];
-        }
-    }
-    // the smallest threshold a unit on another list waits for
-    std::vector<int32_t> xmin(nctr, INT32_MAX);
-    for (int u = 0; u < nU; ++u)
-        for (int d = 0; d < U[u].nd; ++d) {
-            const int c = U[u].dc[d];
-            if (ulist[u] != cl[c]) xmin[c] = std::min(xmin[c], U[u].dt[d]);
-        }
-    for (int u = 0; u < nU; ++u) {
-        const int c = U[u].ctr;
-        bool local = ulist[u] != G && cl[c] >= 0;
-        if (local && xmin[c] != INT32_MAX) local = xmin[c] >= lastLb[c] && unit_task[u] != lastL[c];
-        if (local) U[u].aux0 |= LX_AUX_LOCAL;
-    }
-    std::vector<lx_unit> ordered;
-    ordered.reserve(nU);
-    list_off.assign(nx + 2, 0);
-    for (int q = 0; q <= nx; ++q) {
-        list_off[q] = (int32_t)ordered.size();
-        for (int32_t u : order)
-            if (ulist[u] == q) ordered.push_back(U[u]);
-    }
-    list_off[nx + 1] = (int32_t)ordered.size();
     U.swap(ordered);
 }
 

@@ -33,8 +33,6 @@ struct lx_plan_params {
     int trsm_rows_wide = 64;  // panel TRSM rows per unit off the lookahead tile (64-row passes, one fence)
     int gemv_rows = 128;    // RHS GEMV output rows per unit
     int policy = 1;         // 0: reference priorities + FIFO, 1: critical-path (bottom level)
-    int xcd = 0;            // per-XCD unit lists (factor plans): the number of XCDs, 0 = one shared list
-    int xcd_tail = 16;      // with per-XCD lists: units of the last xcd_tail panel steps stay in a shared list
     int workers = 512;      // workgroups assumed resident in the simulation
     double flop_rate = 1.5e11;   // per-workgroup MFMA rate (flop/s) for GEMM/SYRK
     double potrf_rate = 3.0e10;  // per-workgroup rate of the diagonal factorisation
@@ -81,11 +79,6 @@ class lx_planner {
     // Orders units by simulation and returns the device unit array.
     void finalize();
 
-    // After finalize() with P.xcd > 0: U is grouped into P.xcd + 1 lists, list
-    // q = [list_off[q], list_off[q + 1]): lists 0..xcd-1 hold the units of the
-    // tiles each XCD owns ((i + j) % xcd) in the steps before the last
-    // xcd_tail, list xcd the rest; each list keeps the simulated order.
-    const std::vector<int32_t> &list_offsets() const { return list_off; }
     const std::vector<lx_unit> &units() const { return U; }
     const std::vector<lx_task> &tasks() const { return Tk; }
     int num_counters() const { return nctr; }
@@ -106,10 +99,6 @@ class lx_planner {
     int alloc_counters(int count);
     // aux0 flag of the tile loop's GEMM/SYRK units: 2 = split-BF16 matrix cores
     int upd_flag() const { return P.update_prec >= 1 ? (P.bx_wide ? 6 : 2) | (P.update_prec == 2 ? 8 : 0) : 0; }
-    // Groups the ordered units by XCD and marks the units that may publish
-    // without an L2 write-back (LX_AUX_LOCAL); order[] maps the simulated
-    // execution order to creation order.
-    void split_xcd(const std::vector<int32_t> &order);
     // Units of the diagonal-tile factorisation POTRF(k) of task t (its output
     // counter outc, global column offset k*b, width bs).
     void potrf_units(int t, int s, int kb, int bs, int outc);
@@ -123,7 +112,6 @@ class lx_planner {
     std::vector<int32_t> unit_task;   // creation-order unit -> task
     // per task: dependency list applied to all of its units
     std::vector<std::vector<std::pair<int32_t, int32_t>>> task_deps;
-    std::vector<int32_t> list_off;    // per-XCD list offsets (split_xcd)
     int nctr = 0;
     double sim_time = 0.0;
     bool finalized = false;

@@ -179,8 +179,6 @@ inline int __builtin_amdgcn_readlane(int v, int l) { return lx_emu_xchg(v, l); }
 template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
 
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
-// hardware registers: the emulated workgroup runs on XCD 0
-inline int __builtin_amdgcn_s_getreg(int) { return 0; }
 // lanes are separate threads here, so a wave barrier must really synchronise them
 inline void __builtin_amdgcn_wave_barrier() { lx_emu_wave_barrier[threadIdx.x >> 6].wait(); }
 inline void __builtin_amdgcn_sched_barrier(int) {}

@@ -106,9 +106,9 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
             threadIdx.x = t;
             blockIdx.x = 0;
             if (dtype == 0)
-                lx_exec_kernel<double, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr, nullptr, nullptr);
+                lx_exec_kernel<double, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
             else
-                lx_exec_kernel<float, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr, nullptr, nullptr);
+                lx_exec_kernel<float, LX_FAM_ALL>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, ticks, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
@@ -133,50 +133,6 @@ extern "C" int lx_emu_plan_in_tiles(int variant, int n, int b, int update_prec, 
     else pl.factor_ll(s);
     pl.finalize();
     return lx_units_in_tiles(pl.units(), b) ? 1 : 0;
-}
-
-// A factor plan split into per-XCD lists (what 0 RL, 1 LL): per unit, in
-// list order, {list, local, op, ctr, nd, dc[3], dt[3], c row, c col, a col};
-// offs = the nx + 2 list offsets.  Returns the unit count (or -1 when max is
-// too small).
-extern "C" int lx_emu_plan_xcd(int what, int n, int b, int dtype, int update_prec, int nx, int tail, int max,
-                               int *out, int *offs)
-{
-    lx_plan_params P;
-    P.unit_m = LX_UNIT_M;
-    P.unit_n = LX_UNIT_N;
-    P.elem_bytes = dtype == 0 ? 8 : 4;
-    P.update_prec = update_prec;
-    if (dtype == 0) P.potrf_sub = 256;
-    P.xcd = nx;
-    P.xcd_tail = tail;
-    lx_planner pl(P);
-    int s = pl.add_system(n, b, 0);
-    if (what == 0) pl.factor_rl(s);
-    else pl.factor_ll(s);
-    pl.finalize();
-    const std::vector<lx_unit> &U = pl.units();
-    if ((int)U.size() > max) return -1;
-    const std::vector<int32_t> &lo = pl.list_offsets();
-    for (int q = 0; q < nx + 2; ++q) offs[q] = lo[q];
-    for (size_t u = 0; u < U.size(); ++u) {
-        int list = 0;
-        while (list < nx && (int)u >= lo[list + 1]) ++list;
-        int *o = out + 14 * u;
-        o[0] = list;
-        o[1] = (U[u].aux0 & LX_AUX_LOCAL) ? 1 : 0;
-        o[2] = U[u].op;
-        o[3] = U[u].ctr;
-        o[4] = U[u].nd;
-        for (int d = 0; d < 3; ++d) {
-            o[5 + d] = U[u].dc[d];
-            o[8 + d] = U[u].dt[d];
-        }
-        o[11] = (int)(U[u].c & 0xffffffffll);
-        o[12] = (int)(U[u].c >> 32);
-        o[13] = (int)(U[u].a >> 32);
-    }
-    return (int)U.size();
 }
 
 // The factor of a hypermatrix (tile (i, j) at tiles[j * mt + i], b x b, ld b)
@@ -220,10 +176,10 @@ extern "C" int lx_emu_run_tiles(int dtype, int variant, int mt, int b, void **ti
             blockIdx.x = 0;
             if (dtype == 0)
                 lx_exec_kernel<double, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
-                                                            ticks, nullptr, nullptr, nullptr);
+                                                            ticks, nullptr);
             else
                 lx_exec_kernel<float, LX_FAM_FACTOR, true>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info,
-                                                           ticks, nullptr, nullptr, nullptr);
+                                                           ticks, nullptr);
         });
     for (auto &t : th) t.join();
     *info_out = info == INT_MAX ? 0 : info;
@@ -282,10 +238,10 @@ extern "C" int lx_emu_run_stuck(int dtype, int n, int b, int stuck, long long sp
     run_wg([&] {
         if (dtype == 0)
             lx_exec_kernel<double, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
-                                                  nullptr, nullptr, nullptr);
+                                                  nullptr);
         else
             lx_exec_kernel<float, LX_FAM_FACTOR>(U.data(), (int)U.size(), &sys, ctr.data(), &st, &info, spin_ticks,
-                                                 nullptr, nullptr, nullptr);
+                                                 nullptr);
     });
     st_out[0] = st.abort;
     st_out[1] = st.timeouts;

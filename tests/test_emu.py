@@ -523,102 +523,3 @@ def test_sweep_nan_with_mailbox_bits(orc, dtype, bits, trans):
     assert np.isnan(X[dep, 0]).all()
     assert np.isfinite(X[~dep, 0]).all()
     assert np.isfinite(X[:, 1]).all()
-
-
-def _plan_xcd(what, n, b, dtype, nx, tail):
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
-    I, P = ctypes.c_int, ctypes.c_void_p
-    E.lx_emu_plan_xcd.argtypes = [I] * 8 + [P, P]
-    mx = 200000
-    out = np.zeros((mx, 14), dtype=np.int32)
-    offs = np.zeros(nx + 2, dtype=np.int32)
-    nu = E.lx_emu_plan_xcd(what, n, b, dtype, 2 if dtype == 1 else 0, nx, tail, mx, out.ctypes.data_as(P),
-                           offs.ctypes.data_as(P))
-    assert nu > 0
-    return out[:nu], offs
-
-
-def _simulate_xcd(U, offs, nx, seed):
-    """One worker per XCD, each blocking on its unit until the unit's
-    dependencies are met, taking its own list and then the shared one (as the
-    executor's workgroups do).  Tracks, per XCD, which completed units have
-    been written back from its L2 (a non-local publish writes back everything
-    that XCD has completed so far) and checks that every unit reads counter
-    bumps that are either from its own XCD or written back.  Returns the
-    number of units run (all of them unless the lists deadlock)."""
-    rng = np.random.default_rng(seed)
-    nu = len(U)
-    nctr = int(U[:, 3].max()) + 1 + int(U[:, 5:8].max()) + 1
-    ctr = np.zeros(nctr, dtype=np.int64)
-    bumpers = [[] for _ in range(nctr)]   # completed bumps per counter: (xcd, unit)
-    flushed = np.zeros(nu, dtype=bool)
-    done_on = [[] for _ in range(nx)]     # completed, not yet written back, per XCD
-    head = list(offs[:nx + 1])
-    cur = [None] * nx                      # unit each worker holds
-    own = [True] * nx
-    ran = 0
-
-    def grab(x):
-        if own[x]:
-            if head[x] < offs[x + 1]:
-                head[x] += 1
-                return head[x] - 1
-            own[x] = False
-        if head[nx] < offs[nx + 1]:
-            head[nx] += 1
-            return head[nx] - 1
-        return None
-
-    progress = True
-    while progress:
-        progress = False
-        for x in rng.permutation(nx):
-            if cur[x] is None:
-                cur[x] = grab(x)
-            u = cur[x]
-            if u is None:
-                continue
-            deps = [(U[u, 5 + d], U[u, 8 + d]) for d in range(U[u, 4])]
-            if any(ctr[c] < t for c, t in deps):
-                continue
-            for c, t in deps:  # what the unit reads must be visible on XCD x
-                for (xb, ub) in bumpers[c][:t]:
-                    assert xb == x or flushed[ub], (u, c, t, ub)
-            ctr[U[u, 3]] += 1
-            bumpers[U[u, 3]].append((x, u))
-            done_on[x].append(u)
-            if not U[u, 1]:  # publish with the L2 write-back
-                flushed[done_on[x]] = True
-                done_on[x] = []
-            cur[x] = None
-            ran += 1
-            progress = True
-    return ran
-
-
-@pytest.mark.parametrize("what,n,b,dtype,tail", [(0, 8192, 256, 1, 8), (1, 8192, 256, 1, 8),
-                                                 (0, 6000, 256, 0, 6), (0, 4096, 512, 1, 2)])
-def test_xcd_lists_complete_and_publish_visibly(what, n, b, dtype, tail):
-    """Per-XCD unit lists of a factor plan: every unit of tile (i, j) before
-    the last `tail` steps is on list (i + j) % 8, the rest on the shared list;
-    a simulated run with one blocking worker per XCD completes (no deadlock),
-    and every unit that publishes without an L2 write-back is read on another
-    XCD only after a write-back on its own XCD."""
-    nx = 8
-    U, offs = _plan_xcd(what, n, b, dtype, nx, tail)
-    mt = (n + b - 1) // b
-    ci, cj = U[:, 11] // b, U[:, 12] // b
-    step = np.where(np.isin(U[:, 2], [2, 3]), U[:, 13], U[:, 12]) // b
-    lst = U[:, 0]
-    assert np.all(lst[step >= mt - tail] == nx)
-    inl = lst < nx
-    assert np.all(lst[inl] == (ci + cj)[inl] % nx)
-    assert U[:, 1].sum() > 0.6 * len(U)  # most units publish locally
-    assert not U[lst == nx, 1].any()
-    for seed in range(3):
-        assert _simulate_xcd(U, offs, nx, seed) == len(U)
-    # a planted mistake the check must catch: every unit publishing locally
-    bad = U.copy()
-    bad[:, 1] = 1
-    with pytest.raises(AssertionError):
-        _simulate_xcd(bad, offs, nx, 0)

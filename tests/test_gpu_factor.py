@@ -173,39 +173,6 @@ def test_schol_fp16_planes_bit_identical(lx, orc, dev, monkeypatch, variant, n, 
     assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= 1e-5
 
 
-@pytest.mark.parametrize("dtype,variant,n,b,ld", [
-    (np.float32, "rl", 16384, 512, 16384), (np.float32, "ll", 10000, 256, 10016),
-    (np.float64, "rl", 10000, 256, 10016), (np.float64, "ll", 16384, 512, 16384)])
-def test_chol_xcd_lists_bit_identical(lx, orc, dev, monkeypatch, dtype, variant, n, b, ld):
-    """The factor on per-XCD unit lists (each tile's units on the XCD that owns
-    it, most units publishing without an L2 write-back) equals the factor on
-    one shared list (LINOMPSS_XCD=0) bit for bit: the same units, each with
-    the same arithmetic; a unit that read stale data from another XCD's cache
-    would change the result.  FP32 with the split-FP16 updates and planes."""
-    print(f"[xcd lists] {np.dtype(dtype).name} {variant} n={n} b={b} ld={ld}", flush=True)
-    rng = np.random.default_rng(n)
-    G = rng.standard_normal((n, 256))
-    A0 = G @ G.T / 256 + np.eye(n)
-    A0 /= 2.0 ** np.ceil(np.log2(np.abs(np.diag(A0)).max()))
-    Ah = np.zeros((ld, n), dtype=dtype, order="F")
-    Ah[:n] = A0
-    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2 if dtype == np.float32 else 0)
-    out = []
-    for xcd in ("1", "0"):
-        monkeypatch.setenv("LINOMPSS_XCD", xcd)
-        dA = lx.DeviceMatrix(n, n, dtype, ld=ld)
-        lx.lib.linompss_memcpy(dA.ptr, Ah.ctypes.data, Ah.nbytes, 1)
-        assert lx.chol_dev(dA, b, variant, opts) == 0
-        st = lx.last_exec_stats()
-        assert st.xcd_lists == (8 if xcd == "1" else 0)
-        out.append(np.tril(dA.to_host()))
-        dA.free()
-    assert np.array_equal(out[0], out[1])
-    Lref = np.linalg.cholesky(A0)
-    tol = 1e-5 if dtype == np.float32 else 1e-12
-    assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= tol
-
-
 @pytest.mark.parametrize("dtype,variant,n,b", [(np.float64, "rl", 4096, 512), (np.float64, "rl", 3000, 256),
                                                (np.float32, "rl", 4096, 512), (np.float64, "ll", 4096, 512)])
 def test_chol_tall_units(lx, orc, dev, monkeypatch, dtype, variant, n, b):
