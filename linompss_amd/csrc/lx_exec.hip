@@ -393,8 +393,9 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
                     sh.dtype, sh.n, grid, ms, 100.0 * tot[0] * tick_ms / (ms * grid));
             for (int q = 0; q < LX_OP_NOP; ++q)
                 if (tot[1 + LX_OP_NOP + q] > 0)
-                    fprintf(stderr, "  %s: n=%.0f %.1f%% avg=%.1f us", nm[q], tot[1 + LX_OP_NOP + q],
-                            100.0 * tot[1 + q] * tick_ms / (ms * grid), tot[1 + q] * tick_ms * 1e3 / tot[1 + LX_OP_NOP + q]);
+                    fprintf(stderr, "  %s: n=%.0f %.1f%% avg=%.1f us (publish %.1f)", nm[q], tot[1 + LX_OP_NOP + q],
+                            100.0 * tot[1 + q] * tick_ms / (ms * grid), tot[1 + q] * tick_ms * 1e3 / tot[1 + LX_OP_NOP + q],
+                            tot[1 + 2 * LX_OP_NOP + q] * tick_ms * 1e3 / tot[1 + LX_OP_NOP + q]);
             fprintf(stderr, "\n");
         }
     }

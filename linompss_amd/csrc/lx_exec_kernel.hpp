@@ -224,8 +224,9 @@ __device__ __forceinline__ void lx_wg_publish(int *counter)
 // Optional profile (prof != null): per workgroup, wave 0 accumulates
 // wall-clock ticks spent waiting for dependencies and running each op type,
 // written at exit as LX_PROF_WORDS int64 words per workgroup:
-//   [0] wait ticks, [1 + op] run ticks, [1 + LX_OP_NOP + op] unit counts.
-#define LX_PROF_WORDS (1 + 2 * LX_OP_NOP)
+//   [0] wait ticks, [1 + op] run ticks (body and publication),
+//   [1 + LX_OP_NOP + op] unit counts, [1 + 2 LX_OP_NOP + op] publication ticks.
+#define LX_PROF_WORDS (1 + 3 * LX_OP_NOP)
 
 template <typename T, int FAM, bool HY = false>
 __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd)
@@ -237,7 +238,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
     const int lane = lx_lane();
     __shared__ long long s_prof[LX_PROF_WORDS];  // in LDS: keeps the profile out of the registers
     if (threadIdx.x < LX_PROF_WORDS) s_prof[threadIdx.x] = 0;  // ordered by the loop's first barrier
-    long long t_grab = 0, t_ready = 0;
+    long long t_grab = 0, t_ready = 0, t_body = 0;
     for (;;) {
         if (prof) t_grab = wall_clock64();
         if (lx_wave() == 0) {  // wave-uniform branch
@@ -286,6 +287,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
         if (prof) t_ready = wall_clock64();
         const lx_unit u = units[idx];
         lx_run_unit<T, FAM, HY>(u, sys, info, smem);
+        if (prof) t_body = wall_clock64();
         lx_wg_publish(&ctr[u.ctr]);
         if (lx_wave() == 0) atomicAdd(&st->done, lane == 0 ? 1 : 0);
         if (prof && threadIdx.x == 0 && u.op >= 0 && u.op < LX_OP_NOP) {  // no barrier inside
@@ -293,6 +295,7 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
             s_prof[0] += t_ready - t_grab;
             s_prof[1 + u.op] += t_done - t_ready;
             s_prof[1 + LX_OP_NOP + u.op] += 1;
+            s_prof[1 + 2 * LX_OP_NOP + u.op] += t_done - t_body;
         }
     }
     if (prof && threadIdx.x == 0)
