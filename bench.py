@@ -41,6 +41,15 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def host_threads(requested: int) -> int:
+    """CPU-baseline threads: --cpu-threads, else the host share the environment
+    gives this process (OMP_NUM_THREADS), else every core up to 16."""
+    if requested:
+        return requested
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else min(16, os.cpu_count() or 1)
+
+
 def cpu_baseline(n, b, threads):
     """The oracle's OpenMP-task FP64 RL factor + sweeps on an n x n sample."""
     os.environ.setdefault("OMP_MAX_TASK_PRIORITY", "10000")
@@ -113,7 +122,7 @@ def bench_batched(args):
         f"sweeps max {st['sweeps'].max()}, fallbacks {int(st['fallback'].sum())}")
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = host_threads(args.cpu_threads)
         try:
             cpu = cpu_baseline_batched(sizes, mats, rhs, threads)
             log(f"CPU baseline {cpu['value']} systems/s")
@@ -214,7 +223,7 @@ def bench_gp(args):
     flops = 4.0 * n ** 3 / 3.0  # forward factor n^3/3, recomputed factor n^3/3, adjoint 2n^3/3
     cpu, dell_ref = None, None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = host_threads(args.cpu_threads)
         try:
             dell_ref, cpu = cpu_baseline_gp(P, y, ell0, noise, threads)
             log(f"CPU closed-form gradient {cpu['value']} ms, dnll/dl={dell_ref:.10e}")
@@ -436,7 +445,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--fp64-b", type=int, default=0, help="tile of the plain FP64 factor (0: same as --b)")
-    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=32768)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--update-prec", type=int, default=2, choices=(0, 1, 2),
@@ -562,7 +571,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = host_threads(args.cpu_threads)
         log(f"CPU baseline: n={args.cpu_n} on {threads} threads")
         try:
             cpu = cpu_baseline(args.cpu_n, b, threads)

@@ -99,6 +99,67 @@ int linompss_dtrsm(char side, char uplo, char trans, char diag, int m, int n, in
 int linompss_strsm(char side, char uplo, char trans, char diag, int m, int n, int b, float alpha,
                    const float *A, int lda, float *B, int ldb);
 
+/* ---- The reference's own BLAS-3 and tile-task symbols --------------------
+ * Flags as the reference passes them: 8-byte unions by value
+ * (src/fp/fpmatr.h:14-35), holding a CBLAS enumeration (its MKL build,
+ * fpmatr.c:14-27: 111 NoTrans, 112 Trans, 121 Upper, 122 Lower, 131 NonUnit,
+ * 132 Unit, 141 Left, 142 Right) or pointing at "N"/"T"/"L"/"U"/"R" (its
+ * Fortran-BLAS build, fpmatr.c:36-47); both are accepted.  The descriptor
+ * instances below are the Fortran-BLAS flavour.                            */
+#ifndef __FPMATR_H__
+typedef union {
+    int e;
+    char *s;
+} ompssblas_t;
+typedef union {
+    char e;
+    char *s;
+} ompsslapack_t;
+#endif
+extern ompssblas_t OMPSSBLAS_ROWMAJOR, OMPSSBLAS_COLMAJOR, OMPSSBLAS_TRANSP, OMPSSBLAS_NTRANSP,
+    OMPSSBLAS_UPPERTRIANG, OMPSSBLAS_LOWERTRIANG, OMPSSBLAS_NTRIANG, OMPSSBLAS_DIAGUNIT, OMPSSBLAS_NDIAGUNIT,
+    OMPSSBLAS_LEFT, OMPSSBLAS_RIGHT;
+extern ompsslapack_t OMPSSLAPACK_UPPERTRIANG, OMPSSLAPACK_LOWERTRIANG;
+
+/* Replace ompss_dgemm / ompss_sgemm (src/gemm/ompss_mm.h:27 / :23): the
+ * linompss_{d,s}gemm above with the reference's descriptors; returns 0, -1 /
+ * -2 for an unreadable flag, or -i as linompss_dgemm does. */
+int ompss_dgemm(ompssblas_t transa, ompssblas_t transb, int b, int d, int c, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc);
+int ompss_sgemm(ompssblas_t transa, ompssblas_t transb, int b, int d, int c, int m, int n, int k, float alpha,
+                const float *A, int lda, const float *B, int ldb, float beta, float *C, int ldc);
+/* Replace ompss_dsyrk / ompss_ssyrk (src/syrk/ompss_syrk.h:22 / :20) and
+ * ompss_dtrsm / ompss_strsm (src/trsm/ompss_trsm.h:23 / :20).  void as in the
+ * reference: an argument or device error is printed to stderr. */
+void ompss_dsyrk(ompssblas_t uplo, ompssblas_t trans, int b, int n, int k, double alpha, double *A, int lda,
+                 double beta, double *C, int ldc);
+void ompss_ssyrk(ompssblas_t uplo, ompssblas_t trans, int b, int n, int k, float alpha, float *A, int lda,
+                 float beta, float *C, int ldc);
+void ompss_dtrsm(ompssblas_t side, ompssblas_t uplo, ompssblas_t trans, ompssblas_t diag, int m, int n, int b,
+                 double alpha, double *A, int lda, double *B, int ldb);
+void ompss_strsm(ompssblas_t side, ompssblas_t uplo, ompssblas_t trans, ompssblas_t diag, int m, int n, int b,
+                 float alpha, float *A, int lda, float *B, int ldb);
+/* The tile tasks of libbblas (src/blocked_blas/tasks_potrf.h:23, tasks_trsm.h:24,
+ * tasks_syrk.h:21, task_gemm.h:27, and the s twins): one tile operation on
+ * host memory, run on the device and completed before the call returns (the
+ * reference spawns an OmpSs task; p, its priority, is ignored).  task_dpotrf
+ * factors the uplo triangle in place and, as the reference does, discards
+ * LAPACK's info. */
+void task_dpotrf(ompsslapack_t uplo, int n, double *A, int lda, int p);
+void task_spotrf(ompsslapack_t uplo, int n, float *A, int lda, int p);
+void task_dtrsm(ompssblas_t side, ompssblas_t uplo, ompssblas_t trans, ompssblas_t diag, int bm, int bn, double alpha,
+                double *A, int lda, double *B, int ldb, int p);
+void task_strsm(ompssblas_t side, ompssblas_t uplo, ompssblas_t trans, ompssblas_t diag, int bm, int bn, float alpha,
+                float *A, int lda, float *B, int ldb, int p);
+void task_dsyrk(ompssblas_t uplo, ompssblas_t trans, int n, int k, double alpha, double *A, int lda, double beta,
+                double *C, int ldc, int p);
+void task_ssyrk(ompssblas_t uplo, ompssblas_t trans, int n, int k, float alpha, float *A, int lda, float beta,
+                float *C, int ldc, int p);
+void task_dgemm(ompssblas_t transa, ompssblas_t transb, int bm, int bn, int bk, double alpha, double *A, int lda,
+                double *B, int ldb, double beta, double *C, int ldc, int p);
+void task_sgemm(ompssblas_t transa, ompssblas_t transb, int bm, int bn, int bk, float alpha, float *A, int lda,
+                float *B, int ldb, float beta, float *C, int ldc, int p);
+
 /* ---- Dense matrices in the Octave text format (A.dat / RHS.dat of the
  * refinement app, src/itref/itref_setup.c:98-121, 155-199) ------------------
  * Writer: replaces fprint_dense2mm / print_dense2mm (src/matutil/matfprint.c:

@@ -37,6 +37,7 @@ __all__ = [
     "BatchPlan", "dposv_ir_batched_dev", "gen_batch_spd", "ompss_dchol_hrl", "ompss_dchol_hll",
     "ompss_schol_hrl", "ompss_schol_hll", "ompss_dchol_nhrl", "ompss_dchol_nhll", "ompss_schol_nhrl",
     "ompss_schol_nhll", "chol_tiles_dev", "gemm", "syrk", "trsm", "write_dense", "read_dense", "read_values",
+    "blas_desc", "lapack_desc", "exported_desc", "CBLAS",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -127,6 +128,52 @@ _sig("linompss_last_exec_stats", _i, ctypes.POINTER(ExecStats))
 _sig("linompss_plan_tdg", _i, _i, _i, _i, _i, _ip, _ip, _i, _ip, _i, _ip, _ip)
 _sig("linompss_plan_info", _i, _i, _i, _i, _i, ctypes.POINTER(ExecOpts), _ip, _ip, ctypes.POINTER(_d))
 _sig("linompss_version", ctypes.c_char_p)
+
+# The reference's flag descriptors ompssblas_t / ompsslapack_t
+# (src/fp/fpmatr.h:14-35) are 8-byte unions of an int (a CBLAS value, its MKL
+# build) or a char (LAPACK) with a char * ("N", "T", "U", "L", "R", its
+# Fortran-BLAS build), passed by value.  The x86-64 calling convention passes
+# such a union in one integer register, so ctypes carries it as the 64-bit
+# integer: the CBLAS value / character, or the address of the string.
+_u64 = ctypes.c_uint64
+CBLAS = {"trans": {"N": 111, "T": 112}, "uplo": {"U": 121, "L": 122}, "diag": {"N": 131, "U": 132},
+         "side": {"L": 141, "R": 142}}
+_FLAG_STR = {c: ctypes.create_string_buffer(c.encode()) for c in "NTULRnturlx"}
+
+
+def blas_desc(kind: str, letter: str, flavour: str = "enum") -> int:
+    """An ompssblas_t as the integer ctypes passes: ``flavour`` "enum" gives
+    the CBLAS value of ``letter`` for ``kind`` (trans, uplo, diag, side),
+    "str" the address of a NUL-terminated one-letter string."""
+    if flavour == "enum":
+        return CBLAS[kind][letter.upper()]
+    return ctypes.addressof(_FLAG_STR[letter])
+
+
+def lapack_desc(letter: str, flavour: str = "enum") -> int:
+    """An ompsslapack_t: the character itself ("enum") or the address of a string ("str")."""
+    return ord(letter) if flavour == "enum" else ctypes.addressof(_FLAG_STR[letter])
+
+
+def exported_desc(name: str) -> int:
+    """The library's own descriptor instance ``name`` (OMPSSBLAS_TRANSP, OMPSSLAPACK_LOWERTRIANG, ...)."""
+    return _u64.in_dll(lib, name).value
+
+
+_sig("ompss_dgemm", _i, _u64, _u64, _i, _i, _i, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i)
+_sig("ompss_sgemm", _i, _u64, _u64, _i, _i, _i, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sig("ompss_dsyrk", None, _u64, _u64, _i, _i, _i, _d, _p, _i, _d, _p, _i)
+_sig("ompss_ssyrk", None, _u64, _u64, _i, _i, _i, _f, _p, _i, _f, _p, _i)
+_sig("ompss_dtrsm", None, _u64, _u64, _u64, _u64, _i, _i, _i, _d, _p, _i, _p, _i)
+_sig("ompss_strsm", None, _u64, _u64, _u64, _u64, _i, _i, _i, _f, _p, _i, _p, _i)
+_sig("task_dpotrf", None, _u64, _i, _p, _i, _i)
+_sig("task_spotrf", None, _u64, _i, _p, _i, _i)
+_sig("task_dtrsm", None, _u64, _u64, _u64, _u64, _i, _i, _d, _p, _i, _p, _i, _i)
+_sig("task_strsm", None, _u64, _u64, _u64, _u64, _i, _i, _f, _p, _i, _p, _i, _i)
+_sig("task_dsyrk", None, _u64, _u64, _i, _i, _d, _p, _i, _d, _p, _i, _i)
+_sig("task_ssyrk", None, _u64, _u64, _i, _i, _f, _p, _i, _f, _p, _i, _i)
+_sig("task_dgemm", None, _u64, _u64, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i, _i)
+_sig("task_sgemm", None, _u64, _u64, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i, _i)
 
 ERR_BASE = -1000
 ERR_NAMES = {-1001: "no usable HIP device", -1002: "HIP call failed", -1003: "device allocation failed",

@@ -74,7 +74,10 @@ def build(verbose: bool = False) -> Path:
             sys.stderr.write(f"$ {' '.join(cmd)}\n{out}\n")
         raise RuntimeError("liblinompss build failed")
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
+    # -Bsymbolic: calls between this library's own exported functions (the
+    # tile tasks call ompss_dchol_rl, ...) bind inside it, never to a
+    # same-named symbol another library in the process exports
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic",
           f"-Wl,--version-script={CSRC / 'liblinompss.map'}", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
     return LIB

@@ -57,3 +57,47 @@ def test_package_does_not_import_torch():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                          cwd=str(HDR.parents[1]), check=True).stdout.strip()
     assert out == "False"
+
+
+def test_reference_descriptors_exported(lx):
+    """OMPSSBLAS_* / OMPSSLAPACK_* (src/fp/fpmatr.c:36-47): the Fortran-BLAS
+    flavour, one-letter strings; ROWMAJOR / COLMAJOR are zero there too."""
+    want = {"TRANSP": b"T", "NTRANSP": b"N", "UPPERTRIANG": b"U", "LOWERTRIANG": b"L", "NTRIANG": b"x",
+            "DIAGUNIT": b"U", "NDIAGUNIT": b"N", "LEFT": b"L", "RIGHT": b"R"}
+    for k, v in want.items():
+        assert ctypes.string_at(lx.exported_desc("OMPSSBLAS_" + k)) == v
+    assert lx.exported_desc("OMPSSBLAS_ROWMAJOR") == 0 and lx.exported_desc("OMPSSBLAS_COLMAJOR") == 0
+    assert ctypes.string_at(lx.exported_desc("OMPSSLAPACK_UPPERTRIANG")) == b"U"
+    assert ctypes.string_at(lx.exported_desc("OMPSSLAPACK_LOWERTRIANG")) == b"L"
+
+
+def test_reference_blas_flag_decoding(lx, capfd):
+    """Both descriptor flavours decode to the same flag, and the argument
+    checks that follow run before any device work: a bad flag or size is the
+    reference's BLAS argument number (ompss_dgemm returns it; the void
+    functions print it)."""
+    A = np.zeros((4, 4), order="F")
+    p = A.ctypes.data_as(ctypes.c_void_p)
+    for fl in ("enum", "str"):
+        N, T = lx.blas_desc("trans", "N", fl), lx.blas_desc("trans", "T", fl)
+        assert lx.lib.ompss_dgemm(N, T, 1, 1, 1, -1, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4) == -6
+        assert lx.lib.ompss_dgemm(T, N, 1, 1, 1, 4, 4, 5, 1.0, p, 4, p, 5, 0.0, p, 4) == -11
+        assert lx.lib.ompss_sgemm(N, N, 1, 1, 1, 4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 3) == -16
+    assert lx.lib.ompss_dgemm(0, 111, 1, 1, 1, 4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4) == -1      # no flag at all
+    assert lx.lib.ompss_dgemm(111, 121, 1, 1, 1, 4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4) == -2    # uplo as trans
+    assert lx.lib.ompss_dgemm(lx.blas_desc("uplo", "U", "str"), 111, 1, 1, 1, 4, 4, 4, 1.0, p, 4, p, 4, 0.0,
+                              p, 4) == -1
+    capfd.readouterr()
+    lx.lib.ompss_dsyrk(lx.blas_desc("trans", "N"), 111, 1, 4, 4, 1.0, p, 4, 0.0, p, 4)
+    lx.lib.ompss_dtrsm(141, 122, 111, 131, 4, 4, 0, 1.0, p, 4, p, 4)
+    lx.lib.ompss_strsm(lx.exported_desc("OMPSSBLAS_RIGHT"), lx.exported_desc("OMPSSBLAS_LOWERTRIANG"),
+                       lx.exported_desc("OMPSSBLAS_TRANSP"), lx.exported_desc("OMPSSBLAS_NTRIANG"), 4, 4, 1, 1.0,
+                       p, 4, p, 4)
+    lx.lib.task_dpotrf(ord("X"), 4, p, 4, 0)
+    lx.lib.task_dpotrf(lx.lapack_desc("L", "str"), 4, p, 3, 0)
+    lx.lib.task_dpotrf(lx.exported_desc("OMPSSLAPACK_UPPERTRIANG"), 0, p, 1, 0)  # empty: nothing printed
+    lx.lib.task_dgemm(111, 111, -2, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4, 0)
+    err = capfd.readouterr().err.splitlines()
+    assert err == ["linompss: ompss_dsyrk returned -1", "linompss: ompss_dtrsm returned -7",
+                   "linompss: ompss_strsm returned -4", "linompss: task_dpotrf returned -1",
+                   "linompss: task_dpotrf returned -4", "linompss: task_dgemm returned -6"]
