@@ -431,6 +431,20 @@ int linompss_gen_gauss(int m, int n, uint32_t seed, double *dG, int64_t ldg);
  * dimension lda, in device memory.  Returns 0, -i for argument i (p <= 0 is
  * refused, as the reference returns NULL), or LINOMPSS_ERR_HIP. */
 int linompss_gen_covariant(int n, double p, double k, double *dA, int64_t lda);
+/* The reference's own drand48 generators, bit for bit, in device memory
+ * (dtype 0 FP64, 1 FP32; seed as srand48 takes it, the reference passes
+ * time(NULL)): GENMAT_SPD (src/matutil/genmat.c:49-70), a_ij = a_ji drawn
+ * column by column over i >= j with n added to the diagonal, and GENMAT
+ * (genmat.c:106-128), X(i, j) the (j m + i + 1)-th draw.  Return 0, -i for
+ * argument i, or LINOMPSS_ERR_HIP. */
+int linompss_gen_drand48_spd(int dtype, int n, long seed, void *dA, int64_t lda);
+int linompss_gen_drand48(int dtype, int m, int n, long seed, void *dX, int64_t ldx);
+/* The factor check of the dchol CLI (chol_check.c:12-42): L2 = L L^T from the
+ * lower triangle of dL, then DMAT_RELERR (densutil.c:50-99),
+ * max_j sum_{i>=j} | |L2_ij| - |A_ij| | / max_j sum_{i>=j} |L2_ij|, in
+ * *relerr (column sums in FP64).  Allocates 2 n^2 elements of scratch. */
+int linompss_relerr_lower_dev(int dtype, int n, const void *dL, int64_t ldl, const void *dA, int64_t lda,
+                              double *relerr);
 
 /* ---- Device-memory plumbing (so C and ctypes callers need no other HIP
  * binding).  kind: 1 host->device, 2 device->host, 3 device->device.       */

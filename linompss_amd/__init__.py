@@ -563,6 +563,9 @@ _sig("linompss_gen_gauss_spd", _i, _i, ctypes.c_uint32, _d, _p, _i64, ctypes.POI
 _sig("linompss_gen_uniform", _i, _i, _i, _i, ctypes.c_uint32, _d, _d, _p, _i64)
 _sig("linompss_gen_gauss", _i, _i, _i, ctypes.c_uint32, _p, _i64)
 _sig("linompss_gen_covariant", _i, _i, _d, _d, _p, _i64)
+_sig("linompss_gen_drand48_spd", _i, _i, _i, ctypes.c_long, _p, _i64)
+_sig("linompss_gen_drand48", _i, _i, _i, _i, ctypes.c_long, _p, _i64)
+_sig("linompss_relerr_lower_dev", _i, _i, _i, _p, _i64, _p, _i64, ctypes.POINTER(_d))
 
 
 def gen_gauss_spd(n: int, seed: int, kappa: float) -> tuple[DeviceMatrix, float, float]:
@@ -587,6 +590,32 @@ def gen_covariant(n: int, p: float = 2.0, k: float = 2.0, ld: int | None = None)
     A = DeviceMatrix(n, n, ld=ld)
     _check("linompss_gen_covariant", lib.linompss_gen_covariant(n, p, k, A.ptr, A.ld), False)
     return A
+
+
+def gen_drand48_spd(n: int, seed: int, dtype=np.float64, ld: int | None = None) -> DeviceMatrix:
+    """The reference's GENMAT_SPD (genmat.c:49-70) in device memory, bit for
+    bit: drand48 seeded by ``seed``, drawn column by column over i >= j,
+    mirrored, n added to the diagonal."""
+    A = DeviceMatrix(n, n, dtype, ld=ld)
+    _check("linompss_gen_drand48_spd", lib.linompss_gen_drand48_spd(
+        0 if np.dtype(dtype) == np.float64 else 1, n, seed, A.ptr, A.ld), False)
+    return A
+
+
+def gen_drand48(m: int, n: int, seed: int, dtype=np.float64) -> DeviceMatrix:
+    """The reference's GENMAT (genmat.c:106-128): X(i, j) the (j m + i + 1)-th drand48 draw."""
+    X = DeviceMatrix(m, n, dtype)
+    _check("linompss_gen_drand48", lib.linompss_gen_drand48(
+        0 if np.dtype(dtype) == np.float64 else 1, m, n, seed, X.ptr, X.ld), False)
+    return X
+
+
+def relerr_lower_dev(dL: DeviceMatrix, dA: DeviceMatrix) -> float:
+    """The dchol check (chol_check.c:12-42): DMAT_RELERR of L L^T against A, lower triangles."""
+    out = _d(0.0)
+    _check("linompss_relerr_lower_dev", lib.linompss_relerr_lower_dev(
+        0 if dL.dtype == np.float64 else 1, dL.rows, dL.ptr, dL.ld, dA.ptr, dA.ld, ctypes.byref(out)), False)
+    return out.value
 
 
 def gen_gauss(m: int, n: int, seed: int) -> DeviceMatrix:

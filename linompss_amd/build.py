@@ -83,6 +83,30 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+CLI = PKG / "cli"
+BIN = PKG / "bin"
+# the reference's programs, built per precision / variant as it builds them
+# (chol.c with -DDOUBLE_PRECISION -DUSE_RL, ...; spdsolv.c)
+CLI_PROGRAMS = {
+    "dchol_rl": ("lx_chol_cli.cpp", 0, 0), "dchol_ll": ("lx_chol_cli.cpp", 0, 1),
+    "schol_rl": ("lx_chol_cli.cpp", 1, 0), "schol_ll": ("lx_chol_cli.cpp", 1, 1),
+    "dspdsolv": ("lx_spdsolv_cli.cpp", 0, 0), "sspdsolv": ("lx_spdsolv_cli.cpp", 1, 0),
+}
+
+
+def build_cli() -> Path:
+    """The CLI front ends (host C++ over the C-ABI only) into linompss_amd/bin,
+    linked against the in-tree liblinompss.so."""
+    BIN.mkdir(exist_ok=True)
+    for prog, (src, dtype, variant) in CLI_PROGRAMS.items():
+        tmp = BIN / (prog + ".tmp")
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CLI}", f"-DLX_CLI_DTYPE={dtype}",
+              f"-DLX_CLI_VARIANT={variant}", str(CLI / src), "-o", str(tmp), f"-L{PKG}", "-llinompss",
+              "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, BIN / prog)
+    return BIN
+
+
 def build_oracle() -> Path:
     """Builds the CPU oracle (test infrastructure) with its Makefile."""
     odir = PKG.parent / "oracle"
@@ -95,3 +119,4 @@ def build_oracle() -> Path:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cli())

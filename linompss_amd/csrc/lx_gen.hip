@@ -328,3 +328,94 @@ extern "C" int linompss_gen_covariant(int n, double p, double k, double *dA, int
                        lda);
     return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
 }
+
+// ---- The reference's own generators, GENMAT_SPD and GENMAT
+// (src/matutil/genmat.c:49-70, 106-128), for its CLI front ends: drand48
+// (x <- 0x5DEECE66D x + 0xB mod 2^48, value x / 2^48, seeded x0 = seed << 16
+// | 0x330E) drawn in the reference's loop order.  Each element jumps to its
+// own draw: with x_s = A_s x0 + C_s, the jumps for s = 2^b are tabulated on
+// the host (A_2s = A_s^2, C_2s = (A_s + 1) C_s) and composed over the set bits
+// of s, all modulo 2^64 and masked to 48 bits at the end (2^48 divides 2^64).
+struct lx_lcg_table {
+    uint64_t a[48], c[48];
+};
+
+static lx_lcg_table lx_lcg_make()
+{
+    lx_lcg_table t;
+    uint64_t a = 0x5DEECE66Dull, c = 0xBull;
+    for (int b = 0; b < 48; ++b) {
+        t.a[b] = a;
+        t.c[b] = c;
+        c = (a + 1) * c;
+        a = a * a;
+    }
+    return t;
+}
+
+__device__ inline double lx_drand48_at(const lx_lcg_table &t, uint64_t x0, uint64_t s)
+{
+    uint64_t x = x0;
+    for (int b = 0; s; ++b, s >>= 1)
+        if (s & 1) x = t.a[b] * x + t.c[b];
+    return ldexp((double)(x & 0xFFFFFFFFFFFFull), -48);
+}
+
+// GENMAT_SPD: column j draws rows i = j..n-1 in order, the diagonal gets + n
+// (in fp_t arithmetic), the value is mirrored to (j, i).  Element (i, j) of
+// the lower triangle is draw number j n - j (j - 1) / 2 + (i - j) + 1.
+template <typename T>
+__global__ void lx_genmat_spd_kernel(lx_lcg_table t, uint64_t x0, int n, T *__restrict__ A, int64_t lda)
+{
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / n, i = e % n;
+        const int64_t c = i >= j ? j : i, r = i >= j ? i : j;
+        const uint64_t s = (uint64_t)(c * n - c * (c - 1) / 2 + (r - c) + 1);
+        T v = (T)lx_drand48_at(t, x0, s);
+        if (i == j) v += (T)n;
+        A[j * lda + i] = v;
+    }
+}
+
+// GENMAT: X(i, j) is draw number j m + i + 1, converted to fp_t.
+template <typename T>
+__global__ void lx_genmat_kernel(lx_lcg_table t, uint64_t x0, int m, int n, T *__restrict__ X, int64_t ldx)
+{
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x)
+        X[(e / m) * ldx + e % m] = (T)lx_drand48_at(t, x0, (uint64_t)e + 1);
+}
+
+static uint64_t lx_drand48_seed(long seed) { return ((uint64_t)(uint32_t)seed << 16) | 0x330Eu; }
+
+extern "C" int linompss_gen_drand48_spd(int dtype, int n, long seed, void *dA, int64_t lda)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (n < 0) return -2;
+    if (lda < std::max(1, n)) return -5;
+    if (n == 0) return 0;
+    const lx_lcg_table t = lx_lcg_make();
+    const dim3 g(grid_for((int64_t)n * n)), b(256);
+    if (dtype == 0)
+        hipLaunchKernelGGL(lx_genmat_spd_kernel<double>, g, b, 0, 0, t, lx_drand48_seed(seed), n, (double *)dA, lda);
+    else
+        hipLaunchKernelGGL(lx_genmat_spd_kernel<float>, g, b, 0, 0, t, lx_drand48_seed(seed), n, (float *)dA, lda);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+extern "C" int linompss_gen_drand48(int dtype, int m, int n, long seed, void *dX, int64_t ldx)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (ldx < std::max(1, m)) return -6;
+    if ((int64_t)m * n == 0) return 0;
+    const lx_lcg_table t = lx_lcg_make();
+    const dim3 g(grid_for((int64_t)m * n)), b(256);
+    if (dtype == 0)
+        hipLaunchKernelGGL(lx_genmat_kernel<double>, g, b, 0, 0, t, lx_drand48_seed(seed), m, n, (double *)dX, ldx);
+    else
+        hipLaunchKernelGGL(lx_genmat_kernel<float>, g, b, 0, 0, t, lx_drand48_seed(seed), m, n, (float *)dX, ldx);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
