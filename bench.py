@@ -463,6 +463,10 @@ def main():
     ap.add_argument("--gp-n", type=int, default=8192)
     ap.add_argument("--gp-b", type=int, default=256, help="tile of the GP workload's FP64 factor")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run the factors with the library's default unit shapes (no autotuning)")
+    ap.add_argument("--no-tile-search", action="store_true",
+                    help="skip the extra runs at the tile size the autotuner finds fastest")
     args = ap.parse_args()
     if args.workload in ("batched", "gp", "hyper", "blas3"):
         if args.gpus != 1:
@@ -497,48 +501,93 @@ def main():
     log(f"generated in {time.perf_counter() - t0:.1f} s: lambda_max={lmax:.6f} alpha={alpha:.3e} "
         f"cond2~{(lmax + alpha) / alpha:.3g}")
 
-    # ---- mixed-precision refinement (the metric) ----
-    ir_opts = lx.IrOpts(30, 1e-14, 0.5, args.update_prec)
-    for w in range(args.warmup):
-        st = lx.dposv_ir_dev(A, B, X, b, ir_opts)
-        log(f"warmup {w}: sweeps={st.sweeps} fallback={st.fallback} berr={st.berr:.2e} "
-            f"factor={st.t_factor_s * 1e3:.1f} ms total={st.t_total_s * 1e3:.1f} ms")
-    lx.lib.linompss_dev_sync()
-    t_fac, t_launch, sweeps, berrs, fallbacks = [], [], [], [], []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        st = lx.dposv_ir_dev(A, B, X, b, ir_opts)
-        t_fac.append(st.t_factor_s)
-        # HIP events around the FP32 factor launch, on its stream (the last
-        # executor launch of the solve; the sweeps and residuals use other kernels)
-        t_launch.append(lx.last_exec_stats().exec_ms * 1e-3)
-        sweeps.append(st.sweeps)
-        berrs.append(st.berr)
-        fallbacks.append(st.fallback)
-    lx.lib.linompss_dev_sync()
-    t_ir = (time.perf_counter() - t0) / args.steps
-    berr_ir = lx.dberr_dev(A, X, B)
-    log(f"mixed: {t_ir * 1e3:.1f} ms/solve, factor {np.mean(t_fac) * 1e3:.1f} ms, sweeps {sweeps}, "
-        f"berr {berr_ir:.2e}, fallback {fallbacks}")
+    # ---- autotuning (outside every timed region) ----
+    # the unit shapes of both factors at the configuration's tile size
+    fp64_b = args.fp64_b or b
+    tuned = {}
+    if not args.no_tune:
+        for key, dt, bb, up in (("mixed_factor", np.float32, b, args.update_prec if args.update_prec else 0),
+                                ("fp64_factor", np.float64, fp64_b, 0)):
+            t0 = time.perf_counter()
+            best, tried = lx.tune_factor(n, bb, dt, "rl", up, reps=2)
+            tuned[key] = dict(best.as_dict(), candidates=len(tried), tune_s=round(time.perf_counter() - t0, 1))
+            log(f"tuned {key}: {tuned[key]}")
 
-    # ---- plain FP64 factor + solve (one persistent launch) ----
-    A2 = lx.DeviceMatrix(n, n)
-    X2 = lx.DeviceMatrix(n, nr)
-    t64, berr64 = [], None
-    for k in range(max(1, args.fp64_steps)):
-        lx.copy_dev(A2, A)
-        lx.copy_dev(X2, B)
+    def run_mixed(bb, steps, warmup, tag):
+        ir_opts = lx.IrOpts(30, 1e-14, 0.5, args.update_prec)
+        for w in range(warmup):
+            st = lx.dposv_ir_dev(A, B, X, bb, ir_opts)
+            log(f"{tag} warmup {w}: sweeps={st.sweeps} fallback={st.fallback} berr={st.berr:.2e} "
+                f"factor={st.t_factor_s * 1e3:.1f} ms total={st.t_total_s * 1e3:.1f} ms")
         lx.lib.linompss_dev_sync()
+        r = {"t_fac": [], "t_launch": [], "sweeps": [], "berrs": [], "fallbacks": []}
         t0 = time.perf_counter()
-        info = lx.posv_dev(A2, X2, args.fp64_b or b)
+        for k in range(steps):
+            st = lx.dposv_ir_dev(A, B, X, bb, ir_opts)
+            r["t_fac"].append(st.t_factor_s)
+            # HIP events around the FP32 factor launch, on its stream (the last
+            # executor launch of the solve; the sweeps and residuals use other kernels)
+            r["t_launch"].append(lx.last_exec_stats().exec_ms * 1e-3)
+            r["sweeps"].append(st.sweeps)
+            r["berrs"].append(st.berr)
+            r["fallbacks"].append(st.fallback)
         lx.lib.linompss_dev_sync()
-        t64.append(time.perf_counter() - t0)
-        st64 = lx.last_exec_stats()
-        log(f"fp64 posv {k}: {t64[-1] * 1e3:.1f} ms (launch {st64.exec_ms:.1f} ms, units {st64.units}) info={info}")
-    berr64 = lx.dberr_dev(A, X2, B)
-    t_fp64 = min(t64)
-    A2.free()
-    X2.free()
+        r["t"] = (time.perf_counter() - t0) / steps
+        r["berr"] = lx.dberr_dev(A, X, B)
+        log(f"{tag}: {r['t'] * 1e3:.1f} ms/solve, factor {np.mean(r['t_fac']) * 1e3:.1f} ms, sweeps {r['sweeps']}, "
+            f"berr {r['berr']:.2e}, fallback {r['fallbacks']}")
+        return r
+
+    def run_fp64(bb, steps, tag):
+        A2 = lx.DeviceMatrix(n, n)
+        X2 = lx.DeviceMatrix(n, nr)
+        t64, launch = [], []
+        for k in range(max(1, steps)):
+            lx.copy_dev(A2, A)
+            lx.copy_dev(X2, B)
+            lx.lib.linompss_dev_sync()
+            t0 = time.perf_counter()
+            info = lx.posv_dev(A2, X2, bb)
+            lx.lib.linompss_dev_sync()
+            t64.append(time.perf_counter() - t0)
+            st64 = lx.last_exec_stats()
+            launch.append(st64.exec_ms)
+            log(f"{tag} {k}: {t64[-1] * 1e3:.1f} ms (launch {st64.exec_ms:.1f} ms, units {st64.units}) info={info}")
+        berr = lx.dberr_dev(A, X2, B)
+        A2.free()
+        X2.free()
+        return min(t64), min(launch), berr
+
+    # ---- mixed-precision refinement (the metric) ----
+    mx = run_mixed(b, args.steps, args.warmup, "mixed")
+    t_ir, berr_ir = mx["t"], mx["berr"]
+    t_fac, t_launch, sweeps, fallbacks = mx["t_fac"], mx["t_launch"], mx["sweeps"], mx["fallbacks"]
+
+    # ---- plain FP64 factor + solve (one factor launch + the sweeps) ----
+    t_fp64, t_fp64_launch, berr64 = run_fp64(fp64_b, args.fp64_steps, "fp64 posv")
+
+    # ---- both paths again at the tile size the autotuner finds fastest ----
+    tile_search = None
+    if not args.no_tune and not args.no_tile_search:
+        ts = {}
+        for key, dt, up in (("mixed", np.float32, args.update_prec if args.update_prec else 0),
+                            ("fp64", np.float64, 0)):
+            t0 = time.perf_counter()
+            best, tried = lx.tune_factor(n, 0, dt, "rl", up, reps=2)
+            ts[key] = {"config": best.as_dict(), "candidates": len(tried), "tune_s": round(time.perf_counter() - t0, 1)}
+            log(f"tile search {key}: {ts[key]}")
+        m2 = run_mixed(ts["mixed"]["config"]["b"], max(3, args.steps // 2), 1, "mixed (tuned tile)")
+        f2, f2_launch, f2_berr = run_fp64(ts["fp64"]["config"]["b"], args.fp64_steps, "fp64 posv (tuned tile)")
+        ts["mixed"].update({"ms": round(m2["t"] * 1e3, 3), "gflops": round(flops / m2["t"] / 1e9, 2),
+                            "factor_launch_ms": round(float(np.mean(m2["t_launch"])) * 1e3, 3),
+                            "berr": m2["berr"], "sweeps": int(np.max(m2["sweeps"])),
+                            "fallback": int(np.max(m2["fallbacks"]))})
+        ts["fp64"].update({"ms": round(f2 * 1e3, 3), "gflops": round(flops / f2 / 1e9, 2),
+                           "factor_launch_ms": round(f2_launch, 3),
+                           "factor_frac_of_fp64_peak": round(flops / (f2_launch * 1e-3) / 1e12 / PEAK["fp64"], 4),
+                           "berr": f2_berr})
+        ts["speedup_vs_fp64"] = round(f2 / m2["t"], 3)
+        tile_search = ts
 
     t_kernel = float(np.mean(t_launch))
     achieved = flops / t_kernel / 1e12
@@ -607,10 +656,14 @@ def main():
         "factor_ms": round(float(np.mean(t_fac)) * 1e3, 3),
         "factor_launch_ms": round(t_kernel * 1e3, 3),
         "fp64_posv_ms": round(t_fp64 * 1e3, 3),
-        "fp64_b": args.fp64_b or b,
+        "fp64_b": fp64_b,
         "fp64_posv_gflops": round(flops / t_fp64 / 1e9, 2),
+        "fp64_factor_launch_ms": round(t_fp64_launch, 3),
+        "fp64_factor_frac_of_fp64_peak": round(flops / (t_fp64_launch * 1e-3) / 1e12 / PEAK["fp64"], 4),
         "fp64_berr": berr64,
         "speedup_vs_fp64": round(t_fp64 / t_ir, 3),
+        "tuned": tuned or None,
+        "tile_search": tile_search,
     }
     print(json.dumps(out), flush=True)
 

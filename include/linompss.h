@@ -394,6 +394,54 @@ int linompss_dlogdet_dev(int n, const double *dL, int64_t ldl, double *logdet, v
 int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const double *dX, int64_t ldx,
                        const double *dB, int64_t ldb, double *berr);
 
+/* ---- Factor autotuner (no counterpart in the reference, whose tile size
+ * and task shapes are fixed by its caller and its code).  A configuration is
+ * the tile size plus the shapes of the work units the planner cuts the tile
+ * tasks into (chol_rlmain.c:25-50 / chol_llmain.c:26-53 stay the task graph):
+ *   potrf_sub       diagonal tiles wider than this factor as a sub-DAG of
+ *                   potrf_sub-wide tiles (multiple of 64)
+ *   gemm_wide_m     rows of a trailing-update unit off the lookahead column
+ *                   (multiple of 128)
+ *   gemm_wide_n     columns of a trailing-update unit off the lookahead column
+ *                   (multiple of 128)
+ *   trsm_rows_wide  rows of a panel-solve unit off the lookahead tile
+ *                   (multiple of 64)
+ *   wide_tail       the last wide_tail panels use 128-column update units
+ *                   (-1: half of the panels)
+ * linompss_tune_factor times the factor of an n x n diagonally dominant SPD
+ * matrix it generates on the device (unit diagonal, off-diagonal entries in
+ * [0, 1/n)) under candidate configurations, by coordinate descent from the
+ * library's defaults (each candidate the fastest of reps launches, reps <= 0
+ * means 3), and records the fastest for (dtype, variant, n, best->b,
+ * update_prec): every later factor of that shape (linompss_chol_dev,
+ * linompss_posv_dev, the refinement driver's low-precision factor, the
+ * ompss_* entry points) uses it.  b > 0 keeps the tile size; b = 0 searches
+ * it among 384, 512, 768 and 1024 (those <= n).  Needs n*n elements of
+ * temporary device memory.  update_prec as in linompss_exec_opts (FP32 only).
+ * tried (may be NULL) receives up to max_tried configurations in the order
+ * they were timed, *ntried their count.  Returns 0, -i for argument i, or a
+ * LINOMPSS_ERR_* code.
+ * linompss_set_factor_config records a configuration (e.g. one saved from an
+ * earlier run) with the same effect; linompss_get_factor_config returns the
+ * recorded one (1 if there is one, 0 if the shape uses the defaults, which
+ * are then written to *cfg with ms = 0); linompss_clear_factor_configs forgets
+ * them all.  LINOMPSS_* environment knobs, where set, take precedence. */
+typedef struct {
+    int b;
+    int potrf_sub;
+    int gemm_wide_m;
+    int gemm_wide_n;
+    int trsm_rows_wide;
+    int wide_tail;
+    double ms;  /* factor launch time measured for this configuration */
+} linompss_factor_config;
+
+int linompss_tune_factor(int dtype, int variant, int n, int b, int update_prec, int reps,
+                         linompss_factor_config *best, linompss_factor_config *tried, int max_tried, int *ntried);
+int linompss_set_factor_config(int dtype, int variant, int n, int update_prec, const linompss_factor_config *cfg);
+int linompss_get_factor_config(int dtype, int variant, int n, int b, int update_prec, linompss_factor_config *cfg);
+int linompss_clear_factor_configs(void);
+
 /* Statistics of the last executor launch on this thread. */
 int linompss_last_exec_stats(linompss_exec_stats *st);
 

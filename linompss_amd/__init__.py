@@ -673,6 +673,60 @@ def plan_info(what: int, n: int, b: int, nrhs: int = 0, opts: ExecOpts | None = 
     return u.value, t.value, s.value
 
 
+# ---- factor autotuner -------------------------------------------------------------
+
+class FactorConfig(ctypes.Structure):
+    """Tile size and work-unit shapes of a factor plan (linompss_factor_config)."""
+    _fields_ = [("b", _i), ("potrf_sub", _i), ("gemm_wide_m", _i), ("gemm_wide_n", _i), ("trsm_rows_wide", _i),
+                ("wide_tail", _i), ("ms", _d)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_sig("linompss_tune_factor", _i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(FactorConfig), ctypes.POINTER(FactorConfig),
+     _i, _ip)
+_sig("linompss_set_factor_config", _i, _i, _i, _i, _i, ctypes.POINTER(FactorConfig))
+_sig("linompss_get_factor_config", _i, _i, _i, _i, _i, _i, ctypes.POINTER(FactorConfig))
+_sig("linompss_clear_factor_configs", _i)
+
+_DT = {np.dtype(np.float64): 0, np.dtype(np.float32): 1}
+
+
+def tune_factor(n: int, b: int = 0, dtype=np.float64, variant: str = "rl", update_prec: int = 0, reps: int = 3):
+    """Autotunes the factor of shape (dtype, variant, n, b, update_prec) on the
+    device (b = 0 also searches the tile size) and records the winner, which
+    every later factor of that shape uses.  Returns (best, tried): the winning
+    FactorConfig and the list of timed candidates (dicts, in order)."""
+    best = FactorConfig()
+    tried = (FactorConfig * 64)()
+    nt = _i(0)
+    rc = lib.linompss_tune_factor(_DT[np.dtype(dtype)], {"rl": 0, "ll": 1}[variant], n, b, update_prec, reps,
+                                  ctypes.byref(best), tried, 64, ctypes.byref(nt))
+    _check("linompss_tune_factor", rc, False)
+    return best, [tried[q].as_dict() for q in range(min(nt.value, 64))]
+
+
+def set_factor_config(n: int, cfg: FactorConfig, dtype=np.float64, variant: str = "rl", update_prec: int = 0):
+    _check("linompss_set_factor_config",
+           lib.linompss_set_factor_config(_DT[np.dtype(dtype)], {"rl": 0, "ll": 1}[variant], n, update_prec,
+                                          ctypes.byref(cfg)), False)
+
+
+def get_factor_config(n: int, b: int, dtype=np.float64, variant: str = "rl", update_prec: int = 0):
+    """(recorded, FactorConfig): the configuration a factor of this shape uses."""
+    cfg = FactorConfig()
+    rc = lib.linompss_get_factor_config(_DT[np.dtype(dtype)], {"rl": 0, "ll": 1}[variant], n, b, update_prec,
+                                        ctypes.byref(cfg))
+    if rc < 0:
+        raise LinompssError("linompss_get_factor_config", rc)
+    return rc == 1, cfg
+
+
+def clear_factor_configs() -> None:
+    lib.linompss_clear_factor_configs()
+
+
 # ---- batched systems -----------------------------------------------------------
 
 BATCH_NMAX = 512

@@ -120,7 +120,24 @@ void *scratch_get(lx_dev_ctx *c, int slot, size_t bytes)
     return c->scratch[slot];
 }
 
-void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_opts *o)
+// Factor configurations recorded by the autotuner (or the caller), per
+// (dtype, variant, n, b, update_prec); guarded by g_mu.
+struct lx_cfg_entry {
+    int dtype, variant, n, b, update_prec;
+    linompss_factor_config cfg;
+};
+std::vector<lx_cfg_entry> g_cfgs;
+
+const lx_cfg_entry *cfg_find(int dtype, int variant, int n, int b, int update_prec)
+{
+    for (const auto &e : g_cfgs)
+        if (e.dtype == dtype && e.variant == variant && e.n == n && e.b == b && e.update_prec == update_prec)
+            return &e;
+    return nullptr;
+}
+
+void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_opts *o,
+                 const lx_shape *sh = nullptr)
 {
     P.unit_m = LX_UNIT_M;
     P.unit_n = LX_UNIT_N;
@@ -140,6 +157,26 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
     // while the trailing matrix shrinks; 229.0 against 232.1 ms at n = 32768,
     // b = 512; the FP32 factor measured no gain at any tail length)
     if (dtype == 0) P.wide_tail = -1;
+    // GEMM units taller than 128 rows while the step's parallelism allows,
+    // panel solves of 128 rows off the lookahead tile: each publication writes
+    // the XCD's L2 back.  Measured at n = 32768, b = 512: FP32 factor 113.4 ms
+    // with 128-row GEMM units, 102.1 with 256, 100.9 with 512 everywhere, 99.3
+    // with up to 512 where each step keeps two units per workgroup, 98.7 with
+    // 128-row panel-solve units besides; FP64 226.5 ms with up to 256 rows
+    // against 228.9 with 128 (229.7 with 512, no gain from taller solves)
+    P.gemm_wide_m = dtype == 1 ? 512 : 256;
+    if (dtype == 1) P.trsm_rows_wide = 128;
+    // a configuration recorded for this factor's shape (autotuner)
+    if (sh && (sh->what == LX_W_FACTOR_RL || sh->what == LX_W_FACTOR_LL || sh->what == LX_W_POSV_RL)) {
+        const int variant = sh->what == LX_W_FACTOR_LL ? 1 : 0;
+        if (const lx_cfg_entry *e = cfg_find(sh->dtype, variant, sh->n, sh->b, P.update_prec)) {
+            P.potrf_sub = e->cfg.potrf_sub;
+            P.gemm_wide_m = e->cfg.gemm_wide_m;
+            P.gemm_wide_n = e->cfg.gemm_wide_n;
+            P.trsm_rows_wide = e->cfg.trsm_rows_wide;
+            P.wide_tail = e->cfg.wide_tail;
+        }
+    }
     // tuning knobs: the sub-tile of diagonal-tile factorisations, the width
     // of GEMM units off the lookahead column, 128 x 256 split-BF16 blocks
     if (const char *ev = getenv("LINOMPSS_POTRF_SUB")) {
@@ -155,15 +192,6 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         const int v = atoi(ev);
         if (v >= -1) P.wide_tail = v;
     }
-    // GEMM units taller than 128 rows while the step's parallelism allows,
-    // panel solves of 128 rows off the lookahead tile: each publication writes
-    // the XCD's L2 back.  Measured at n = 32768, b = 512: FP32 factor 113.4 ms
-    // with 128-row GEMM units, 102.1 with 256, 100.9 with 512 everywhere, 99.3
-    // with up to 512 where each step keeps two units per workgroup, 98.7 with
-    // 128-row panel-solve units besides; FP64 226.5 ms with up to 256 rows
-    // against 228.9 with 128 (229.7 with 512, no gain from taller solves)
-    P.gemm_wide_m = dtype == 1 ? 512 : 256;
-    if (dtype == 1) P.trsm_rows_wide = 128;
     if (const char *ev = getenv("LINOMPSS_WIDE_M")) {
         const int v = atoi(ev);
         if (v >= 128 && v % 128 == 0) P.gemm_wide_m = v;
@@ -205,7 +233,7 @@ bool build_plan(lx_planner &pl, const lx_shape &sh)
 lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linompss_exec_opts *o)
 {
     lx_plan_params P;
-    fill_params(P, sh.dtype, workers, o);
+    fill_params(P, sh.dtype, workers, o, &sh);
     char key[256];
     snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d",
              sh.what, sh.dtype, sh.n, sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec,
@@ -802,11 +830,98 @@ extern "C" int linompss_plan_tdg(int what, int n, int b, int nrhs, int *types, i
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Recorded factor configurations (the search itself is in lx_tune.hip).
+
+static bool cfg_valid(const linompss_factor_config *c)
+{
+    return c->b > 0 && c->potrf_sub >= 64 && c->potrf_sub % 64 == 0 && c->gemm_wide_m >= 128 &&
+           c->gemm_wide_m % 128 == 0 && c->gemm_wide_n >= 128 && c->gemm_wide_n % 128 == 0 &&
+           c->trsm_rows_wide >= 64 && c->trsm_rows_wide % 64 == 0 && c->wide_tail >= -1;
+}
+
+extern "C" int linompss_set_factor_config(int dtype, int variant, int n, int update_prec,
+                                          const linompss_factor_config *cfg)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (variant != 0 && variant != 1) return -2;
+    if (n <= 0) return -3;
+    if (update_prec < 0 || update_prec > 2 || (dtype == 0 && update_prec != 0)) return -4;
+    if (!cfg || !cfg_valid(cfg)) return -5;
+    std::lock_guard<std::mutex> g(g_mu);
+    for (auto &e : g_cfgs)
+        if (e.dtype == dtype && e.variant == variant && e.n == n && e.b == cfg->b && e.update_prec == update_prec) {
+            e.cfg = *cfg;
+            return 0;
+        }
+    g_cfgs.push_back({dtype, variant, n, cfg->b, update_prec, *cfg});
+    return 0;
+}
+
+extern "C" int linompss_get_factor_config(int dtype, int variant, int n, int b, int update_prec,
+                                          linompss_factor_config *cfg)
+{
+    if (dtype != 0 && dtype != 1) return -1;
+    if (variant != 0 && variant != 1) return -2;
+    if (n <= 0) return -3;
+    if (b <= 0) return -4;
+    if (update_prec < 0 || update_prec > 2 || (dtype == 0 && update_prec != 0)) return -5;
+    if (!cfg) return -6;
+    std::lock_guard<std::mutex> g(g_mu);
+    if (const lx_cfg_entry *e = cfg_find(dtype, variant, n, b, update_prec)) {
+        *cfg = e->cfg;
+        return 1;
+    }
+    // the defaults (environment knobs included)
+    lx_cfg_defaults(dtype, update_prec, b, cfg);
+    return 0;
+}
+
+bool lx_cfg_lookup(int dtype, int variant, int n, int b, int update_prec, linompss_factor_config *cfg)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    const lx_cfg_entry *e = cfg_find(dtype, variant, n, b, update_prec);
+    if (e) *cfg = e->cfg;
+    return e != nullptr;
+}
+
+void lx_cfg_forget(int dtype, int variant, int n, int b, int update_prec)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cfgs.erase(std::remove_if(g_cfgs.begin(), g_cfgs.end(),
+                                [&](const lx_cfg_entry &e) {
+                                    return e.dtype == dtype && e.variant == variant && e.n == n && e.b == b &&
+                                           e.update_prec == update_prec;
+                                }),
+                 g_cfgs.end());
+}
+
+void lx_cfg_defaults(int dtype, int update_prec, int b, linompss_factor_config *cfg)
+{
+    linompss_exec_opts o = {};
+    o.policy = 1;
+    o.update_prec = update_prec;
+    lx_plan_params P;
+    fill_params(P, dtype, 512, &o);
+    *cfg = {b, P.potrf_sub, P.gemm_wide_m, P.gemm_wide_n, P.trsm_rows_wide, P.wide_tail, 0.0};
+}
+
+extern "C" int linompss_clear_factor_configs(void)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cfgs.clear();
+    return 0;
+}
+
 extern "C" int linompss_plan_info(int what, int n, int b, int nrhs, const linompss_exec_opts *opts, int *units,
                                   int *tasks, double *sim_ms)
 {
     lx_plan_params P;
-    fill_params(P, 0, 512, opts);
+    {
+        std::lock_guard<std::mutex> g(g_mu);  // the recorded configurations
+        const lx_shape sh = {what, 0, n, b, nrhs};
+        fill_params(P, 0, 512, opts, &sh);
+    }
     lx_planner pl(P);
     if (plan_host(what, n, b, nrhs, opts, pl)) return -2;
     if (units) *units = (int)pl.units().size();

@@ -261,7 +261,9 @@ __global__ void __launch_bounds__(LX_WG_THREADS, lx_fam_occ<FAM>::waves_per_simd
                             go = 0;
                             break;
                         }
-                        if (wall_clock64() - t0 > spin_ticks) {
+                        // the clock is a scalar read (one value per wave); taken
+                        // through readfirstlane so no lane can decide apart
+                        if (lx_uniform(wall_clock64() - t0 > spin_ticks ? 1 : 0)) {
                             // every lane stores the same values (no lane branch)
                             atomicAdd(&st->timeouts, lane == 0 ? 1 : 0);
                             atomicOr(&st->abort, lane == 0 ? 1 : 0);  // read-modify-write: seen at once

@@ -45,6 +45,12 @@ std::mutex &lx_dev_mutex();
 // Grow-only device scratch owned by the library (one per slot and device).
 void *lx_scratch(int slot, size_t bytes);
 
+// Recorded factor configurations (lx_exec.hip): lookup (false when the shape
+// has none), removal, and the library's defaults for a tile size.
+bool lx_cfg_lookup(int dtype, int variant, int n, int b, int update_prec, linompss_factor_config *cfg);
+void lx_cfg_forget(int dtype, int variant, int n, int b, int update_prec);
+void lx_cfg_defaults(int dtype, int update_prec, int b, linompss_factor_config *cfg);
+
 // Elementwise helpers enqueued on a stream.
 void lx_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
 

@@ -71,7 +71,7 @@ __device__ __forceinline__ void lx_sw_wait_ge(const int *p, int need, int *abort
         const long long t0 = wall_clock64();
         while (lx_sw_uniform(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
             if (lx_sw_uniform(__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
-            if (wall_clock64() - t0 > spin_ticks) {
+            if (lx_sw_uniform(wall_clock64() - t0 > spin_ticks ? 1 : 0)) {  // one decision per wave
                 atomicOr(abort, (threadIdx.x & 63) == 0 ? 1 : 0);  // read-modify-write: seen at once
                 break;
             }
