@@ -442,6 +442,23 @@ int linompss_set_factor_config(int dtype, int variant, int n, int update_prec, c
 int linompss_get_factor_config(int dtype, int variant, int n, int b, int update_prec, linompss_factor_config *cfg);
 int linompss_clear_factor_configs(void);
 
+/* ---- Gaussian-process kernel on the device (no counterpart in the
+ * reference; it feeds the factor and the adjoints above so that a GP
+ * likelihood's forward and backward passes keep K on the device).
+ * linompss_gp_kernel_dev: K_ij = sf2 exp(-|x_i - x_j|^2 / (2 ell^2)) +
+ * noise delta_ij for the n points X (n x d column-major, ldx >= n, d <= 16),
+ * both triangles of K (ldk >= n) written.
+ * linompss_gp_grad_dev: with Kbar the adjoint of K in the lower representation
+ * (dLoss = sum_{i>=j} Kbar_ij dK_ij, as linompss_dpotrf_adjoint_dev writes
+ * it), grads[0] = dLoss/d ell, grads[1] = dLoss/d sf2, grads[2] = dLoss/d noise
+ * (host array of 3), and, when dXbar is not NULL, dXbar = dLoss/dX (n x d,
+ * device).  Sums in a fixed order (deterministic).  Both return 0, -i for
+ * argument i, or a LINOMPSS_ERR_* code, after the stream has completed. */
+int linompss_gp_kernel_dev(int n, int d, const double *dX, int64_t ldx, double ell, double sf2, double noise,
+                           double *dK, int64_t ldk, void *stream);
+int linompss_gp_grad_dev(int n, int d, const double *dX, int64_t ldx, double ell, double sf2, const double *dKbar,
+                         int64_t ldkb, double *grads, double *dXbar, int64_t ldxb, void *stream);
+
 /* Statistics of the last executor launch on this thread. */
 int linompss_last_exec_stats(linompss_exec_stats *st);
 

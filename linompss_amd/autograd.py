@@ -41,6 +41,12 @@ lib.linompss_dlogdet_dev.restype = _i
 lib.linompss_dlogdet_dev.argtypes = [_i, _p, _i64, ctypes.POINTER(_d), _p]
 
 
+lib.linompss_gp_kernel_dev.restype = _i
+lib.linompss_gp_kernel_dev.argtypes = [_i, _i, _p, _i64, _d, _d, _d, _p, _i64, _p]
+lib.linompss_gp_grad_dev.restype = _i
+lib.linompss_gp_grad_dev.argtypes = [_i, _i, _p, _i64, _d, _d, _p, _i64, ctypes.POINTER(_d), _p, _i64, _p]
+
+
 def _f64(t: torch.Tensor) -> np.ndarray:
     """Column-major float64 copy of a 2-D CPU tensor."""
     return np.asfortranarray(t.detach().to(dtype=torch.float64, device="cpu").numpy())
@@ -213,3 +219,149 @@ def gp_nll_fused(points: torch.Tensor, y: torch.Tensor, lengthscale: torch.Tenso
     if K is None:
         K = gp_kernel(points, lengthscale, noise)
     return GaussianNLL.apply(K, y, b, timing)
+
+
+# ---- Gaussian-process likelihood with K built on the device --------------------
+
+class _GPWork:
+    """Device buffers of one problem shape, reused across calls: K (the factor
+    in place), Kbar, the points, y, a = L^-1 y, bbar and dX."""
+
+    def __init__(self, n: int, d: int):
+        self.n, self.d = n, d
+        self.K = DeviceMatrix(n, n)
+        self.Kb = DeviceMatrix(n, n)
+        self.P = DeviceMatrix(n, d)
+        self.y = DeviceMatrix(n, 1)
+        self.a = DeviceMatrix(n, 1)
+        self.bb = DeviceMatrix(n, 1)
+        self.dX = DeviceMatrix(n, d)
+
+    def free(self):
+        for m in (self.K, self.Kb, self.P, self.y, self.a, self.bb, self.dX):
+            m.free()
+
+
+_GP_WS: dict = {}
+
+
+def _gp_work(n: int, d: int) -> _GPWork:
+    ws = _GP_WS.get((n, d))
+    if ws is None:
+        for k in list(_GP_WS):  # one shape at a time: K and Kbar are n^2 each
+            _GP_WS.pop(k).free()
+        ws = _GP_WS[(n, d)] = _GPWork(n, d)
+    return ws
+
+
+def gp_release_workspace() -> None:
+    """Frees the device buffers GPMarginalLikelihood keeps between calls."""
+    for k in list(_GP_WS):
+        _GP_WS.pop(k).free()
+
+
+def gp_kernel_dev(points: np.ndarray, ell: float, sf2: float = 1.0, noise: float = 0.0) -> np.ndarray:
+    """K built on the device (linompss_gp_kernel_dev), downloaded."""
+    P = np.asfortranarray(points, dtype=np.float64)
+    n, d = P.shape
+    dP = DeviceMatrix.from_host(P)
+    dK = DeviceMatrix(n, n)
+    try:
+        _check("linompss_gp_kernel_dev",
+               lib.linompss_gp_kernel_dev(n, d, dP.ptr, n, float(ell), float(sf2), float(noise), dK.ptr, n, None),
+               False)
+        return dK.to_host()
+    finally:
+        dP.free()
+        dK.free()
+
+
+def _scalar(v) -> float:
+    return float(v.detach()) if isinstance(v, torch.Tensor) else float(v)
+
+
+def _gp_forward_dev(ws: _GPWork, ell: float, sf2: float, noise: float, b: int) -> float:
+    """K from the points, L in place, a = L^-1 y; returns log det K."""
+    n, d = ws.n, ws.d
+    _check("linompss_gp_kernel_dev",
+           lib.linompss_gp_kernel_dev(n, d, ws.P.ptr, n, ell, sf2, noise, ws.K.ptr, n, None), False)
+    _check("linompss_chol_dev", lib.linompss_chol_dev(0, 0, n, b, ws.K.ptr, n, None, None))
+    copy_dev(ws.a, ws.y)
+    _check("linompss_trsm_dev", lib.linompss_trsm_dev(0, 0, n, 1, ws.K.ptr, n, ws.a.ptr, n, None), False)
+    ld = _d(0.0)
+    _check("linompss_dlogdet_dev", lib.linompss_dlogdet_dev(n, ws.K.ptr, n, ctypes.byref(ld), None), False)
+    return ld.value
+
+
+class GPMarginalLikelihood(torch.autograd.Function):
+    """nll(X, y; ell, sf2, noise) = y^T K^-1 y / 2 + log det K / 2 + n log(2 pi) / 2
+    with K_ij = sf2 exp(-|x_i - x_j|^2 / (2 ell^2)) + noise delta_ij built on
+    the device from the points: only X (n x d), y and scalars cross PCIe.
+    Forward: K, FP64 factor in place, one sweep, log det.  Backward: K and the
+    factor recomputed (nothing n x n is kept between the passes), the solve
+    adjoint, the blocked Cholesky adjoint with the log det term, then the
+    contraction of Kbar with dK/dell, dK/dsf2, dK/dnoise and dK/dX on the
+    device.  Gradients flow to X, y and to ell / sf2 / noise when they are
+    tensors.  ``timing`` (a dict, optional) receives each pass's device seconds."""
+
+    @staticmethod
+    def forward(ctx, points, y, ell, sf2, noise, b: int = 256, timing: dict | None = None):
+        import time
+        P = _f64(points)
+        n, d = P.shape
+        ws = _gp_work(n, d)
+        ws.P.upload(P)
+        ws.y.upload(y.detach().to(torch.float64).numpy().reshape(-1, 1))
+        vals = (_scalar(ell), _scalar(sf2), _scalar(noise))
+        lib.linompss_dev_sync()
+        t0 = time.perf_counter()
+        logdet = _gp_forward_dev(ws, *vals, b)
+        lib.linompss_dev_sync()
+        if timing is not None:
+            timing["forward_s"] = time.perf_counter() - t0
+        a = ws.a.to_host()[:, 0]
+        ctx.save_for_backward(points, y)
+        ctx.vals, ctx.b, ctx.timing = vals, b, timing
+        ctx.is_tensor = [isinstance(v, torch.Tensor) for v in (ell, sf2, noise)]
+        nll = 0.5 * float(a @ a) + 0.5 * logdet + 0.5 * n * math.log(2.0 * math.pi)
+        return torch.tensor(nll, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        import time
+        points, y = ctx.saved_tensors
+        ell, sf2, noise = ctx.vals
+        P = _f64(points)
+        n, d = P.shape
+        ws = _gp_work(n, d)
+        ws.P.upload(P)
+        ws.y.upload(y.detach().to(torch.float64).numpy().reshape(-1, 1))
+        lib.linompss_dev_sync()
+        t0 = time.perf_counter()
+        _gp_forward_dev(ws, ell, sf2, noise, ctx.b)  # K, L and a recomputed
+        copy_dev(ws.bb, ws.a)  # abar = a (from a^T a / 2)
+        # bbar = L^-T a (= K^-1 y) and Lbar = -tril(bbar a^T), written without reading Lbar
+        _check("linompss_dtrsm_adjoint_dev", lib.linompss_dtrsm_adjoint_dev(
+            0, n, 1, ws.K.ptr, n, ws.a.ptr, n, ws.bb.ptr, n, ws.Kb.ptr, n, 0, None), False)
+        # + the log det term, then the reverse sweep: Lbar -> Kbar (lower representation)
+        _check("linompss_dpotrf_adjoint_dev", lib.linompss_dpotrf_adjoint_dev(
+            n, ws.K.ptr, n, ws.Kb.ptr, n, 0.5, None), False)
+        grads = (_d * 3)()
+        want_x = ctx.needs_input_grad[0]
+        _check("linompss_gp_grad_dev", lib.linompss_gp_grad_dev(
+            n, d, ws.P.ptr, n, ell, sf2, ws.Kb.ptr, n, grads, ws.dX.ptr if want_x else None, n, None), False)
+        lib.linompss_dev_sync()
+        if ctx.timing is not None:
+            ctx.timing["backward_s"] = time.perf_counter() - t0
+        gv = float(g)
+        gx = torch.from_numpy(ws.dX.to_host() * gv).to(points.dtype) if want_x else None
+        gy = torch.from_numpy(ws.bb.to_host()[:, 0] * gv).to(y.dtype) if ctx.needs_input_grad[1] else None
+        gs = [torch.tensor(grads[q] * gv, dtype=torch.float64) if ctx.is_tensor[q] else None for q in range(3)]
+        return gx, gy, gs[0], gs[1], gs[2], None, None
+
+
+def gp_nll_device(points: torch.Tensor, y: torch.Tensor, lengthscale, sf2=1.0, noise=1e-2, b: int = 256,
+                  timing: dict | None = None) -> torch.Tensor:
+    """The GP negative log marginal likelihood with K built on the device
+    (GPMarginalLikelihood)."""
+    return GPMarginalLikelihood.apply(points, y, lengthscale, sf2, noise, b, timing)

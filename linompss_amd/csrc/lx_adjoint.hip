@@ -36,25 +36,45 @@
 //           (n passed as jb), one workgroup per 64-column block J:
 //           Z_J = L[J, J]^-1 into Zb + 4096 J (64 x 64, ld 64; identity past
 //           the block's order);
-//   mode 1: Pbar = level-2 adjoint of D = chol(P) applied to Dbar (Abar[J, J],
-//           lower), written back in place; Sb (64 x 64, ld 64) = Pbar + Pbar^T.
-// Level-2 adjoint, columns j = jb-1 .. 0 of the block (the scalar case of the
-// block recurrences above): cbar = Dbar[>j, j] / d_j; Dbar[j, j] -= cbar . c;
-// pbar = Dbar[j, j] / (2 d_j); Dbar[>j, <j] -= cbar L[j, <j];
-// Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j].
+//   mode 1: Pbar = the adjoint of D = chol(P) applied to Dbar (Abar[J, J],
+//           lower), written back in place (lower representation); Sb (64 x 64,
+//           ld 64) = Pbar + Pbar^T.  Zb holds Z = D^-1 from mode 0.
+// Mode 1 is Murray's closed form for one block (eq. 10 of the paper in the
+// file header), three 64 x 64 x 64 products on the matrix cores:
+//     M = D^T Dbar,  Phi = tril(M) with its diagonal halved,
+//     G = Z^T Phi Z,  Pbar_ij = G_ij + G_ji (i > j),  Pbar_ii = G_ii,
+// the same result as the column-by-column (level-2) recurrences
+//     cbar = Dbar[>j, j] / d_j; Dbar[j, j] -= cbar . c; pbar = Dbar[j, j] / (2 d_j);
+//     Dbar[>j, <j] -= cbar L[j, <j]; Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j]
+// (which took 64 dependent steps of three barriers each, about 120 us per block).
+// A block of order jb < 64 is padded: D and Z with the identity, Dbar with
+// zeros, so every product keeps the padding out of the leading jb x jb part.
+__device__ __forceinline__ void lx_adj_mm64(lx_d4 (&acc)[4], const double (*A)[65], bool at, const double (*B)[65])
+{
+    // the wave's 16-row strip of op(A) B (op(A) = A^T when at), 64 x 64, from LDS
+    const int lane = lx_lane(), lr = lane & 15, lk = lane >> 4, r0 = 16 * lx_wave();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = lx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+        const double a = at ? A[k0 + lk][r0 + lr] : A[r0 + lr][k0 + lk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = lx_mfma<double>::mac(a, B[k0 + lk][16 * j + lr], acc[j]);
+    }
+}
+
 __global__ void __launch_bounds__(LX_WG_THREADS)
 lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl, double *__restrict__ Ab,
                    int64_t lda, double *__restrict__ Zb, double *__restrict__ Sb)
 {
     __shared__ double sL[64][65], sX[64][65];
-    __shared__ double s_red[4];
     if (mode == 0) {
         const int j0 = 64 * (int)blockIdx.x;
         jb = min(64, jb - j0);
         Ld += (int64_t)j0 * ldl + j0;
         Zb += (int64_t)blockIdx.x * 4096;
     }
-    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    const int tid = lx_tid(), lane = tid & 63, lr = lane & 15, lk = lane >> 4, r0 = 16 * lx_wave();
     for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
         const int r = e & 63, c = e >> 6;
         const bool in = r < jb && c < jb && r >= c;
@@ -70,56 +90,40 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
         }
         return;
     }
-    __shared__ double s_part[4][64];
-    const int cc = tid & 63, part = tid >> 6;  // column of the row update, and a quarter of the rows
-    for (int j = jb - 1; j >= 0; --j) {
-        const double d = sL[j][j];
-        // cbar (rows > j of column j), and its dot with c (wave 0, lane = row)
-        if (w == 0) {
-            double cb = 0.0;
-            if (lane > j && lane < jb) {
-                cb = sX[lane][j] / d;
-                sX[lane][j] = cb;
-            }
-            double t = cb * (lane > j && lane < jb ? sL[lane][j] : 0.0);
+    lx_d4 acc[4];
+    // M = D^T Dbar, then Phi into sX and Z into sL
+    lx_adj_mm64(acc, sL, true, sX);
+    __syncthreads();
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-            if (lane == 0) {
-                const double pbar = (sX[j][j] - t) / (2.0 * d);
-                sX[j][j] = pbar;
-                s_red[0] = pbar;
-            }
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lx_mfma<double>::row(lk, e), c = 16 * j + lr;
+            sX[r][c] = r > c ? acc[j][e] : (r == c ? 0.5 * acc[j][e] : 0.0);
         }
-        __syncthreads();
-        // thread (column cc < j, rows 16 part .. 16 part + 15):
-        //   Dbar[r, cc] -= cbar_r L[j, cc] for rows r > j, and the partial
-        //   sum over those rows of cbar_r L[r, cc] for the row update below
-        double ps = 0.0;
-        if (cc < j) {
-            const double ljc = sL[j][cc];
-#pragma unroll 4
-            for (int q = 0; q < 16; ++q) {
-                const int r = part * 16 + q;
-                if (r > j && r < jb) {
-                    const double cb = sX[r][j];
-                    sX[r][cc] -= cb * ljc;
-                    ps += cb * sL[r][cc];
-                }
-            }
-        }
-        s_part[part][cc] = ps;
-        __syncthreads();
-        // Dbar[j, <j] -= cbar^T L[>j, <j] + 2 pbar L[j, <j]
-        if (part == 0 && cc < j)
-            sX[j][cc] -= ((s_part[0][cc] + s_part[1][cc]) + (s_part[2][cc] + s_part[3][cc])) +
-                         2.0 * s_red[0] * sL[j][cc];
-        __syncthreads();
-    }
+    for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) sL[e & 63][e >> 6] = Zb[e];
+    __syncthreads();
+    // T = Z^T Phi into sX
+    lx_adj_mm64(acc, sL, true, sX);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sX[r0 + lx_mfma<double>::row(lk, e)][16 * j + lr] = acc[j][e];
+    __syncthreads();
+    // G = T Z into sX
+    lx_adj_mm64(acc, sX, false, sL);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sX[r0 + lx_mfma<double>::row(lk, e)][16 * j + lr] = acc[j][e];
+    __syncthreads();
     for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
         const int r = e & 63, c = e >> 6;
-        if (r < jb && c < jb && r >= c) Ab[(int64_t)c * lda + r] = sX[r][c];
-        const double v = r == c ? 2.0 * sX[r][r] : (r > c ? sX[r][c] : sX[c][r]);
-        Sb[c * 64 + r] = (r < jb && c < jb) ? v : 0.0;
+        const double g = sX[r][c] + sX[c][r];
+        if (r < jb && c < jb && r >= c) Ab[(int64_t)c * lda + r] = r > c ? g : sX[r][r];
+        Sb[c * 64 + r] = (r < jb && c < jb) ? g : 0.0;
     }
 }
 
@@ -220,7 +224,8 @@ int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda
                 break;
         }
         // Pbar (in place of Dbar) and S = Pbar + Pbar^T
-        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda, nullptr, Sb);
+        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda,
+                           Zall + (size_t)J * 4096, Sb);
         if (p > 0) {
             const double *R = L + j0;           // L[J, <J]: jb x p, ld ldl
             double *Rbar = Ab + j0;            // Abar[J, <J]

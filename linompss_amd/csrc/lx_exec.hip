@@ -65,8 +65,8 @@ struct lx_dev_ctx {
     int wall_khz = 0;
     int occ[2][3] = {};  // resident WGs per CU: [FP64 / FP32][op family]
     std::list<lx_dplan> plans;  // most recent first
-    void *scratch[12] = {};
-    size_t scratch_sz[12] = {};
+    void *scratch[16] = {};
+    size_t scratch_sz[16] = {};
 };
 
 std::mutex g_mu;
@@ -103,7 +103,7 @@ lx_dev_ctx *ctx_get()
 
 void *scratch_get(lx_dev_ctx *c, int slot, size_t bytes)
 {
-    if (slot < 0 || slot >= 12) return nullptr;
+    if (slot < 0 || slot >= 16) return nullptr;
     if (c->scratch_sz[slot] >= bytes && c->scratch[slot]) return c->scratch[slot];
     if (c->scratch[slot]) {
         (void)hipDeviceSynchronize();
@@ -286,7 +286,9 @@ void *lx_scratch(int slot, size_t bytes)
 
 // Scratch slots: 0 = executor control (status, counters, info, sys table),
 // 1 = factor workspace (inverse diagonal blocks), 2 = executor profile,
-// 3 = sweep flags, 4..7 = refinement workspaces, 8 = residual partial sums.
+// 3 = sweep flags, 4..7 = refinement workspaces, 8 = residual partial sums,
+// 9 = adjoint workspace, 10 = log det, 11 = FP16 operand planes, 12 = GP
+// gradient partials.
 static bool lx_trace_on()
 {
     static int on = -1;

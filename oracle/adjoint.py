@@ -87,15 +87,15 @@ def trsm_adjoint(trans: str, L: np.ndarray, X: np.ndarray, Xbar: np.ndarray):
     return Bbar, Lbar
 
 
-def gp_kernel(P: np.ndarray, ell: float, noise: float) -> np.ndarray:
-    """K_ij = exp(-|p_i - p_j|^2 / (2 ell^2)) + noise delta_ij."""
+def gp_kernel(P: np.ndarray, ell: float, noise: float, sf2: float = 1.0) -> np.ndarray:
+    """K_ij = sf2 exp(-|p_i - p_j|^2 / (2 ell^2)) + noise delta_ij."""
     d2 = ((P[:, None, :] - P[None, :, :]) ** 2).sum(-1)
-    return np.exp(-d2 / (2.0 * ell * ell)) + noise * np.eye(P.shape[0])
+    return sf2 * np.exp(-d2 / (2.0 * ell * ell)) + noise * np.eye(P.shape[0])
 
 
-def gp_nll(P: np.ndarray, y: np.ndarray, ell: float, noise: float) -> float:
+def gp_nll(P: np.ndarray, y: np.ndarray, ell: float, noise: float, sf2: float = 1.0) -> float:
     """y^T K^-1 y / 2 + log det K / 2 + n log(2 pi) / 2."""
-    K = gp_kernel(P, ell, noise)
+    K = gp_kernel(P, ell, noise, sf2)
     L = np.linalg.cholesky(K)
     a = solve_triangular(L, y, lower=True)
     return float(0.5 * a @ a + np.log(np.diag(L)).sum() + 0.5 * len(y) * np.log(2.0 * np.pi))
@@ -112,3 +112,21 @@ def gp_nll_grad(P: np.ndarray, y: np.ndarray, ell: float, noise: float):
     dK = 0.5 * (Ki - np.outer(a, a))
     dK = 0.5 * (dK + dK.T)
     return dK, float((dK * K0 * d2).sum() / ell ** 3)
+
+
+def gp_nll_grad_all(P: np.ndarray, y: np.ndarray, ell: float, noise: float, sf2: float = 1.0) -> dict:
+    """Every gradient of gp_nll through the symmetric dK = (K^-1 - a a^T) / 2:
+    d ell = sum_ij dK_ij k_ij |p_i - p_j|^2 / ell^3, d sf2 = sum_ij dK_ij k_ij / sf2,
+    d noise = trace(dK), dP_i = -2 / ell^2 sum_j dK_ij k_ij (p_i - p_j)
+    (k = sf2 exp(...), the noise-free part of K)."""
+    diff = P[:, None, :] - P[None, :, :]
+    d2 = (diff ** 2).sum(-1)
+    k = sf2 * np.exp(-d2 / (2.0 * ell * ell))
+    K = k + noise * np.eye(P.shape[0])
+    Ki = np.linalg.inv(K)
+    a = Ki @ y
+    dK = 0.5 * (Ki - np.outer(a, a))
+    dK = 0.5 * (dK + dK.T)
+    W = dK * k
+    return {"dell": float((W * d2).sum() / ell ** 3), "dsf2": float(W.sum() / sf2), "dnoise": float(np.trace(dK)),
+            "dP": -2.0 / (ell * ell) * (W[:, :, None] * diff).sum(1), "dy": a}

@@ -80,3 +80,32 @@ def test_gp_gradient_central_differences():
         E[i, j] = E[j, i] = 1.0
         fdk = (nll_K(K + h * E) - nll_K(K - h * E)) / (2 * h)
         assert abs(np.sum(dK * E) - fdk) <= 1e-6 * max(1.0, abs(fdk))
+
+
+def test_gp_all_gradients_central_differences():
+    """gp_nll_grad_all (closed forms through dK) against central differences of
+    gp_nll in every parameter: the length scale, the amplitude, the noise, one
+    coordinate of one point, and one entry of y."""
+    n, ell, noise, sf2, h = 36, 0.35, 2e-2, 1.7, 1e-6
+    rng = np.random.default_rng(11)
+    P = rng.uniform(0, 1, (n, 3))
+    y = np.linalg.cholesky(adj.gp_kernel(P, ell, noise, sf2)) @ rng.standard_normal(n)
+    g = adj.gp_nll_grad_all(P, y, ell, noise, sf2)
+    f = lambda **kw: adj.gp_nll(kw.get("P", P), kw.get("y", y), kw.get("ell", ell), kw.get("noise", noise),
+                                kw.get("sf2", sf2))
+    fd = lambda name, v: (f(**{name: v + h}) - f(**{name: v - h})) / (2 * h)
+    for name, v in (("ell", ell), ("noise", noise), ("sf2", sf2)):
+        assert abs(g["d" + name] - fd(name, v)) <= 1e-6 * max(1.0, abs(g["d" + name])), name
+    for (i, q) in [(4, 0), (17, 2)]:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i, q] += h
+        Pm[i, q] -= h
+        fdp = (f(P=Pp) - f(P=Pm)) / (2 * h)
+        assert abs(g["dP"][i, q] - fdp) <= 1e-6 * max(1.0, abs(fdp))
+    yp, ym = y.copy(), y.copy()
+    yp[9] += h
+    ym[9] -= h
+    assert abs(g["dy"][9] - (f(y=yp) - f(y=ym)) / (2 * h)) <= 1e-6 * max(1.0, abs(g["dy"][9]))
+    # with sf2 = 1 the older function agrees
+    assert adj.gp_nll_grad(P, y, ell, noise)[1] == pytest.approx(adj.gp_nll_grad_all(P, y, ell, noise)["dell"],
+                                                                 rel=1e-12)

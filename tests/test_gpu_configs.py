@@ -124,3 +124,33 @@ def test_gp_gradient_8192(lx, dev):
     fd = (fp - fm) / (2 * h)
     print(f"  central difference {fd:.12e}", flush=True)
     assert abs(g - fd) <= 1e-5 * abs(fd)
+
+
+@pytest.mark.timeout(900)
+def test_gp_device_gradients_8192(lx, dev):
+    """The same workload through GPMarginalLikelihood (K built on the device
+    from the points): the value and every gradient against the oracle's FP64
+    closed forms (pinned by central differences in test_adjoint_oracle.py)."""
+    import torch
+    from oracle import adjoint as adj
+    import linompss_amd.autograd as ag
+    n = 8192
+    print(f"[gp device] n={n}", flush=True)
+    rng = np.random.default_rng(3)
+    P = rng.uniform(0, 1, (n, 3))
+    ell0, noise = 0.3, 1e-2
+    y = np.linalg.cholesky(adj.gp_kernel(P, ell0, noise)) @ rng.standard_normal(n)
+    pts = torch.from_numpy(P).requires_grad_(True)
+    yt = torch.from_numpy(y).requires_grad_(True)
+    ell = torch.tensor(ell0, dtype=torch.float64, requires_grad=True)
+    nz = torch.tensor(noise, dtype=torch.float64, requires_grad=True)
+    loss = ag.gp_nll_device(pts, yt, ell, 1.0, nz)
+    loss.backward()
+    ref = adj.gp_nll_grad_all(P, y, ell0, noise)
+    print(f"  dl={ell.grad.item():.12e} ref={ref['dell']:.12e}", flush=True)
+    assert abs(loss.item() - adj.gp_nll(P, y, ell0, noise)) <= 1e-10 * abs(loss.item())
+    assert abs(ell.grad.item() - ref["dell"]) <= 1e-9 * abs(ref["dell"])
+    assert abs(nz.grad.item() - ref["dnoise"]) <= 1e-9 * abs(ref["dnoise"])
+    assert np.abs(pts.grad.numpy() - ref["dP"]).max() <= 1e-8 * np.abs(ref["dP"]).max()
+    assert np.abs(yt.grad.numpy() - ref["dy"]).max() <= 1e-9 * np.abs(ref["dy"]).max()
+    ag.gp_release_workspace()
