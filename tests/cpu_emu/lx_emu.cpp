@@ -462,3 +462,13 @@ extern "C" void lx_emu_adj_gemm(int ta, int tb, int M, int N, int K, double alph
                         lx_gemm64_kernel<double, 1, 1>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower, ksplit, kchunk, part);
                 });
 }
+
+// Positive control for the sanitizer build: element i of an 8-element heap
+// array (i = 8 reads past its end, which AddressSanitizer must report).
+extern "C" int lx_emu_heap_probe(int i)
+{
+    int *p = new int[8]();
+    const int v = p[i];
+    delete[] p;
+    return v;
+}

@@ -7,6 +7,7 @@ are checked against the oracle and LAPACK.  This pins the kernels' indexing,
 barrier matching and dependency waits without a GPU.
 """
 import ctypes
+import os
 import subprocess
 from pathlib import Path
 
@@ -14,12 +15,20 @@ import numpy as np
 import pytest
 
 EMU = Path(__file__).resolve().parent / "cpu_emu"
+# LX_EMU_SANITIZE=1 (set by test_emu_sanitized.py, with the sanitizer runtime
+# preloaded): the same sources built with AddressSanitizer and
+# UndefinedBehaviorSanitizer
+SANITIZE = os.environ.get("LX_EMU_SANITIZE") == "1"
+
+
+def _load_emu():
+    subprocess.run(["make", "-s", "-C", str(EMU), "asan" if SANITIZE else "all"], check=True)
+    return ctypes.CDLL(str(EMU / "_build" / ("liblx_emu_asan.so" if SANITIZE else "liblx_emu.so")))
 
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     E.lx_emu_run.restype = ctypes.c_int
     E.lx_emu_run.argtypes = [ctypes.c_int] * 5 + [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
                                                    ctypes.c_longlong, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
@@ -50,7 +59,7 @@ def test_factor_fp64(emu, orc, what, n, b):
 def test_factor_fp32_split_bf16_updates(emu, orc):
     """The FP32 factor with its tile-loop SYRK/GEMM units on the split-BF16
     unit bodies: FP32-level agreement with LAPACK in FP64."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     n, b = 300, 128
     A0 = orc.genmat_spd(n, 4, diag_mode=1)
     Af = A0.astype(np.float32, order="F")
@@ -85,7 +94,7 @@ def test_gemm_unit_split_bf16(emu, flags, M, N, K, gdiag):
     accumulation error (plus the split's error per product) of the FP64
     product, entries outside the lower mask and rows / columns past M / N
     untouched."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     F, I, L, P = ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
     E.lx_emu_gemm_unit.argtypes = [I, I, I, I, P, L, P, L, I]
     rng = np.random.default_rng(M * 1000 + N + K)
@@ -114,7 +123,7 @@ def test_factor_fp32_split_fp16_updates(emu, orc):
     with LAPACK; with entries of the panels the trailing units read past
     FP16's range (x 2^12) the factor reports a non-positive pivot instead of
     a wrong factor."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     n, b = 300, 128
     A0 = orc.genmat_spd(n, 4, diag_mode=1) / 512.0  # diagonal 300/512
     Af = A0.astype(np.float32, order="F")
@@ -143,7 +152,7 @@ def test_gemm_unit_from_planes(emu, flags, M, N, K, gdiag):
     operands from the FP16 planes (the FP32 operands it could also read are
     zeros here): C -= A B^T to the split's accuracy, entries outside the lower
     mask and past M / N untouched."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     I, L, P = ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
     E.lx_emu_gemm_unit_planes.argtypes = [I, I, I, I, P, L, P, L, I]
     rng = np.random.default_rng(M * 1000 + N + K)
@@ -172,7 +181,7 @@ def test_factor_fp32_fp16_planes(emu, orc, variant, n, b):
     values split the same way and multiplied in the same order, so the factor
     equals the in-place split's bit for bit, on ragged orders too (n = 200:
     the last panel's 8 columns are not a whole slab)."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     A0 = orc.genmat_spd(n, 5, diag_mode=1) / (2.0 * n)
     E.lx_emu_set_update_prec(2)
     try:
@@ -232,7 +241,7 @@ def test_info_on_indefinite(emu, orc):
 
 @pytest.fixture(scope="module")
 def emu_sweep(emu):
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     E.lx_emu_sweep.restype = ctypes.c_int
     E.lx_emu_sweep.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
                                                    ctypes.c_longlong]
@@ -263,8 +272,7 @@ def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
 
 def _emu_batch(sizes, mats, rhs, max_sweeps=30, tol=1e-14, stall=0.5):
     """lx_batch_ir_kernel on the CPU: one workgroup takes every system."""
-    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     cnt = len(sizes)
     P = ctypes.c_void_p
     X = [np.zeros(max(int(n), 1)) for n in sizes]
@@ -322,8 +330,7 @@ def test_batch_kernel_fallback(orc):
 
 
 def _emu_lib():
-    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     P, I, L, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
     E.lx_emu_adj_diag.argtypes = [I, I, P, L, P, L, P, P]
     E.lx_emu_adj_gemm.argtypes = [I, I, I, I, I, D, P, L, P, L, D, P, L, I, I, I, P]
@@ -390,7 +397,7 @@ def test_plans_cut_units_inside_tiles(emu, variant):
     """The hypermatrix path needs every unit's operands inside one b x b tile:
     the factor plans of n = mt * b over tile sizes, both diagonal sub-tiles the
     library uses (FP32 128, FP64 256), both update precisions and GEMM widths."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     E.lx_emu_plan_in_tiles.argtypes = [ctypes.c_int] * 6
     for b in (1, 37, 64, 96, 128, 200, 256, 384, 512, 1024):
         for mt in (1, 2, 5):
@@ -403,7 +410,7 @@ def test_factor_hypermatrix(emu, orc, dtype, variant, mt, b):
     """The tiled executor instantiation on a hypermatrix (separate tile
     buffers): each tile of the factor against the oracle's CHOL_HRL / CHOL_HLL
     on the same tiles and against LAPACK on the assembled matrix."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     E.lx_emu_run_tiles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.POINTER(ctypes.c_int)]
     n = mt * b
@@ -430,7 +437,7 @@ def test_factor_hypermatrix(emu, orc, dtype, variant, mt, b):
 def test_gemm64_float_and_upper(emu, ta, tb, uplo):
     """The shared 64 x 64-tile GEMM kernel in FP32 (v_mfma_f32_16x16x4f32),
     every operand orientation, with the lower and upper masks of SYRK."""
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     F, I, L, P = ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
     E.lx_emu_gemm64s.argtypes = [I, I, I, I, I, F, P, L, P, L, F, P, L, I]
     M, N, K = 70, 90, 37
@@ -452,8 +459,7 @@ def test_gemm64_float_and_upper(emu, ta, tb, uplo):
 
 
 def _emu_raw():
-    subprocess.run(["make", "-s", "-C", str(EMU)], check=True)
-    E = ctypes.CDLL(str(EMU / "_build" / "liblx_emu.so"))
+    E = _load_emu()
     E.lx_emu_run_stuck.restype = ctypes.c_int
     E.lx_emu_run_stuck.argtypes = [ctypes.c_int] * 4 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                                         ctypes.POINTER(ctypes.c_int)]
