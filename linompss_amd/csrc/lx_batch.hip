@@ -81,15 +81,26 @@ __device__ __forceinline__ void lx_b_lower_to(const double *__restrict__ A, int6
 {
     const int tid = lx_tid(), l = tid & 63, g = tid >> 6;
     const int nb = (n + 63) / 64;
+    // the next tile's 16 loads are issued before the current tile is stored
+    // (the compiler cannot move loads of A above stores to L itself)
+    double an[16];
+    auto load_tile = [&](int I, int J) {
+        const int row = I * 64 + l;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int col = J * 64 + g * 16 + k;
+            an[k] = (row < n && col < n && row >= col) ? A[(int64_t)col * lda + row] : 0.0;
+        }
+    };
+    load_tile(0, 0);
     for (int J = 0; J < nb; ++J)
         for (int I = J; I < nb; ++I) {
             const int row = I * 64 + l;
             double a[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int col = J * 64 + g * 16 + k;
-                a[k] = (row < n && col < n && row >= col) ? A[(int64_t)col * lda + row] : 0.0;
-            }
+            for (int k = 0; k < 16; ++k) a[k] = an[k];
+            if (I + 1 < nb) load_tile(I + 1, J);  // uniform
+            else if (J + 1 < nb) load_tile(J + 1, J + 1);
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int col = J * 64 + g * 16 + k;
@@ -217,16 +228,33 @@ __device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t
         }
     }
     __syncthreads();
-    // entry (g, i) of racc / aacc is only ever updated by thread (i & 63, g)
+    // entry (g, i) of racc / aacc is only ever updated by thread (i & 63, g).
+    // Tiles in the order (J, I >= J); the 16 loads of the next tile are issued
+    // before the current one is used, so a tile's memory round trip overlaps
+    // the previous tile's arithmetic and barriers.
+    double an[16];
+    auto load_tile = [&](int I, int J) {
+        const int row = I * 64 + l;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int col = J * 64 + g * 16 + k;
+            an[k] = (row < n && col < n && row >= col) ? A[(int64_t)col * lda + row] : 0.0;
+        }
+    };
+    load_tile(0, 0);
     for (int J = 0; J < nb; ++J)
         for (int I = J; I < nb; ++I) {
             const int row = I * 64 + l;
+            double a_t[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a_t[k] = an[k];
+            if (I + 1 < nb) load_tile(I + 1, J);  // uniform
+            else if (J + 1 < nb) load_tile(J + 1, J + 1);
             double s = 0.0, sa = 0.0;
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int cl = g * 16 + k, col = J * 64 + cl;
-                double a = 0.0;
-                if (row < n && col < n && row >= col) a = A[(int64_t)col * lda + row];
+                const double a = a_t[k];
                 s += a * sx[col < n ? col : 0];
                 sa += fabs(a);
                 sT[cl][l] = row > col ? a : 0.0;  // the transposed use leaves the diagonal out
@@ -265,9 +293,30 @@ __device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t
     if (want_anorm) anorm = lx_b_max(am, red);
 }
 
+// Optional phase profile (prof != null): per workgroup, wall-clock ticks of
+// [0] the FP32 copy, [1] the FP32 factor, [2] the first solve, [3] the
+// residuals, [4] the correction solves, [5] the FP64 fallback, [6] systems.
+#define LX_B_PROF_WORDS 8
+struct lx_b_clock {
+    long long *acc;  // LDS, LX_B_PROF_WORDS (null: off)
+    long long t = 0;
+    __device__ void start()
+    {
+        if (acc) t = wall_clock64();
+    }
+    __device__ void lap(int q)
+    {
+        if (!acc) return;
+        const long long now = wall_clock64();
+        if (lx_tid() == 0) acc[q] += now - t;
+        t = now;
+    }
+};
+
 // Factor + sweeps + refinement of one system by the whole workgroup.
 __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot &slot, lx_bstat *stat,
-                                               int max_sweeps, double tol, double stall, char *smem)
+                                               int max_sweeps, double tol, double stall, char *smem,
+                                               lx_b_clock &clk)
 {
     const int tid = lx_tid();
     const int n = S.n;
@@ -291,10 +340,13 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
     float *F = reinterpret_cast<float *>(slot.F);
     float *Wf = reinterpret_cast<float *>(slot.W);
     float *vf = reinterpret_cast<float *>(slot.v);
+    clk.start();
     lx_b_lower_to<float>(S.A, S.lda, n, F, ld);
+    clk.lap(0);
     int info = 0;
     lx_unit_potrf<float>(F, ld, n, Wf, slot.ldw, &info, smem);
     info = __builtin_amdgcn_readfirstlane(info);
+    clk.lap(1);
     bool fb = info != 0;
     double anorm = 0.0, rmax = 0.0, xmax = 0.0, berr = 0.0;
     if (!fb) {
@@ -304,10 +356,12 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
         lx_b_sweep<float>(F, ld, Wf, slot.ldw, n, 1, vf, smem);
         for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] = bm * (double)vf[i];
         __syncthreads();
+        clk.lap(2);
         double prev = INFINITY;
         for (int it = 0;; ++it) {
             lx_b_resid(S.A, S.lda, n, S.X, S.B, slot.r, it == 0, smem, rmax, xmax, anorm);
             berr = rmax / (anorm * xmax);
+            clk.lap(3);
             if (berr <= tol) break;
             if (it >= max_sweeps || !(rmax <= stall * prev) || rmax != rmax) {
                 fb = true;
@@ -321,6 +375,7 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
             for (int i = tid; i < n; i += LX_WG_THREADS) S.X[i] += rmax * (double)vf[i];
             __syncthreads();
             out.sweeps = it + 1;
+            clk.lap(4);
         }
     }
     if (fb) {
@@ -339,6 +394,7 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
         __syncthreads();
         lx_b_resid(S.A, S.lda, n, S.X, S.B, slot.r, true, smem, rmax, xmax, anorm);
         berr = rmax / (anorm * xmax);
+        clk.lap(5);
     }
     out.fallback = fb ? 1 : 0;
     out.info = info;
@@ -375,11 +431,15 @@ template <int MINB>
 __global__ void __launch_bounds__(LX_WG_THREADS, MINB)
 lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ order, int count, char *ws,
                    int64_t slot_bytes, int nmax, unsigned *ticket, lx_bstat *stat, int max_sweeps, double tol,
-                   double stall)
+                   double stall, long long *prof)
 {
     __shared__ __attribute__((aligned(16))) char smem[LX_BATCH_SMEM];
     __shared__ int s_t;
+    __shared__ long long s_prof[LX_B_PROF_WORDS];
     const lx_bslot slot = lx_b_slot(ws, slot_bytes, nmax);
+    lx_b_clock clk;
+    clk.acc = prof ? s_prof : nullptr;
+    if (prof && lx_tid() < LX_B_PROF_WORDS) s_prof[lx_tid()] = 0;  // ordered by the loop's first barrier
     for (;;) {
         if (lx_tid() == 0) s_t = (int)atomicAdd(ticket, 1u);
         __syncthreads();
@@ -388,8 +448,11 @@ lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ orde
         if (t >= count) break;
         const int k = order[t];
         const lx_bsys S = sys[k];
-        lx_b_solve_one(S, slot, stat + k, max_sweeps, tol, stall, smem);
+        lx_b_solve_one(S, slot, stat + k, max_sweeps, tol, stall, smem, clk);
+        if (prof && lx_tid() == 0) s_prof[6] += 1;
     }
+    if (prof && lx_tid() == 0)
+        for (int q = 0; q < LX_B_PROF_WORDS; ++q) prof[(int64_t)blockIdx.x * LX_B_PROF_WORDS + q] = s_prof[q];
 }
 
 __global__ void lx_batch_reset_kernel(unsigned *ticket) { ticket[0] = 0u; }
@@ -397,6 +460,8 @@ __global__ void lx_batch_reset_kernel(unsigned *ticket) { ticket[0] = 0u; }
 #ifndef LX_BATCH_KERNELS_ONLY
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -418,6 +483,7 @@ struct linompss_batch_plan {
     lx_bstat *d_stat = nullptr;
     unsigned *d_ticket = nullptr;
     char *d_ws = nullptr;
+    long long *d_prof = nullptr;  // phase profile (LINOMPSS_BATCH_PROFILE set at plan creation)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -436,11 +502,11 @@ int lx_b_launch(linompss_batch_plan *p, int wg, int ord, hipStream_t s)
     if (wg == 1)
         hipLaunchKernelGGL(lx_batch_ir_kernel<1>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
                            p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
-                           p->stall);
+                           p->stall, p->d_prof);
     else
         hipLaunchKernelGGL(lx_batch_ir_kernel<2>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
                            p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
-                           p->stall);
+                           p->stall, p->d_prof);
     LX_HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -455,6 +521,7 @@ void lx_b_release(linompss_batch_plan *p)
     (void)hipFree(p->d_stat);
     (void)hipFree(p->d_ticket);
     (void)hipFree(p->d_ws);
+    (void)hipFree(p->d_prof);
     delete p;
 }
 
@@ -558,6 +625,9 @@ extern "C" int linompss_batch_plan_create(int count, const int *n, const double 
         hipMalloc(&p->d_ticket, 256) != hipSuccess ||
         hipMalloc(&p->d_ws, (size_t)p->slot_bytes * gmax) != hipSuccess)
         return fail(LINOMPSS_ERR_NOMEM);
+    if (getenv("LINOMPSS_BATCH_PROFILE") &&
+        hipMalloc(&p->d_prof, sizeof(long long) * LX_B_PROF_WORDS * gmax) != hipSuccess)
+        return fail(LINOMPSS_ERR_NOMEM);
     if (hipMemcpy(p->d_sys, hs.data(), sizeof(lx_bsys) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->d_order[0], ord_desc.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->d_order[1], ord_given.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess)
@@ -621,6 +691,19 @@ extern "C" int linompss_batch_run(linompss_batch_plan *p, void *stream, linompss
     }
     if (st) LX_HIP_TRY(hipMemcpyAsync(st, p->d_stat, sizeof(lx_bstat) * p->count, hipMemcpyDeviceToHost, s));
     LX_HIP_TRY(hipStreamSynchronize(s));
+    if (p->d_prof) {
+        std::vector<long long> h((size_t)LX_B_PROF_WORDS * p->grid);
+        LX_HIP_TRY(hipMemcpy(h.data(), p->d_prof, sizeof(long long) * h.size(), hipMemcpyDeviceToHost));
+        double tot[LX_B_PROF_WORDS] = {}, all = 0.0;
+        for (int g = 0; g < p->grid; ++g)
+            for (int q = 0; q < LX_B_PROF_WORDS; ++q) tot[q] += (double)h[(size_t)g * LX_B_PROF_WORDS + q];
+        for (int q = 0; q < 6; ++q) all += tot[q];
+        static const char *nm[6] = {"copy", "factor", "solve0", "resid", "corr", "fallback"};
+        fprintf(stderr, "[linompss batch profile] grid=%d systems=%.0f busy per WG %.3f ms:", p->grid, tot[6],
+                all / 1e5 / p->grid);
+        for (int q = 0; q < 6; ++q) fprintf(stderr, " %s %.1f%%", nm[q], 100.0 * tot[q] / (all > 0 ? all : 1.0));
+        fprintf(stderr, "\n");
+    }
     return 0;
 }
 

@@ -421,7 +421,7 @@ extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, con
     blockDim.x = 256;
     run_wg([&] {
         lx_batch_ir_kernel<2>(sys.data(), order, count, (char *)ws.data(), sb, nmax, &ticket, (lx_bstat *)stat,
-                           max_sweeps, tol, stall);
+                           max_sweeps, tol, stall, nullptr);
     });
     return ticket == (unsigned)count + 1 ? 0 : -1;
 }
