@@ -434,9 +434,9 @@ def traffic_from(fetch_csv, write_csv, out, n, update_prec):
         "hbm_bytes_per_launch": round((ff * fs + wf * ws) * 1024),
         "commands": [
             "rocprofv3 --pmc FETCH_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
-            f"--warmup 0 --fp64-steps 1 --no-cpu --update-prec {update_prec}",
+            f"--warmup 0 --fp64-steps 1 --no-cpu --no-tune --update-prec {update_prec}",
             "rocprofv3 --pmc WRITE_SIZE -d <dir> -o run --output-format csv -- python3 bench.py --steps 1 "
-            f"--warmup 0 --fp64-steps 1 --no-cpu --update-prec {update_prec}",
+            f"--warmup 0 --fp64-steps 1 --no-cpu --no-tune --update-prec {update_prec}",
             "python3 bench.py --traffic-from <fetch>/run_counter_collection.csv "
             f"<write>/run_counter_collection.csv --update-prec {update_prec} "
             "--traffic-out profiles/traffic_fp32_factor_n32768.json"],
