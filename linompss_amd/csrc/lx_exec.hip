@@ -157,6 +157,11 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
     // while the trailing matrix shrinks; 229.0 against 232.1 ms at n = 32768,
     // b = 512; the FP32 factor measured no gain at any tail length)
     if (dtype == 0) P.wide_tail = -1;
+    // paired trailing updates (each tile off the lookahead columns read and
+    // written once per two panels): at n = 32768, b = 512 the FP32 factor with
+    // split-FP16 updates 98.9 -> 79.8 ms, FP64 225 -> 216 ms; at b = 1024
+    // FP32 78.2 -> 71.7 ms, FP64 unchanged
+    P.pair_updates = 1;
     // GEMM units taller than 128 rows while the step's parallelism allows,
     // panel solves of 128 rows off the lookahead tile: each publication writes
     // the XCD's L2 back.  Measured at n = 32768, b = 512: FP32 factor 113.4 ms
@@ -175,6 +180,7 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
             P.gemm_wide_n = e->cfg.gemm_wide_n;
             P.trsm_rows_wide = e->cfg.trsm_rows_wide;
             P.wide_tail = e->cfg.wide_tail;
+            P.pair_updates = e->cfg.pair_updates;
         }
     }
     // tuning knobs: the sub-tile of diagonal-tile factorisations, the width
@@ -204,6 +210,9 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         const double v = atof(ev);
         if (v >= 0.0) P.wide_units = v;
     }
+    if (const char *ev = getenv("LINOMPSS_PAIR")) P.pair_updates = atoi(ev) != 0;
+    // a hypermatrix's tiles are separate buffers: no unit may span two panels
+    if (sh && sh->tiled) P.pair_updates = 0;
 }
 
 bool build_plan(lx_planner &pl, const lx_shape &sh)
@@ -235,9 +244,10 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
     lx_plan_params P;
     fill_params(P, sh.dtype, workers, o, &sh);
     char key[256];
-    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d",
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d pu%d",
              sh.what, sh.dtype, sh.n, sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec,
-             P.potrf_sub, P.gemm_wide_n, P.gemm_wide_m, P.wide_units, P.trsm_rows_wide, P.bx_wide, P.wide_tail);
+             P.potrf_sub, P.gemm_wide_n, P.gemm_wide_m, P.wide_units, P.trsm_rows_wide, P.bx_wide, P.wide_tail,
+             P.pair_updates);
     for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
         if (it->key == key) {
             c->plans.splice(c->plans.begin(), c->plans, it);
@@ -318,7 +328,9 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
     int wpc = occ;
     if (opts && opts->wg_per_cu > 0) wpc = std::min(opts->wg_per_cu, occ);
     const int workers = c->cus * wpc;
-    lx_dplan *pl = plan_get(c, sh, workers, opts);
+    lx_shape shp = sh;
+    shp.tiled = dtiles != nullptr;
+    lx_dplan *pl = plan_get(c, shp, workers, opts);
     if (!pl) return LINOMPSS_ERR_PLAN;
     LX_TRACE("plan units=%d tasks=%d ctr=%d", pl->nunits, pl->ntasks, pl->nctr);
     if (dtiles && !pl->in_tiles) return LINOMPSS_ERR_PLAN;
@@ -839,7 +851,8 @@ static bool cfg_valid(const linompss_factor_config *c)
 {
     return c->b > 0 && c->potrf_sub >= 64 && c->potrf_sub % 64 == 0 && c->gemm_wide_m >= 128 &&
            c->gemm_wide_m % 128 == 0 && c->gemm_wide_n >= 128 && c->gemm_wide_n % 128 == 0 &&
-           c->trsm_rows_wide >= 64 && c->trsm_rows_wide % 64 == 0 && c->wide_tail >= -1;
+           c->trsm_rows_wide >= 64 && c->trsm_rows_wide % 64 == 0 && c->wide_tail >= -1 &&
+           (c->pair_updates == 0 || c->pair_updates == 1);
 }
 
 extern "C" int linompss_set_factor_config(int dtype, int variant, int n, int update_prec,
@@ -905,7 +918,7 @@ void lx_cfg_defaults(int dtype, int update_prec, int b, linompss_factor_config *
     o.update_prec = update_prec;
     lx_plan_params P;
     fill_params(P, dtype, 512, &o);
-    *cfg = {b, P.potrf_sub, P.gemm_wide_m, P.gemm_wide_n, P.trsm_rows_wide, P.wide_tail, 0.0};
+    *cfg = {b, P.potrf_sub, P.gemm_wide_m, P.gemm_wide_n, P.trsm_rows_wide, P.wide_tail, P.pair_updates, 0.0};
 }
 
 extern "C" int linompss_clear_factor_configs(void)

@@ -21,6 +21,7 @@ enum lx_what : int {
 
 struct lx_shape {
     int what, dtype, n, b, nrhs;
+    int tiled = 0;  // a hypermatrix factor: every unit's operands inside one b x b tile
 };
 
 // Runs one plan over nsys systems sharing the same shape family (the plan was

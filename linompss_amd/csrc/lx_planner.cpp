@@ -2,6 +2,7 @@
 #include "lx_planner.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <queue>
 #include <stdexcept>
 
@@ -207,8 +208,24 @@ void lx_planner::factor_rl(int s)
 {
     const sysinfo &sy = S[s];
     const int n = sy.n, b = sy.b, mt = sy.mt;
+    const int tail = P.wide_tail < 0 ? mt / 2 : P.wide_tail;
+    // Paired updates (pair_updates): for an even step k whose successor k + 1
+    // is outside the narrow tail, the tasks GEMM(i,j,k) / SYRK(i,k) of tiles
+    // with j >= k + 3 get no units of their own; at step k + 1 the units of
+    // GEMM(i,j,k+1) / SYRK(i,k+1) apply both panels at once (K = 2b over the
+    // two contiguous panel columns), so such a tile is read and written once
+    // for two updates.  The task graph is unchanged: no task reads a tile
+    // j >= k + 3 between the two steps, and the panel-k inputs of the combined
+    // unit precede its panel-(k+1) inputs (TRSM(i,k) -> GEMM(i,k+1,k) ->
+    // TRSM(i,k+1)).  Columns k + 1 (the lookahead) and k + 2 keep one unit
+    // pass per panel, so the diagonal chain's next tile is never delayed.
+    auto pair_ok = [&](int k) { return P.pair_updates && k % 2 == 0 && k + 1 < mt && mt - (k + 1) > tail; };
+    auto deferred = [&](int k, int j) { return pair_ok(k) && j >= k + 3; };
+    auto combined = [&](int k, int j) { return k % 2 == 1 && pair_ok(k - 1) && j >= k + 2; };
     for (int k = 0; k < mt; ++k) {
         const int bs = std::min(b, n - k * b);
+        // a combined unit reaches back one panel: K and the operands' first column
+        const int kb_pair = (k - 1) * b, bs_pair = b + bs;
         {   // POTRF(k)  chol_rlmain.c:30
             int outc = tile_ctr(s, k, k);
             int t = new_task(LX_T_POTRF, s, k, k, k, PRIO_HI, outc, nullptr, 0);
@@ -242,11 +259,15 @@ void lx_planner::factor_rl(int s)
                 const int jbs = std::min(b, n - j * b);
                 int outc = tile_ctr(s, i, j), in[2] = {tile_ctr(s, i, k), tile_ctr(s, j, k)};
                 int t = new_task(LX_T_GEMM, s, i, j, k, PRIO_LO, outc, in, 2);
+                if (deferred(k, j)) {  // applied by GEMM(i,j,k+1)'s units
+                    last_writer[outc] = t;
+                    continue;
+                }
+                const bool comb = combined(k, j);
                 // the next panel's column (j = k+1) feeds the critical path: fine units;
                 // elsewhere wide units, so that one release fence covers more work
                 // (and in the last wide_tail panels, whose trailing matrix no longer
                 // gives every workgroup a wide unit)
-                const int tail = P.wide_tail < 0 ? mt / 2 : P.wide_tail;
                 const bool fine = j == k + 1 || mt - k <= tail;
                 const int un = fine ? P.unit_n : std::max(P.unit_n, P.gemm_wide_n);
                 // each unit's publication writes its XCD's L2 back (measured at
@@ -269,20 +290,25 @@ void lx_planner::factor_rl(int s)
                         u.aux0 = upd_flag();
                         u.M = std::min(um, ibs - r0);
                         u.N = std::min(un, jbs - c0);
-                        u.K = bs;
+                        u.K = comb ? bs_pair : bs;
                         u.ctr = outc;
                         u.c = rc(i * b + r0, j * b + c0);
-                        u.a = rc(i * b + r0, k * b);
-                        u.b = rc(j * b + c0, k * b);
+                        u.a = rc(i * b + r0, comb ? kb_pair : k * b);
+                        u.b = rc(j * b + c0, comb ? kb_pair : k * b);
                         add_unit(t, u);
                     }
-                Tk[t].unit_cost = 2.0 * um * un * bs / P.flop_rate + P.unit_overhead;
+                Tk[t].unit_cost = 2.0 * um * un * (comb ? bs_pair : bs) / P.flop_rate + P.unit_overhead;
                 issued[outc] += Tk[t].nunits;
                 last_writer[outc] = t;
             }
             {   // SYRK(i,k)  chol_rlmain.c:48
                 int outc = tile_ctr(s, i, i), in[1] = {tile_ctr(s, i, k)};
                 int t = new_task(LX_T_SYRK, s, i, i, k, PRIO_LO, outc, in, 1);
+                if (deferred(k, i)) {  // applied by SYRK(i,k+1)'s units
+                    last_writer[outc] = t;
+                    continue;
+                }
+                const bool comb = combined(k, i);
                 for (int c0 = 0; c0 < ibs; c0 += P.unit_n)
                     for (int r0 = 0; r0 < ibs; r0 += P.unit_m) {
                         if (r0 + P.unit_m - 1 < c0) continue;  // strictly upper
@@ -290,16 +316,16 @@ void lx_planner::factor_rl(int s)
                         u.aux0 = upd_flag();
                         u.M = std::min(P.unit_m, ibs - r0);
                         u.N = std::min(P.unit_n, ibs - c0);
-                        u.K = bs;
+                        u.K = comb ? bs_pair : bs;
                         u.ctr = outc;
                         u.c = rc(i * b + r0, i * b + c0);
-                        u.a = rc(i * b + r0, k * b);
-                        u.b = rc(i * b + c0, k * b);
+                        u.a = rc(i * b + r0, comb ? kb_pair : k * b);
+                        u.b = rc(i * b + c0, comb ? kb_pair : k * b);
                         u.aux0 |= 1;
                         u.aux1 = r0 - c0;
                         add_unit(t, u);
                     }
-                Tk[t].unit_cost = 2.0 * P.unit_m * P.unit_n * bs / P.flop_rate + P.unit_overhead;
+                Tk[t].unit_cost = 2.0 * P.unit_m * P.unit_n * (comb ? bs_pair : bs) / P.flop_rate + P.unit_overhead;
                 issued[outc] += Tk[t].nunits;
                 last_writer[outc] = t;
             }
@@ -525,7 +551,14 @@ void lx_planner::finalize()
     };
     auto ecmp = [](const EV &a, const EV &b) { return a.t > b.t; };
     std::priority_queue<EV, std::vector<EV>, decltype(ecmp)> events(ecmp);
-    auto push_task = [&](int t) {
+    // a task with no units of its own (its work done by a later task's units)
+    // completes as soon as it is ready
+    std::function<void(int)> push_task = [&](int t) {
+        if (Tk[t].nunits == 0) {
+            for (int q : succ[t])
+                if (--pending[q] == 0) push_task(q);
+            return;
+        }
         for (int q = 0; q < Tk[t].nunits; ++q) ready.push({key(t), t, Tk[t].first_unit + q});
     };
     for (int t = 0; t < nt; ++t)

@@ -5,11 +5,14 @@
 // chol_rlmain.c:25-50) and takes b from its caller.  Here a tile task is cut
 // into work units whose shapes (diagonal sub-tiles, trailing-update unit
 // heights and widths, panel-solve heights, narrow units in the last panels)
-// decide how well the persistent launch fills the chip; the best shapes
+// decide how well the persistent launch fills the chip (and whether tiles off
+// the lookahead columns take two panels' updates per pass); the best shapes
 // depend on n, b, the precision and the update arithmetic.  The search is a
 // coordinate descent from the library's defaults over a small grid per knob:
 // each knob in turn takes each of its values with the others held at the
-// best so far, and keeps the fastest.  A candidate's time is the fastest of
+// best so far, and keeps the fastest; when the tile size is searched too,
+// every candidate tile is timed with its defaults and the descent runs from
+// the two fastest.  A candidate's time is the fastest of
 // `reps` factor launches (HIP events on the launch stream) of one generated
 // matrix, restored from a device copy before every launch.  The winner is
 // recorded for its shape, so every later factor of that shape runs with it.
@@ -121,38 +124,58 @@ extern "C" int linompss_tune_factor(int dtype, int variant, int n, int b, int up
         rc = hipDeviceSynchronize() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
     }
 
-    // start: the library's defaults for the middle candidate tile (512 when searched)
-    linompss_factor_config cur = {};
-    const int b0 = b > 0 ? b : (std::find(tiles.begin(), tiles.end(), 512) != tiles.end() ? 512 : tiles.back());
-    lx_cfg_defaults(dtype, update_prec, b0, &cur);
-    double tbest = rc == 0 ? time_cfg(t, cur, &rc) : 0.0;
-
-    // one pass per knob, tile size first (the unit shapes are tuned for it)
+    // one pass of coordinate descent over the unit shapes for a fixed tile,
+    // from that tile's defaults (timed as t0)
     struct knob {
         int linompss_factor_config::*f;
         std::vector<int> v;
     };
-    std::vector<knob> knobs;
-    if (b == 0) knobs.push_back({&linompss_factor_config::b, tiles});
-    knobs.push_back({&linompss_factor_config::potrf_sub, {128, 256, 512}});
-    knobs.push_back({&linompss_factor_config::gemm_wide_m, {128, 256, 512}});
-    knobs.push_back({&linompss_factor_config::gemm_wide_n, {256, 512, 1024}});
-    knobs.push_back({&linompss_factor_config::trsm_rows_wide, {64, 128, 256}});
-    knobs.push_back({&linompss_factor_config::wide_tail, {0, 8, 16, -1}});
-    for (size_t q = 0; q < knobs.size() && rc == 0; ++q) {
-        const linompss_factor_config base = cur;
-        for (int v : knobs[q].v) {
-            linompss_factor_config c = base;
-            if (c.*knobs[q].f == v) continue;
-            c.*knobs[q].f = v;
-            if (knobs[q].f == &linompss_factor_config::potrf_sub && v >= c.b && base.potrf_sub >= c.b)
-                continue;  // no sub-DAG either way
-            const double ms = time_cfg(t, c, &rc);
-            if (rc) break;
-            if (ms < tbest) {
-                tbest = ms;
-                cur = c;
+    const std::vector<knob> knobs = {
+        {&linompss_factor_config::pair_updates, {0, 1}},
+        {&linompss_factor_config::potrf_sub, {128, 256, 512}},
+        {&linompss_factor_config::gemm_wide_m, {128, 256, 512}},
+        {&linompss_factor_config::gemm_wide_n, {256, 512, 1024}},
+        {&linompss_factor_config::trsm_rows_wide, {64, 128, 256}},
+        {&linompss_factor_config::wide_tail, {0, 8, 16, -1}},
+    };
+    auto descend = [&](linompss_factor_config &cur, double &tbest) {
+        for (size_t q = 0; q < knobs.size() && rc == 0; ++q) {
+            const linompss_factor_config base = cur;
+            for (int v : knobs[q].v) {
+                linompss_factor_config c = base;
+                if (c.*knobs[q].f == v) continue;
+                c.*knobs[q].f = v;
+                if (knobs[q].f == &linompss_factor_config::potrf_sub && v >= c.b && base.potrf_sub >= c.b)
+                    continue;  // no sub-DAG either way
+                const double ms = time_cfg(t, c, &rc);
+                if (rc) break;
+                if (ms < tbest) {
+                    tbest = ms;
+                    cur = c;
+                }
             }
+        }
+    };
+    // every candidate tile with its defaults, then the descent from the two
+    // fastest (a tile's best unit shapes differ from another's, so the tile is
+    // not fixed before the shapes are searched)
+    std::vector<std::pair<double, linompss_factor_config>> start;
+    for (size_t q = 0; q < tiles.size() && rc == 0; ++q) {
+        linompss_factor_config c = {};
+        lx_cfg_defaults(dtype, update_prec, tiles[q], &c);
+        const double ms = time_cfg(t, c, &rc);
+        if (rc == 0) start.push_back({ms, c});
+    }
+    std::sort(start.begin(), start.end(), [](const auto &a, const auto &b2) { return a.first < b2.first; });
+    linompss_factor_config cur = {};
+    double tbest = 1e30;
+    for (size_t q = 0; q < start.size() && q < 2 && rc == 0; ++q) {
+        linompss_factor_config c = start[q].second;
+        double tb = start[q].first;
+        descend(c, tb);
+        if (rc == 0 && tb < tbest) {
+            tbest = tb;
+            cur = c;
         }
     }
 

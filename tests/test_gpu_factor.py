@@ -200,3 +200,29 @@ def test_chol_tall_units(lx, orc, dev, monkeypatch, dtype, variant, n, b):
     Lref = np.linalg.cholesky(A0)
     tol = 1e-12 if dtype == np.float64 else 1e-5
     assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= tol
+
+
+@pytest.mark.parametrize("dtype,n,b", [(np.float64, 4096, 256), (np.float64, 3000, 256), (np.float64, 2100, 128),
+                                       (np.float32, 4096, 256), (np.float32, 3001, 512)])
+def test_chol_paired_updates(lx, orc, dev, monkeypatch, dtype, n, b):
+    """Paired trailing updates (the default for right-looking plans: tiles two
+    columns or more right of the lookahead column take panels k and k + 1 in one
+    K = 2b pass) against one pass per panel (LINOMPSS_PAIR=0) and LAPACK, on
+    exact and ragged tilings; FP32 with the split-FP16 updates from the planes."""
+    print(f"[paired] {np.dtype(dtype).name} n={n} b={b}", flush=True)
+    A0 = orc.genmat_spd(n, seed=10, diag_mode=1) / 2.0 ** np.ceil(np.log2(n))
+    opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2 if dtype == np.float32 else 0)
+    out, units = [], []
+    for pair in ("1", "0"):
+        monkeypatch.setenv("LINOMPSS_PAIR", pair)
+        dA = lx.DeviceMatrix.from_host(A0.astype(dtype, order="F"))
+        assert lx.chol_dev(dA, b, "rl", opts) == 0
+        units.append(lx.last_exec_stats().units)
+        out.append(dA.to_host())
+        dA.free()
+    assert units[0] < units[1]
+    Lref = np.linalg.cholesky(A0)
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    for o in out:
+        assert np.linalg.norm(np.tril(o) - Lref) / np.linalg.norm(Lref) <= tol
+        assert np.array_equal(np.triu(o, 1), np.triu(A0.astype(dtype), 1))
