@@ -193,53 +193,93 @@ int lx_adj_gemm(int ta, int tb, int M, int N, int K, double alpha, const double 
                              lx_adj_cus(), s);
 }
 
-// The reverse sweep over 64-column blocks (see the header comment).
+__global__ void lx_adj_zero_kernel(double *__restrict__ X, int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) X[e] = 0.0;
+}
+
+// The reverse sweep over 64-column blocks (see the header comment), in groups
+// of four blocks (256 columns, from the last group to the first).  Inside a
+// group each block runs the recurrences with its updates restricted to the
+// group's columns left of it; the updates of the columns left of the group
+// are gathered and applied once per group as two rank-256 products.  With
+// Q_G the group's Qbar blocks side by side (block J's column block holds
+// Qbar_J in the rows below block J, zeros above), rows from the group's first
+// row g0 to n, and L_G = L[group rows, 0:g0]:
+//     Abar[g0:n, 0:g0]     -= Q_G L_G                (the Bbar terms)
+//     Abar[group, 0:g0]    -= Q_G^T L[g0:n, 0:g0]    (the Qbar^T B terms)
+// while each block's (Pbar + Pbar^T) R term spans all its left columns at once.
+// The same sums as the block-by-block sweep in another order; the rank-64
+// updates of the whole left part, each reading and writing it, become two
+// rank-256 ones.
 int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda, hipStream_t s)
 {
     const int nb = (n + 63) / 64;
+    constexpr int GB = 4;  // blocks per group
     const size_t part_elems = (size_t)16 * 64 * std::max(n, 64);
-    double *ws = (double *)lx_scratch(9, sizeof(double) * ((size_t)nb * 4096 + 64 * 64 + (size_t)n * 64 + part_elems));
+    const size_t qg_elems = (size_t)std::max(n, 1) * 64 * GB;
+    double *ws = (double *)lx_scratch(
+        9, sizeof(double) * ((size_t)nb * 4096 + 64 * 64 + qg_elems + part_elems));
     if (!ws) return LINOMPSS_ERR_NOMEM;
-    double *Zall = ws, *Sb = Zall + (size_t)nb * 4096, *Qb = Sb + 64 * 64, *part = Qb + (size_t)n * 64;
+    double *Zall = ws, *Sb = Zall + (size_t)nb * 4096, *QG = Sb + 64 * 64, *part = QG + qg_elems;
     // the inverses of the diagonal blocks do not depend on the sweep: all at once
     hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(nb), dim3(LX_WG_THREADS), 0, s, 0, n, L, ldl, Ab, lda, Zall, Sb);
     int rc = 0;
-    for (int J = nb - 1; J >= 0 && rc == 0; --J) {
-        const int j0 = 64 * J, jb = std::min(64, n - j0), m = n - j0 - jb, p = j0;
-        const double *D = L + (int64_t)j0 * ldl + j0;
-        double *Dbar = Ab + (int64_t)j0 * lda + j0;
-        const int64_t ldq = std::max(m, 1);
-        const double *Zb = Zall + (size_t)J * 4096;
-        if (m > 0) {
-            double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
-            const double *C = L + (int64_t)j0 * ldl + j0 + jb;
-            // Qbar = Cbar D^-1; Abar_C = Qbar
-            if ((rc = lx_adj_gemm(0, 0, m, jb, jb, 1.0, Cbar, lda, Zb, 64, 0.0, Qb, ldq, 0, part, part_elems, s)))
-                break;
-            const int64_t tot = (int64_t)m * jb;
-            hipLaunchKernelGGL(lx_adj_copy_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, jb, Qb,
-                               ldq, Cbar, lda);
-            // Dbar -= tril(Qbar^T C)
-            if ((rc = lx_adj_gemm(1, 0, jb, jb, m, -1.0, Qb, ldq, C, ldl, 1.0, Dbar, lda, 1, part, part_elems, s)))
-                break;
-        }
-        // Pbar (in place of Dbar) and S = Pbar + Pbar^T
-        hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda,
-                           Zall + (size_t)J * 4096, Sb);
-        if (p > 0) {
-            const double *R = L + j0;           // L[J, <J]: jb x p, ld ldl
-            double *Rbar = Ab + j0;            // Abar[J, <J]
+    for (int G = (nb - 1) / GB; G >= 0 && rc == 0; --G) {
+        const int gJ0 = G * GB, gJ1 = std::min(nb, gJ0 + GB) - 1;  // blocks of the group
+        const int g0 = 64 * gJ0, gw = std::min(n, 64 * (gJ1 + 1)) - g0;
+        const int64_t ldqg = std::max(n - g0, 1);
+        const int64_t qtot = ldqg * gw;
+        hipLaunchKernelGGL(lx_adj_zero_kernel, dim3((unsigned)((qtot + 255) / 256)), dim3(256), 0, s, QG, qtot);
+        for (int J = gJ1; J >= gJ0 && rc == 0; --J) {
+            const int j0 = 64 * J, jb = std::min(64, n - j0), m = n - j0 - jb;
+            const int pin = j0 - g0;  // the group's columns left of block J
+            const double *D = L + (int64_t)j0 * ldl + j0;
+            double *Dbar = Ab + (int64_t)j0 * lda + j0;
+            // Qbar_J lives in Q_G: column block J - gJ0, rows below block J
+            double *Qb = QG + (int64_t)(j0 - g0) * ldqg + (j0 + jb - g0);
+            const double *Zb = Zall + (size_t)J * 4096;
             if (m > 0) {
-                const double *Bm = L + j0 + jb;  // L[>J, <J]: m x p
-                double *Bbar = Ab + j0 + jb;
-                // Bbar -= Qbar R ;  Rbar -= Qbar^T B
-                if ((rc = lx_adj_gemm(0, 0, m, p, jb, -1.0, Qb, ldq, R, ldl, 1.0, Bbar, lda, 0, part, part_elems, s)))
+                double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
+                const double *C = L + (int64_t)j0 * ldl + j0 + jb;
+                // Qbar = Cbar D^-1; Abar_C = Qbar
+                if ((rc = lx_adj_gemm(0, 0, m, jb, jb, 1.0, Cbar, lda, Zb, 64, 0.0, Qb, ldqg, 0, part, part_elems, s)))
                     break;
-                if ((rc = lx_adj_gemm(1, 0, jb, p, m, -1.0, Qb, ldq, Bm, ldl, 1.0, Rbar, lda, 0, part, part_elems, s)))
+                const int64_t tot = (int64_t)m * jb;
+                hipLaunchKernelGGL(lx_adj_copy_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, jb, Qb,
+                                   ldqg, Cbar, lda);
+                // Dbar -= tril(Qbar^T C)
+                if ((rc = lx_adj_gemm(1, 0, jb, jb, m, -1.0, Qb, ldqg, C, ldl, 1.0, Dbar, lda, 1, part, part_elems,
+                                      s)))
                     break;
             }
-            // Rbar -= (Pbar + Pbar^T) R
-            if ((rc = lx_adj_gemm(0, 0, jb, p, jb, -1.0, Sb, 64, R, ldl, 1.0, Rbar, lda, 0, part, part_elems, s)))
+            // Pbar (in place of Dbar) and S = Pbar + Pbar^T
+            hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda,
+                               Zall + (size_t)J * 4096, Sb);
+            if (pin > 0 && m > 0) {
+                // inside the group: Bbar -= Qbar R, Rbar -= Qbar^T B (columns g0 .. j0)
+                const double *Rin = L + (int64_t)g0 * ldl + j0;
+                const double *Bin = L + (int64_t)g0 * ldl + j0 + jb;
+                if ((rc = lx_adj_gemm(0, 0, m, pin, jb, -1.0, Qb, ldqg, Rin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0 + jb,
+                                      lda, 0, part, part_elems, s)))
+                    break;
+                if ((rc = lx_adj_gemm(1, 0, jb, pin, m, -1.0, Qb, ldqg, Bin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0,
+                                      lda, 0, part, part_elems, s)))
+                    break;
+            }
+            // Rbar -= (Pbar + Pbar^T) R over every column left of block J
+            if (j0 > 0 && (rc = lx_adj_gemm(0, 0, jb, j0, jb, -1.0, Sb, 64, L + j0, ldl, 1.0, Ab + j0, lda, 0, part,
+                                             part_elems, s)))
+                break;
+        }
+        if (rc == 0 && g0 > 0) {
+            // the columns left of the group: the rank-gw products with Q_G
+            if ((rc = lx_adj_gemm(0, 0, n - g0, g0, gw, -1.0, QG, ldqg, L + g0, ldl, 1.0, Ab + g0, lda, 0, part,
+                                  part_elems, s)))
+                break;
+            if ((rc = lx_adj_gemm(1, 0, gw, g0, n - g0, -1.0, QG, ldqg, L + g0, ldl, 1.0, Ab + g0, lda, 0, part,
+                                  part_elems, s)))
                 break;
         }
     }
