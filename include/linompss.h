@@ -408,12 +408,12 @@ int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const dou
  *                   (multiple of 64)
  *   wide_tail       the last wide_tail panels use 128-column update units
  *                   (-1: half of the panels)
- *   pair_updates    1: right-looking plans give tiles two columns or more
- *                   right of the lookahead column the updates of panels k
- *                   (even) and k + 1 in one pass (each such tile read and
- *                   written once per two GEMM / SYRK tasks; the task graph is
- *                   unchanged); 0: one pass per task.  Hypermatrix plans
- *                   always use 0.
+ *   update_group    g (1 to 8): right-looking plans take the steps in groups
+ *                   of g and give the tiles right of a group's columns one
+ *                   pass for the group's g panels (each such tile read and
+ *                   written once per g GEMM / SYRK tasks; the task graph is
+ *                   unchanged); 1: one pass per task.  Hypermatrix plans
+ *                   always use 1.
  * linompss_tune_factor times the factor of an n x n diagonally dominant SPD
  * matrix it generates on the device (unit diagonal, off-diagonal entries in
  * [0, 1/n)) under candidate configurations, by coordinate descent from the
@@ -439,7 +439,7 @@ typedef struct {
     int gemm_wide_n;
     int trsm_rows_wide;
     int wide_tail;
-    int pair_updates;
+    int update_group;
     double ms;  /* factor launch time measured for this configuration */
 } linompss_factor_config;
 

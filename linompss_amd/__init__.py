@@ -678,7 +678,7 @@ def plan_info(what: int, n: int, b: int, nrhs: int = 0, opts: ExecOpts | None = 
 class FactorConfig(ctypes.Structure):
     """Tile size and work-unit shapes of a factor plan (linompss_factor_config)."""
     _fields_ = [("b", _i), ("potrf_sub", _i), ("gemm_wide_m", _i), ("gemm_wide_n", _i), ("trsm_rows_wide", _i),
-                ("wide_tail", _i), ("pair_updates", _i), ("ms", _d)]
+                ("wide_tail", _i), ("update_group", _i), ("ms", _d)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -136,6 +136,10 @@ const lx_cfg_entry *cfg_find(int dtype, int variant, int n, int b, int update_pr
     return nullptr;
 }
 
+// Panels per trailing-update pass for tiles right of a group of steps:
+// about 2048 columns (groups of 4 at b = 512, 2 at b = 1024).
+int default_update_group(int b) { return b > 0 ? std::max(1, std::min(8, 2048 / b)) : 1; }
+
 void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_opts *o,
                  const lx_shape *sh = nullptr)
 {
@@ -157,11 +161,12 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
     // while the trailing matrix shrinks; 229.0 against 232.1 ms at n = 32768,
     // b = 512; the FP32 factor measured no gain at any tail length)
     if (dtype == 0) P.wide_tail = -1;
-    // paired trailing updates (each tile off the lookahead columns read and
-    // written once per two panels): at n = 32768, b = 512 the FP32 factor with
-    // split-FP16 updates 98.9 -> 79.8 ms, FP64 225 -> 216 ms; at b = 1024
-    // FP32 78.2 -> 71.7 ms, FP64 unchanged
-    P.pair_updates = 1;
+    // grouped trailing updates (each tile right of a group of steps read and
+    // written once per group), about 2048 columns of panels per pass.  At
+    // n = 32768: b = 512, groups of 1 / 2 / 3 / 4: FP32 factor with split-FP16
+    // updates 98.4 / 79.0 / 73.9 / 72.6 ms, FP64 225.7 / 218.1 / 215.4 /
+    // 213.1 ms; b = 1024, groups of 1 / 2 / 4: FP32 78.1 / 70.2 / 72.8 ms
+    P.update_group = sh ? default_update_group(sh->b) : 1;
     // GEMM units taller than 128 rows while the step's parallelism allows,
     // panel solves of 128 rows off the lookahead tile: each publication writes
     // the XCD's L2 back.  Measured at n = 32768, b = 512: FP32 factor 113.4 ms
@@ -180,7 +185,7 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
             P.gemm_wide_n = e->cfg.gemm_wide_n;
             P.trsm_rows_wide = e->cfg.trsm_rows_wide;
             P.wide_tail = e->cfg.wide_tail;
-            P.pair_updates = e->cfg.pair_updates;
+            P.update_group = e->cfg.update_group;
         }
     }
     // tuning knobs: the sub-tile of diagonal-tile factorisations, the width
@@ -210,9 +215,12 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
         const double v = atof(ev);
         if (v >= 0.0) P.wide_units = v;
     }
-    if (const char *ev = getenv("LINOMPSS_PAIR")) P.pair_updates = atoi(ev) != 0;
+    if (const char *ev = getenv("LINOMPSS_UPDATE_GROUP")) {
+        const int v = atoi(ev);
+        if (v >= 1 && v <= 8) P.update_group = v;
+    }
     // a hypermatrix's tiles are separate buffers: no unit may span two panels
-    if (sh && sh->tiled) P.pair_updates = 0;
+    if (sh && sh->tiled) P.update_group = 1;
 }
 
 bool build_plan(lx_planner &pl, const lx_shape &sh)
@@ -244,10 +252,10 @@ lx_dplan *plan_get(lx_dev_ctx *c, const lx_shape &sh, int workers, const linomps
     lx_plan_params P;
     fill_params(P, sh.dtype, workers, o, &sh);
     char key[256];
-    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d pu%d",
+    snprintf(key, sizeof key, "w%d d%d n%d b%d r%d p%d um%d un%d W%d u%d ps%d wn%d wm%d wu%g tr%d bw%d wt%d ug%d",
              sh.what, sh.dtype, sh.n, sh.b, sh.nrhs, P.policy, P.unit_m, P.unit_n, P.workers, P.update_prec,
              P.potrf_sub, P.gemm_wide_n, P.gemm_wide_m, P.wide_units, P.trsm_rows_wide, P.bx_wide, P.wide_tail,
-             P.pair_updates);
+             P.update_group);
     for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
         if (it->key == key) {
             c->plans.splice(c->plans.begin(), c->plans, it);
@@ -852,7 +860,7 @@ static bool cfg_valid(const linompss_factor_config *c)
     return c->b > 0 && c->potrf_sub >= 64 && c->potrf_sub % 64 == 0 && c->gemm_wide_m >= 128 &&
            c->gemm_wide_m % 128 == 0 && c->gemm_wide_n >= 128 && c->gemm_wide_n % 128 == 0 &&
            c->trsm_rows_wide >= 64 && c->trsm_rows_wide % 64 == 0 && c->wide_tail >= -1 &&
-           (c->pair_updates == 0 || c->pair_updates == 1);
+           c->update_group >= 1 && c->update_group <= 8;
 }
 
 extern "C" int linompss_set_factor_config(int dtype, int variant, int n, int update_prec,
@@ -918,7 +926,12 @@ void lx_cfg_defaults(int dtype, int update_prec, int b, linompss_factor_config *
     o.update_prec = update_prec;
     lx_plan_params P;
     fill_params(P, dtype, 512, &o);
-    *cfg = {b, P.potrf_sub, P.gemm_wide_m, P.gemm_wide_n, P.trsm_rows_wide, P.wide_tail, P.pair_updates, 0.0};
+    P.update_group = default_update_group(b);
+    if (const char *ev = getenv("LINOMPSS_UPDATE_GROUP")) {
+        const int v = atoi(ev);
+        if (v >= 1 && v <= 8) P.update_group = v;
+    }
+    *cfg = {b, P.potrf_sub, P.gemm_wide_m, P.gemm_wide_n, P.trsm_rows_wide, P.wide_tail, P.update_group, 0.0};
 }
 
 extern "C" int linompss_clear_factor_configs(void)

@@ -209,23 +209,33 @@ void lx_planner::factor_rl(int s)
     const sysinfo &sy = S[s];
     const int n = sy.n, b = sy.b, mt = sy.mt;
     const int tail = P.wide_tail < 0 ? mt / 2 : P.wide_tail;
-    // Paired updates (pair_updates): for an even step k whose successor k + 1
-    // is outside the narrow tail, the tasks GEMM(i,j,k) / SYRK(i,k) of tiles
-    // with j >= k + 3 get no units of their own; at step k + 1 the units of
-    // GEMM(i,j,k+1) / SYRK(i,k+1) apply both panels at once (K = 2b over the
-    // two contiguous panel columns), so such a tile is read and written once
-    // for two updates.  The task graph is unchanged: no task reads a tile
-    // j >= k + 3 between the two steps, and the panel-k inputs of the combined
-    // unit precede its panel-(k+1) inputs (TRSM(i,k) -> GEMM(i,k+1,k) ->
-    // TRSM(i,k+1)).  Columns k + 1 (the lookahead) and k + 2 keep one unit
-    // pass per panel, so the diagonal chain's next tile is never delayed.
-    auto pair_ok = [&](int k) { return P.pair_updates && k % 2 == 0 && k + 1 < mt && mt - (k + 1) > tail; };
-    auto deferred = [&](int k, int j) { return pair_ok(k) && j >= k + 3; };
-    auto combined = [&](int k, int j) { return k % 2 == 1 && pair_ok(k - 1) && j >= k + 2; };
+    // Grouped updates (update_group = g > 1): the steps come in groups
+    // k0 .. k0 + g - 1 (k0 a multiple of g, the whole group before the narrow
+    // tail).  Tiles with j >= k0 + g + 1 take no unit pass at the group's first
+    // g - 1 steps (their GEMM(i,j,k) / SYRK(i,k) tasks keep their place in the
+    // task graph with no units of their own); at the group's last step the
+    // units of GEMM(i,j,k0+g-1) / SYRK(i,k0+g-1) apply all g panels at once
+    // (K = g b over the contiguous panel columns), so such a tile is read and
+    // written once per g updates.  No task reads those tiles during the group,
+    // and every panel of the group precedes the last one's inputs
+    // (TRSM(i,k) -> GEMM(i,k+1,k) -> TRSM(i,k+1), with column k + 1 always
+    // updated one panel at a time).  Columns up to k0 + g keep one pass per
+    // panel, so the diagonal chain's next tiles are never delayed.
+    const int g = std::max(1, P.update_group);
+    auto group_ok = [&](int k0) { return g > 1 && k0 % g == 0 && k0 + g - 1 < mt && mt - (k0 + g - 1) > tail; };
+    auto deferred = [&](int k, int j) {
+        const int k0 = k - k % g;
+        return k - k0 < g - 1 && group_ok(k0) && j >= k0 + g + 1;
+    };
+    auto combined = [&](int k, int j) {
+        const int k0 = k - k % g;
+        return k - k0 == g - 1 && group_ok(k0) && j >= k0 + g + 1;
+    };
     for (int k = 0; k < mt; ++k) {
         const int bs = std::min(b, n - k * b);
-        // a combined unit reaches back one panel: K and the operands' first column
-        const int kb_pair = (k - 1) * b, bs_pair = b + bs;
+        // a combined unit reaches back to its group's first panel: K and the
+        // operands' first column
+        const int kb_pair = (k - (g - 1)) * b, bs_pair = (g - 1) * b + bs;
         {   // POTRF(k)  chol_rlmain.c:30
             int outc = tile_ctr(s, k, k);
             int t = new_task(LX_T_POTRF, s, k, k, k, PRIO_HI, outc, nullptr, 0);
@@ -259,7 +269,7 @@ void lx_planner::factor_rl(int s)
                 const int jbs = std::min(b, n - j * b);
                 int outc = tile_ctr(s, i, j), in[2] = {tile_ctr(s, i, k), tile_ctr(s, j, k)};
                 int t = new_task(LX_T_GEMM, s, i, j, k, PRIO_LO, outc, in, 2);
-                if (deferred(k, j)) {  // applied by GEMM(i,j,k+1)'s units
+                if (deferred(k, j)) {  // applied by the group's last GEMM(i,j,.) units
                     last_writer[outc] = t;
                     continue;
                 }
@@ -304,7 +314,7 @@ void lx_planner::factor_rl(int s)
             {   // SYRK(i,k)  chol_rlmain.c:48
                 int outc = tile_ctr(s, i, i), in[1] = {tile_ctr(s, i, k)};
                 int t = new_task(LX_T_SYRK, s, i, i, k, PRIO_LO, outc, in, 1);
-                if (deferred(k, i)) {  // applied by SYRK(i,k+1)'s units
+                if (deferred(k, i)) {  // applied by the group's last SYRK(i,.) units
                     last_writer[outc] = t;
                     continue;
                 }

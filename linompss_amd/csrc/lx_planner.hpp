@@ -27,8 +27,8 @@ struct lx_plan_params {
     double wide_units = 2.0;  // taller units only while a step still has wide_units units per workgroup
     int wide_tail = 0;      // the last wide_tail panels (-1: half of them) use unit_n-column GEMM units throughout
     int potrf_sub = 128;    // diagonal tiles wider than this factor as a sub-DAG of sub-tiles of this size
-    int pair_updates = 0;   // RL: trailing tiles two columns or more right of the lookahead column take
-                            // the updates of panels k (even) and k + 1 in one pass (K = 2b)
+    int update_group = 1;   // RL: panels per trailing-update pass for tiles right of a group of that
+                            // many steps (1: one pass per panel; g: K = g b for most of the matrix)
     int update_prec = 0;    // trailing GEMM/SYRK units of the tile loop: 1 split-BF16, 2 split-FP16 matrix cores
     int bx_wide = 1;        // split-BF16 units: 128 x 256 blocks (0: 128 x 128 blocks only)
     int trsm_rows = 64;     // panel TRSM rows per unit

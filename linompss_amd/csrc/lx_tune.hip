@@ -5,8 +5,8 @@
 // chol_rlmain.c:25-50) and takes b from its caller.  Here a tile task is cut
 // into work units whose shapes (diagonal sub-tiles, trailing-update unit
 // heights and widths, panel-solve heights, narrow units in the last panels)
-// decide how well the persistent launch fills the chip (and whether tiles off
-// the lookahead columns take two panels' updates per pass); the best shapes
+// decide how well the persistent launch fills the chip (and how many panels'
+// updates a tile right of the diagonal chain takes per pass); the best shapes
 // depend on n, b, the precision and the update arithmetic.  The search is a
 // coordinate descent from the library's defaults over a small grid per knob:
 // each knob in turn takes each of its values with the others held at the
@@ -131,7 +131,7 @@ extern "C" int linompss_tune_factor(int dtype, int variant, int n, int b, int up
         std::vector<int> v;
     };
     const std::vector<knob> knobs = {
-        {&linompss_factor_config::pair_updates, {0, 1}},
+        {&linompss_factor_config::update_group, {1, 2, 3, 4, 6, 8}},
         {&linompss_factor_config::potrf_sub, {128, 256, 512}},
         {&linompss_factor_config::gemm_wide_m, {128, 256, 512}},
         {&linompss_factor_config::gemm_wide_n, {256, 512, 1024}},

@@ -44,8 +44,8 @@ extern "C" void lx_emu_set_units(int wide_m, int wide_n, int trsm_wide, int work
 
 // trailing-update precision of the next lx_emu_run plans (0 FP32, 1 split BF16)
 extern "C" void lx_emu_set_update_prec(int u) { g_update_prec = u; }
-static int g_pair = 0;  // RL plans: paired trailing updates
-extern "C" void lx_emu_set_pair(int v) { g_pair = v; }
+static int g_group = 1;  // RL plans: panels per trailing-update pass
+extern "C" void lx_emu_set_update_group(int v) { g_group = v; }
 // 1: FP32 factors with split-FP16 updates get the plane buffer (TRSM units
 // write the FP16 planes, eligible GEMM / SYRK units read them), as the
 // library sets it up
@@ -63,7 +63,7 @@ extern "C" int lx_emu_run(int dtype, int what, int n, int b, int nrhs, void *A, 
     P.gemm_wide_n = g_wide_n;
     P.trsm_rows_wide = g_trsm_wide;
     P.workers = g_workers;
-    P.pair_updates = g_pair;
+    P.update_group = g_group;
     lx_planner pl(P);
     int s = pl.add_system(n, b, nrhs);
     switch (what) {

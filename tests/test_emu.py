@@ -56,44 +56,45 @@ def test_factor_fp64(emu, orc, what, n, b):
     assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
 
 
-@pytest.mark.parametrize("dtype,n,b", [(np.float64, 150, 32), (np.float32, 160, 32)])
-def test_factor_paired_updates(emu, orc, dtype, n, b):
-    """RL plans with paired trailing updates (tiles two columns or more right
-    of the lookahead column take panels k and k + 1 in one K = 2b pass): the
-    factor still matches LAPACK, on a ragged last tile too, and the plan has
-    fewer units; FP32 with its split-FP16 updates read from the planes, and
-    the single-launch solve over such a plan."""
+@pytest.mark.parametrize("dtype,n,b,g", [(np.float64, 150, 32, 2), (np.float32, 160, 32, 2),
+                                         (np.float64, 150, 32, 3), (np.float64, 192, 32, 4)])
+def test_factor_grouped_updates(emu, orc, dtype, n, b, g):
+    """RL plans with grouped trailing updates (tiles right of a group of g
+    steps take the group's g panels in one K = g b pass): the factor matches
+    LAPACK and the plan with one pass per panel, on ragged last tiles too;
+    FP32 with its split-FP16 updates read from the planes; the single-launch
+    solve over a grouped plan."""
     E = _load_emu()
     A0 = orc.genmat_spd(n, 8, diag_mode=1) / (2.0 * n)
     Lref = np.linalg.cholesky(A0)
     up = 2 if dtype == np.float32 else 0
+    dt = 1 if dtype == np.float32 else 0
     E.lx_emu_set_update_prec(up)
     E.lx_emu_set_planes(1 if up else 0)
     try:
         A = A0.astype(dtype, order="F")
-        E.lx_emu_set_pair(1)
-        assert emu(1 if dtype == np.float32 else 0, 0, n, b, 0, A) == 0
-        Ap = A.copy(order="F")
-        E.lx_emu_set_pair(0)
+        E.lx_emu_set_update_group(g)
+        assert emu(dt, 0, n, b, 0, A) == 0
+        E.lx_emu_set_update_group(1)
         A1 = A0.astype(dtype, order="F")
-        assert emu(1 if dtype == np.float32 else 0, 0, n, b, 0, A1) == 0
+        assert emu(dt, 0, n, b, 0, A1) == 0
     finally:
-        E.lx_emu_set_pair(0)
+        E.lx_emu_set_update_group(1)
         E.lx_emu_set_planes(0)
         E.lx_emu_set_update_prec(0)
     tol = 1e-13 if dtype == np.float64 else 1e-6
-    assert np.linalg.norm(np.tril(Ap) - Lref) / np.linalg.norm(Lref) <= tol
-    assert np.linalg.norm(np.tril(Ap) - np.tril(A1)) / np.linalg.norm(Lref) <= tol
-    assert np.array_equal(np.triu(Ap, 1), np.triu(A0.astype(dtype), 1))
-    if dtype == np.float64:
-        # the fused factor + sweeps launch over a paired plan
+    assert np.linalg.norm(np.tril(A) - Lref) / np.linalg.norm(Lref) <= tol
+    assert np.linalg.norm(np.tril(A) - np.tril(A1)) / np.linalg.norm(Lref) <= tol
+    assert np.array_equal(np.triu(A, 1), np.triu(A0.astype(dtype), 1))
+    if dtype == np.float64 and g == 2:
+        # the fused factor + sweeps launch over a grouped plan
         X = np.asfortranarray(A0 @ np.ones((n, 1)))
         A2 = A0.copy(order="F")
-        E.lx_emu_set_pair(1)
+        E.lx_emu_set_update_group(g)
         try:
             assert emu(0, 2, n, b, 1, A2, X) == 0
         finally:
-            E.lx_emu_set_pair(0)
+            E.lx_emu_set_update_group(1)
         assert np.abs(X - 1.0).max() <= 1e-10
 
 

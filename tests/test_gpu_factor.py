@@ -202,19 +202,20 @@ def test_chol_tall_units(lx, orc, dev, monkeypatch, dtype, variant, n, b):
     assert np.linalg.norm(out[0] - Lref) / np.linalg.norm(Lref) <= tol
 
 
-@pytest.mark.parametrize("dtype,n,b", [(np.float64, 4096, 256), (np.float64, 3000, 256), (np.float64, 2100, 128),
-                                       (np.float32, 4096, 256), (np.float32, 3001, 512)])
-def test_chol_paired_updates(lx, orc, dev, monkeypatch, dtype, n, b):
-    """Paired trailing updates (the default for right-looking plans: tiles two
-    columns or more right of the lookahead column take panels k and k + 1 in one
-    K = 2b pass) against one pass per panel (LINOMPSS_PAIR=0) and LAPACK, on
-    exact and ragged tilings; FP32 with the split-FP16 updates from the planes."""
-    print(f"[paired] {np.dtype(dtype).name} n={n} b={b}", flush=True)
+@pytest.mark.parametrize("dtype,n,b,g", [(np.float64, 4096, 256, 2), (np.float64, 3000, 256, 3),
+                                         (np.float64, 2100, 128, 4), (np.float32, 4096, 256, 2),
+                                         (np.float32, 3001, 512, 3), (np.float32, 4100, 256, 4)])
+def test_chol_grouped_updates(lx, orc, dev, monkeypatch, dtype, n, b, g):
+    """Grouped trailing updates (tiles right of a group of g steps take the
+    group's g panels in one K = g b pass; g = 2 is the default) against one
+    pass per panel (LINOMPSS_UPDATE_GROUP=1) and LAPACK, on exact and ragged
+    tilings; FP32 with the split-FP16 updates from the planes."""
+    print(f"[grouped] {np.dtype(dtype).name} n={n} b={b} g={g}", flush=True)
     A0 = orc.genmat_spd(n, seed=10, diag_mode=1) / 2.0 ** np.ceil(np.log2(n))
     opts = lx.ExecOpts(1, 128, 128, 0, 30.0, 0, 2 if dtype == np.float32 else 0)
     out, units = [], []
-    for pair in ("1", "0"):
-        monkeypatch.setenv("LINOMPSS_PAIR", pair)
+    for grp in (str(g), "1"):
+        monkeypatch.setenv("LINOMPSS_UPDATE_GROUP", grp)
         dA = lx.DeviceMatrix.from_host(A0.astype(dtype, order="F"))
         assert lx.chol_dev(dA, b, "rl", opts) == 0
         units.append(lx.last_exec_stats().units)

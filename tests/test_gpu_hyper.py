@@ -55,7 +55,7 @@ def test_hypermatrix_equals_colmajor(lx, orc, dev, variant):
     assert fn(A, b) == 0  # the default plan
     lx.clear_factor_configs()
     _, cfg = lx.get_factor_config(n, b, np.float64, variant)
-    cfg.pair_updates = 0
+    cfg.update_group = 1
     lx.set_factor_config(n, cfg, np.float64, variant)
     try:
         As = A0.copy(order="F")

@@ -51,7 +51,7 @@ def test_tune_records_and_factors_stay_exact(lx, orc, dev, dtype, variant, updat
 
 def test_tune_fixed_tile_and_prior_kept(lx, dev):
     """b > 0 keeps the tile; a configuration recorded for another tile size is untouched."""
-    other = lx.FactorConfig(1024, 128, 128, 256, 64, 0, 0, 1.0)
+    other = lx.FactorConfig(1024, 128, 128, 256, 64, 0, 1, 1.0)
     lx.set_factor_config(1536, other)
     best, tried = lx.tune_factor(1536, 384, np.float64, "rl", 0, reps=1)
     assert best.b == 384 and all(t["b"] == 384 for t in tried)
