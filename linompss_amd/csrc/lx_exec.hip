@@ -306,7 +306,8 @@ void *lx_scratch(int slot, size_t bytes)
 // 1 = factor workspace (inverse diagonal blocks), 2 = executor profile,
 // 3 = sweep flags, 4..7 = refinement workspaces, 8 = residual partial sums,
 // 9 = adjoint workspace, 10 = log det, 11 = FP16 operand planes, 12 = GP
-// gradient partials.
+// gradient partials, 13 = step matrices of a 128-row sweep, 14 = the
+// refinement loop's step matrices (kept for all its solves).
 static bool lx_trace_on()
 {
     static int on = -1;

@@ -425,13 +425,28 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     info = rc;
     bool fallback = info != 0;
     double prev = INFINITY, cur_berr = INFINITY;
+    // one right-hand side: the 128-row sweeps, their step matrices computed
+    // once from this factor and kept for every solve of the loop
+    float *prep = nullptr;
+    if (!fallback && lx_sw2_enabled(1, n, nrhs)) {
+        std::lock_guard<std::mutex> g(lx_dev_mutex());
+        prep = (float *)lx_scratch(14, lx_sw2_prep_bytes(n));
+        if (!prep) return LINOMPSS_ERR_NOMEM;
+        rc = lx_sw2_prep(n, Af, n, prep, s);
+        if (rc) return rc;
+    }
+    auto fsolve = [&]() -> int {
+        if (!prep) return linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
+        std::lock_guard<std::mutex> g(lx_dev_mutex());
+        return lx_sw2_sweeps(n, Af, n, Sf, 3, prep, s);
+    };
     if (!fallback) {
         // x0 = solve(b) with per-column scaling by ||b_c||_inf
         // the FP32 right-hand side D b / max|D b| lies in [-1, 1] whatever D is
         hipLaunchKernelGGL(lx_colmax_kernel, dim3(nrhs), dim3(1024), 0, s, dB, ldb, n, Deq, bmax);
         hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, dB, ldb, bmax,
                            Deq, Sf);
-        rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
+        rc = fsolve();
         if (rc) return rc;
         hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf, bmax, Deq, dX,
                            ldx, 0);
@@ -462,7 +477,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
             // d = solve(r / ||r||); x += ||r|| d
             hipLaunchKernelGGL(lx_d2s_scaled_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, R,
                                (int64_t)n, Deq ? smax : rmax, Deq, Sf);
-            rc = linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
+            rc = fsolve();
             if (rc) return rc;
             hipLaunchKernelGGL(lx_update_kernel, dim3(grid_for((int64_t)nr)), dim3(256), 0, s, n, nrhs, Sf,
                                Deq ? smax : rmax, Deq, dX, ldx, 1);

@@ -28,6 +28,12 @@
 
 #define LX_SW_NR 16
 
+// waves per SIMD a kernel's registers are budgeted for (the CPU build of the
+// kernels defines it away)
+#ifndef LX_WAVES_PER_EU
+#define LX_WAVES_PER_EU(w) __attribute__((amdgpu_waves_per_eu(w, w)))
+#endif
+
 __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Work units of one sweep over nb 64-row blocks, in "step" order s = 0..nb-1
@@ -45,17 +51,21 @@ __device__ __forceinline__ int lx_sw_uniform(int v) { return __builtin_amdgcn_re
 // on a unit listed earlier, so a persistent grid that takes units in list
 // order cannot deadlock.  Unit code: (s << 16) | c.
 #define LX_SW_GAP 3  // the solve unit's direct products (s-3, s-2) use sD and sA
-__host__ __device__ constexpr int lx_sw_nparts(int s, int C) { return s > LX_SW_GAP ? (s - LX_SW_GAP + C - 1) / C : 0; }
-static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase)
+__host__ __device__ constexpr int lx_sw_nparts(int s, int C, int gap = LX_SW_GAP)
+{
+    return s > gap ? (s - gap + C - 1) / C : 0;
+}
+static inline void lx_sw_build_units(int nb, int C, std::vector<int> &units, std::vector<int> &rowbase,
+                                     int gap = LX_SW_GAP)
 {
     units.clear();
     rowbase.assign(nb + 1, 0);
-    for (int s = 0; s < nb; ++s) rowbase[s + 1] = rowbase[s] + lx_sw_nparts(s, C);
+    for (int s = 0; s < nb; ++s) rowbase[s + 1] = rowbase[s] + lx_sw_nparts(s, C, gap);
     for (int key = -1; key <= nb - 2; ++key) {
         units.push_back((key + 1) << 16);
         if (key < 0) continue;
         for (int c = 1;; ++c) {
-            const long long st = (long long)key + LX_SW_GAP + 1 + (long long)(c - 1) * C;
+            const long long st = (long long)key + gap + 1 + (long long)(c - 1) * C;
             if (st >= nb) break;
             units.push_back((int)(st << 16) | c);
         }
@@ -148,6 +158,22 @@ __device__ __forceinline__ T lx_sw_mb_get(const T *mb, int *abort, long long spi
         }
     }
     return __builtin_bit_cast(T, v);
+}
+
+// One load of a mailbox slot (its bits; all-ones = not yet written), so that
+// a lane can issue the loads of several slots before it polls any of them.
+template <typename T>
+__device__ __forceinline__ typename lx_sw_bits<T>::U lx_sw_mb_peek(const T *mb)
+{
+    return __hip_atomic_load((const typename lx_sw_bits<T>::U *)mb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The value of a peeked slot, polling (bounded) while it read empty.
+template <typename T>
+__device__ __forceinline__ T lx_sw_mb_finish(const T *mb, typename lx_sw_bits<T>::U v, int *abort,
+                                             long long spin_ticks)
+{
+    typedef typename lx_sw_bits<T>::U U;
+    return v == ~U(0) ? lx_sw_mb_get<T>(mb, abort, spin_ticks) : __builtin_bit_cast(T, v);
 }
 
 // trans = 0: forward L Y = B;  trans = 1: backward L^T X = Y.  X (n x nrhs,
@@ -484,6 +510,338 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
     }
 }
 
+// ---------------------------------------------------------------------------
+// One right-hand side: 128-row steps over precomputed step matrices.
+//
+// The chain of the sweep above costs about one cross-XCD mailbox round trip
+// per 64-row step.  With one right-hand side the chain can take 128-row steps
+// when the solve unit's matrices come from a pre-pass over L alone: for each
+// 128-row block i, with D = L_ii (padded with the identity past n),
+//   forward:  Z = D^-1,   M1 = D^-1 L_{i,i-1},     M2 = D^-1 L_{i,i-2}
+//   backward: Z = D^-T,   M1 = D^-T L_{i+1,i}^T,   M2 = D^-T L_{i+2,i}^T
+// so that x_i = Z (b_i - partial sums) - M2 x_{i-/+2} - M1 x_{i-/+1}: the
+// serial part of a step is one 128 x 128 product with the previous step's x,
+// from registers.  The pre-pass reads only L, so a refinement loop runs it
+// once per factor and reuses it for every sweep.  Layout of the pre-pass
+// output: for direction d (0 forward, 1 backward) and block i, three
+// column-major 128 x 128 matrices Z, M1, M2 at ((d nb + i) 3 + m) 128^2
+// (a coupling past either end of L is stored as zeros).
+#define LX_S2 128
+#define LX_S2_GAP 2  // partial sums skip the two newest steps (the solve unit's M2 and M1)
+#define LX_S2_MAT (LX_S2 * LX_S2)
+
+// One workgroup per block: grid nb, one launch per direction (DIR 0 forward,
+// 1 backward).  Every load loop has a constant trip count with all its loads
+// issued before the LDS stores, so a thread has its 16 or 64 loads in flight
+// at once.
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) lx_sw2_prep_kernel(int n, const T *__restrict__ L, int64_t ldl,
+                                                          T *__restrict__ prep)
+{
+    constexpr int WS = LX_S2 + 4;                                  // row stride of sW: 16-byte aligned rows
+    __shared__ T sZ[LX_S2][LX_S2 + 1];                             // D^-1 (lower)
+    __shared__ __attribute__((aligned(16))) T sW[LX_S2][WS];       // scratch, then a coupling block
+    const int nb = (n + LX_S2 - 1) / LX_S2, ib = blockIdx.x;
+    const int r0 = ib * LX_S2, hb = min(LX_S2, n - r0);
+    const int tid = lx_tid();
+    T(*sA)[65] = reinterpret_cast<T(*)[65]>(&sW[0][0]);
+    T(*sB)[65] = reinterpret_cast<T(*)[65]>(&sW[0][0] + 64 * 65);
+    // a 64 x 64 block of L at (row o1, column o2) of the diagonal block, rows
+    // past hb (and, on the diagonal, columns past hb) as the identity's
+    auto load64 = [&](T(*dst)[65], int o1, int o2, bool diag) {
+        T v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int e = tid + i * LX_WG_THREADS, r = e % 64, k = e / 64;
+            v[i] = (o1 + r < hb && o2 + k < hb) ? L[(int64_t)(r0 + o2 + k) * ldl + r0 + o1 + r]
+                                                : (diag && r == k ? T(1) : T(0));
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int e = tid + i * LX_WG_THREADS;
+            dst[e % 64][e / 64] = v[i];
+        }
+    };
+    for (int e = tid; e < LX_S2 * (LX_S2 + 1); e += LX_WG_THREADS) (&sZ[0][0])[e] = T(0);
+    // the two 64 x 64 diagonal blocks of D, inverted in turn
+    for (int h = 0; h < 2; ++h) {
+        load64(sA, 64 * h, 64 * h, true);
+        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) sB[e % 64][e / 64] = T(0);
+        __syncthreads();
+        lx_tri_inv64<T>(sA, sB);  // ends with a barrier
+        for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
+            const int r = e % 64, k = e / 64;
+            sZ[64 * h + r][64 * h + k] = sB[r][k];
+        }
+        __syncthreads();
+    }
+    // Z21 = -Z22 (L21 Z11): sB = L21 (rows past n zero), sA = L21 Z11
+    load64(sB, 64, 0, false);
+    __syncthreads();
+    {
+        const int i = tid & 63, j0 = 16 * (tid >> 6);
+        T acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = T(0);
+        for (int k = 0; k < 64; ++k) {
+            const T a = sB[i][k];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] += a * sZ[k][j0 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) sA[i][j0 + q] = acc[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = T(0);
+        for (int k = 0; k < 64; ++k) {
+            const T a = sZ[64 + i][64 + k];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] += a * sA[k][j0 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) sZ[64 + i][j0 + q] = -acc[q];
+    }
+    __syncthreads();
+    T *pz = prep + (size_t)(DIR * nb + ib) * 3 * LX_S2_MAT;
+    for (int e = tid; e < LX_S2_MAT; e += LX_WG_THREADS) {
+        const int r = e % LX_S2, c = e / LX_S2;
+        pz[e] = DIR ? sZ[c][r] : sZ[r][c];
+    }
+    // M_d = op(Z) C_d: C[k][c] = L[r0 + k, ip*128 + c] (forward, ip = ib - d) or
+    // L[ip*128 + c, r0 + k] (backward, ip = ib + d); thread: row r, 64 columns.
+    // Load e = tid + 256 i of the block: lo = tid % 128 (k forward, c
+    // backward), hi = tid / 128 + 2 i, consecutive lanes along a column of L.
+#pragma unroll 1
+    for (int d = 1; d <= 2; ++d) {
+        const int ip = DIR ? ib + d : ib - d;
+        T *pm = pz + (size_t)d * LX_S2_MAT;
+        if (ip < 0 || ip >= nb) {  // uniform
+            for (int e = tid; e < LX_S2_MAT; e += LX_WG_THREADS) pm[e] = T(0);
+            continue;
+        }
+        const int hp = min(LX_S2, n - ip * LX_S2);
+        const int lo = tid & (LX_S2 - 1), hi0 = tid >> 7;
+        const bool lo_ok = DIR ? lo < hp : lo < hb;
+        const int hi_lim = DIR ? hb : hp;
+        const T *src = DIR ? L + (int64_t)(r0 + hi0) * ldl + ip * LX_S2 + lo : L + (int64_t)(ip * LX_S2 + hi0) * ldl + r0 + lo;
+        const int64_t step = 2 * ldl;
+        T v[LX_S2_MAT / LX_WG_THREADS];
+#pragma unroll
+        for (int i = 0; i < LX_S2_MAT / LX_WG_THREADS; ++i)
+            v[i] = (lo_ok && hi0 + 2 * i < hi_lim) ? src[i * step] : T(0);
+        __syncthreads();  // sW free
+#pragma unroll
+        for (int i = 0; i < LX_S2_MAT / LX_WG_THREADS; ++i) {
+            if (!DIR) sW[lo][hi0 + 2 * i] = v[i];
+            else sW[hi0 + 2 * i][lo] = v[i];
+        }
+        __syncthreads();
+        const int r = tid & (LX_S2 - 1), c0 = 64 * (tid >> 7);
+        T acc[64];
+#pragma unroll
+        for (int q = 0; q < 64; ++q) acc[q] = T(0);
+        for (int k = 0; k < LX_S2; ++k) {
+            const T z = DIR ? sZ[k][r] : sZ[r][k];
+            const T *wk = &sW[k][c0];
+#pragma unroll
+            for (int q = 0; q < 64; ++q) acc[q] += z * wk[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 64; ++q) pm[(size_t)(c0 + q) * LX_S2 + r] = acc[q];
+    }
+}
+
+// Padded LDS index of a 128-vector read by two half-waves at once (entries k
+// and 64 + k land in different banks).
+__device__ __forceinline__ int lx_s2_ix(int k) { return k + 4 * (k >> 6); }
+
+// trans = 0: forward L y = x;  trans = 1: backward L^T x = y; one column X,
+// overwritten in place.  Units as for lx_sweep_kernel with 128-row steps and
+// a gap of 2.  Partial sums are mailboxes too: P(s, c) writes its 128 values
+// with atomic exchanges and S(s) polls them by value, so the path from the
+// newest step a part reads to the solve unit that adds it is two memory
+// round trips (no counter, no release fence), within the two steps of slack
+// the gap gives.  ctl = [ticket, abort, ...] zero on entry; part
+// (rowbase[nb] * 128) and mb (nb * 128) all-ones bits on entry; prep from
+// lx_sw2_prep_kernel over the same L.
+template <typename T, int TRANS, int WPE>
+__global__ void __launch_bounds__(256) LX_WAVES_PER_EU(WPE) lx_sweep2_kernel(int n, const T *__restrict__ L, int64_t ldl, T *__restrict__ X,
+                                                        const T *__restrict__ prep, const int *__restrict__ units,
+                                                        int nunits, const int *__restrict__ rowbase, int C,
+                                                        T *__restrict__ part, int *ctl, T *__restrict__ mb,
+                                                        long long spin_ticks)
+{
+    constexpr int KC = 128 / sizeof(T);  // backward partial sums: L rows per LDS slab
+    constexpr int CPI = 64 / KC;
+    constexpr int V = LX_S2 + 8;  // padded 128-vector (lx_s2_ix)
+    __shared__ __attribute__((aligned(16))) T stile[4][64][KC + 1];
+    __shared__ __attribute__((aligned(16))) T sxw[4][V];  // per wave: x of a block (P: 64, S: x_{s-1})
+    __shared__ __attribute__((aligned(16))) T sy[V], sx2[V];
+    __shared__ T sred[2][LX_S2];
+    __shared__ int s_unit;
+    const int nb = (n + LX_S2 - 1) / LX_S2;
+    int *ticket = ctl, *abort = ctl + 1;
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave();
+    for (;;) {
+        if (w == 0) {
+            const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
+            s_unit = t < nunits ? units[t] : -1;  // every lane stores the same value
+        }
+        __syncthreads();
+        const int code = lx_sw_uniform(s_unit);
+        __syncthreads();
+        if (code < 0) break;
+        const int s = code >> 16, c = code & 0xffff;
+        const int ib = TRANS ? nb - 1 - s : s;
+        const int r0 = ib * LX_S2, hb = min(LX_S2, n - r0);
+        if (c == 0) {
+            // ---------------- S(s) ----------------
+            // thread: row rl of the block, columns 64 hc .. 64 hc + 63 of Z, M1, M2
+            const int rl = 32 * w + (lane & 31), hc = lane >> 5;
+            const T *pz = prep + (size_t)(TRANS * nb + ib) * 3 * LX_S2_MAT + (size_t)(64 * hc) * LX_S2 + rl;
+            T mz[64], m1[64], m2[64];
+#pragma unroll
+            for (int q = 0; q < 64; ++q) {
+                mz[q] = pz[q * LX_S2];
+                m2[q] = pz[2 * LX_S2_MAT + q * LX_S2];
+                m1[q] = pz[LX_S2_MAT + q * LX_S2];
+            }
+            // waves 0, 1: y = b minus the partial sums (each slot's load issued
+            // before any is polled, added in part order); waves 2, 3: x_{s-2}
+            const int need = lx_sw_nparts(s, C, LX_S2_GAP);
+            if (tid < LX_S2) {  // waves 0 and 1
+                T y = tid < hb ? X[r0 + tid] : T(0);
+                const T *pp = part + (size_t)rowbase[s] * LX_S2 + tid;
+                for (int q0 = 0; q0 < need; q0 += 8) {  // the host keeps need <= 8
+                    typename lx_sw_bits<T>::U v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q0 + q < need) v[q] = lx_sw_mb_peek<T>(pp + (size_t)(q0 + q) * LX_S2);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q0 + q < need) y -= lx_sw_mb_finish<T>(pp + (size_t)(q0 + q) * LX_S2, v[q], abort, spin_ticks);
+                }
+                sy[lx_s2_ix(tid)] = y;
+            } else {
+                const int k = 64 * (w - 2) + lane;
+                T v = T(0);
+                if (s >= 2) {
+                    const int ip = TRANS ? ib + 2 : ib - 2;
+                    if (k < min(LX_S2, n - ip * LX_S2)) v = lx_sw_mb_get<T>(mb + ip * LX_S2 + k, abort, spin_ticks);
+                }
+                sx2[lx_s2_ix(k)] = v;
+            }
+            __syncthreads();
+            // u = Z y - M2 x_{s-2}; the two half-waves' sums meet by a lane swap
+            const T *yv = sy + 68 * hc, *x2v = sx2 + 68 * hc;  // = lx_s2_ix(64 hc + q) - q
+            T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);
+#pragma unroll
+            for (int q = 0; q < 64; q += 2) {
+                a0 += mz[q] * yv[q];
+                a1 += mz[q + 1] * yv[q + 1];
+                b0 += m2[q] * x2v[q];
+                b1 += m2[q + 1] * x2v[q + 1];
+            }
+            T a = a0 + a1, bb = b0 + b1;
+            a += __shfl_xor(a, 32);
+            bb += __shfl_xor(bb, 32);
+            T u = a - bb;
+            if (s >= 1) {  // uniform: the chain, x = u - M1 x_{s-1}
+                const int ip = TRANS ? ib + 1 : ib - 1;
+                const int hp = min(LX_S2, n - ip * LX_S2);
+                T *sx1 = sxw[w];
+                const T *p0 = mb + ip * LX_S2 + lane, *p1 = p0 + 64;
+                typename lx_sw_bits<T>::U e0 = 0, e1 = 0;  // zero bits: the value 0 past hp
+                if (lane < hp) e0 = lx_sw_mb_peek<T>(p0);
+                if (64 + lane < hp) e1 = lx_sw_mb_peek<T>(p1);
+                sx1[lx_s2_ix(lane)] = lx_sw_mb_finish<T>(p0, e0, abort, spin_ticks);
+                sx1[lx_s2_ix(64 + lane)] = lx_sw_mb_finish<T>(p1, e1, abort, spin_ticks);
+                __builtin_amdgcn_wave_barrier();
+                const T *x1v = sx1 + 68 * hc;
+                T c0 = T(0), c1 = T(0);
+#pragma unroll
+                for (int q = 0; q < 64; q += 2) {
+                    c0 += m1[q] * x1v[q];
+                    c1 += m1[q + 1] * x1v[q + 1];
+                }
+                T cs = c0 + c1;
+                cs += __shfl_xor(cs, 32);
+                u -= cs;
+            }
+            if (hc == 0 && rl < hb) {
+                X[r0 + rl] = u;
+                lx_sw_mb_put<T>(mb + r0 + rl, u);
+            }
+        } else {
+            // ---------------- P(s, c) ----------------
+            // steps [t_lo, t_hi); wave w: rows 64 (w >> 1) .. of the block against
+            // columns 64 (w & 1) .. of each earlier block
+            const int t_hi = s - LX_S2_GAP - (c - 1) * C, t_lo = max(0, t_hi - C);
+            const int rh = w >> 1, ch = w & 1;
+            const int hr = min(64, hb - 64 * rh);
+            T acc = T(0);
+            T *sx = sxw[w];
+            if (hr > 0) {  // wave-uniform
+                const int row = r0 + 64 * rh + min(lane, hr - 1);  // clamped: rows past hr never stored
+                for (int t = t_lo; t < t_hi; ++t) {
+                    const int jb = TRANS ? nb - 1 - t : t;
+                    const int c0 = jb * LX_S2 + 64 * ch, hj = min(64, n - c0);
+                    if (hj <= 0) continue;  // uniform (a ragged last block, backward)
+                    // the block of L before the wait: only x depends on step t
+                    T lv[64];
+                    if (!TRANS) {
+                        const T *Lb = L + (int64_t)c0 * ldl + row;
+#pragma unroll
+                        for (int k = 0; k < 64; ++k) lv[k] = Lb[(int64_t)k * ldl];
+                    } else {
+                        const int kk = lane % KC, cq = lane / KC;
+#pragma unroll
+                        for (int m = 0; m < 64 / KC; ++m)
+#pragma unroll
+                            for (int q = 0; q < 64 / CPI; ++q) {
+                                const int colg = r0 + 64 * rh + min(q * CPI + cq, hr - 1);
+                                const int rowg = c0 + min(m * KC + kk, hj - 1);
+                                lv[m * (64 / CPI) + q] = L[(int64_t)colg * ldl + rowg];
+                            }
+                    }
+                    sx[lane] = lane < hj ? lx_sw_mb_get<T>(mb + c0 + lane, abort, spin_ticks) : T(0);
+                    __builtin_amdgcn_wave_barrier();
+                    if (!TRANS) {
+                        T p0 = T(0), p1 = T(0);
+#pragma unroll
+                        for (int k = 0; k < 64; k += 2) {
+                            p0 += lv[k] * sx[k];
+                            p1 += lv[k + 1] * sx[k + 1];
+                        }
+                        acc += p0 + p1;
+                    } else {
+                        const int kk = lane % KC, cq = lane / KC;
+#pragma unroll
+                        for (int m = 0; m < 64 / KC; ++m) {
+#pragma unroll
+                            for (int q = 0; q < 64 / CPI; ++q) stile[w][q * CPI + cq][kk] = lv[m * (64 / CPI) + q];
+                            __builtin_amdgcn_wave_barrier();
+                            T p0 = T(0), p1 = T(0);
+#pragma unroll
+                            for (int k = 0; k < KC; k += 2) {  // rows past hj: x is zero there
+                                p0 += stile[w][lane][k] * sx[m * KC + k];
+                                p1 += stile[w][lane][k + 1] * sx[m * KC + k + 1];
+                            }
+                            acc += p0 + p1;
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            sred[ch][64 * rh + lane] = acc;
+            __syncthreads();
+            T *pp = part + ((size_t)rowbase[s] + (c - 1)) * LX_S2;
+            if (tid < LX_S2) lx_sw_mb_put<T>(pp + tid, tid < hb ? sred[0][tid] + sred[1][tid] : T(0));
+        }
+    }
+}
+
 #ifndef LX_SWEEP_KERNELS_ONLY
 #include <map>
 #include <mutex>
@@ -657,11 +1015,134 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
 
 // which: 1 forward, 2 backward, 3 both.  Returns after the sweeps finished
 // (each launch is checked for a timed-out wait).
+// ---------------------------------------------------------------------------
+// 128-row sweeps of one FP32 right-hand side (lx_sweep2_kernel) over the
+// step matrices of lx_sw2_prep_kernel.
+namespace {
+// 128-row blocks per partial-sum unit: at least 4, and at most 8 parts per step
+constexpr int lx_s2_chunk(int nb) { return (nb - 1 + 7) / 8 > 4 ? (nb - 1 + 7) / 8 : 4; }
+std::map<int, std::map<int, lx_sw_plan>> lx_s2_plans;  // [device][number of blocks]
+int lx_s2_occ_tab[LX_SW_MAXDEV][2][2];  // [device][waves per SIMD - 1][trans]
+}  // namespace
+
+int lx_sw2_enabled(int dtype, int n, int nrhs)
+{
+    if (dtype != 1 || nrhs != 1) return 0;
+    int min_n = 2048;  // below it the pre-pass costs more than the shorter chain saves
+    if (const char *e = getenv("LINOMPSS_SWEEP2_MIN_N")) min_n = atoi(e);
+    return min_n > 0 && n >= min_n;
+}
+
+size_t lx_sw2_prep_bytes(int n) { return sizeof(float) * 6 * (size_t)LX_S2_MAT * (size_t)((n + LX_S2 - 1) / LX_S2); }
+
+int lx_sw2_prep(int n, const float *L, int64_t ldl, float *prep, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    const int nb = (n + LX_S2 - 1) / LX_S2;
+    hipLaunchKernelGGL((lx_sw2_prep_kernel<float, 0>), dim3(nb), dim3(256), 0, s, n, L, ldl, prep);
+    hipLaunchKernelGGL((lx_sw2_prep_kernel<float, 1>), dim3(nb), dim3(256), 0, s, n, L, ldl, prep);
+    return hipGetLastError() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
+}
+
+int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const float *prep, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    const int nb = (n + LX_S2 - 1) / LX_S2, C = lx_s2_chunk(nb);
+    const lx_sw_props *d = nullptr;
+    lx_sw_plan pl;
+    int occ[2];
+    int wpe = 1;  // LINOMPSS_SWEEP2_WPE=2: the build budgeted for two waves per SIMD
+    if (const char *e = getenv("LINOMPSS_SWEEP2_WPE")) wpe = atoi(e) == 2 ? 2 : 1;
+    {
+        std::lock_guard<std::mutex> g(lx_sw_mu);
+        int dev = 0;
+        d = lx_sw_props_get(&dev);
+        if (!d) return LINOMPSS_ERR_HIP;
+        if (lx_s2_occ_tab[dev][0][0] == 0) {
+            auto occ_of = [](const void *k) {
+                int o = 0;
+                const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, 256, 0);
+                return (e == hipSuccess && o >= 1) ? o : 1;
+            };
+            lx_s2_occ_tab[dev][0][0] = occ_of((const void *)lx_sweep2_kernel<float, 0, 1>);
+            lx_s2_occ_tab[dev][0][1] = occ_of((const void *)lx_sweep2_kernel<float, 1, 1>);
+            lx_s2_occ_tab[dev][1][0] = occ_of((const void *)lx_sweep2_kernel<float, 0, 2>);
+            lx_s2_occ_tab[dev][1][1] = occ_of((const void *)lx_sweep2_kernel<float, 1, 2>);
+        }
+        occ[0] = lx_s2_occ_tab[dev][wpe - 1][0];
+        occ[1] = lx_s2_occ_tab[dev][wpe - 1][1];
+        auto &plans = lx_s2_plans[dev];
+        auto it = plans.find(nb);
+        if (it == plans.end()) {
+            std::vector<int> units, rowbase;
+            lx_sw_build_units(nb, C, units, rowbase, LX_S2_GAP);
+            lx_sw_plan np;
+            np.nunits = (int)units.size();
+            np.nparts = rowbase[nb];
+            if (hipMalloc(&np.d_units, sizeof(int) * units.size()) != hipSuccess) return LINOMPSS_ERR_NOMEM;
+            if (hipMalloc(&np.d_rowbase, sizeof(int) * rowbase.size()) != hipSuccess) {
+                (void)hipFree(np.d_units);
+                return LINOMPSS_ERR_NOMEM;
+            }
+            if (hipMemcpy(np.d_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(np.d_rowbase, rowbase.data(), sizeof(int) * rowbase.size(), hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                return LINOMPSS_ERR_HIP;
+            it = plans.emplace(nb, np).first;
+        }
+        pl = it->second;
+    }
+    // workspace: ctl (ticket, abort) | part | mailbox
+    const size_t ctl_ints = 64;
+    const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
+    const size_t mb_off = (part_off + sizeof(float) * (size_t)std::max(1, pl.nparts) * LX_S2 + 255) / 256 * 256;
+    const size_t bytes = mb_off + sizeof(float) * (size_t)nb * LX_S2;
+    char *ws = (char *)lx_scratch(3, bytes);
+    if (!ws) return LINOMPSS_ERR_NOMEM;
+    int *ctl = (int *)ws;
+    float *part = (float *)(ws + part_off), *mb = (float *)(ws + mb_off);
+    double tmo = 30.0;
+    if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
+    const long long ticks = (long long)(tmo * d->wall_khz * 1000.0);
+    int aborted = 0;
+    for (int tr = 0; tr < 2; ++tr) {
+        if (!(which & (1 << tr))) continue;
+        const int grid = std::max(1, std::min(pl.nunits, d->cus * occ[tr]));
+        lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
+        // all-ones: empty (the parts and the mailbox are contiguous)
+        lx_fill_i32((int32_t *)part, -1, (int64_t)((mb_off - part_off) / sizeof(float)) + (int64_t)nb * LX_S2, s);
+#define LX_S2_LAUNCH(TR, WPE)                                                                                      \
+    hipLaunchKernelGGL((lx_sweep2_kernel<float, TR, WPE>), dim3(grid), dim3(256), 0, s, n, L, ldl, X, prep,       \
+                       pl.d_units, pl.nunits, pl.d_rowbase, C, part, ctl, mb, ticks)
+        if (tr == 0) {
+            if (wpe == 2) LX_S2_LAUNCH(0, 2);
+            else LX_S2_LAUNCH(0, 1);
+        } else {
+            if (wpe == 2) LX_S2_LAUNCH(1, 2);
+            else LX_S2_LAUNCH(1, 1);
+        }
+#undef LX_S2_LAUNCH
+        if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
+        int ab = 0;
+        LX_HIP_TRY(hipMemcpyAsync(&ab, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        LX_HIP_TRY(hipStreamSynchronize(s));
+        aborted |= ab;
+    }
+    return aborted ? LINOMPSS_ERR_SCHED : 0;
+}
+
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
                   hipStream_t s)
 {
     if (n <= 0 || nrhs <= 0) return 0;
     if (dtype == 0) return sweeps<double>(n, nrhs, (const double *)L, ldl, (double *)X, ldx, which, s);
+    if (lx_sw2_enabled(dtype, n, nrhs)) {
+        float *prep = (float *)lx_scratch(13, lx_sw2_prep_bytes(n));
+        if (!prep) return LINOMPSS_ERR_NOMEM;
+        int rc = lx_sw2_prep(n, (const float *)L, ldl, prep, s);
+        if (rc) return rc;
+        return lx_sw2_sweeps(n, (const float *)L, ldl, (float *)X, which, prep, s);
+    }
     return sweeps<float>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, s);
 }
 

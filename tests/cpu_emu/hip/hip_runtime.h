@@ -28,6 +28,8 @@
 #define __device__
 #define __host__
 #define __global__
+// kernel attributes of the register allocator have no meaning here
+#define LX_WAVES_PER_EU(w)
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static

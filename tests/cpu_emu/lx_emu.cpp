@@ -405,6 +405,50 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
     return lx_emu_sweep_ex(dtype, trans, n, nrhs, L, ldl, X, ldx, -1, 100000000LL * 20);
 }
 
+// One FP32 sweep with 128-row steps: the step matrices through
+// lx_sw2_prep_kernel (grid nb, each direction, one workgroup at a time), then
+// lx_sweep2_kernel as one workgroup taking every unit in list order.
+// Returns 0, -3 when a wait gave up, -1 when a partial sum was never written,
+// -2 when the ticket does not add up.
+extern "C" int lx_emu_sweep2(int trans, int n, const float *L, long long ldl, float *X, int C)
+{
+    const int nb = (n + LX_S2 - 1) / LX_S2;
+    std::vector<float> prep((size_t)6 * LX_S2_MAT * nb);
+    gridDim.x = nb;
+    gridDim.y = 2;
+    gridDim.z = 1;
+    blockDim.x = 256;
+    for (int y = 0; y < 2; ++y)
+        for (int x = 0; x < nb; ++x)
+            run_wg([&] {
+                blockIdx.x = x;
+                blockIdx.y = y;
+                blockIdx.z = 0;
+                if (y) lx_sw2_prep_kernel<float, 1>(n, L, ldl, prep.data());
+                else lx_sw2_prep_kernel<float, 0>(n, L, ldl, prep.data());
+            });
+    gridDim.x = 1;
+    gridDim.y = 1;
+    std::vector<int> units, rowbase;
+    lx_sw_build_units(nb, C, units, rowbase, LX_S2_GAP);
+    std::vector<int> ctl(64, 0);
+    std::vector<unsigned int> part((size_t)std::max(1, rowbase[nb]) * LX_S2, ~0u);  // all-ones: empty
+    std::vector<unsigned int> mbv((size_t)nb * LX_S2, ~0u);
+    const long long ticks = 100000000LL * 20;
+    run_wg([&] {
+        if (trans)
+            lx_sweep2_kernel<float, 1, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(), C,
+                                       (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
+        else
+            lx_sweep2_kernel<float, 0, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(), C,
+                                       (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
+    });
+    if (ctl[1] != 0) return -3;
+    for (int q = 0; q < rowbase[nb] * LX_S2; ++q)  // every partial sum written
+        if (part[q] == ~0u) return -1;
+    return ctl[0] == (int)units.size() + 1 ? 0 : -2;
+}
+
 // Batched refinement solves through lx_batch_ir_kernel, one workgroup taking
 // every system from the queue.  stat: count x {sweeps, fallback, info, pad, berr}.
 extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, const long long *lda,

@@ -312,6 +312,40 @@ def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
         assert np.abs(X - Xo).max() <= tol * np.abs(Xo).max()
 
 
+@pytest.fixture(scope="module")
+def emu_sweep2(emu):
+    E = _load_emu()
+    E.lx_emu_sweep2.restype = ctypes.c_int
+    E.lx_emu_sweep2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+                                ctypes.c_int]
+
+    def run(trans, L, X, C):
+        rc = E.lx_emu_sweep2(trans, L.shape[0], L.ctypes.data_as(ctypes.c_void_p), L.shape[0],
+                             X.ctypes.data_as(ctypes.c_void_p), C)
+        assert rc == 0
+    return run
+
+
+@pytest.mark.parametrize("n,C", [(1, 4), (128, 4), (129, 4), (300, 1), (520, 2), (1000, 3)])
+def test_sweep2_kernels(emu_sweep2, orc, n, C):
+    """The 128-row FP32 sweeps (step matrices from the pre-pass, then the
+    sweep kernel with C blocks per partial sum) against the oracle's blocked
+    sweeps in FP64 on the same FP32 inputs: a Cholesky factor of a Gram
+    matrix plus 1e-2 I, ragged orders included.  The bound is FP32 rounding
+    of a solve with cond(L) of a few hundred (the 64-row kernel measures the
+    same 1e-7 to 4e-7 on these inputs)."""
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    L = np.asfortranarray(np.linalg.cholesky(G @ G.T / n + 1e-2 * np.eye(n))).astype(np.float32)
+    b = rng.uniform(-1, 1, (n, 1)).astype(np.float32)
+    for trans in (0, 1):
+        X = np.asfortranarray(b.copy())
+        emu_sweep2(trans, L, X, C)
+        Xo = b.astype(np.float64, order="F")
+        orc.ltrsm_lower(trans, np.asfortranarray(L.astype(np.float64)), Xo, 32)
+        assert np.abs(X - Xo).max() <= 2e-6 * np.abs(Xo).max()
+
+
 def _emu_batch(sizes, mats, rhs, max_sweeps=30, tol=1e-14, stall=0.5):
     """lx_batch_ir_kernel on the CPU: one workgroup takes every system."""
     E = _load_emu()

@@ -133,6 +133,33 @@ def test_sltrsm_multi_rhs(lx, orc, dev, trans, nr):
     assert rel <= 2e-5, rel
 
 
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m", [300, 1000, 4100, 8192])
+def test_sltrsm_128_row_steps(lx, orc, dev, monkeypatch, trans, m):
+    """One FP32 right-hand side with 128-row steps (the step matrices' pre-pass
+    and the 128-row sweep kernel; forced down to any order here) against the
+    oracle's blocked sweeps in FP64 on the same FP32 inputs, on a Cholesky
+    factor of a Gram matrix plus 1e-2 I; ragged orders included.  The 64-row
+    kernel on the same inputs sets the scale: the error may not exceed twice
+    its error or the FP32 bound."""
+    print(f"[sltrsm 128 {trans}] m={m}", flush=True)
+    rng = np.random.default_rng(m)
+    G = rng.standard_normal((m, m))
+    L = np.asfortranarray(np.linalg.cholesky(G @ G.T / m + 1e-2 * np.eye(m))).astype(np.float32)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, 1))).astype(np.float32)
+    Bo = B0.astype(np.float64, order="F")
+    orc.ltrsm_lower(0 if trans == "N" else 1, np.asfortranarray(L.astype(np.float64)), Bo, 256, 1.0)
+    errs = {}
+    for min_n in ("1", "0"):  # 128-row steps, then the 64-row kernel
+        monkeypatch.setenv("LINOMPSS_SWEEP2_MIN_N", min_n)
+        B = B0.copy(order="F")
+        lx.sltrsm(trans, L, B, 256, 1.0)
+        assert np.isfinite(B).all()
+        errs[min_n] = np.abs(B - Bo).max() / np.abs(Bo).max()
+    print(f"  rel err 128-row {errs['1']:.2e}, 64-row {errs['0']:.2e}", flush=True)
+    assert errs["1"] <= max(2 * errs["0"], 2e-6), errs
+
+
 def test_dposv_ir_fallback(lx, dev):
     """cond(A) ~ 1e9 is beyond what an FP32 factor can refine: the FP64 factor answers."""
     n, b = 600, 128
