@@ -39,15 +39,16 @@ int lx_run_single(const lx_shape &sh, void *dA, int64_t lda, void *dX, int64_t l
 int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *X, int64_t ldx, int which,
                   hipStream_t s);
 
-// 128-row sweeps of one FP32 right-hand side (lx_sweep.hip): whether a solve
-// of this shape takes them (LINOMPSS_SWEEP2_MIN_N: the smallest order, 0 =
-// never; default 2048), the bytes of their step matrices, the pre-pass that
-// writes them from L (enqueued only), and the sweeps over them (which as
-// above; synchronises the stream).  The caller holds lx_dev_mutex().
+// 128-row FP32 sweeps (lx_sweep.hip): whether a solve of this shape takes
+// them (LINOMPSS_SWEEP2_MIN_N: the smallest order, 0 = never; default 2048),
+// the bytes of their step matrices, the pre-pass that writes them from L
+// (enqueued only), and the sweeps over them (which as above; synchronises the
+// stream).  The caller holds lx_dev_mutex().
 int lx_sw2_enabled(int dtype, int n, int nrhs);
 size_t lx_sw2_prep_bytes(int n);
 int lx_sw2_prep(int n, const float *L, int64_t ldl, float *prep, hipStream_t s);
-int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const float *prep, hipStream_t s);
+int lx_sw2_sweeps(int n, int nrhs, const float *L, int64_t ldl, float *X, int64_t ldx, int which, const float *prep,
+                  hipStream_t s);
 
 // The lock that serialises the executor's plans, its scratch slots and the
 // sweep kernels on a device.

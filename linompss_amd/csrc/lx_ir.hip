@@ -425,8 +425,8 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     info = rc;
     bool fallback = info != 0;
     double prev = INFINITY, cur_berr = INFINITY;
-    // one right-hand side: the 128-row sweeps, their step matrices computed
-    // once from this factor and kept for every solve of the loop
+    // the 128-row sweeps, their step matrices computed once from this factor
+    // and kept for every solve of the loop
     float *prep = nullptr;
     if (!fallback && lx_sw2_enabled(1, n, nrhs)) {
         std::lock_guard<std::mutex> g(lx_dev_mutex());
@@ -438,7 +438,7 @@ extern "C" int linompss_dposv_ir_dev(int n, int nrhs, int b, const double *dA, i
     auto fsolve = [&]() -> int {
         if (!prep) return linompss_potrs_dev(1, n, nrhs, b, Af, n, Sf, n, s, &eo);
         std::lock_guard<std::mutex> g(lx_dev_mutex());
-        return lx_sw2_sweeps(n, Af, n, Sf, 3, prep, s);
+        return lx_sw2_sweeps(n, nrhs, Af, n, Sf, n, 3, prep, s);
     };
     if (!fallback) {
         // x0 = solve(b) with per-column scaling by ||b_c||_inf

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "linompss.h"
@@ -530,6 +531,50 @@ __global__ void __launch_bounds__(256) lx_sweep_kernel(int n, int nrhs, const T 
 #define LX_S2_GAP 2  // partial sums skip the two newest steps (the solve unit's M2 and M1)
 #define LX_S2_MAT (LX_S2 * LX_S2)
 
+// Partial sums of step s, of sizes growing geometrically from C0 steps: part
+// c = 1, 2, ... covers the steps [hi - C0 2^(c-1), hi) with
+// hi = s - gap - C0 (2^(c-1) - 1), clipped at 0.  The part the solve unit
+// waits for last (c = 1, the newest steps) is short, so it is done soon after
+// its newest step is; the larger parts end at older steps and are done long
+// before they are needed.
+__host__ __device__ inline void lx_s2_part(int s, int c, int C0, int &lo, int &hi)
+{
+    const int sz = C0 << (c - 1);
+    hi = s - LX_S2_GAP - (sz - C0);
+    lo = hi - sz > 0 ? hi - sz : 0;
+}
+__host__ __device__ inline int lx_s2_nparts(int s, int C0)
+{
+    int m = 0;
+    for (int done = 0; done < s - LX_S2_GAP; ++m) done += C0 << m;
+    return m;
+}
+// Unit list of a 128-row sweep over nb blocks: S(s) keyed by s - 1, P(s, c)
+// by the newest step it reads (hi - 1), S first within a key, then by s.
+// Every wait is on a unit listed earlier, so a persistent grid that takes
+// units in list order cannot deadlock.  rowbase[s]: index of step s's first
+// part.
+static inline void lx_s2_build_units(int nb, int C0, std::vector<int> &units, std::vector<int> &rowbase)
+{
+    rowbase.assign(nb + 1, 0);
+    std::vector<std::array<int, 3>> v;  // key, 0 = S / 1 = P, code
+    for (int s = 0; s < nb; ++s) {
+        const int np = lx_s2_nparts(s, C0);
+        rowbase[s + 1] = rowbase[s] + np;
+        v.push_back({s - 1, 0, s << 16});
+        for (int c = 1; c <= np; ++c) {
+            int lo, hi;
+            lx_s2_part(s, c, C0, lo, hi);
+            v.push_back({hi - 1, 1, (s << 16) | c});
+        }
+    }
+    std::stable_sort(v.begin(), v.end(), [](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+        return a[0] != b[0] ? a[0] < b[0] : a[1] < b[1];
+    });
+    units.clear();
+    for (const auto &e : v) units.push_back(e[2]);
+}
+
 // One workgroup per block: grid nb, one launch per direction (DIR 0 forward,
 // 1 backward).  Every load loop has a constant trip count with all its loads
 // issued before the LDS stores, so a thread has its 16 or 64 loads in flight
@@ -656,8 +701,8 @@ __global__ void __launch_bounds__(256) lx_sw2_prep_kernel(int n, const T *__rest
 __device__ __forceinline__ int lx_s2_ix(int k) { return k + 4 * (k >> 6); }
 
 // trans = 0: forward L y = x;  trans = 1: backward L^T x = y; one column X,
-// overwritten in place.  Units as for lx_sweep_kernel with 128-row steps and
-// a gap of 2.  Partial sums are mailboxes too: P(s, c) writes its 128 values
+// overwritten in place.  Units of lx_s2_build_units (128-row steps, a gap of
+// 2, parts of C, 2C, 4C, ... steps).  Partial sums are mailboxes too: P(s, c) writes its 128 values
 // with atomic exchanges and S(s) polls them by value, so the path from the
 // newest step a part reads to the solve unit that adds it is two memory
 // round trips (no counter, no release fence), within the two steps of slack
@@ -708,11 +753,11 @@ __global__ void __launch_bounds__(256) LX_WAVES_PER_EU(WPE) lx_sweep2_kernel(int
             }
             // waves 0, 1: y = b minus the partial sums (each slot's load issued
             // before any is polled, added in part order); waves 2, 3: x_{s-2}
-            const int need = lx_sw_nparts(s, C, LX_S2_GAP);
+            const int need = lx_s2_nparts(s, C);
             if (tid < LX_S2) {  // waves 0 and 1
                 T y = tid < hb ? X[r0 + tid] : T(0);
                 const T *pp = part + (size_t)rowbase[s] * LX_S2 + tid;
-                for (int q0 = 0; q0 < need; q0 += 8) {  // the host keeps need <= 8
+                for (int q0 = 0; q0 < need; q0 += 8) {
                     typename lx_sw_bits<T>::U v[8];
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
@@ -776,7 +821,8 @@ __global__ void __launch_bounds__(256) LX_WAVES_PER_EU(WPE) lx_sweep2_kernel(int
             // ---------------- P(s, c) ----------------
             // steps [t_lo, t_hi); wave w: rows 64 (w >> 1) .. of the block against
             // columns 64 (w & 1) .. of each earlier block
-            const int t_hi = s - LX_S2_GAP - (c - 1) * C, t_lo = max(0, t_hi - C);
+            int t_lo, t_hi;
+            lx_s2_part(s, c, C, t_lo, t_hi);
             const int rh = w >> 1, ch = w & 1;
             const int hr = min(64, hb - 64 * rh);
             T acc = T(0);
@@ -838,6 +884,267 @@ __global__ void __launch_bounds__(256) LX_WAVES_PER_EU(WPE) lx_sweep2_kernel(int
             __syncthreads();
             T *pp = part + ((size_t)rowbase[s] + (c - 1)) * LX_S2;
             if (tid < LX_S2) lx_sw_mb_put<T>(pp + tid, tid < hb ? sred[0][tid] + sred[1][tid] : T(0));
+        }
+    }
+}
+
+// Up to 16 right-hand sides (nrhs columns of X, ld ldx; the mailbox and the
+// parts hold 16 columns, ld ldmb = nb * 128 and 128): the same units and step
+// matrices as lx_sweep2_kernel, the products on the matrix cores
+// (v_mfma_f32_16x16x4f32, lane l: A[l%16][l/16], B[l/16][l%16]).  The solve
+// unit holds Z, M1 and M2 as A operands in registers (wave w: rows 32w ..
+// 32w + 31 as two 16-row tiles, 32 k-steps each); Y = b - parts, x_{s-2} and
+// x_{s-1} pass through LDS as B operands ([column][row], row stride 132: the
+// 64 lanes of a read hit 64 banks).  A partial-sum unit's wave (rows 64 rh ..,
+// columns 64 ch .. of each earlier block) runs four 16-row tiles over its
+// block's 64 columns, L from global memory (forward) or from the slab
+// transposed in LDS (backward).
+template <typename T, int TRANS>
+__global__ void __launch_bounds__(256) LX_WAVES_PER_EU(1) lx_sweep2m_kernel(
+    int n, int nrhs, const T *__restrict__ L, int64_t ldl, T *__restrict__ X, int64_t ldx, const T *__restrict__ prep,
+    const int *__restrict__ units, int nunits, const int *__restrict__ rowbase, int C, T *__restrict__ part, int *ctl,
+    T *__restrict__ mb, int64_t ldmb, long long spin_ticks)
+{
+    typedef typename lx_sw_bits<T>::U U;
+    typedef lx_mfma<T> MF;
+    constexpr int KC = 128 / sizeof(T);
+    constexpr int CPI = 64 / KC;
+    constexpr int VS = LX_S2 + 4;  // row stride of the [column][row] B operands
+    __shared__ __attribute__((aligned(16))) T stile[4][64][KC + 1];
+    __shared__ __attribute__((aligned(16))) T sxp[4][16][68];  // P: per wave, x of a 64-row block
+    __shared__ __attribute__((aligned(16))) T sY[16][VS], sX2[16][VS], sX1[16][VS];
+    __shared__ T sred[2][16][LX_S2];
+    __shared__ int s_unit;
+    const int nb = (n + LX_S2 - 1) / LX_S2;
+    int *ticket = ctl, *abort = ctl + 1;
+    const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    for (;;) {
+        if (w == 0) {
+            const int t = lx_sw_uniform(atomicAdd(ticket, lane == 0 ? 1 : 0));
+            s_unit = t < nunits ? units[t] : -1;  // every lane stores the same value
+        }
+        __syncthreads();
+        const int code = lx_sw_uniform(s_unit);
+        __syncthreads();
+        if (code < 0) break;
+        const int s = code >> 16, c = code & 0xffff;
+        const int ib = TRANS ? nb - 1 - s : s;
+        const int r0 = ib * LX_S2, hb = min(LX_S2, n - r0);
+        if (c == 0) {
+            // ---------------- S(s) ----------------
+            const T *pz = prep + (size_t)(TRANS * nb + ib) * 3 * LX_S2_MAT + 32 * w + lr + (size_t)lk * LX_S2;
+            T mz[2][32], m1[2][32], m2[2][32];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int kk = 0; kk < 32; ++kk) {
+                    const size_t off = (size_t)(4 * kk) * LX_S2 + 16 * t;
+                    mz[t][kk] = pz[off];
+                    m2[t][kk] = pz[2 * LX_S2_MAT + off];
+                    m1[t][kk] = pz[LX_S2_MAT + off];
+                }
+            // thread: row r, columns cb + 2i; every slot's load issued before any poll
+            const int r = tid & (LX_S2 - 1), cb = tid >> 7;
+            const int need = lx_s2_nparts(s, C);
+            const T *pp = part + (size_t)rowbase[s] * 16 * LX_S2 + r;
+            const int ip2 = TRANS ? ib + 2 : ib - 2;
+            const bool h2 = s >= 2 && r < min(LX_S2, n - ip2 * LX_S2);
+            const T *p2 = mb + ip2 * LX_S2 + r;
+            {
+                T y[8];
+                U xv[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int cc = cb + 2 * i;
+                    xv[i] = (h2 && cc < nrhs) ? lx_sw_mb_peek<T>(p2 + cc * ldmb) : U(0);
+                    y[i] = (cc < nrhs && r < hb) ? X[(int64_t)cc * ldx + r0 + r] : T(0);
+                }
+                // the parts in groups of four, oldest group first (the newest part,
+                // q = 0, is the one still being computed)
+                for (int q0 = (need - 1) / 4 * 4; q0 >= 0; q0 -= 4) {
+                    U pv[4][8];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const int cc = cb + 2 * i;
+                            pv[q][i] = (q0 + q < need && cc < nrhs)
+                                           ? lx_sw_mb_peek<T>(pp + (size_t)((q0 + q) * 16 + cc) * LX_S2)
+                                           : U(0);
+                        }
+#pragma unroll
+                    for (int q = 3; q >= 0; --q)
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const int cc = cb + 2 * i;
+                            if (q0 + q < need && cc < nrhs)
+                                y[i] -= lx_sw_mb_finish<T>(pp + (size_t)((q0 + q) * 16 + cc) * LX_S2, pv[q][i], abort,
+                                                           spin_ticks);
+                        }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int cc = cb + 2 * i;
+                    sY[cc][r] = y[i];
+                    sX2[cc][r] = (h2 && cc < nrhs) ? lx_sw_mb_finish<T>(p2 + cc * ldmb, xv[i], abort, spin_ticks) : T(0);
+                }
+            }
+            __syncthreads();
+            // U = Z Y - M2 x_{s-2}: tile t of wave w, value i = row 32w + 16t + 4lk + i, column lr
+            typename MF::acc ua[2], ub[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) ua[t][i] = ub[t][i] = T(0);
+#pragma unroll
+            for (int kk = 0; kk < 32; ++kk) {
+                const T by = sY[lr][4 * kk + lk], bx = sX2[lr][4 * kk + lk];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    ua[t] = MF::mac(mz[t][kk], by, ua[t]);
+                    ub[t] = MF::mac(m2[t][kk], bx, ub[t]);
+                }
+            }
+            T u[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[t][i] = ua[t][i] - ub[t][i];
+            if (s >= 1) {  // uniform: the chain, x = U - M1 x_{s-1}
+                const int ip1 = TRANS ? ib + 1 : ib - 1;
+                const bool h1 = r < min(LX_S2, n - ip1 * LX_S2);
+                const T *p1 = mb + ip1 * LX_S2 + r;
+                U e[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int cc = cb + 2 * i;
+                    e[i] = (h1 && cc < nrhs) ? lx_sw_mb_peek<T>(p1 + cc * ldmb) : U(0);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int cc = cb + 2 * i;
+                    sX1[cc][r] = (h1 && cc < nrhs) ? lx_sw_mb_finish<T>(p1 + cc * ldmb, e[i], abort, spin_ticks) : T(0);
+                }
+                __syncthreads();
+                typename MF::acc xa[2][2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) xa[t][h][i] = T(0);
+#pragma unroll
+                for (int kk = 0; kk < 32; ++kk) {
+                    const T bx = sX1[lr][4 * kk + lk];
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) xa[t][kk & 1] = MF::mac(m1[t][kk], bx, xa[t][kk & 1]);
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) u[t][i] -= xa[t][0][i] + xa[t][1][i];
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 32 * w + 16 * t + MF::row(lk, i);
+                    if (row < hb && lr < nrhs) {
+                        X[(int64_t)lr * ldx + r0 + row] = u[t][i];
+                        lx_sw_mb_put<T>(mb + lr * ldmb + r0 + row, u[t][i]);
+                    }
+                }
+        } else {
+            // ---------------- P(s, c) ----------------
+            int t_lo, t_hi;
+            lx_s2_part(s, c, C, t_lo, t_hi);
+            const int rh = w >> 1, ch = w & 1;
+            const int hr = min(64, hb - 64 * rh);
+            typename MF::acc macc[4];
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) macc[tt][i] = T(0);
+            T(*sx)[68] = sxp[w];
+            if (hr > 0) {  // wave-uniform
+                const int rbase = r0 + 64 * rh;
+                for (int t = t_lo; t < t_hi; ++t) {
+                    const int jb = TRANS ? nb - 1 - t : t;
+                    const int c0 = jb * LX_S2 + 64 * ch, hj = min(64, n - c0);
+                    if (hj <= 0) continue;  // uniform (a ragged last block, backward)
+                    // the block of L before the wait: only x depends on step t
+                    T a[64];
+                    if (!TRANS) {
+                        // a[tt*16 + kk] = L[rbase + 16 tt + lr, c0 + 4 kk + lk] (rows past hr clamped)
+#pragma unroll
+                        for (int tt = 0; tt < 4; ++tt) {
+                            const T *Lb = L + (int64_t)(c0 + lk) * ldl + rbase + min(16 * tt + lr, hr - 1);
+#pragma unroll
+                            for (int kk = 0; kk < 16; ++kk) a[tt * 16 + kk] = Lb[(int64_t)(4 * kk) * ldl];
+                        }
+                    } else {
+                        const int kk = lane % KC, cq = lane / KC;
+#pragma unroll
+                        for (int m = 0; m < 64 / KC; ++m)
+#pragma unroll
+                            for (int q = 0; q < 64 / CPI; ++q) {
+                                const int colg = rbase + min(q * CPI + cq, hr - 1);
+                                const int rowg = c0 + min(m * KC + kk, hj - 1);
+                                a[m * (64 / CPI) + q] = L[(int64_t)colg * ldl + rowg];
+                            }
+                    }
+                    // x of the block: lane = row, 16 columns, loads issued before any poll
+                    {
+                        U e[16];
+                        const T *px = mb + c0 + lane;
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) e[j] = (lane < hj && j < nrhs) ? lx_sw_mb_peek<T>(px + j * ldmb) : U(0);
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            sx[j][lane] = (lane < hj && j < nrhs) ? lx_sw_mb_finish<T>(px + j * ldmb, e[j], abort, spin_ticks) : T(0);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (!TRANS) {
+#pragma unroll
+                        for (int kk = 0; kk < 16; ++kk) {
+                            const T bv = sx[lr][4 * kk + lk];
+#pragma unroll
+                            for (int tt = 0; tt < 4; ++tt) macc[tt] = MF::mac(a[tt * 16 + kk], bv, macc[tt]);
+                        }
+                    } else {
+                        const int kk = lane % KC, cq = lane / KC;
+#pragma unroll
+                        for (int m = 0; m < 64 / KC; ++m) {
+#pragma unroll
+                            for (int q = 0; q < 64 / CPI; ++q) stile[w][q * CPI + cq][kk] = a[m * (64 / CPI) + q];
+                            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                            for (int k4 = 0; k4 < KC; k4 += 4) {  // rows past hj: x is zero there
+                                const T bv = sx[lr][m * KC + k4 + lk];
+#pragma unroll
+                                for (int tt = 0; tt < 4; ++tt)
+                                    macc[tt] = MF::mac(stile[w][16 * tt + lr][k4 + lk], bv, macc[tt]);
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            // the two column halves of each row half meet in LDS, in a fixed order
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sred[ch][lr][64 * rh + 16 * tt + MF::row(lk, i)] = macc[tt][i];
+            __syncthreads();
+            T *pp = part + ((size_t)rowbase[s] + (c - 1)) * 16 * LX_S2;
+            const int r = tid & (LX_S2 - 1), cb = tid >> 7;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int cc = cb + 2 * i;
+                if (cc < nrhs)
+                    lx_sw_mb_put<T>(pp + (size_t)cc * LX_S2 + r, r < hb ? sred[0][cc][r] + sred[1][cc][r] : T(0));
+            }
         }
     }
 }
@@ -1016,18 +1323,17 @@ static int sweeps(int n, int nrhs, const T *L, int64_t ldl, T *X, int64_t ldx, i
 // which: 1 forward, 2 backward, 3 both.  Returns after the sweeps finished
 // (each launch is checked for a timed-out wait).
 // ---------------------------------------------------------------------------
-// 128-row sweeps of one FP32 right-hand side (lx_sweep2_kernel) over the
-// step matrices of lx_sw2_prep_kernel.
+// 128-row FP32 sweeps (lx_sweep2_kernel for one right-hand side,
+// lx_sweep2m_kernel for up to 16 a launch) over the step matrices of
+// lx_sw2_prep_kernel.
 namespace {
-// 128-row blocks per partial-sum unit: at least 4, and at most 8 parts per step
-constexpr int lx_s2_chunk(int nb) { return (nb - 1 + 7) / 8 > 4 ? (nb - 1 + 7) / 8 : 4; }
-std::map<int, std::map<int, lx_sw_plan>> lx_s2_plans;  // [device][number of blocks]
-int lx_s2_occ_tab[LX_SW_MAXDEV][2][2];  // [device][waves per SIMD - 1][trans]
+std::map<int, std::map<int, lx_sw_plan>> lx_s2_plans;  // [device][1024 * blocks + newest part's steps]
+int lx_s2_occ_tab[LX_SW_MAXDEV][3][2];  // [device][one column at 1 or 2 waves per SIMD, 16 columns][trans]
 }  // namespace
 
 int lx_sw2_enabled(int dtype, int n, int nrhs)
 {
-    if (dtype != 1 || nrhs != 1) return 0;
+    if (dtype != 1 || nrhs < 1) return 0;
     int min_n = 2048;  // below it the pre-pass costs more than the shorter chain saves
     if (const char *e = getenv("LINOMPSS_SWEEP2_MIN_N")) min_n = atoi(e);
     return min_n > 0 && n >= min_n;
@@ -1044,14 +1350,19 @@ int lx_sw2_prep(int n, const float *L, int64_t ldl, float *prep, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : LINOMPSS_ERR_HIP;
 }
 
-int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const float *prep, hipStream_t s)
+int lx_sw2_sweeps(int n, int nrhs, const float *L, int64_t ldl, float *X, int64_t ldx, int which, const float *prep,
+                  hipStream_t s)
 {
-    if (n <= 0) return 0;
-    const int nb = (n + LX_S2 - 1) / LX_S2, C = lx_s2_chunk(nb);
+    if (n <= 0 || nrhs <= 0) return 0;
+    const int nb = (n + LX_S2 - 1) / LX_S2;
+    const bool multi = nrhs > 1;  // 16 columns a launch on the matrix cores, else one column
+    int C = 2;  // steps of the newest partial sum (then 2C, 4C, ...)
+    if (const char *e = getenv("LINOMPSS_SWEEP2_PART")) C = std::min(1023, std::max(1, atoi(e)));
+    const int W = multi ? 16 : 1;  // columns held by the mailbox and the parts
     const lx_sw_props *d = nullptr;
     lx_sw_plan pl;
     int occ[2];
-    int wpe = 1;  // LINOMPSS_SWEEP2_WPE=2: the build budgeted for two waves per SIMD
+    int wpe = 1;  // LINOMPSS_SWEEP2_WPE=2: the one-column build budgeted for two waves per SIMD
     if (const char *e = getenv("LINOMPSS_SWEEP2_WPE")) wpe = atoi(e) == 2 ? 2 : 1;
     {
         std::lock_guard<std::mutex> g(lx_sw_mu);
@@ -1068,14 +1379,18 @@ int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const
             lx_s2_occ_tab[dev][0][1] = occ_of((const void *)lx_sweep2_kernel<float, 1, 1>);
             lx_s2_occ_tab[dev][1][0] = occ_of((const void *)lx_sweep2_kernel<float, 0, 2>);
             lx_s2_occ_tab[dev][1][1] = occ_of((const void *)lx_sweep2_kernel<float, 1, 2>);
+            lx_s2_occ_tab[dev][2][0] = occ_of((const void *)lx_sweep2m_kernel<float, 0>);
+            lx_s2_occ_tab[dev][2][1] = occ_of((const void *)lx_sweep2m_kernel<float, 1>);
         }
-        occ[0] = lx_s2_occ_tab[dev][wpe - 1][0];
-        occ[1] = lx_s2_occ_tab[dev][wpe - 1][1];
+        const int oi = multi ? 2 : wpe - 1;
+        occ[0] = lx_s2_occ_tab[dev][oi][0];
+        occ[1] = lx_s2_occ_tab[dev][oi][1];
         auto &plans = lx_s2_plans[dev];
-        auto it = plans.find(nb);
+        const int key = nb * 1024 + C;
+        auto it = plans.find(key);
         if (it == plans.end()) {
             std::vector<int> units, rowbase;
-            lx_sw_build_units(nb, C, units, rowbase, LX_S2_GAP);
+            lx_s2_build_units(nb, C, units, rowbase);
             lx_sw_plan np;
             np.nunits = (int)units.size();
             np.nparts = rowbase[nb];
@@ -1088,15 +1403,16 @@ int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const
                 hipMemcpy(np.d_rowbase, rowbase.data(), sizeof(int) * rowbase.size(), hipMemcpyHostToDevice) !=
                     hipSuccess)
                 return LINOMPSS_ERR_HIP;
-            it = plans.emplace(nb, np).first;
+            it = plans.emplace(key, np).first;
         }
         pl = it->second;
     }
     // workspace: ctl (ticket, abort) | part | mailbox
     const size_t ctl_ints = 64;
     const size_t part_off = (ctl_ints * sizeof(int) + 255) / 256 * 256;
-    const size_t mb_off = (part_off + sizeof(float) * (size_t)std::max(1, pl.nparts) * LX_S2 + 255) / 256 * 256;
-    const size_t bytes = mb_off + sizeof(float) * (size_t)nb * LX_S2;
+    const size_t mb_off = (part_off + sizeof(float) * (size_t)std::max(1, pl.nparts) * LX_S2 * W + 255) / 256 * 256;
+    const int64_t ldmb = (int64_t)nb * LX_S2;
+    const size_t bytes = mb_off + sizeof(float) * (size_t)ldmb * W;
     char *ws = (char *)lx_scratch(3, bytes);
     if (!ws) return LINOMPSS_ERR_NOMEM;
     int *ctl = (int *)ws;
@@ -1105,28 +1421,39 @@ int lx_sw2_sweeps(int n, const float *L, int64_t ldl, float *X, int which, const
     if (const char *ev = getenv("LINOMPSS_TIMEOUT_S")) tmo = atof(ev) > 0 ? atof(ev) : tmo;
     const long long ticks = (long long)(tmo * d->wall_khz * 1000.0);
     int aborted = 0;
-    for (int tr = 0; tr < 2; ++tr) {
-        if (!(which & (1 << tr))) continue;
-        const int grid = std::max(1, std::min(pl.nunits, d->cus * occ[tr]));
-        lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
-        // all-ones: empty (the parts and the mailbox are contiguous)
-        lx_fill_i32((int32_t *)part, -1, (int64_t)((mb_off - part_off) / sizeof(float)) + (int64_t)nb * LX_S2, s);
+    for (int c0 = 0; c0 < nrhs; c0 += 16) {
+        const int nc = multi ? std::min(16, nrhs - c0) : 1;
+        float *Xc = X + (int64_t)c0 * ldx;
+        for (int tr = 0; tr < 2; ++tr) {
+            if (!(which & (1 << tr))) continue;
+            const int grid = std::max(1, std::min(pl.nunits, d->cus * occ[tr]));
+            lx_fill_i32(ctl, 0, (int64_t)ctl_ints, s);
+            // all-ones: empty (the parts and the mailbox are contiguous)
+            lx_fill_i32((int32_t *)part, -1, (int64_t)((mb_off - part_off) / sizeof(float)) + ldmb * W, s);
 #define LX_S2_LAUNCH(TR, WPE)                                                                                      \
-    hipLaunchKernelGGL((lx_sweep2_kernel<float, TR, WPE>), dim3(grid), dim3(256), 0, s, n, L, ldl, X, prep,       \
+    hipLaunchKernelGGL((lx_sweep2_kernel<float, TR, WPE>), dim3(grid), dim3(256), 0, s, n, L, ldl, Xc, prep,      \
                        pl.d_units, pl.nunits, pl.d_rowbase, C, part, ctl, mb, ticks)
-        if (tr == 0) {
-            if (wpe == 2) LX_S2_LAUNCH(0, 2);
-            else LX_S2_LAUNCH(0, 1);
-        } else {
-            if (wpe == 2) LX_S2_LAUNCH(1, 2);
-            else LX_S2_LAUNCH(1, 1);
-        }
+#define LX_S2M_LAUNCH(TR)                                                                                          \
+    hipLaunchKernelGGL((lx_sweep2m_kernel<float, TR>), dim3(grid), dim3(256), 0, s, n, nc, L, ldl, Xc, ldx, prep, \
+                       pl.d_units, pl.nunits, pl.d_rowbase, C, part, ctl, mb, ldmb, ticks)
+            if (multi) {
+                if (tr == 0) LX_S2M_LAUNCH(0);
+                else LX_S2M_LAUNCH(1);
+            } else if (tr == 0) {
+                if (wpe == 2) LX_S2_LAUNCH(0, 2);
+                else LX_S2_LAUNCH(0, 1);
+            } else {
+                if (wpe == 2) LX_S2_LAUNCH(1, 2);
+                else LX_S2_LAUNCH(1, 1);
+            }
 #undef LX_S2_LAUNCH
-        if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
-        int ab = 0;
-        LX_HIP_TRY(hipMemcpyAsync(&ab, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        LX_HIP_TRY(hipStreamSynchronize(s));
-        aborted |= ab;
+#undef LX_S2M_LAUNCH
+            if (hipGetLastError() != hipSuccess) return LINOMPSS_ERR_HIP;
+            int ab = 0;
+            LX_HIP_TRY(hipMemcpyAsync(&ab, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            LX_HIP_TRY(hipStreamSynchronize(s));
+            aborted |= ab;
+        }
     }
     return aborted ? LINOMPSS_ERR_SCHED : 0;
 }
@@ -1141,7 +1468,7 @@ int lx_sweeps_dev(int dtype, int n, int nrhs, const void *L, int64_t ldl, void *
         if (!prep) return LINOMPSS_ERR_NOMEM;
         int rc = lx_sw2_prep(n, (const float *)L, ldl, prep, s);
         if (rc) return rc;
-        return lx_sw2_sweeps(n, (const float *)L, ldl, (float *)X, which, prep, s);
+        return lx_sw2_sweeps(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, prep, s);
     }
     return sweeps<float>(n, nrhs, (const float *)L, ldl, (float *)X, ldx, which, s);
 }

@@ -410,42 +410,53 @@ extern "C" int lx_emu_sweep(int dtype, int trans, int n, int nrhs, const void *L
 // lx_sweep2_kernel as one workgroup taking every unit in list order.
 // Returns 0, -3 when a wait gave up, -1 when a partial sum was never written,
 // -2 when the ticket does not add up.
-extern "C" int lx_emu_sweep2(int trans, int n, const float *L, long long ldl, float *X, int C)
+extern "C" int lx_emu_sweep2(int trans, int n, int nrhs, const float *L, long long ldl, float *X, long long ldx,
+                             int C)
 {
-    const int nb = (n + LX_S2 - 1) / LX_S2;
+    const int nb = (n + LX_S2 - 1) / LX_S2, W = nrhs > 1 ? 16 : 1;
     std::vector<float> prep((size_t)6 * LX_S2_MAT * nb);
     gridDim.x = nb;
-    gridDim.y = 2;
+    gridDim.y = 1;
     gridDim.z = 1;
     blockDim.x = 256;
     for (int y = 0; y < 2; ++y)
         for (int x = 0; x < nb; ++x)
             run_wg([&] {
                 blockIdx.x = x;
-                blockIdx.y = y;
+                blockIdx.y = 0;
                 blockIdx.z = 0;
                 if (y) lx_sw2_prep_kernel<float, 1>(n, L, ldl, prep.data());
                 else lx_sw2_prep_kernel<float, 0>(n, L, ldl, prep.data());
             });
     gridDim.x = 1;
-    gridDim.y = 1;
     std::vector<int> units, rowbase;
-    lx_sw_build_units(nb, C, units, rowbase, LX_S2_GAP);
+    lx_s2_build_units(nb, C, units, rowbase);
     std::vector<int> ctl(64, 0);
-    std::vector<unsigned int> part((size_t)std::max(1, rowbase[nb]) * LX_S2, ~0u);  // all-ones: empty
-    std::vector<unsigned int> mbv((size_t)nb * LX_S2, ~0u);
+    std::vector<unsigned int> part((size_t)std::max(1, rowbase[nb]) * LX_S2 * W, ~0u);  // all-ones: empty
+    const int64_t ldmb = (int64_t)nb * LX_S2;
+    std::vector<unsigned int> mbv((size_t)ldmb * W, ~0u);
     const long long ticks = 100000000LL * 20;
     run_wg([&] {
-        if (trans)
-            lx_sweep2_kernel<float, 1, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(), C,
-                                       (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
+        if (nrhs > 1) {
+            if (trans)
+                lx_sweep2m_kernel<float, 1>(n, nrhs, L, ldl, X, ldx, prep.data(), units.data(), (int)units.size(),
+                                            rowbase.data(), C, (float *)part.data(), ctl.data(), (float *)mbv.data(),
+                                            ldmb, ticks);
+            else
+                lx_sweep2m_kernel<float, 0>(n, nrhs, L, ldl, X, ldx, prep.data(), units.data(), (int)units.size(),
+                                            rowbase.data(), C, (float *)part.data(), ctl.data(), (float *)mbv.data(),
+                                            ldmb, ticks);
+        } else if (trans)
+            lx_sweep2_kernel<float, 1, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(),
+                                          C, (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
         else
-            lx_sweep2_kernel<float, 0, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(), C,
-                                       (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
+            lx_sweep2_kernel<float, 0, 1>(n, L, ldl, X, prep.data(), units.data(), (int)units.size(), rowbase.data(),
+                                          C, (float *)part.data(), ctl.data(), (float *)mbv.data(), ticks);
     });
     if (ctl[1] != 0) return -3;
-    for (int q = 0; q < rowbase[nb] * LX_S2; ++q)  // every partial sum written
-        if (part[q] == ~0u) return -1;
+    for (int p = 0; p < rowbase[nb]; ++p)  // every partial sum written, in its first nrhs columns
+        for (int q = 0; q < LX_S2 * std::min(W, nrhs); ++q)
+            if (part[(size_t)p * LX_S2 * W + q] == ~0u) return -1;
     return ctl[0] == (int)units.size() + 1 ? 0 : -2;
 }
 

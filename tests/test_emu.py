@@ -316,34 +316,37 @@ def test_sweep_kernels(emu_sweep, orc, dtype, n, nr):
 def emu_sweep2(emu):
     E = _load_emu()
     E.lx_emu_sweep2.restype = ctypes.c_int
-    E.lx_emu_sweep2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
-                                ctypes.c_int]
+    E.lx_emu_sweep2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
+                                ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int]
 
     def run(trans, L, X, C):
-        rc = E.lx_emu_sweep2(trans, L.shape[0], L.ctypes.data_as(ctypes.c_void_p), L.shape[0],
-                             X.ctypes.data_as(ctypes.c_void_p), C)
+        rc = E.lx_emu_sweep2(trans, L.shape[0], X.shape[1], L.ctypes.data_as(ctypes.c_void_p), L.shape[0],
+                             X.ctypes.data_as(ctypes.c_void_p), X.shape[0], C)
         assert rc == 0
     return run
 
 
-@pytest.mark.parametrize("n,C", [(1, 4), (128, 4), (129, 4), (300, 1), (520, 2), (1000, 3)])
-def test_sweep2_kernels(emu_sweep2, orc, n, C):
+@pytest.mark.parametrize("n,nr,C", [(1, 1, 4), (128, 1, 4), (129, 1, 4), (300, 1, 1), (520, 1, 2), (1000, 1, 3),
+                                    (129, 3, 4), (300, 16, 1), (700, 5, 2)])
+def test_sweep2_kernels(emu_sweep2, orc, n, nr, C):
     """The 128-row FP32 sweeps (step matrices from the pre-pass, then the
-    sweep kernel with C blocks per partial sum) against the oracle's blocked
-    sweeps in FP64 on the same FP32 inputs: a Cholesky factor of a Gram
-    matrix plus 1e-2 I, ragged orders included.  The bound is FP32 rounding
-    of a solve with cond(L) of a few hundred (the 64-row kernel measures the
-    same 1e-7 to 4e-7 on these inputs)."""
+    one-column kernel, or the 16-column kernel on the emulated matrix cores,
+    with C blocks per partial sum) against the oracle's blocked sweeps in
+    FP64 on the same FP32 inputs: a Cholesky factor of a Gram matrix plus
+    1e-2 I, ragged orders included.  The bound is FP32 rounding of a solve
+    with cond(L) of a few hundred (the 64-row kernel measures the same 1e-7 to
+    4e-7 on these inputs)."""
     rng = np.random.default_rng(n)
     G = rng.standard_normal((n, n))
     L = np.asfortranarray(np.linalg.cholesky(G @ G.T / n + 1e-2 * np.eye(n))).astype(np.float32)
-    b = rng.uniform(-1, 1, (n, 1)).astype(np.float32)
+    b = np.asfortranarray(rng.uniform(-1, 1, (n + 3, nr)).astype(np.float32))  # ldx = n + 3
     for trans in (0, 1):
-        X = np.asfortranarray(b.copy())
+        X = b.copy(order="F")
         emu_sweep2(trans, L, X, C)
-        Xo = b.astype(np.float64, order="F")
+        Xo = b[:n].astype(np.float64, order="F")
         orc.ltrsm_lower(trans, np.asfortranarray(L.astype(np.float64)), Xo, 32)
-        assert np.abs(X - Xo).max() <= 2e-6 * np.abs(Xo).max()
+        assert np.abs(X[:n] - Xo).max() <= 2e-6 * np.abs(Xo).max()
+        assert np.array_equal(X[n:], b[n:])  # rows past n untouched
 
 
 def _emu_batch(sizes, mats, rhs, max_sweeps=30, tol=1e-14, stall=0.5):

@@ -134,19 +134,20 @@ def test_sltrsm_multi_rhs(lx, orc, dev, trans, nr):
 
 
 @pytest.mark.parametrize("trans", ["N", "T"])
-@pytest.mark.parametrize("m", [300, 1000, 4100, 8192])
-def test_sltrsm_128_row_steps(lx, orc, dev, monkeypatch, trans, m):
-    """One FP32 right-hand side with 128-row steps (the step matrices' pre-pass
-    and the 128-row sweep kernel; forced down to any order here) against the
-    oracle's blocked sweeps in FP64 on the same FP32 inputs, on a Cholesky
-    factor of a Gram matrix plus 1e-2 I; ragged orders included.  The 64-row
-    kernel on the same inputs sets the scale: the error may not exceed twice
-    its error or the FP32 bound."""
-    print(f"[sltrsm 128 {trans}] m={m}", flush=True)
+@pytest.mark.parametrize("m,nr", [(300, 1), (1000, 1), (4100, 1), (8192, 1), (1000, 5), (4100, 16), (2000, 21)])
+def test_sltrsm_128_row_steps(lx, orc, dev, monkeypatch, trans, m, nr):
+    """FP32 sweeps with 128-row steps (the step matrices' pre-pass, then the
+    one-column kernel or, for several right-hand sides, the 16-column kernel
+    on the matrix cores; forced down to any order here) against the oracle's
+    blocked sweeps in FP64 on the same FP32 inputs, on a Cholesky factor of a
+    Gram matrix plus 1e-2 I; ragged orders and a second 16-column launch
+    (21 columns) included.  The 64-row kernel on the same inputs sets the
+    scale: the error may not exceed twice its error or the FP32 bound."""
+    print(f"[sltrsm 128 {trans}] m={m} nr={nr}", flush=True)
     rng = np.random.default_rng(m)
     G = rng.standard_normal((m, m))
     L = np.asfortranarray(np.linalg.cholesky(G @ G.T / m + 1e-2 * np.eye(m))).astype(np.float32)
-    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, 1))).astype(np.float32)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, nr))).astype(np.float32)
     Bo = B0.astype(np.float64, order="F")
     orc.ltrsm_lower(0 if trans == "N" else 1, np.asfortranarray(L.astype(np.float64)), Bo, 256, 1.0)
     errs = {}

@@ -2,10 +2,14 @@
 allocates nothing (its workspace is a fixed device array of the library).
 
 Checked through the C-ABI against linompss_potrs_dev (the allocating entry
-point with the same kernels): bit-identical results, on two calls in a row of
-which the second is smaller (the array keeps the first call's contents), for
-both element types and for more than 16 right-hand sides (several launches);
-sizes past the array's limit are refused before any device work.
+point) with the same kernels, the 64-row sweeps: bit-identical results, on two
+calls in a row of which the second is smaller (the array keeps the first
+call's contents), for both element types and for more than 16 right-hand
+sides (several launches); sizes past the array's limit are refused before any
+device work.  For FP32 from n = 2048 on, linompss_potrs_dev itself takes the
+128-row sweeps (a step-matrix pre-pass the fixed array has no room for): the
+comparison then forces its 64-row kernels (LINOMPSS_SWEEP2_MIN_N=0), and the
+default path is held to FP32 rounding of the same solve.
 """
 import ctypes
 
@@ -32,12 +36,17 @@ def _pair(lx, dL, B, b):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_noalloc_matches_potrs_two_calls(lx, orc, dev, dtype):
+def test_noalloc_matches_potrs_two_calls(lx, orc, dev, monkeypatch, dtype):
     for n, nr, seed in ((3000, 5, 1), (700, 1, 2)):  # the second call is smaller
         print(f"[noalloc] dtype={np.dtype(dtype).name} n={n} nr={nr}", flush=True)
         dL = _factor(lx, orc, n, dtype, seed)
         B = np.asfortranarray(np.random.default_rng(seed).uniform(-1, 1, (n, nr))).astype(dtype, order="F")
+        if dtype == np.float32:  # the default path (128-row sweeps from n = 2048): FP32 rounding apart
+            X0, X2 = _pair(lx, dL, B, 256)
+            assert np.abs(X0 - X2).max() <= 1e-5 * np.abs(X2).max()
+        monkeypatch.setenv("LINOMPSS_SWEEP2_MIN_N", "0")  # linompss_potrs_dev on the 64-row sweeps
         X1, X2 = _pair(lx, dL, B, 256)
+        monkeypatch.delenv("LINOMPSS_SWEEP2_MIN_N")
         assert np.array_equal(X1, X2)
         if dtype == np.float64:
             A = orc.genmat_spd(n, seed, diag_mode=1)
