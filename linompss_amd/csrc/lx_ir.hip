@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "linompss.h"
+#include "lx_device.hpp"
 #include "lx_internal.hpp"
 
 
@@ -152,6 +153,145 @@ __global__ void __launch_bounds__(256) lx_symv_blk_kernel(const double *__restri
                     const int r = e % 64, c = e / 64;
                     Pcol[(size_t)c * SN + b * 64 + r] =
                         (sred[0][c][r] + sred[1][c][r]) + (sred[2][c][r] + sred[3][c][r]);
+                }
+            }
+        }
+    }
+}
+
+// The same residual pass for up to 16 right-hand sides at once, the block
+// products on the matrix cores (v_mfma_f64_16x16x4f64; lane l supplies
+// A[l%16][l/16] and B[l/16][l%16], value i of the result is row l/16 + 4i of
+// column l%16).  Per 64 x 64 block, wave q forms the 16 x 16 tiles
+//   rows 16q.. of A_IJ x_J       (row part; K = the block's 64 columns)
+//   rows 16q.. of A_IJ^T x_I     (column part; strictly lower on a diagonal block)
+// with A from the block staged in LDS and x from LDS ([b][column][row], row
+// stride 66); the |A| row and column sums of the first pass are summed from
+// the same operand values, each lane over its k = lk mod 4, then over the
+// four lanes of a row.  P and the reduction are laid out as for
+// lx_symv_blk_kernel with NR = 16.
+template <bool ABS>
+__global__ void __launch_bounds__(256) lx_symv_mm_kernel(const double *__restrict__ A, int64_t lda, int n,
+                                                         const double *__restrict__ X, int64_t ldx, int nrhs,
+                                                         double *__restrict__ P)
+{
+    constexpr int NR = 16, NC = NR + (ABS ? 1 : 0), SB = LX_SY_SB, SN = SB * LX_SY_BS, XS = 66;
+    typedef lx_mfma<double> MF;
+    __shared__ double sA[64][65];  // sA[c][r] = A[I*64 + r, J*64 + c]
+    __shared__ double sxI[SB][NR][XS];
+    __shared__ double sxJ[SB][NR][XS];
+    __shared__ double sred[NC][65];
+    const int nb = (n + 63) / 64, ns = (nb + SB - 1) / SB;
+    const int64_t nsb = (int64_t)ns * (ns + 1) / 2;
+    const int tid = threadIdx.x, l = tid & 63, q = tid >> 6, lr = l & 15, lk = l >> 4;
+    for (int64_t sbk = blockIdx.x; sbk < nsb; sbk += gridDim.x) {
+        int SI = (int)((sqrt(8.0 * (double)sbk + 1.0) - 1.0) * 0.5);
+        while ((int64_t)SI * (SI + 1) / 2 > sbk) --SI;
+        while ((int64_t)(SI + 1) * (SI + 2) / 2 <= sbk) ++SI;
+        const int SJ = (int)(sbk - (int64_t)SI * (SI + 1) / 2);
+        __syncthreads();  // the previous super-block is done with the LDS
+        for (int e = tid; e < SB * NR * 64; e += 256) {
+            const int k = e % 64, c = (e / 64) % NR, b = e / (64 * NR);
+            const int ri = SI * SN + b * 64 + k, rj = SJ * SN + b * 64 + k;
+            sxI[b][c][k] = (c < nrhs && ri < n) ? X[(int64_t)c * ldx + ri] : 0.0;
+            sxJ[b][c][k] = (c < nrhs && rj < n) ? X[(int64_t)c * ldx + rj] : 0.0;
+        }
+        typename MF::acc accR[SB], accC[SB];
+        double absR[SB], absC[SB];
+#pragma unroll
+        for (int b = 0; b < SB; ++b) {
+            absR[b] = absC[b] = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) accR[b][i] = accC[b][i] = 0.0;
+        }
+        auto blk_ok = [&](int t) {
+            const int I = SI * SB + t / SB, J = SJ * SB + t % SB;
+            return J <= I && I < nb;
+        };
+        double nxt[16];
+        auto gload = [&](int t) {
+            const int I = SI * SB + t / SB, J = SJ * SB + t % SB;
+            const int r0 = I * 64, c0 = J * 64, hr = min(64, n - r0), hc = min(64, n - c0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int e = tid + 256 * i, r = e & 63, c = e >> 6;
+                nxt[i] = (r < hr && c < hc && (I != J || r >= c)) ? A[(int64_t)(c0 + c) * lda + r0 + r] : 0.0;
+            }
+        };
+        {
+            int t = 0;
+            while (t < SB * SB && !blk_ok(t)) ++t;  // uniform
+            if (t < SB * SB) gload(t);
+        }
+#pragma unroll
+        for (int bi = 0; bi < SB; ++bi) {
+#pragma unroll
+            for (int bj = 0; bj < SB; ++bj) {
+                const int I = SI * SB + bi, J = SJ * SB + bj;
+                if (J > I || I >= nb) continue;  // uniform
+                __syncthreads();  // sA is free; also orders the sx writes above
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int e = tid + 256 * i;
+                    sA[e >> 6][e & 63] = nxt[i];
+                }
+                __syncthreads();
+                {
+                    int t = bi * SB + bj + 1;
+                    while (t < SB * SB && !blk_ok(t)) ++t;  // uniform
+                    if (t < SB * SB) gload(t);
+                }
+                const bool diag = I == J;
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) {
+                    const int k = 4 * kk + lk;
+                    // row part: A[16q + lr][k] x_J[k][lr]
+                    const double ar = sA[k][16 * q + lr];
+                    accR[bi] = MF::mac(ar, sxJ[bj][lr][k], accR[bi]);
+                    // column part: A^T[16q + lr][k] = A[k][16q + lr], strictly lower on the diagonal
+                    const double at = (!diag || k > 16 * q + lr) ? sA[16 * q + lr][k] : 0.0;
+                    accC[bj] = MF::mac(at, sxI[bi][lr][k], accC[bj]);
+                    if (ABS) {  // |A| sums of row / column 16q + lr over the lane's k = lk mod 4
+                        absR[bi] += fabs(ar);
+                        absC[bj] += fabs(at);
+                    }
+                }
+            }
+        }
+        // out to P: the MFMA tiles hold rows 16q + lk + 4i of column lr; the |A|
+        // sums of row 16q + lr meet over the four lanes lr + 16 lk
+        double *Prow = P + ((size_t)SI * ns + SJ) * SN * NC;
+        double *Pcol = P + ((size_t)SJ * ns + SI) * SN * NC;
+        auto lanes4 = [](double v) {
+            v += __shfl_xor(v, 16);
+            return v + __shfl_xor(v, 32);
+        };
+        for (int b = 0; b < SB; ++b) {
+            const double ar = ABS ? lanes4(absR[b]) : 0.0, ac = ABS ? lanes4(absC[b]) : 0.0;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sred[lr][16 * q + MF::row(lk, i)] = accR[b][i];
+            if (ABS && lk == 0) sred[NR][16 * q + lr] = ar;
+            __syncthreads();
+            if (SI == SJ) {  // uniform: both parts land in P[SI][SI]
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sred[lr][16 * q + MF::row(lk, i)] += accC[b][i];
+                if (ABS && lk == 0) sred[NR][16 * q + lr] += ac;
+                __syncthreads();
+            }
+            for (int e = tid; e < NC * 64; e += 256) {
+                const int r = e % 64, c = e / 64;
+                Prow[(size_t)c * SN + b * 64 + r] = sred[c][r];
+            }
+            if (SI != SJ) {
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sred[lr][16 * q + MF::row(lk, i)] = accC[b][i];
+                if (ABS && lk == 0) sred[NR][16 * q + lr] = ac;
+                __syncthreads();
+                for (int e = tid; e < NC * 64; e += 256) {
+                    const int r = e % 64, c = e / 64;
+                    Pcol[(size_t)c * SN + b * 64 + r] = sred[c][r];
                 }
             }
         }
@@ -329,8 +469,8 @@ __global__ void __launch_bounds__(1024) lx_max_kernel(int n, const double *__res
 
 static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
-// r = b - A x, four right-hand sides per pass (one for a single column);
-// rowabs (when non-null) receives the row sums of |A|.
+// r = b - A x for nc <= NR right-hand sides in one pass over A; rowabs (when
+// non-null) receives the row sums of |A|.
 template <int NR>
 static int residual_nr(int n, int nc, const double *A, int64_t lda, const double *X, int64_t ldx, const double *B,
                        int64_t ldb, double *R, double *rowabs, double *P, hipStream_t s)
@@ -339,6 +479,18 @@ static int residual_nr(int n, int nc, const double *A, int64_t lda, const double
     const int64_t nsb = (int64_t)ns * (ns + 1) / 2;
     const int grid = (int)std::min<int64_t>(nsb, 2048);
     const int64_t red = (int64_t)ns * LX_SY_SB * LX_SY_BS * (NR + 1);
+    if constexpr (NR == 16) {  // the matrix-core pass
+        if (rowabs) {
+            hipLaunchKernelGGL((lx_symv_mm_kernel<true>), dim3(grid), dim3(256), 0, s, A, lda, n, X, ldx, nc, P);
+            hipLaunchKernelGGL((lx_symv_reduce_kernel<NR, true>), dim3(grid_for(red)), dim3(256), 0, s, n, nc, P, B,
+                               ldb, R, rowabs);
+        } else {
+            hipLaunchKernelGGL((lx_symv_mm_kernel<false>), dim3(grid), dim3(256), 0, s, A, lda, n, X, ldx, nc, P);
+            hipLaunchKernelGGL((lx_symv_reduce_kernel<NR, false>), dim3(grid_for(red)), dim3(256), 0, s, n, nc, P, B,
+                               ldb, R, nullptr);
+        }
+        return 0;
+    }
     if (rowabs) {
         hipLaunchKernelGGL((lx_symv_blk_kernel<NR, true>), dim3(grid), dim3(256), 0, s, A, lda, n, X, ldx, nc, P);
         hipLaunchKernelGGL((lx_symv_reduce_kernel<NR, true>), dim3(grid_for(red)), dim3(256), 0, s, n, nc, P, B, ldb, R,
@@ -355,7 +507,9 @@ static int residual(int n, int nrhs, const double *A, int64_t lda, const double 
                     int64_t ldb, double *R, double *rowabs, hipStream_t s)
 {
     const int nb = (n + 63) / 64, ns = (nb + LX_SY_SB - 1) / LX_SY_SB;
-    const int nrp = nrhs == 1 ? 1 : 4;
+    // one column on the vector units; 2-4 four at a time; more in passes of 16
+    // on the matrix cores
+    const int nrp = nrhs == 1 ? 1 : nrhs <= 4 ? 4 : 16;
     double *P = (double *)lx_scratch(8, sizeof(double) * (size_t)ns * ns * LX_SY_SB * LX_SY_BS * (nrp + 1));
     if (!P) return LINOMPSS_ERR_NOMEM;
     for (int c0 = 0; c0 < nrhs; c0 += nrp) {
@@ -363,8 +517,10 @@ static int residual(int n, int nrhs, const double *A, int64_t lda, const double 
         double *ra = c0 == 0 ? rowabs : nullptr;
         if (nrp == 1)
             residual_nr<1>(n, nc, A, lda, X + c0 * ldx, ldx, B + c0 * ldb, ldb, R + (int64_t)c0 * n, ra, P, s);
-        else
+        else if (nrp == 4)
             residual_nr<4>(n, nc, A, lda, X + c0 * ldx, ldx, B + c0 * ldb, ldb, R + (int64_t)c0 * n, ra, P, s);
+        else
+            residual_nr<16>(n, nc, A, lda, X + c0 * ldx, ldx, B + c0 * ldb, ldb, R + (int64_t)c0 * n, ra, P, s);
     }
     LX_HIP_TRY(hipGetLastError());
     return 0;
