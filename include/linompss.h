@@ -324,6 +324,10 @@ typedef struct {
     int wg_per_cu;        /* resident workgroups per CU: 1 or 2; 0 = autotuned */
     int order;            /* 0 largest system first, 1 as given; -1 = autotuned */
     int use_graph;        /* 1: runs replay a captured hipGraph; 0: plain launches */
+    int panel;            /* FP32 factor panel width: 64 or 128 for every system; 0 =
+                             per size class (orders 1-64, 65-128, ...): measured on the
+                             batch's own systems when the plan tunes its schedule, else
+                             128 above order 256 and 64 below */
 } linompss_batch_opts;
 
 typedef struct {
@@ -337,7 +341,7 @@ typedef struct {
 typedef struct linompss_batch_plan linompss_batch_plan;
 
 /* n, dA, lda, dB, dX: host arrays of count entries (the pointers are device
- * pointers).  opts NULL = {30, 1e-14, 0.5, 0, -1, 1}.  Returns 0, -i for an
+ * pointers).  opts NULL = {30, 1e-14, 0.5, 0, -1, 1, 0}.  Returns 0, -i for an
  * invalid argument i, or LINOMPSS_ERR_*. */
 int linompss_batch_plan_create(int count, const int *n, const double *const *dA, const int64_t *lda,
                                const double *const *dB, double *const *dX, const linompss_batch_opts *opts,
@@ -346,6 +350,9 @@ int linompss_batch_plan_create(int count, const int *n, const double *const *dA,
 int linompss_batch_run(linompss_batch_plan *plan, void *stream, linompss_batch_status *st);
 int linompss_batch_plan_info(const linompss_batch_plan *plan, int *grid, int *wg_per_cu, int *order,
                              double *tune_ms);
+/* Bit c of *mask: systems of order 64c+1 .. 64c+64 factor in 128-column
+ * panels (else 64).  No counterpart in the reference. */
+int linompss_batch_plan_panels(const linompss_batch_plan *plan, unsigned *mask);
 int linompss_batch_plan_destroy(linompss_batch_plan *plan);
 /* One-shot: plan (opts NULL = one workgroup per CU, largest first, no graph),
  * run, destroy. */

@@ -734,7 +734,7 @@ BATCH_NMAX = 512
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("max_sweeps", _i), ("tol", _d), ("stall_ratio", _d), ("wg_per_cu", _i), ("order", _i),
-                ("use_graph", _i)]
+                ("use_graph", _i), ("panel", _i)]
 
 
 class BatchStatus(ctypes.Structure):
@@ -750,6 +750,7 @@ _sig("linompss_batch_plan_create", _i, _i, _ip, _pp, ctypes.POINTER(_i64), _pp, 
 _sig("linompss_batch_run", _i, _p, _p, _p)
 _sig("linompss_batch_plan_info", _i, _p, _ip, _ip, _ip, ctypes.POINTER(_d))
 _sig("linompss_batch_plan_destroy", _i, _p)
+_sig("linompss_batch_plan_panels", _i, _p, ctypes.POINTER(ctypes.c_uint))
 _sig("linompss_dposv_ir_batched_dev", _i, _i, _ip, _pp, ctypes.POINTER(_i64), _pp, _pp, _p,
      ctypes.POINTER(BatchOpts), _p)
 
@@ -819,7 +820,10 @@ class BatchPlan:
         g, w, o, t = _i(0), _i(0), _i(0), _d(0.0)
         _check("linompss_batch_plan_info", lib.linompss_batch_plan_info(self._h, ctypes.byref(g), ctypes.byref(w),
                                                                         ctypes.byref(o), ctypes.byref(t)), False)
-        return {"grid": g.value, "wg_per_cu": w.value, "order": o.value, "tune_ms": t.value}
+        m = ctypes.c_uint(0)
+        _check("linompss_batch_plan_panels", lib.linompss_batch_plan_panels(self._h, ctypes.byref(m)), False)
+        return {"grid": g.value, "wg_per_cu": w.value, "order": o.value, "tune_ms": t.value,
+                "panel_mask": m.value}
 
     def close(self) -> None:
         if self._h:

@@ -293,6 +293,42 @@ __device__ __forceinline__ void lx_b_resid(const double *__restrict__ A, int64_t
     if (want_anorm) anorm = lx_b_max(am, red);
 }
 
+// FP32/FP64 factor of one system, in place: pw = 64 is lx_unit_potrf over the
+// whole system (a trailing update per 64-column panel); pw = 128 factors
+// 128-column tall panels with it (two 64-column steps each) and updates the
+// trailing matrix once per panel with K = 128, so the trailing part is read
+// and written half as often.  W and info as for lx_unit_potrf (info: the
+// 1-based column of the first non-positive pivot).
+template <typename T>
+__device__ __forceinline__ void lx_b_potrf(T *__restrict__ F, int64_t ld, int n, T *__restrict__ W, int64_t ldw,
+                                           int *info, char *smem, int pw)
+{
+    if (pw <= 64) {
+        lx_unit_potrf<T>(F, ld, n, W, ldw, info, smem);
+        return;
+    }
+    int bad = 0;
+    for (int p = 0; p < n; p += pw) {
+        const int pc = n - p < pw ? n - p : pw;
+        int loc = 0;
+        lx_unit_potrf<T>(F + p * ld + p, ld, n - p, W + p, ldw, &loc, smem, pc);
+        loc = __builtin_amdgcn_readfirstlane(loc);
+        if (loc && !bad) bad = p + loc;
+        const int r_beg = p + pc, nrow = n - r_beg;
+        if (nrow > 0) {  // uniform: A[r_beg:n, r_beg:n] -= P P^T (lower), P = the panel's rows below
+            const T *Pp = F + p * ld + r_beg;
+            T *Ct = F + r_beg * ld + r_beg;
+            constexpr int BS = 128;
+            for (int cc = 0; cc < nrow; cc += BS)
+                for (int rr = cc; rr < nrow; rr += BS)
+                    lx_unit_gemm_lds<T>(Ct + cc * ld + rr, ld, Pp + rr, ld, Pp + cc, ld, nrow - rr < BS ? nrow - rr : BS,
+                                        nrow - cc < BS ? nrow - cc : BS, pc, rr == cc, rr - cc, smem);
+            __syncthreads();  // the updated trailing part before the next panel's loads
+        }
+    }
+    *info = bad;
+}
+
 // Optional phase profile (prof != null): per workgroup, wall-clock ticks of
 // [0] the FP32 copy, [1] the FP32 factor, [2] the first solve, [3] the
 // residuals, [4] the correction solves, [5] the FP64 fallback, [6] systems.
@@ -313,10 +349,11 @@ struct lx_b_clock {
     }
 };
 
-// Factor + sweeps + refinement of one system by the whole workgroup.
+// Factor + sweeps + refinement of one system by the whole workgroup; pw =
+// the FP32 factor's panel width (lx_b_potrf).
 __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot &slot, lx_bstat *stat,
                                                int max_sweeps, double tol, double stall, char *smem,
-                                               lx_b_clock &clk)
+                                               lx_b_clock &clk, int pw)
 {
     const int tid = lx_tid();
     const int n = S.n;
@@ -344,7 +381,7 @@ __device__ __forceinline__ void lx_b_solve_one(const lx_bsys &S, const lx_bslot 
     lx_b_lower_to<float>(S.A, S.lda, n, F, ld);
     clk.lap(0);
     int info = 0;
-    lx_unit_potrf<float>(F, ld, n, Wf, slot.ldw, &info, smem);
+    lx_b_potrf<float>(F, ld, n, Wf, slot.ldw, &info, smem, pw);
     info = __builtin_amdgcn_readfirstlane(info);
     clk.lap(1);
     bool fb = info != 0;
@@ -426,12 +463,13 @@ __host__ __device__ inline int64_t lx_b_slot_bytes(int nmax)
 // Persistent launch: workgroups take systems order[0], order[1], ... from
 // *ticket (zero at launch) until all count are done.  MINB = resident
 // workgroups per CU the registers are budgeted for (1: no spills, 2: twice
-// the latency hiding); the autotuner picks.
+// the latency hiding); the autotuner picks.  Bit c of pw_mask: systems of
+// order 64c+1 .. 64c+64 take 128-column factor panels (else 64).
 template <int MINB>
 __global__ void __launch_bounds__(LX_WG_THREADS, MINB)
 lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ order, int count, char *ws,
                    int64_t slot_bytes, int nmax, unsigned *ticket, lx_bstat *stat, int max_sweeps, double tol,
-                   double stall, long long *prof)
+                   double stall, long long *prof, unsigned pw_mask)
 {
     __shared__ __attribute__((aligned(16))) char smem[LX_BATCH_SMEM];
     __shared__ int s_t;
@@ -448,7 +486,8 @@ lx_batch_ir_kernel(const lx_bsys *__restrict__ sys, const int *__restrict__ orde
         if (t >= count) break;
         const int k = order[t];
         const lx_bsys S = sys[k];
-        lx_b_solve_one(S, slot, stat + k, max_sweeps, tol, stall, smem, clk);
+        const int pw = (S.n > 0 && ((pw_mask >> ((S.n - 1) / 64)) & 1u)) ? 128 : 64;
+        lx_b_solve_one(S, slot, stat + k, max_sweeps, tol, stall, smem, clk, pw);
         if (prof && lx_tid() == 0) s_prof[6] += 1;
     }
     if (prof && lx_tid() == 0)
@@ -475,6 +514,7 @@ static_assert(sizeof(lx_bstat) == sizeof(linompss_batch_status), "status layout"
 struct linompss_batch_plan {
     int count = 0, nmax = 0, cus = 0;
     int wg_per_cu = 1, order = 0, grid = 0, use_graph = 1;
+    unsigned pw_mask = 0;  // bit c: orders 64c+1 .. 64c+64 factor in 128-column panels
     int max_sweeps = 30;
     double tol = 1e-14, stall = 0.5, tune_ms = 0.0;
     int64_t slot_bytes = 0;
@@ -490,25 +530,39 @@ struct linompss_batch_plan {
 
 namespace {
 
-// Autotuned (workgroups per CU, order) by batch signature: device, count,
-// largest order and the total factor work.
+// Autotuned (workgroups per CU, order, panel mask) by batch signature:
+// device, count, largest order and the total factor work.
+struct lx_bt_choice {
+    int wg, order;
+    unsigned pw_mask;
+    bool pw_tuned;  // pw_mask was measured (not a fixed width of the plan that stored it)
+};
 std::mutex lx_bt_mu;
-std::map<std::tuple<int, int, int, long long>, std::pair<int, int>> lx_bt_cache;
+std::map<std::tuple<int, int, int, long long>, lx_bt_choice> lx_bt_cache;
+
+// The panel rule without measurements: 128-column panels above order 256.
+unsigned lx_b_default_mask() { return 0xF0u; }
+
+// One persistent launch over cnt systems taken from d_ord.
+int lx_b_launch_ex(linompss_batch_plan *p, int wg, const int *d_ord, int cnt, unsigned mask, hipStream_t s)
+{
+    const int grid = std::min(cnt, std::max(1, wg * p->cus));
+    hipLaunchKernelGGL(lx_batch_reset_kernel, dim3(1), dim3(1), 0, s, p->d_ticket);
+    if (wg == 1)
+        hipLaunchKernelGGL(lx_batch_ir_kernel<1>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, d_ord, cnt,
+                           p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol, p->stall,
+                           p->d_prof, mask);
+    else
+        hipLaunchKernelGGL(lx_batch_ir_kernel<2>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, d_ord, cnt,
+                           p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol, p->stall,
+                           p->d_prof, mask);
+    LX_HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 int lx_b_launch(linompss_batch_plan *p, int wg, int ord, hipStream_t s)
 {
-    const int grid = std::min(p->count, std::max(1, wg * p->cus));
-    hipLaunchKernelGGL(lx_batch_reset_kernel, dim3(1), dim3(1), 0, s, p->d_ticket);
-    if (wg == 1)
-        hipLaunchKernelGGL(lx_batch_ir_kernel<1>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
-                           p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
-                           p->stall, p->d_prof);
-    else
-        hipLaunchKernelGGL(lx_batch_ir_kernel<2>, dim3(grid), dim3(LX_WG_THREADS), 0, s, p->d_sys, p->d_order[ord],
-                           p->count, p->d_ws, p->slot_bytes, p->nmax, p->d_ticket, p->d_stat, p->max_sweeps, p->tol,
-                           p->stall, p->d_prof);
-    LX_HIP_TRY(hipGetLastError());
-    return 0;
+    return lx_b_launch_ex(p, wg, p->d_order[ord], p->count, p->pw_mask, s);
 }
 
 void lx_b_release(linompss_batch_plan *p)
@@ -523,6 +577,57 @@ void lx_b_release(linompss_batch_plan *p)
     (void)hipFree(p->d_ws);
     (void)hipFree(p->d_prof);
     delete p;
+}
+
+// The panel width of each size class, measured on the batch's own systems:
+// for every class with at least 16 systems, the class alone (largest first)
+// is run with 64- and with 128-column panels under the chosen workgroups per
+// CU, and a width at least 2 % faster sets the class's bit (the runs write
+// X).  Classes with fewer systems, or closer timings, keep the default rule.
+int lx_b_tune_panels(linompss_batch_plan *p, const int *n, int wg, unsigned *mask_out, const int *ord_desc)
+{
+    hipStream_t s = nullptr;
+    LX_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int *d_sub = nullptr;
+    int rc = 0;
+    unsigned mask = lx_b_default_mask();
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMalloc(&d_sub, sizeof(int) * std::max(1, p->count)) != hipSuccess)
+        rc = LINOMPSS_ERR_HIP;
+    for (int c = 0; c < LX_BATCH_NMAX / 64 && rc == 0; ++c) {
+        std::vector<int> sub;
+        for (int i = 0; i < p->count; ++i) {
+            const int k = ord_desc[i];
+            if (n[k] > 64 * c && n[k] <= 64 * c + 64) sub.push_back(k);
+        }
+        if (sub.size() < 16) continue;
+        if (hipMemcpy(d_sub, sub.data(), sizeof(int) * sub.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = LINOMPSS_ERR_HIP;
+            break;
+        }
+        float t[2] = {0.f, 0.f};
+        for (int v = 0; v < 2 && rc == 0; ++v) {
+            const unsigned m = v ? (mask | (1u << c)) : (mask & ~(1u << c));
+            if ((rc = lx_b_launch_ex(p, wg, d_sub, (int)sub.size(), m, s)) != 0) break;  // warm-up
+            if (hipEventRecord(e0, s) != hipSuccess || (rc = lx_b_launch_ex(p, wg, d_sub, (int)sub.size(), m, s)) != 0 ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t[v], e0, e1) != hipSuccess) {
+                if (rc == 0) rc = LINOMPSS_ERR_HIP;
+                break;
+            }
+            p->tune_ms += t[v];
+        }
+        // a width wins by 2 % or more; closer than that the default rule stays
+        if (rc == 0 && t[1] < 0.98f * t[0]) mask |= 1u << c;
+        if (rc == 0 && t[0] < 0.98f * t[1]) mask &= ~(1u << c);
+    }
+    if (d_sub) (void)hipFree(d_sub);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc == 0) *mask_out = mask;
+    return rc;
 }
 
 // Times each (workgroups per CU, order) candidate on the batch itself (the
@@ -573,9 +678,10 @@ extern "C" int linompss_batch_plan_create(int count, const int *n, const double 
     if (!plan) return -8;
     *plan = nullptr;
     if (count < 0) return -1;
-    linompss_batch_opts o = {30, 1e-14, 0.5, 0, -1, 1};
+    linompss_batch_opts o = {30, 1e-14, 0.5, 0, -1, 1, 0};
     if (opts) o = *opts;
     if (o.wg_per_cu < 0 || o.wg_per_cu > 2 || o.order < -1 || o.order > 1 || o.max_sweeps < 0) return -7;
+    if (o.panel != 0 && o.panel != 64 && o.panel != 128) return -7;
     if (count > 0 && (!n || !dA || !lda || !dB || !dX)) return -2;
     int nmax = 0;
     double work = 0.0;
@@ -601,6 +707,7 @@ extern "C" int linompss_batch_plan_create(int count, const int *n, const double 
     p->use_graph = o.use_graph ? 1 : 0;
     p->wg_per_cu = o.wg_per_cu ? o.wg_per_cu : 1;
     p->order = o.order < 0 ? 0 : o.order;
+    p->pw_mask = o.panel == 64 ? 0u : o.panel == 128 ? 0xFFu : lx_b_default_mask();
     p->slot_bytes = lx_b_slot_bytes(p->nmax);
     auto fail = [&](int rc) {
         lx_b_release(p);
@@ -632,30 +739,50 @@ extern "C" int linompss_batch_plan_create(int count, const int *n, const double 
         hipMemcpy(p->d_order[0], ord_desc.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->d_order[1], ord_given.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess)
         return fail(LINOMPSS_ERR_HIP);
-    // schedule: given, cached for this batch signature, or measured now
+    // schedule: given, cached for this batch signature, or measured now (the
+    // panel mask too, when it is left per size class)
     const bool tune_wg = o.wg_per_cu == 0, tune_order = o.order < 0;
     if (tune_wg || tune_order) {
+        const bool tune_pw = o.panel == 0;
         // total work in buckets of about 1 % (log scale)
         const long long sig = std::llround(std::log2(work + 1.0) * 64.0);
         const auto key = std::make_tuple(dev, count, nmax, sig);
-        bool hit = false;
+        bool hit = false, pw_hit = false;
         {
             std::lock_guard<std::mutex> g(lx_bt_mu);
             auto it = lx_bt_cache.find(key);
             if (it != lx_bt_cache.end()) {
-                if (tune_wg) p->wg_per_cu = it->second.first;
-                if (tune_order) p->order = it->second.second;
+                if (tune_wg) p->wg_per_cu = it->second.wg;
+                if (tune_order) p->order = it->second.order;
+                if (tune_pw && it->second.pw_tuned) {
+                    p->pw_mask = it->second.pw_mask;
+                    pw_hit = true;
+                }
                 hit = true;
             }
         }
-        if (!hit) {
-            int wg = p->wg_per_cu, od = p->order;
-            const int rc = lx_b_tune(p, tune_wg, tune_order, &wg, &od);
-            if (rc) return fail(rc);
-            p->wg_per_cu = wg;
-            p->order = od;
+        if (!hit || (tune_pw && !pw_hit)) {
+            int wg = p->wg_per_cu, od = p->order, rc = 0;
+            if (!hit) {
+                if ((rc = lx_b_tune(p, tune_wg, tune_order, &wg, &od)) != 0) return fail(rc);
+                p->wg_per_cu = wg;
+                p->order = od;
+            }
+            bool pw_tuned = false;
+            if (tune_pw) {
+                unsigned m = p->pw_mask;
+                if ((rc = lx_b_tune_panels(p, n, p->wg_per_cu, &m, ord_desc.data())) != 0) return fail(rc);
+                p->pw_mask = m;
+                pw_tuned = true;
+            }
             std::lock_guard<std::mutex> g(lx_bt_mu);
-            lx_bt_cache[key] = {wg, od};
+            auto it = lx_bt_cache.find(key);
+            if (it == lx_bt_cache.end())
+                lx_bt_cache[key] = {p->wg_per_cu, p->order, p->pw_mask, pw_tuned};
+            else if (pw_tuned) {  // keep the stored schedule, add the measured panels
+                it->second.pw_mask = p->pw_mask;
+                it->second.pw_tuned = true;
+            }
         }
     }
     p->grid = std::min(count, std::max(1, p->wg_per_cu * p->cus));
@@ -707,6 +834,14 @@ extern "C" int linompss_batch_run(linompss_batch_plan *p, void *stream, linompss
     return 0;
 }
 
+extern "C" int linompss_batch_plan_panels(const linompss_batch_plan *p, unsigned *mask)
+{
+    if (!p) return -1;
+    if (!mask) return -2;
+    *mask = p->pw_mask;
+    return 0;
+}
+
 extern "C" int linompss_batch_plan_info(const linompss_batch_plan *p, int *grid, int *wg_per_cu, int *order,
                                         double *tune_ms)
 {
@@ -731,7 +866,7 @@ extern "C" int linompss_dposv_ir_batched_dev(int count, const int *n, const doub
                                              const double *const *dB, double *const *dX, void *stream,
                                              const linompss_batch_opts *opts, linompss_batch_status *st)
 {
-    linompss_batch_opts o = {30, 1e-14, 0.5, 1, 0, 0};  // one launch, no tuning, no graph
+    linompss_batch_opts o = {30, 1e-14, 0.5, 1, 0, 0, 0};  // one launch, no tuning, no graph
     if (opts) o = *opts;
     linompss_batch_plan *p = nullptr;
     int rc = linompss_batch_plan_create(count, n, dA, lda, dB, dX, &o, &p);

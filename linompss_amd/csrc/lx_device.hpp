@@ -1084,17 +1084,21 @@ __device__ __forceinline__ void lx_trsm_update_lds(typename lx_mfma<T>::acc (&ac
 // of the first non-positive pivot, as LAPACK's info; the factorisation keeps
 // going past it (as LAPACK, results there are not meaningful).
 // LDS: max(2 * 64 * 65 * sizeof(T) + 16, lx_lgemm_smem<T>::bytes) bytes.
+// Nc < M factors only the leading Nc columns of the M-row matrix (a tall
+// panel): the same steps, with the trailing updates restricted to the
+// columns left of Nc (the caller updates the rest with the whole panel).
 template <typename T>
 __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, int M, T *__restrict__ W,
-                                              long long ldw, int *info_local, char *smem)
+                                              long long ldw, int *info_local, char *smem, int Nc = -1)
 {
     constexpr int PB = 64, LD = 65;
     T(*sD)[LD] = reinterpret_cast<T(*)[LD]>(smem);
     T(*sZ)[LD] = reinterpret_cast<T(*)[LD]>(smem + sizeof(T) * PB * LD);
     int *s_bad = reinterpret_cast<int *>(smem + 2 * sizeof(T) * PB * LD);
     const int tid = lx_tid(), lane = tid & 63, w = lx_wave(), lr = lane & 15, lk = lane >> 4;
+    const int NC = (Nc < 0 || Nc > M) ? M : Nc;
     int first_bad = 0;
-    for (int p = 0; p < M; p += PB) {
+    for (int p = 0; p < NC; p += PB) {
         const int pb = M - p < PB ? M - p : PB;
         // load D (full square, identity pad past pb); clear Z.  A constant trip
         // count, unrolled, so several loads are in flight per thread
@@ -1169,8 +1173,20 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
         __syncthreads();
         // 4) trailing update of A[r_beg:M, r_beg:M] (lower) with the panel, K = pb,
         // in staged 128 x 128 blocks (D and Z are no longer needed: the blocks
-        // take the whole LDS arena)
-        if (nrow > 0 && nrow <= 64 && pb == 64) {  // uniform: the last panel of a 128-row tile
+        // take the whole LDS arena); a tall panel only updates its own columns
+        if (NC < M) {  // uniform
+            const int ncol = NC - r_beg;
+            if (ncol > 0) {
+                const T *Pp = A + (long long)p * lda + r_beg;
+                T *Ct = A + (long long)r_beg * lda + r_beg;
+                constexpr int BS = 128;
+                for (int cc = 0; cc < ncol; cc += BS)
+                    for (int rr = cc; rr < nrow; rr += BS)
+                        lx_unit_gemm_lds<T>(Ct + (long long)cc * lda + rr, lda, Pp + rr, lda, Pp + cc, lda,
+                                            nrow - rr < BS ? nrow - rr : BS, ncol - cc < BS ? ncol - cc : BS, pb,
+                                            rr == cc, rr - cc, smem);
+            }
+        } else if (nrow > 0 && nrow <= 64 && pb == 64) {  // uniform: the last panel of a 128-row tile
             // one 64 x 64 block through the shared LDS slabs of the TRSM update,
             // then C -= acc on and below the diagonal
             const T *Pp = A + (long long)p * lda + r_beg;

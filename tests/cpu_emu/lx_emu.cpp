@@ -460,6 +460,11 @@ extern "C" int lx_emu_sweep2(int trans, int n, int nrhs, const float *L, long lo
     return ctl[0] == (int)units.size() + 1 ? 0 : -2;
 }
 
+// Panel mask of the next lx_emu_batch runs (bit c: orders 64c+1 .. 64c+64
+// factor in 128-column panels).
+static unsigned g_batch_pw_mask = 0;
+extern "C" void lx_emu_set_batch_panels(unsigned mask) { g_batch_pw_mask = mask; }
+
 // Batched refinement solves through lx_batch_ir_kernel, one workgroup taking
 // every system from the queue.  stat: count x {sweeps, fallback, info, pad, berr}.
 extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, const long long *lda,
@@ -479,7 +484,7 @@ extern "C" int lx_emu_batch(int count, const int *n, const double *const *A, con
     blockDim.x = 256;
     run_wg([&] {
         lx_batch_ir_kernel<2>(sys.data(), order, count, (char *)ws.data(), sb, nmax, &ticket, (lx_bstat *)stat,
-                           max_sweeps, tol, stall, nullptr);
+                           max_sweeps, tol, stall, nullptr, g_batch_pw_mask);
     });
     return ticket == (unsigned)count + 1 ? 0 : -1;
 }

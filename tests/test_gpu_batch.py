@@ -45,9 +45,9 @@ def _check_solutions(lx, orc, batch, sizes, mats, rhs, st, n_oracle=6):
         assert err <= 2 * cond * 1e-14, (i, n, err, cond)
 
 
-@pytest.mark.parametrize("use_graph", [1, 0])
-def test_batch_uneven(lx, orc, dev, use_graph):
-    print(f"[batch] 96 systems in [1, 512], graph={use_graph}", flush=True)
+@pytest.mark.parametrize("use_graph,panel", [(1, 0), (0, 0), (1, 128), (1, 64)])
+def test_batch_uneven(lx, orc, dev, use_graph, panel):
+    print(f"[batch] 96 systems in [1, 512], graph={use_graph}, panel={panel}", flush=True)
     sizes, mats, rhs = lx.gen_batch_spd(92, 1, 512, seed=7)
     # edges: the maximum order, one row, one and two 64-row blocks
     for n in (512, 1, 64, 65):
@@ -57,7 +57,10 @@ def test_batch_uneven(lx, orc, dev, use_graph):
         rhs.insert(0, r1[0])
     batch = lx.DeviceBatch(sizes)
     batch.upload(mats, rhs)
-    plan = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 1, 0, use_graph))
+    plan = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 1, 0, use_graph, panel))
+    # a fixed width applies to every size class; 0 without tuning keeps the rule
+    # (128-column panels above order 256)
+    assert plan.info()["panel_mask"] == {0: 0xF0, 64: 0, 128: 0xFF}[panel]
     st = plan.run()
     _check_solutions(lx, orc, batch, sizes, mats, rhs, st)
     plan.close()
@@ -103,9 +106,37 @@ def test_batch_autotune(lx, dev):
     # a second plan of the same batch takes the cached choice without measuring
     plan2 = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 0, -1, 1))
     i2 = plan2.info()
-    assert (i2["wg_per_cu"], i2["order"], i2["tune_ms"]) == (info["wg_per_cu"], info["order"], 0.0)
+    assert (i2["wg_per_cu"], i2["order"], i2["tune_ms"], i2["panel_mask"]) == \
+        (info["wg_per_cu"], info["order"], 0.0, info["panel_mask"])
     plan.close()
     plan2.close()
+    batch.free()
+
+
+def test_batch_autotune_panels(lx, dev):
+    """Panel widths per size class: a plan with a fixed width leaves no panel
+    choice in the cache, so a later per-class plan of the same batch measures
+    the classes itself (tune_ms > 0) and then shares its mask."""
+    print("[batch] autotuned panel widths", flush=True)
+    sizes, mats, rhs = lx.gen_batch_spd(240, 200, 400, seed=13)
+    batch = lx.DeviceBatch(sizes)
+    batch.upload(mats, rhs)
+    p64 = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 0, -1, 1, 64))
+    assert p64.info()["panel_mask"] == 0
+    p0 = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 0, -1, 1, 0))
+    i0 = p0.info()
+    assert i0["tune_ms"] > 0
+    # classes with no systems (orders 1-192 and 449-512) keep the rule
+    assert i0["panel_mask"] & 0x07 == 0 and i0["panel_mask"] & 0x80 == 0x80
+    st = p0.run()
+    X = batch.solutions()
+    assert (st["fallback"] == 0).all()
+    for i in range(len(sizes)):
+        assert _berr(mats[i], X[i], rhs[i]) <= 1e-14
+    p1 = lx.BatchPlan(batch, lx.BatchOpts(30, 1e-14, 0.5, 0, -1, 1, 0))
+    assert p1.info()["panel_mask"] == i0["panel_mask"] and p1.info()["tune_ms"] == 0.0
+    for p in (p64, p0, p1):
+        p.close()
     batch.free()
 
 
@@ -151,7 +182,14 @@ def test_batch_arguments(lx, dev):
     assert lx.lib.linompss_batch_plan_create(-1, n, p, ld, p, p, None, ctypes.byref(h)) == -1
     assert lx.lib.linompss_batch_plan_create(0, None, None, None, None, None, None, ctypes.byref(h)) == 0
     assert lx.lib.linompss_batch_run(h, None, None) == 0
+    m = ctypes.c_uint(7)
+    assert lx.lib.linompss_batch_plan_panels(h, ctypes.byref(m)) == 0 and m.value == 0xF0
+    assert lx.lib.linompss_batch_plan_panels(h, None) == -2
     assert lx.lib.linompss_batch_plan_destroy(h) == 0
+    n[1] = 8
+    ld[1] = 8
+    bad = lx.BatchOpts(30, 1e-14, 0.5, 1, 0, 0, 96)  # a panel width other than 0, 64, 128
+    assert lx.lib.linompss_batch_plan_create(2, n, p, ld, p, p, ctypes.byref(bad), ctypes.byref(h)) == -7
 
 
 def test_batch_workload_4096(lx, orc, dev):
