@@ -37,8 +37,10 @@
 //           Z_J = L[J, J]^-1 into Zb + 4096 J (64 x 64, ld 64; identity past
 //           the block's order);
 //   mode 1: Pbar = the adjoint of D = chol(P) applied to Dbar (Abar[J, J],
-//           lower), written back in place (lower representation); Sb (64 x 64,
-//           ld 64) = Pbar + Pbar^T.  Zb holds Z = D^-1 from mode 0.
+//           lower), written back in place (lower representation); the
+//           leading jb x jb part of Sb (ld lds, which may be jb) =
+//           sscale (Pbar + Pbar^T), nothing past it.  Zb holds Z = D^-1 from
+//           mode 0.
 // Mode 1 is Murray's closed form for one block (eq. 10 of the paper in the
 // file header), three 64 x 64 x 64 products on the matrix cores:
 //     M = D^T Dbar,  Phi = tril(M) with its diagonal halved,
@@ -65,7 +67,7 @@ __device__ __forceinline__ void lx_adj_mm64(lx_d4 (&acc)[4], const double (*A)[6
 
 __global__ void __launch_bounds__(LX_WG_THREADS)
 lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl, double *__restrict__ Ab,
-                   int64_t lda, double *__restrict__ Zb, double *__restrict__ Sb)
+                   int64_t lda, double *__restrict__ Zb, double *__restrict__ Sb, int64_t lds, double sscale)
 {
     __shared__ double sL[64][65], sX[64][65];
     if (mode == 0) {
@@ -123,7 +125,7 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
         const int r = e & 63, c = e >> 6;
         const double g = sX[r][c] + sX[c][r];
         if (r < jb && c < jb && r >= c) Ab[(int64_t)c * lda + r] = r > c ? g : sX[r][r];
-        Sb[c * 64 + r] = (r < jb && c < jb) ? g : 0.0;
+        if (r < jb && c < jb) Sb[c * lds + r] = sscale * g;
     }
 }
 
@@ -149,6 +151,18 @@ __global__ void __launch_bounds__(LX_WG_THREADS) lx_adj_logdet_sum_kernel(int n,
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = 2.0 * red[0];
+}
+
+// The group's Qbar blocks into Abar: column c of Q (rows x cols, ld ldq; the
+// group's first row and column at Q[0]) holds Qbar of block c / 64 in the
+// rows below that block, which are Abar's final values there.
+__global__ void lx_adj_qcopy_kernel(int rows, int cols, const double *__restrict__ Q, int64_t ldq,
+                                    double *__restrict__ Ab, int64_t lda)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    if (r >= (c / 64) * 64 + 64) Ab[(int64_t)c * lda + r] = Q[(int64_t)c * ldq + r];
 }
 
 // Y (ld) := X, a rows x cols block copy.
@@ -204,15 +218,20 @@ __global__ void lx_adj_zero_kernel(double *__restrict__ X, int64_t n)
 // group each block runs the recurrences with its updates restricted to the
 // group's columns left of it; the updates of the columns left of the group
 // are gathered and applied once per group as two rank-256 products.  With
-// Q_G the group's Qbar blocks side by side (block J's column block holds
-// Qbar_J in the rows below block J, zeros above), rows from the group's first
-// row g0 to n, and L_G = L[group rows, 0:g0]:
-//     Abar[g0:n, 0:g0]     -= Q_G L_G                (the Bbar terms)
-//     Abar[group, 0:g0]    -= Q_G^T L[g0:n, 0:g0]    (the Qbar^T B terms)
-// while each block's (Pbar + Pbar^T) R term spans all its left columns at once.
-// The same sums as the block-by-block sweep in another order; the rank-64
+// Q_G the group's blocks side by side (block J's column block holds S_J / 2
+// in block J's own rows, S_J = Pbar_J + Pbar_J^T, and Qbar_J in the rows
+// below; zeros above), rows from the group's first row g0 to n, and
+// L_G = L[group rows, 0:g0]:
+//     Abar[g0:n, 0:g0]     -= Q_G L_G                (the Bbar terms, and S R / 2)
+//     Abar[group, 0:g0]    -= Q_G^T L[g0:n, 0:g0]    (the Qbar^T B terms, and S^T R / 2)
+// S is symmetric, so the two halves add up to the S R term of the block's
+// left columns; inside the group the same column block of Q_G, from block
+// J's first row, gives the in-group terms of both kinds in two products.  The
+// same sums as the block-by-block sweep in another order: the rank-64
 // updates of the whole left part, each reading and writing it, become two
-// rank-256 ones.
+// rank-256 ones, and a block's chain of launches carries no S R product.
+// Abar_C = Qbar_J is final once formed, so the group's blocks are copied into
+// Abar once, at the group's end.
 int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda, hipStream_t s)
 {
     const int nb = (n + 63) / 64;
@@ -222,9 +241,10 @@ int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda
     double *ws = (double *)lx_scratch(
         9, sizeof(double) * ((size_t)nb * 4096 + 64 * 64 + qg_elems + part_elems));
     if (!ws) return LINOMPSS_ERR_NOMEM;
-    double *Zall = ws, *Sb = Zall + (size_t)nb * 4096, *QG = Sb + 64 * 64, *part = QG + qg_elems;
+    double *Zall = ws, *QG = Zall + (size_t)nb * 4096 + 64 * 64, *part = QG + qg_elems;
     // the inverses of the diagonal blocks do not depend on the sweep: all at once
-    hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(nb), dim3(LX_WG_THREADS), 0, s, 0, n, L, ldl, Ab, lda, Zall, Sb);
+    hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(nb), dim3(LX_WG_THREADS), 0, s, 0, n, L, ldl, Ab, lda, Zall,
+                       nullptr, (int64_t)64, 0.0);
     int rc = 0;
     for (int G = (nb - 1) / GB; G >= 0 && rc == 0; --G) {
         const int gJ0 = G * GB, gJ1 = std::min(nb, gJ0 + GB) - 1;  // blocks of the group
@@ -237,43 +257,42 @@ int lx_chol_adjoint(int n, const double *L, int64_t ldl, double *Ab, int64_t lda
             const int pin = j0 - g0;  // the group's columns left of block J
             const double *D = L + (int64_t)j0 * ldl + j0;
             double *Dbar = Ab + (int64_t)j0 * lda + j0;
-            // Qbar_J lives in Q_G: column block J - gJ0, rows below block J
-            double *Qb = QG + (int64_t)(j0 - g0) * ldqg + (j0 + jb - g0);
+            // block J's column block of Q_G from its first row: S_J / 2, then Qbar_J
+            double *Qs = QG + (int64_t)(j0 - g0) * ldqg + (j0 - g0);
+            double *Qb = Qs + jb;
             const double *Zb = Zall + (size_t)J * 4096;
             if (m > 0) {
-                double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
+                const double *Cbar = Ab + (int64_t)j0 * lda + j0 + jb;
                 const double *C = L + (int64_t)j0 * ldl + j0 + jb;
-                // Qbar = Cbar D^-1; Abar_C = Qbar
+                // Qbar = Cbar D^-1 (Abar_C = Qbar, copied at the group's end)
                 if ((rc = lx_adj_gemm(0, 0, m, jb, jb, 1.0, Cbar, lda, Zb, 64, 0.0, Qb, ldqg, 0, part, part_elems, s)))
                     break;
-                const int64_t tot = (int64_t)m * jb;
-                hipLaunchKernelGGL(lx_adj_copy_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, jb, Qb,
-                                   ldqg, Cbar, lda);
                 // Dbar -= tril(Qbar^T C)
                 if ((rc = lx_adj_gemm(1, 0, jb, jb, m, -1.0, Qb, ldqg, C, ldl, 1.0, Dbar, lda, 1, part, part_elems,
                                       s)))
                     break;
             }
-            // Pbar (in place of Dbar) and S = Pbar + Pbar^T
+            // Pbar (in place of Dbar) and S_J / 2 into Q_G
             hipLaunchKernelGGL(lx_adj_diag_kernel, dim3(1), dim3(LX_WG_THREADS), 0, s, 1, jb, D, ldl, Dbar, lda,
-                               Zall + (size_t)J * 4096, Sb);
-            if (pin > 0 && m > 0) {
-                // inside the group: Bbar -= Qbar R, Rbar -= Qbar^T B (columns g0 .. j0)
+                               Zall + (size_t)J * 4096, Qs, ldqg, 0.5);
+            if (pin > 0) {
+                // inside the group (columns g0 .. j0), rows from block J on:
+                //   Abar[j0:n, .]    -= [S/2; Qbar] R         (Bbar and S R / 2)
+                //   Abar[J, .]       -= [S/2; Qbar]^T L[j0:n] (Qbar^T B and S^T R / 2)
                 const double *Rin = L + (int64_t)g0 * ldl + j0;
-                const double *Bin = L + (int64_t)g0 * ldl + j0 + jb;
-                if ((rc = lx_adj_gemm(0, 0, m, pin, jb, -1.0, Qb, ldqg, Rin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0 + jb,
+                if ((rc = lx_adj_gemm(0, 0, m + jb, pin, jb, -1.0, Qs, ldqg, Rin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0,
                                       lda, 0, part, part_elems, s)))
                     break;
-                if ((rc = lx_adj_gemm(1, 0, jb, pin, m, -1.0, Qb, ldqg, Bin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0,
+                if ((rc = lx_adj_gemm(1, 0, jb, pin, m + jb, -1.0, Qs, ldqg, Rin, ldl, 1.0, Ab + (int64_t)g0 * lda + j0,
                                       lda, 0, part, part_elems, s)))
                     break;
             }
-            // Rbar -= (Pbar + Pbar^T) R over every column left of block J
-            if (j0 > 0 && (rc = lx_adj_gemm(0, 0, jb, j0, jb, -1.0, Sb, 64, L + j0, ldl, 1.0, Ab + j0, lda, 0, part,
-                                             part_elems, s)))
-                break;
         }
-        if (rc == 0 && g0 > 0) {
+        if (rc) break;
+        // the group's Qbar blocks into Abar (rows below each block)
+        hipLaunchKernelGGL(lx_adj_qcopy_kernel, dim3((unsigned)((qtot + 255) / 256)), dim3(256), 0, s, (int)ldqg, gw,
+                           QG, ldqg, Ab + (int64_t)g0 * lda + g0, lda);
+        if (g0 > 0) {
             // the columns left of the group: the rank-gw products with Q_G
             if ((rc = lx_adj_gemm(0, 0, n - g0, g0, gw, -1.0, QG, ldqg, L + g0, ldl, 1.0, Ab + g0, lda, 0, part,
                                   part_elems, s)))

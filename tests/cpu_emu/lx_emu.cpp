@@ -495,7 +495,7 @@ extern "C" void lx_emu_adj_diag(int mode, int jb, const double *L, long long ldl
 {
     gridDim.x = 1;
     blockDim.x = 256;
-    run_wg([&] { lx_adj_diag_kernel(mode, jb, L, ldl, Ab, lda, Zb, Sb); });
+    run_wg([&] { lx_adj_diag_kernel(mode, jb, L, ldl, Ab, lda, Zb, Sb, 64, 1.0); });
 }
 
 // The adjoint's GEMM kernel over its whole grid, one workgroup at a time.

@@ -458,7 +458,7 @@ def test_adjoint_diag_kernel(jb):
     Lb = np.asfortranarray(np.linalg.cholesky(G @ G.T / jb + np.eye(jb)))
     Db = np.asfortranarray(np.tril(rng.standard_normal((jb, jb))))
     Z = np.zeros((64, 64), order="F")
-    S = np.zeros((64, 64), order="F")
+    S = np.full((64, 64), 7.0, order="F")  # entries past the block's order stay untouched
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     E.lx_emu_adj_diag(0, jb, p(Lb), jb, p(Db), jb, p(Z), p(S))
     assert np.abs(Z[:jb, :jb] - np.linalg.inv(Lb)).max() <= 1e-12 * np.abs(np.linalg.inv(Lb)).max()
@@ -469,6 +469,7 @@ def test_adjoint_diag_kernel(jb):
     assert np.abs(np.tril(out) - ref).max() <= 1e-12 * np.abs(ref).max()
     assert np.array_equal(np.triu(out, 1), np.triu(Db, 1))
     assert np.abs(S[:jb, :jb] - (ref + ref.T)).max() <= 1e-12 * np.abs(ref).max()
+    assert (S[jb:, :] == 7.0).all() and (S[:, jb:] == 7.0).all()
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
