@@ -198,13 +198,16 @@ int lx_adj_cus()
 }
 
 // C = alpha op(A) op(B) + beta C (lx_gemm64, FP64), split-K through part.
+// The sweep's skinny products (64 x 64 or 64 x 192 outputs over thousands of
+// rows) split K into chunks of 64: at 256 a chunk's workgroup walked its 16
+// slabs one load round trip at a time (about 20 us).
 int lx_adj_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, int64_t lda, const double *B,
                 int64_t ldb, double beta, double *C, int64_t ldc, int lower, double *part, size_t part_elems,
                 hipStream_t s)
 {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     return lx_gemm64<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower ? 1 : 0, part, part_elems,
-                             lx_adj_cus(), s);
+                             lx_adj_cus(), s, 64);
 }
 
 __global__ void lx_adj_zero_kernel(double *__restrict__ X, int64_t n)

@@ -126,17 +126,19 @@ __global__ void lx_gemm64_reduce_kernel(int M, int N, int ksplit, const T *__res
 
 // Launches C = alpha op(A) op(B) + beta C (ta / tb: 0 = N, 1 = T) with split-K
 // through part (part_elems values) when M x N has fewer than two tiles per CU
-// and K >= 512.  Returns 0 or LINOMPSS_ERR_HIP.
+// and K >= 2 min_chunk, in chunks of at least min_chunk (a multiple of 16).
+// Returns 0 or LINOMPSS_ERR_HIP.
 template <typename T>
 int lx_gemm64(int ta, int tb, int M, int N, int K, T alpha, const T *A, int64_t lda, const T *B, int64_t ldb,
-              T beta, T *C, int64_t ldc, int uplo, T *part, size_t part_elems, int cus, hipStream_t s)
+              T beta, T *C, int64_t ldc, int uplo, T *part, size_t part_elems, int cus, hipStream_t s,
+              int min_chunk = 256)
 {
     if (M <= 0 || N <= 0) return 0;
     const int tm = (M + LX_G64_T - 1) / LX_G64_T, tn = (N + LX_G64_T - 1) / LX_G64_T;
     const int target = 2 * cus;
     int ksplit = 1;
-    if ((int64_t)tm * tn < target && K >= 512 && part) {
-        ksplit = (int)std::min<int64_t>((target + (int64_t)tm * tn - 1) / ((int64_t)tm * tn), K / 256);
+    if ((int64_t)tm * tn < target && K >= 2 * min_chunk && part) {
+        ksplit = (int)std::min<int64_t>((target + (int64_t)tm * tn - 1) / ((int64_t)tm * tn), K / min_chunk);
         ksplit = (int)std::min<int64_t>(ksplit, (int64_t)(part_elems / ((size_t)M * N)));
         ksplit = std::max(1, ksplit);
     }
