@@ -106,7 +106,9 @@ lx_gemm64_kernel(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t 
             }
 }
 
-// C = beta C + sum_z part[z] (fixed order), within the uplo triangle.
+// C = beta C + sum_z part[z], within the uplo triangle.  The partials go into
+// eight running sums (z mod 8), so eight loads are in flight per thread, and
+// the sums meet in a fixed order: the same result on every run.
 template <typename T>
 __global__ void lx_gemm64_reduce_kernel(int M, int N, int ksplit, const T *__restrict__ part, T beta,
                                         T *__restrict__ C, int64_t ldc, int uplo)
@@ -115,8 +117,20 @@ __global__ void lx_gemm64_reduce_kernel(int M, int N, int ksplit, const T *__res
     if (e >= (int64_t)M * N) return;
     const int r = (int)(e % M), c = (int)(e / M);
     if ((uplo == 1 && r < c) || (uplo == 2 && r > c)) return;
-    T s = T(0);
-    for (int z = 0; z < ksplit; ++z) s += part[((int64_t)z * N + c) * M + r];
+    const T *p0 = part + (int64_t)c * M + r;
+    const int64_t zs = (int64_t)N * M;
+    T a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = T(0);
+    int z = 0;
+    for (; z + 8 <= ksplit; z += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += p0[(z + u) * zs];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (z + u < ksplit) a[u] += p0[(z + u) * zs];
+    const T s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     T *p = C + (int64_t)c * ldc + r;
     *p = beta == T(0) ? s : s + beta * *p;
 }
