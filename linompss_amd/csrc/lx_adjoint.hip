@@ -77,11 +77,21 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
         Zb += (int64_t)blockIdx.x * 4096;
     }
     const int tid = lx_tid(), lane = tid & 63, lr = lane & 15, lk = lane >> 4, r0 = 16 * lx_wave();
-    for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) {
-        const int r = e & 63, c = e >> 6;
+    // the blocks' loads (and, in mode 1, Z's) all issued before the LDS stores
+    double vl[16], vx[16], vz[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = tid + i * LX_WG_THREADS, r = e & 63, c = e >> 6;
         const bool in = r < jb && c < jb && r >= c;
-        sL[r][c] = in ? Ld[(int64_t)c * ldl + r] : (r == c ? 1.0 : 0.0);
-        sX[r][c] = (mode == 1 && in) ? Ab[(int64_t)c * lda + r] : 0.0;
+        vl[i] = in ? Ld[(int64_t)c * ldl + r] : (r == c ? 1.0 : 0.0);
+        vx[i] = (mode == 1 && in) ? Ab[(int64_t)c * lda + r] : 0.0;
+        vz[i] = mode == 1 ? Zb[e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = tid + i * LX_WG_THREADS, r = e & 63, c = e >> 6;
+        sL[r][c] = vl[i];
+        sX[r][c] = vx[i];
     }
     __syncthreads();
     if (mode == 0) {
@@ -103,7 +113,11 @@ lx_adj_diag_kernel(int mode, int jb, const double *__restrict__ Ld, int64_t ldl,
             const int r = r0 + lx_mfma<double>::row(lk, e), c = 16 * j + lr;
             sX[r][c] = r > c ? acc[j][e] : (r == c ? 0.5 * acc[j][e] : 0.0);
         }
-    for (int e = tid; e < 64 * 64; e += LX_WG_THREADS) sL[e & 63][e >> 6] = Zb[e];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = tid + i * LX_WG_THREADS;
+        sL[e & 63][e >> 6] = vz[i];
+    }
     __syncthreads();
     // T = Z^T Phi into sX
     lx_adj_mm64(acc, sL, true, sX);
