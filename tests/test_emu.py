@@ -349,6 +349,29 @@ def test_sweep2_kernels(emu_sweep2, orc, n, nr, C):
         assert np.array_equal(X[n:], b[n:])  # rows past n untouched
 
 
+@pytest.mark.parametrize("nr", [1, 3])
+@pytest.mark.parametrize("trans", [0, 1])
+def test_sweep2_nan_with_mailbox_bits(emu_sweep2, trans, nr):
+    """The 128-row sweeps with an input NaN whose bits equal the 'empty' mark
+    of the solved-value and partial-sum mailboxes: it reaches both as the
+    canonical quiet NaN, so the sweep ends (rc 0, no abort) with NaN exactly
+    where the solution depends on it."""
+    n, row = 520, 200
+    rng = np.random.default_rng(12)
+    L = np.asfortranarray(np.tril(rng.uniform(0.5, 1, (n, n))) + n * np.eye(n)).astype(np.float32)
+    X = np.asfortranarray(rng.uniform(-1, 1, (n, nr))).astype(np.float32)
+    X.view(np.uint32)[row, 0] = 0xFFFFFFFF
+    assert np.isnan(X[row, 0])
+    emu_sweep2(trans, L, X, 2)  # asserts rc == 0
+    # the 128-row block of `row` is multiplied by its full inverse: all of it turns NaN
+    blk = row // 128 * 128
+    dep = np.arange(n) >= blk if trans == 0 else np.arange(n) < blk + 128
+    assert np.isnan(X[dep, 0]).all()
+    assert np.isfinite(X[~dep, 0]).all()
+    if nr > 1:
+        assert np.isfinite(X[:, 1:]).all()
+
+
 def _emu_batch(sizes, mats, rhs, max_sweeps=30, tol=1e-14, stall=0.5):
     """lx_batch_ir_kernel on the CPU: one workgroup takes every system."""
     E = _load_emu()
