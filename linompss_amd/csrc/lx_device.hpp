@@ -941,19 +941,40 @@ __device__ __forceinline__ double lx_readlane(double v, int l)
 // sZ must be zero on entry and sL/sZ visible to the whole workgroup; all four
 // waves must call it (it contains workgroup barriers); sZ is complete and
 // visible on return.
+// 1/sqrt(x) for x > 0 to about 1 ulp: the hardware estimate, then Newton
+// steps r <- r + r (1 - x r^2) / 2 (two for FP64, one for FP32).  A short
+// dependency chain (the Cholesky pivots are sequential); x <= 0 gives NaN or
+// infinity, as 1/sqrt(x) does.
+template <typename T> __device__ __forceinline__ T lx_rsqrt(T x)
+{
+    if constexpr (sizeof(T) == 8) {
+        double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+        for (int it = 0; it < 2; ++it) r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+        return r;
+    } else {
+        float r = __builtin_amdgcn_rsqf(x);
+        return fmaf(0.5f * r, fmaf(-x * r, r, 1.0f), r);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void lx_tri_inv64(T (*sL)[65], T (*sZ)[65])
 {
     const int lane = lx_lane(), w = lx_wave(), lr = lane & 15, lk = lane >> 4;
     {
         const int b0 = 16 * w, c = lr;
-        T z[16];
+        T z[16], id[16];
+        // the reciprocals of the diagonal first: no division on the chain
+        // of the substitution
+#pragma unroll
+        for (int r = 0; r < 16; ++r) id[r] = T(1) / sL[b0 + r][b0 + r];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             T s = r == c ? T(1) : T(0);
 #pragma unroll
             for (int k = 0; k < r; ++k) s -= sL[b0 + r][b0 + k] * z[k];
-            z[r] = r >= c ? s / sL[b0 + r][b0 + r] : T(0);
+            z[r] = r >= c ? s * id[r] : T(0);
         }
         if (lane < 16) {
 #pragma unroll
@@ -1125,18 +1146,19 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
                 T a[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) a[j] = sD[lane][j0 + j];
+                // lanes above the diagonal of a column are the upper triangle,
+                // never read as data (the inverse below writes its scratch
+                // there before reading it; only the lower part is stored), so
+                // every lane takes the same arithmetic, without selects
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const int cc = j0 + c;
                     const T piv = lx_readlane(a[c], cc);
                     if (cc < pb && !(piv > T(0)) && first_bad == 0) first_bad = p + cc + 1;  // uniform
-                    const T d = sqrt(piv), rd = T(1) / d;
-                    a[c] = lane == cc ? d : (lane > cc ? a[c] * rd : a[c]);
+                    const T rd = lx_rsqrt<T>(piv);
+                    a[c] = lane == cc ? piv * rd : a[c] * rd;
 #pragma unroll
-                    for (int c2 = c + 1; c2 < 16; ++c2) {
-                        const T l2 = lx_readlane(a[c], j0 + c2);
-                        a[c2] = lane >= j0 + c2 ? a[c2] - a[c] * l2 : a[c2];
-                    }
+                    for (int c2 = c + 1; c2 < 16; ++c2) a[c2] -= a[c] * lx_readlane(a[c], j0 + c2);
                 }
 #pragma unroll
                 for (int j = 0; j < 16; ++j) sD[lane][j0 + j] = a[j];

@@ -15,10 +15,15 @@
 // * MFMA operand/result layout: lane l supplies A[l%16][l/16], B[l/16][l%16];
 //   lane l receives column l%16, rows 4*(l/16)+i (f32) or (l/16)+4*i (f64).
 #pragma once
+#include <linux/futex.h>
+#include <sys/syscall.h>
+#include <time.h>
+#include <unistd.h>
+
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
-#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -40,27 +45,40 @@ extern thread_local lx_emu_dim3 threadIdx;
 extern thread_local lx_emu_dim3 blockIdx;
 extern lx_emu_dim3 blockDim, gridDim;
 
+// A generation barrier on a futex: arrivals count up, the last one resets the
+// count, advances the generation and wakes the sleepers, which re-check the
+// generation themselves (no mutex to re-take, so a release does not queue the
+// 255 other lanes of a workgroup behind one lock).
 struct lx_emu_barrier {
-    std::mutex m;
-    std::condition_variable cv;
-    int n = 0, count = 0;
-    long gen = 0;
+    std::atomic<int> count{0}, gen{0};
+    int n = 0;
     const char *name = "";
-    void init(int parties, const char *nm) { n = parties; count = 0; gen = 0; name = nm; }
+    void init(int parties, const char *nm)
+    {
+        n = parties;
+        count.store(0);
+        gen.store(0);
+        name = nm;
+    }
     void wait()
     {
-        std::unique_lock<std::mutex> lk(m);
-        long g = gen;
-        if (++count == n) {
-            count = 0;
-            ++gen;
-            cv.notify_all();
+        const int g = gen.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+            count.store(0, std::memory_order_relaxed);
+            gen.fetch_add(1, std::memory_order_release);
+            syscall(SYS_futex, reinterpret_cast<int *>(&gen), FUTEX_WAKE_PRIVATE, INT_MAX, nullptr, nullptr, 0);
             return;
         }
-        if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return gen != g; })) {
-            fprintf(stderr, "lx_emu: %s barrier timeout (%d of %d arrived) -- mismatched barrier\n", name, count, n);
-            fflush(stderr);
-            std::_Exit(86);
+        const auto t0 = std::chrono::steady_clock::now();
+        while (gen.load(std::memory_order_acquire) == g) {
+            timespec ts{0, 200000000};  // re-check the deadline every 0.2 s
+            syscall(SYS_futex, reinterpret_cast<int *>(&gen), FUTEX_WAIT_PRIVATE, g, &ts, nullptr, 0);
+            if (gen.load(std::memory_order_acquire) == g && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+                fprintf(stderr, "lx_emu: %s barrier timeout (%d of %d arrived) -- mismatched barrier\n", name,
+                        count.load(), n);
+                fflush(stderr);
+                std::_Exit(86);
+            }
         }
     }
 };
@@ -180,6 +198,11 @@ inline int __builtin_amdgcn_readfirstlane(int v) { return lx_emu_xchg(v, 0); }
 inline int __builtin_amdgcn_readlane(int v, int l) { return lx_emu_xchg(v, l); }
 template <typename T> inline T __shfl_xor(T v, int mask) { return lx_emu_xchg(v, (threadIdx.x & 63) ^ mask); }
 
+// reciprocal square root estimates: the FP64 one at single precision only (a
+// coarser seed than the hardware's), so the emulated kernels show that their
+// Newton steps reach full precision from it
+inline double __builtin_amdgcn_rsq(double x) { return (double)(float)(1.0 / std::sqrt(x)); }
+inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
 // lanes are separate threads here, so a wave barrier must really synchronise them
 inline void __builtin_amdgcn_wave_barrier() { lx_emu_wave_barrier[threadIdx.x >> 6].wait(); }

@@ -411,7 +411,11 @@ def test_batch_kernel(orc):
         assert st["fallback"][i] == 0 and st["info"][i] == 0
         assert be <= 1e-14 and st["berr"][i] <= 1e-14
         xr = np.linalg.solve(A, b)
-        assert np.abs(x - xr).max() / np.abs(xr).max() <= np.linalg.cond(A) * 1e-14
+        # x - xr = A^-1 (r_xr - r_x): in the infinity norm the two backward
+        # errors bound the difference through cond_inf(A)
+        xn, rn = np.abs(x).max(), np.abs(xr).max()
+        bound = np.linalg.cond(A, np.inf) * (be * xn / rn + orc.backward_error(A, xr, b))
+        assert np.abs(x - xr).max() / rn <= 1.0001 * bound
 
 
 def test_batch_kernel_128_panels(orc):
