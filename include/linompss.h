@@ -429,7 +429,8 @@ int linompss_dberr_dev(int n, int nrhs, const double *dA, int64_t lda, const dou
  * update_prec): every later factor of that shape (linompss_chol_dev,
  * linompss_posv_dev, the refinement driver's low-precision factor, the
  * ompss_* entry points) uses it.  b > 0 keeps the tile size; b = 0 searches
- * it among 384, 512, 768 and 1024 (those <= n).  Needs n*n elements of
+ * it among 384, 512, 768 and 1024, and 128 and 256 when n <= 16384 (those
+ * <= n).  Needs n*n elements of
  * temporary device memory.  update_prec as in linompss_exec_opts (FP32 only).
  * tried (may be NULL) receives up to max_tried configurations in the order
  * they were timed, *ntried their count.  Returns 0, -i for argument i, or a

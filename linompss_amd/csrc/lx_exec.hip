@@ -141,8 +141,9 @@ const lx_cfg_entry *cfg_find(int dtype, int variant, int n, int b, int update_pr
 int default_update_group(int b) { return b > 0 ? std::max(1, std::min(8, 2048 / b)) : 1; }
 
 void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_opts *o,
-                 const lx_shape *sh = nullptr)
+                 const lx_shape *sh = nullptr, int b_hint = 0)
 {
+    const int tb = sh ? sh->b : b_hint;
     P.unit_m = LX_UNIT_M;
     P.unit_n = LX_UNIT_N;
     P.elem_bytes = dtype == 0 ? 8 : 4;
@@ -154,9 +155,12 @@ void fill_params(lx_plan_params &P, int dtype, int workers, const linompss_exec_
     }
     if (o) P.policy = o->policy;
     if (o && dtype == 1 && (o->update_prec == 1 || o->update_prec == 2)) P.update_prec = o->update_prec;
-    // FP64: 256-wide diagonal sub-tiles (fewer, longer steps on the critical
-    // path; measured at n = 32768, b = 512: 235 ms against 255 ms with 128)
-    if (dtype == 0) P.potrf_sub = 256;
+    // FP64: 256-wide diagonal sub-tiles for tiles of 512 and more (fewer,
+    // longer steps on the critical path; measured at n = 32768, b = 512: 208.2
+    // ms against 209.0 with 128), 128-wide below, where a 256-wide unit's own
+    // trailing updates dominate it (n = 8192: b = 256 12.25 ms against 13.50,
+    // b = 384 13.5 against 15.1)
+    if (dtype == 0) P.potrf_sub = tb > 0 && tb < 512 ? 128 : 256;
     // FP64: 128-column GEMM units in the second half of the panels (more units
     // while the trailing matrix shrinks; 229.0 against 232.1 ms at n = 32768,
     // b = 512; the FP32 factor measured no gain at any tail length)
@@ -926,7 +930,7 @@ void lx_cfg_defaults(int dtype, int update_prec, int b, linompss_factor_config *
     o.policy = 1;
     o.update_prec = update_prec;
     lx_plan_params P;
-    fill_params(P, dtype, 512, &o);
+    fill_params(P, dtype, 512, &o, nullptr, b);
     P.update_group = default_update_group(b);
     if (const char *ev = getenv("LINOMPSS_UPDATE_GROUP")) {
         const int v = atoi(ev);

@@ -22,12 +22,13 @@ def test_tune_records_and_factors_stay_exact(lx, orc, dev, dtype, variant, updat
     n = 2048
     print(f"[tune] n={n} dtype={np.dtype(dtype)} {variant} up={update_prec}", flush=True)
     best, tried = lx.tune_factor(n, 0, dtype, variant, update_prec, reps=1)
-    assert best.b in (384, 512, 768, 1024) and best.ms > 0
+    tiles = {128, 256, 384, 512, 768, 1024}  # the small tiles are searched up to n = 16384
+    assert best.b in tiles and best.ms > 0
     assert len(tried) >= 10 and min(t["ms"] for t in tried) == pytest.approx(best.ms)
-    assert {t["b"] for t in tried} == {384, 512, 768, 1024}
+    assert {t["b"] for t in tried} == tiles
     rec, got = lx.get_factor_config(n, best.b, dtype, variant, update_prec)
     assert rec and got.as_dict() == best.as_dict()
-    for b in {384, 512, 768, 1024} - {best.b}:  # losing tile sizes leave nothing behind
+    for b in tiles - {best.b}:  # losing tile sizes leave nothing behind
         assert not lx.get_factor_config(n, b, dtype, variant, update_prec)[0]
     # a factor of that shape now runs with the recorded configuration
     A0 = orc.genmat_spd(n, seed=5, diag_mode=1)

@@ -14,6 +14,9 @@ def test_defaults_and_roundtrip(lx):
     assert not rec and d64.b == 512 and d64.ms == 0.0
     assert (d64.potrf_sub, d64.gemm_wide_m, d64.gemm_wide_n, d64.trsm_rows_wide, d64.wide_tail,
             d64.update_group) == (256, 256, 512, 64, -1, 4)
+    # FP64 tiles below 512: 128-wide diagonal sub-tiles
+    assert lx.get_factor_config(8192, 256)[1].potrf_sub == 128
+    assert lx.get_factor_config(8192, 384)[1].potrf_sub == 128
     rec, d32 = lx.get_factor_config(32768, 512, np.float32, update_prec=2)
     assert not rec and (d32.potrf_sub, d32.gemm_wide_m, d32.trsm_rows_wide, d32.wide_tail, d32.update_group) == \
         (128, 512, 128, 0, 4)
