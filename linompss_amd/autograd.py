@@ -6,7 +6,8 @@ The functions take and return CPU float64 tensors; every factor, solve and
 adjoint runs on the GPU inside liblinompss (host arrays are staged through
 device buffers by the library).  The Cholesky backward recomputes the factor
 from A on the device instead of keeping it from the forward pass
-(linompss_dpotrf_adjoint).
+(linompss_dpotrf_adjoint); the Gaussian-process likelihoods keep their factor
+on the device between the passes.
 
 Import this module (or ``linompss_amd``) before anything that initialises
 torch's own GPU runtime: torch bundles a HIP runtime with the same soname,
@@ -15,6 +16,7 @@ and the process keeps whichever copy is loaded first.
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
 
 import numpy as np
@@ -130,23 +132,21 @@ class TriSolve(torch.autograd.Function):
 
 
 class GaussianNLL(torch.autograd.Function):
-    """nll(K, y) = y^T K^-1 y / 2 + log det K / 2 + n log(2 pi) / 2 with K kept
-    on the device between the passes: forward = FP64 factor, one sweep and the
-    log det; backward = the factor recomputed from K, the solve adjoint and the
-    Cholesky adjoint (with the log det term), so only K goes up and dK
-    (symmetric) comes down.  ``timing`` (a dict, optional) receives the device
-    seconds of each pass."""
+    """nll(K, y) = y^T K^-1 y / 2 + log det K / 2 + n log(2 pi) / 2 with the
+    factor L kept on the device between the passes: forward = FP64 factor, one
+    sweep and the log det; backward = the solve adjoint and the Cholesky
+    adjoint (with the log det term) on the forward's L, so only K goes up and
+    dK (symmetric) comes down.  ``timing`` (a dict, optional) receives the
+    device seconds of each pass."""
 
     @staticmethod
     def forward(ctx, K: torch.Tensor, y: torch.Tensor, b: int = 256, timing: dict | None = None):
         import time
         n = K.shape[0]
-        dK = DeviceMatrix.from_host(_f64(K))
-        dL = DeviceMatrix(n, n)
+        dL = DeviceMatrix.from_host(_f64(K))
         da = DeviceMatrix.from_host(y.detach().to(torch.float64).numpy().reshape(-1, 1))
         lib.linompss_dev_sync()
         t0 = time.perf_counter()
-        copy_dev(dL, dK)
         _check("linompss_chol_dev", lib.linompss_chol_dev(0, 0, n, b, dL.ptr, n, None, None))
         _check("linompss_trsm_dev", lib.linompss_trsm_dev(0, 0, n, 1, dL.ptr, n, da.ptr, n, None), False)
         ld = _d(0.0)
@@ -155,8 +155,7 @@ class GaussianNLL(torch.autograd.Function):
         if timing is not None:
             timing["forward_s"] = time.perf_counter() - t0
         a = da.to_host()[:, 0]
-        dL.free()  # the factor is recomputed in backward
-        ctx.dK, ctx.da, ctx.b, ctx.timing = dK, da, b, timing
+        ctx.dL, ctx.da, ctx.b, ctx.timing = dL, da, b, timing
         ctx.dtype = K.dtype
         nll = 0.5 * float(a @ a) + 0.5 * ld.value + 0.5 * n * math.log(2.0 * math.pi)
         return torch.tensor(nll, dtype=K.dtype)
@@ -164,15 +163,12 @@ class GaussianNLL(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g: torch.Tensor):
         import time
-        dK, da, b = ctx.dK, ctx.da, ctx.b
-        n = dK.rows
-        dL = DeviceMatrix(n, n)
+        dL, da = ctx.dL, ctx.da
+        n = dL.rows
         dLb = DeviceMatrix(n, n)
         dbb = DeviceMatrix(n, 1)
         lib.linompss_dev_sync()
         t0 = time.perf_counter()
-        copy_dev(dL, dK)
-        _check("linompss_chol_dev", lib.linompss_chol_dev(0, 0, n, b, dL.ptr, n, None, None))
         copy_dev(dbb, da)  # abar = a (from a^T a / 2)
         # bbar = L^-T a (= K^-1 y) and Lbar = -tril(bbar a^T), written without reading Lbar
         _check("linompss_dtrsm_adjoint_dev", lib.linompss_dtrsm_adjoint_dev(
@@ -185,9 +181,10 @@ class GaussianNLL(torch.autograd.Function):
             ctx.timing["backward_s"] = time.perf_counter() - t0
         Ab = dLb.to_host()
         ybar = dbb.to_host()[:, 0]
-        # K and a stay on the device until ctx is released, so a second
-        # backward (retain_graph=True) recomputes the same gradient
-        for m in (dL, dLb, dbb):
+        # L and a are only read here and stay on the device until ctx is
+        # released, so a second backward (retain_graph=True) gives the same
+        # gradient
+        for m in (dLb, dbb):
             m.free()
         low = np.tril(Ab, -1)
         S = 0.5 * (low + low.T) + np.diag(np.diag(Ab))
@@ -225,10 +222,13 @@ def gp_nll_fused(points: torch.Tensor, y: torch.Tensor, lengthscale: torch.Tenso
 
 class _GPWork:
     """Device buffers of one problem shape, reused across calls: K (the factor
-    in place), Kbar, the points, y, a = L^-1 y, bbar and dX."""
+    in place), Kbar, the points, y, a = L^-1 y, bbar and dX.  ``owner`` names
+    the forward pass whose points, y, factor and a the buffers hold (0: none);
+    the backward passes only read those four."""
 
     def __init__(self, n: int, d: int):
         self.n, self.d = n, d
+        self.owner = 0
         self.K = DeviceMatrix(n, n)
         self.Kb = DeviceMatrix(n, n)
         self.P = DeviceMatrix(n, d)
@@ -243,6 +243,7 @@ class _GPWork:
 
 
 _GP_WS: dict = {}
+_GP_PASS = itertools.count(1)  # forward passes of GPMarginalLikelihood
 
 
 def _gp_work(n: int, d: int) -> _GPWork:
@@ -297,9 +298,10 @@ class GPMarginalLikelihood(torch.autograd.Function):
     """nll(X, y; ell, sf2, noise) = y^T K^-1 y / 2 + log det K / 2 + n log(2 pi) / 2
     with K_ij = sf2 exp(-|x_i - x_j|^2 / (2 ell^2)) + noise delta_ij built on
     the device from the points: only X (n x d), y and scalars cross PCIe.
-    Forward: K, FP64 factor in place, one sweep, log det.  Backward: K and the
-    factor recomputed (nothing n x n is kept between the passes), the solve
-    adjoint, the blocked Cholesky adjoint with the log det term, then the
+    Forward: K, FP64 factor in place, one sweep, log det.  Backward: the
+    forward's factor and a = L^-1 y from the shape's workspace (recomputed only
+    when another forward of the same shape has used the workspace since), the
+    solve adjoint, the blocked Cholesky adjoint with the log det term, then the
     contraction of Kbar with dK/dell, dK/dsf2, dK/dnoise and dK/dX on the
     device.  Gradients flow to X, y and to ell / sf2 / noise when they are
     tensors.  ``timing`` (a dict, optional) receives each pass's device seconds."""
@@ -316,6 +318,7 @@ class GPMarginalLikelihood(torch.autograd.Function):
         lib.linompss_dev_sync()
         t0 = time.perf_counter()
         logdet = _gp_forward_dev(ws, *vals, b)
+        ws.owner = ctx.owner = next(_GP_PASS)
         lib.linompss_dev_sync()
         if timing is not None:
             timing["forward_s"] = time.perf_counter() - t0
@@ -334,11 +337,15 @@ class GPMarginalLikelihood(torch.autograd.Function):
         P = _f64(points)
         n, d = P.shape
         ws = _gp_work(n, d)
-        ws.P.upload(P)
-        ws.y.upload(y.detach().to(torch.float64).numpy().reshape(-1, 1))
+        recompute = ws.owner != ctx.owner
+        if recompute:
+            ws.P.upload(P)
+            ws.y.upload(y.detach().to(torch.float64).numpy().reshape(-1, 1))
         lib.linompss_dev_sync()
         t0 = time.perf_counter()
-        _gp_forward_dev(ws, ell, sf2, noise, ctx.b)  # K, L and a recomputed
+        if recompute:  # another forward of this shape ran in between
+            _gp_forward_dev(ws, ell, sf2, noise, ctx.b)
+            ws.owner = ctx.owner
         copy_dev(ws.bb, ws.a)  # abar = a (from a^T a / 2)
         # bbar = L^-T a (= K^-1 y) and Lbar = -tril(bbar a^T), written without reading Lbar
         _check("linompss_dtrsm_adjoint_dev", lib.linompss_dtrsm_adjoint_dev(

@@ -120,7 +120,7 @@ def test_gp_autograd(lx, dev, n):
 
 @pytest.mark.parametrize("n", [300, 2048])
 def test_gp_fused(lx, dev, n):
-    """GaussianNLL (K resident on the device; factor recomputed in backward):
+    """GaussianNLL (the forward's factor kept on the device for backward):
     value, dL/dl, dL/dK and dL/dy against the FP64 closed forms."""
     import torch
     from oracle import adjoint as adj
@@ -148,9 +148,9 @@ def test_gp_fused(lx, dev, n):
 
 
 def test_gp_fused_second_backward(lx, dev):
-    """backward twice on one graph (retain_graph=True): K and a stay on the
-    device until the graph is released, so the second pass recomputes the
-    same gradient instead of reading freed buffers."""
+    """backward twice on one graph (retain_graph=True): L and a stay on the
+    device until the graph is released and backward only reads them, so the
+    second pass gives the same gradient instead of reading freed buffers."""
     import torch
     from oracle import adjoint as adj
     import linompss_amd.autograd as ag

@@ -87,3 +87,35 @@ def test_gp_device_scalars_as_floats_and_reuse(lx, dev):
         grads.append(ell.grad.item())
     assert grads[1] == grads[2] and grads[0] != grads[1]
     ag.gp_release_workspace()
+
+
+def test_gp_device_interleaved_passes(lx, dev):
+    """Backward reuses the forward's factor from the shape's workspace only
+    while no other forward of that shape has run: two graphs of one shape
+    with their forwards first (the first backward recomputes its factor), and
+    a retained second backward, give the gradients of a forward followed at
+    once by its backward, bit for bit."""
+    import torch
+    import linompss_amd.autograd as ag
+    n = 700
+    print(f"[gp interleaved] n={n}", flush=True)
+    P, y = _problem(n, 11, 0.3, 1e-2)
+
+    def run(ell0):
+        ell = torch.tensor(ell0, dtype=torch.float64, requires_grad=True)
+        return ell, ag.gp_nll_device(torch.from_numpy(P), torch.from_numpy(y), ell, 1.0, 1e-2)
+
+    ref = {}
+    for ell0 in (0.3, 0.35):
+        ell, loss = run(ell0)
+        loss.backward()
+        ref[ell0] = (loss.item(), ell.grad.item())
+    (ea, la), (eb, lb) = run(0.3), run(0.35)
+    la.backward(retain_graph=True)  # the workspace holds the 0.35 forward: recomputed
+    lb.backward()                   # and now the 0.3 one: recomputed again
+    assert (la.item(), ea.grad.item()) == ref[0.3]
+    assert (lb.item(), eb.grad.item()) == ref[0.35]
+    ea.grad = None
+    la.backward()                   # the workspace still holds the 0.3 pass: reused
+    assert ea.grad.item() == ref[0.3][1]
+    ag.gp_release_workspace()
