@@ -130,10 +130,22 @@ __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, 
     for (int i = tid; i < n; i += LX_WG_THREADS) sv[i] = v[i];
     for (int qi = 0; qi < nb; ++qi) {
         const int q = trans ? nb - 1 - qi : qi, p = q * 64, pb = n - p < 64 ? n - p : 64;
+        // four loads in flight per thread, then their LDS stores (left to
+        // itself the scheduler of this kernel issues one load at a time)
 #pragma unroll
-        for (int q2 = 0; q2 < 64 * 64 / LX_WG_THREADS; ++q2) {
-            const int e = tid + q2 * LX_WG_THREADS, r = e & 63, c = e >> 6;
-            sZ[r][c] = W[(int64_t)c * ldw + p + r];
+        for (int q0 = 0; q0 < 64 * 64 / LX_WG_THREADS; q0 += 4) {
+            T zv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = tid + (q0 + k) * LX_WG_THREADS;
+                zv[k] = W[(int64_t)(e >> 6) * ldw + p + (e & 63)];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = tid + (q0 + k) * LX_WG_THREADS;
+                sZ[e & 63][e >> 6] = zv[k];
+            }
         }
         __syncthreads();  // sZ, and sv after the previous block's updates
         T z = T(0);
@@ -159,11 +171,14 @@ __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, 
                 const T *Lr = L + (int64_t)p * ld + r;
 #pragma unroll
                 for (int c0 = 0; c0 < 64; c0 += 16) {
+                    // unconditional loads (columns clamped into the block, the
+                    // clamped ones zeroed after): a load under a condition is
+                    // a branch and waits for its data before the next issues
                     T a[16];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lr[(int64_t)(c0 + k) * ld] : T(0);
+                    for (int k = 0; k < 16; ++k) a[k] = Lr[(int64_t)(c0 + k < pb ? c0 + k : pb - 1) * ld];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
+                    for (int k = 0; k < 16; ++k) acc[k & 3] += (c0 + k < pb ? a[k] : T(0)) * sy[c0 + k];
                     __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
                 }
             } else if (pb == 64) {
@@ -191,9 +206,9 @@ __device__ __forceinline__ void lx_b_sweep(const T *__restrict__ L, int64_t ld, 
                 for (int c0 = 0; c0 < 64; c0 += 16) {
                     T a[16];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) a[k] = c0 + k < pb ? Lc[c0 + k] : T(0);
+                    for (int k = 0; k < 16; ++k) a[k] = Lc[c0 + k < pb ? c0 + k : pb - 1];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) acc[k & 3] += a[k] * sy[c0 + k];
+                    for (int k = 0; k < 16; ++k) acc[k & 3] += (c0 + k < pb ? a[k] : T(0)) * sy[c0 + k];
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }

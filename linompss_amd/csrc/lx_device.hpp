@@ -192,6 +192,8 @@ __device__ __forceinline__ void lx_acc_sub_store(typename lx_mfma<T>::acc (&acc)
                 v[j][e] = C[(long long)cc * ldc + rr];
             }
         }
+        // keep the row's loads together where the kernel is short of registers
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int c = c0 + j * 16 + lr;
@@ -1121,12 +1123,16 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
     int first_bad = 0;
     for (int p = 0; p < NC; p += PB) {
         const int pb = M - p < PB ? M - p : PB;
-        // load D (full square, identity pad past pb); clear Z.  A constant trip
-        // count, unrolled, so several loads are in flight per thread
+        // load D (full square, identity pad past pb); clear Z.  The loads are
+        // unconditional, their indices clamped into the block (a load under a
+        // per-lane condition is a branch, and each one then waits for its own
+        // data before the next issues); a constant trip count, unrolled, so
+        // several loads are in flight per thread
 #pragma unroll 4
         for (int q = 0; q < PB * PB / LX_WG_THREADS; ++q) {
             const int e = tid + q * LX_WG_THREADS, r = e % PB, c = e / PB;
-            sD[r][c] = (r < pb && c < pb) ? A[(long long)(p + c) * lda + p + r] : (r == c ? T(1) : T(0));
+            const T v = A[(long long)(p + (c < pb ? c : pb - 1)) * lda + p + (r < pb ? r : pb - 1)];
+            sD[r][c] = (r < pb && c < pb) ? v : (r == c ? T(1) : T(0));
             sZ[r][c] = T(0);
         }
         __syncthreads();
@@ -1216,12 +1222,23 @@ __device__ __forceinline__ void lx_unit_potrf(T *__restrict__ A, long long lda, 
             typename lx_mfma<T>::acc acc[1][4];
             lx_acc_zero<T, 1, 4>(acc);
             lx_trsm_update_lds<T>(acc, Pp, lda, nrow, Pp, lda, nrow, PB, smem);
+            // every load of C first (indices clamped into the block), then the
+            // masked stores: a read-modify-write under the mask would wait a
+            // memory round trip per element
+            T cv[4][4];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int r = 16 * w + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
-                    if (r < nrow && c <= r) Ct[(long long)c * lda + r] -= acc[0][j][e];
+                    cv[j][e] = Ct[(long long)(c < nrow ? c : nrow - 1) * lda + (r < nrow ? r : nrow - 1)];
+                }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * w + lx_mfma<T>::row(lk, e), c = j * 16 + lr;
+                    if (r < nrow && c <= r) Ct[(long long)c * lda + r] = cv[j][e] - acc[0][j][e];
                 }
         } else if (nrow > 0) {
             constexpr int BS = 128;
