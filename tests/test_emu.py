@@ -56,8 +56,8 @@ def test_factor_fp64(emu, orc, what, n, b):
     assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
 
 
-@pytest.mark.parametrize("dtype,n,b,g", [(np.float64, 150, 32, 2), (np.float32, 160, 32, 2),
-                                         (np.float64, 150, 32, 3), (np.float64, 192, 32, 4)])
+@pytest.mark.parametrize("dtype,n,b,g", [(np.float64, 112, 32, 2), (np.float32, 160, 32, 2),
+                                         (np.float64, 150, 32, 3), (np.float64, 160, 32, 4)])
 def test_factor_grouped_updates(emu, orc, dtype, n, b, g):
     """RL plans with grouped trailing updates (tiles right of a group of g
     steps take the group's g panels in one K = g b pass): the factor matches
